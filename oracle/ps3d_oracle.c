@@ -1,0 +1,884 @@
+/*
+ * ps3d_oracle.c -- CPU restatement of the ps3D model time step (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path, never the thing
+ * measured or shipped.  See ps3d_oracle.h for who may load it.
+ *
+ * Every routine follows one reference routine (cited per function) loop by
+ * loop, with the same index ranges and the same association of floating-point
+ * operations, so that on the same inputs it reproduces the reference to the
+ * last bit where the reference is compiled without FMA contraction.  Build
+ * with -ffp-contract=off.
+ *
+ * The FFT is not the reference's (P3DFFT over FFTW, absent here): it is a
+ * plain mixed-radix Cooley-Tukey DFT with exact table twiddles, unnormalised,
+ * e^{-i} forward, c2r ignoring the imaginary parts of the kx=0 and kx=nx/2
+ * modes (the FFTW/P3DFFT conventions used at my_fft.f90:159-205).
+ */
+#include "ps3d_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct { double re, im; } cplx;
+
+typedef struct {
+    int n;
+    int nfac;
+    int fac[32];
+    cplx *tw;      /* tw[t] = exp(-2 pi i t / n), t = 0..n-1 */
+    cplx *scratch; /* 2n */
+} plan1d;
+
+struct orc_model {
+    orc_params p;
+    int64_t X, Y, Z, H;       /* padded extents and field size */
+    int64_t nkx, S;           /* nx/2+1 and spectral size */
+    double *f[ORC_NFIELDS];
+    int64_t len[ORC_NFIELDS];
+    double *aloc;             /* nx*ny*nz */
+    double *del2;             /* H, the automatic array of mixing.f90 */
+    plan1d px, py, pz;
+    cplx *row;                /* max(nx,ny,nz) */
+};
+
+#define ONX 2
+
+static inline int64_t IX(const orc_model *m, int i, int j, int k)
+{
+    return (int64_t)(i + 1) + m->X * ((int64_t)(j + 1) + m->Y * (int64_t)(k + 1));
+}
+#define A(arr, i, j, k) (arr)[IX(m, (i), (j), (k))]
+
+/* ------------------------------------------------------------------------ */
+/* 1-D complex DFT                                                           */
+/* ------------------------------------------------------------------------ */
+
+static void plan_init(plan1d *pl, int n)
+{
+    pl->n = n;
+    pl->nfac = 0;
+    int r = n;
+    const int pref[3] = {4, 2, 3};
+    (void)pref;
+    for (int f = 2; r > 1;) {
+        if (r % f == 0) { pl->fac[pl->nfac++] = f; r /= f; }
+        else f = (f == 2) ? 3 : f + 2;
+    }
+    pl->tw = (cplx *)malloc(sizeof(cplx) * (size_t)(n > 0 ? n : 1));
+    for (int t = 0; t < n; ++t) {
+        double ang = 2.0 * M_PI * (double)t / (double)n;
+        pl->tw[t].re = cos(ang);
+        pl->tw[t].im = -sin(ang);
+    }
+    pl->scratch = (cplx *)malloc(sizeof(cplx) * (size_t)(2 * n + 2));
+}
+
+static void plan_free(plan1d *pl)
+{
+    free(pl->tw);
+    free(pl->scratch);
+}
+
+/* Recursive decimation in time.  in: n values at stride s; out: n contiguous.
+   W_n^{q} = tw[(q * (N/n)) mod N] of the top-level table (size N). */
+static void dft_rec(const plan1d *pl, int fi, int n, const cplx *in, int64_t s,
+                    cplx *out, int sign)
+{
+    if (n == 1) { out[0] = in[0]; return; }
+    const int N = pl->n;
+    const int p = pl->fac[fi];
+    const int mlen = n / p;
+    for (int r = 0; r < p; ++r)
+        dft_rec(pl, fi + 1, mlen, in + r * s, s * p, out + r * mlen, sign);
+    /* combine: X[k] = sum_r W_n^{r k} Y_r[k mod m] */
+    cplx tmp[64];
+    cplx *acc = (n <= 64) ? tmp : (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    const int stride = N / n;
+    for (int k = 0; k < n; ++k) {
+        const int km = k % mlen;
+        double re = out[km].re, im = out[km].im;
+        for (int r = 1; r < p; ++r) {
+            int q = (int)(((int64_t)r * k) % n) * stride;
+            cplx w = pl->tw[q];
+            if (sign > 0) w.im = -w.im;
+            const cplx y = out[r * mlen + km];
+            re += y.re * w.re - y.im * w.im;
+            im += y.re * w.im + y.im * w.re;
+        }
+        acc[k].re = re;
+        acc[k].im = im;
+    }
+    memcpy(out, acc, sizeof(cplx) * (size_t)n);
+    if (acc != tmp) free(acc);
+}
+
+static void cfft_strided(const plan1d *pl, cplx *data, int64_t stride, int sign)
+{
+    const int n = pl->n;
+    cplx *in = pl->scratch;
+    cplx *out = pl->scratch + n;
+    for (int t = 0; t < n; ++t) in[t] = data[t * stride];
+    dft_rec(pl, 0, n, in, 1, out, sign);
+    for (int t = 0; t < n; ++t) data[t * stride] = out[t];
+}
+
+void orc_cfft(int n, double *data, int sign)
+{
+    plan1d pl;
+    plan_init(&pl, n);
+    cfft_strided(&pl, (cplx *)data, 1, sign);
+    plan_free(&pl);
+}
+
+/* ------------------------------------------------------------------------ */
+/* model                                                                     */
+/* ------------------------------------------------------------------------ */
+
+orc_model *orc_create(const orc_params *prm)
+{
+    orc_model *m = (orc_model *)calloc(1, sizeof(orc_model));
+    m->p = *prm;
+    const int nx = prm->nx, ny = prm->ny, nz = prm->nz;
+    m->X = nx + 2 * ONX;
+    m->Y = ny + 2 * ONX;
+    m->Z = nz + 2 * ONX;
+    m->H = m->X * m->Y * m->Z;
+    m->nkx = nx / 2 + 1;
+    m->S = m->nkx * ny * nz;
+    for (int id = ORC_U; id <= ORC_P; ++id) m->len[id] = m->H;
+    for (int id = ORC_DU; id <= ORC_DB; ++id) m->len[id] = 3 * m->H;
+    for (int id = ORC_FLUX_EAST; id <= ORC_FLUX_TOP; ++id) m->len[id] = m->H;
+    m->len[ORC_FS] = 2 * m->S;
+    m->len[ORC_KSQR] = prm->enable_vertical_boundaries ? m->nkx * ny : m->S;
+    m->len[ORC_KX] = m->nkx;
+    m->len[ORC_KY] = ny;
+    m->len[ORC_KZ] = nz;
+    for (int id = 0; id < ORC_NFIELDS; ++id)
+        m->f[id] = (double *)calloc((size_t)m->len[id], sizeof(double));
+    m->aloc = (double *)calloc((size_t)nx * ny * nz, sizeof(double));
+    m->del2 = (double *)calloc((size_t)m->H, sizeof(double));
+    plan_init(&m->px, nx);
+    plan_init(&m->py, ny);
+    plan_init(&m->pz, nz);
+    int nmax = nx > ny ? nx : ny;
+    nmax = nmax > nz ? nmax : nz;
+    m->row = (cplx *)malloc(sizeof(cplx) * (size_t)nmax);
+    orc_wavenumbers(m);
+    return m;
+}
+
+void orc_destroy(orc_model *m)
+{
+    if (!m) return;
+    for (int id = 0; id < ORC_NFIELDS; ++id) free(m->f[id]);
+    free(m->aloc);
+    free(m->del2);
+    plan_free(&m->px);
+    plan_free(&m->py);
+    plan_free(&m->pz);
+    free(m->row);
+    free(m);
+}
+
+double *orc_ptr(orc_model *m, int field)
+{
+    if (field < 0 || field >= ORC_NFIELDS) return NULL;
+    return m->f[field];
+}
+
+int64_t orc_field_len(const orc_model *m, int field)
+{
+    if (field < 0 || field >= ORC_NFIELDS) return -1;
+    return m->len[field];
+}
+
+/* eigenvectors.f90:224-229 */
+double orc_hat_ksqr(double k, double dx)
+{
+    return 2.0 * (1.0 - cos(k * dx)) / (dx * dx);
+}
+
+/* my_fft.f90:87-156 (wavenumber grid and ksqr; the vertical modes phi, psi,
+   psib belong to the balancing diagnostic and are not needed here) */
+void orc_wavenumbers(orc_model *m)
+{
+    const orc_params *p = &m->p;
+    const double pi = acos(0.0) * 2.0;   /* main_module.f90:116 */
+    double *kx = m->f[ORC_KX], *ky = m->f[ORC_KY], *kz = m->f[ORC_KZ];
+    for (int i = 1; i <= p->nx / 2 + 1; ++i) kx[i - 1] = (double)(2 * (i - 1)) * pi / p->Lx;
+    for (int j = 1; j <= (p->ny + 1) / 2; ++j) ky[j - 1] = (double)(2 * (j - 1)) * pi / p->Ly;
+    for (int j = (p->ny + 1) / 2 + 1; j <= p->ny; ++j)
+        ky[j - 1] = -((double)(2 * (p->ny - j + 1)) * pi / p->Ly);
+    if (p->enable_vertical_boundaries) {
+        for (int k = 1; k <= p->nz; ++k) kz[k - 1] = (double)(float)(k - 1.0f) * pi / p->Lz;
+    } else {
+        for (int k = 1; k <= (p->nz + 1) / 2; ++k) kz[k - 1] = (double)(2 * (k - 1)) * pi / p->Lz;
+        for (int k = (p->nz + 1) / 2 + 1; k <= p->nz; ++k)
+            kz[k - 1] = -((double)(2 * (p->nz - k + 1)) * pi / p->Lz);
+    }
+    double *ks = m->f[ORC_KSQR];
+    const int64_t nkx = m->nkx;
+    if (p->enable_vertical_boundaries) {
+        for (int j = 0; j < p->ny; ++j)
+            for (int i = 0; i < nkx; ++i)
+                ks[i + nkx * j] = orc_hat_ksqr(kx[i], p->dx) + orc_hat_ksqr(ky[j], p->dy);
+    } else {
+        for (int k = 0; k < p->nz; ++k)
+            for (int j = 0; j < p->ny; ++j)
+                for (int i = 0; i < nkx; ++i)
+                    ks[i + nkx * (j + (int64_t)p->ny * k)] =
+                        orc_hat_ksqr(kx[i], p->dx) + orc_hat_ksqr(ky[j], p->dy) +
+                        orc_hat_ksqr(kz[k], p->dz) / p->dsqr;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* advection.f90                                                             */
+/* ------------------------------------------------------------------------ */
+
+static void lid_zero_flux_top(orc_model *m, double *ft, int top_from)
+{
+    /* flux_top(:,:,1-onx:0) = 0 ; flux_top(:,:,top_from:nz+onx) = 0 */
+    const orc_params *p = &m->p;
+    if (!p->enable_vertical_boundaries) return;
+    for (int k = 1 - ONX; k <= 0; ++k)
+        memset(&A(ft, -1, -1, k), 0, sizeof(double) * (size_t)(m->X * m->Y));
+    for (int k = top_from; k <= p->nz + ONX; ++k)
+        memset(&A(ft, -1, -1, k), 0, sizeof(double) * (size_t)(m->X * m->Y));
+}
+
+/* advection.f90:2-48 */
+void orc_buoyancy_advection_2nd(orc_model *m, double *db_loc)
+{
+    const orc_params *p = &m->p;
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    double *u = m->f[ORC_U], *v = m->f[ORC_V], *w = m->f[ORC_W], *b = m->f[ORC_B];
+    double *fe = m->f[ORC_FLUX_EAST], *fn = m->f[ORC_FLUX_NORTH], *ft = m->f[ORC_FLUX_TOP];
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 0; i <= nx; ++i)
+                A(fe, i, j, k) = 0.5 * (A(b, i, j, k) + A(b, i + 1, j, k)) * A(u, i, j, k);
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(fn, i, j, k) = 0.5 * (A(b, i, j, k) + A(b, i, j + 1, k)) * A(v, i, j, k);
+    for (int k = 0; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(ft, i, j, k) = 0.5 * (A(b, i, j, k) + A(b, i, j, k + 1)) * A(w, i, j, k);
+    lid_zero_flux_top(m, ft, nz);
+    const double Ro = p->Ro, dx = p->dx, dy = p->dy, dz = p->dz;
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(db_loc, i, j, k) = -(Ro * ((A(fe, i, j, k) - A(fe, i - 1, j, k)) / dx +
+                                             (A(fn, i, j, k) - A(fn, i, j - 1, k)) / dy +
+                                             (A(ft, i, j, k) - A(ft, i, j, k - 1)) / dz));
+}
+
+/* advection.f90:53-184 */
+void orc_momentum_advection_2nd(orc_model *m, double *du_loc, double *dv_loc, double *dw_loc)
+{
+    const orc_params *p = &m->p;
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    double *u = m->f[ORC_U], *v = m->f[ORC_V], *w = m->f[ORC_W];
+    double *fe = m->f[ORC_FLUX_EAST], *fn = m->f[ORC_FLUX_NORTH], *ft = m->f[ORC_FLUX_TOP];
+    const double Ro = p->Ro, dx = p->dx, dy = p->dy, dz = p->dz, dsqr = p->dsqr;
+
+    /* zonal momentum (:62-100) */
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 0; i <= nx; ++i)
+                A(fe, i, j, k) = 0.25 * (A(u, i, j, k) + A(u, i + 1, j, k)) *
+                                 (A(u, i + 1, j, k) + A(u, i, j, k));
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(fn, i, j, k) = 0.25 * (A(u, i, j, k) + A(u, i, j + 1, k)) *
+                                 (A(v, i + 1, j, k) + A(v, i, j, k));
+    for (int k = 0; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(ft, i, j, k) = 0.25 * (A(u, i, j, k + 1) + A(u, i, j, k)) *
+                                 (A(w, i, j, k) + A(w, i + 1, j, k));
+    lid_zero_flux_top(m, ft, nz);
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(du_loc, i, j, k) = -(Ro * ((A(fe, i, j, k) - A(fe, i - 1, j, k)) / dx +
+                                             (A(fn, i, j, k) - A(fn, i, j - 1, k)) / dy +
+                                             (A(ft, i, j, k) - A(ft, i, j, k - 1)) / dz));
+
+    /* meridional momentum (:104-140) */
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 0; i <= nx; ++i)
+                A(fe, i, j, k) = 0.25 * (A(v, i, j, k) + A(v, i + 1, j, k)) *
+                                 (A(u, i, j + 1, k) + A(u, i, j, k));
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(fn, i, j, k) = 0.25 * (A(v, i, j, k) + A(v, i, j + 1, k)) *
+                                 (A(v, i, j + 1, k) + A(v, i, j, k));
+    for (int k = 0; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(ft, i, j, k) = 0.25 * (A(v, i, j, k + 1) + A(v, i, j, k)) *
+                                 (A(w, i, j, k) + A(w, i, j + 1, k));
+    lid_zero_flux_top(m, ft, nz);
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(dv_loc, i, j, k) = -(Ro * ((A(fe, i, j, k) - A(fe, i - 1, j, k)) / dx +
+                                             (A(fn, i, j, k) - A(fn, i, j - 1, k)) / dy +
+                                             (A(ft, i, j, k) - A(ft, i, j, k - 1)) / dz));
+
+    /* vertical momentum (:145-181); the lid zeroing starts at nz-1 (:169) */
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 0; i <= nx; ++i)
+                A(fe, i, j, k) = 0.25 * (A(w, i, j, k) + A(w, i + 1, j, k)) *
+                                 (A(u, i, j, k) + A(u, i, j, k + 1));
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(fn, i, j, k) = 0.25 * (A(w, i, j, k) + A(w, i, j + 1, k)) *
+                                 (A(v, i, j, k) + A(v, i, j, k + 1));
+    for (int k = 0; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(ft, i, j, k) = 0.25 * (A(w, i, j, k + 1) + A(w, i, j, k)) *
+                                 (A(w, i, j, k) + A(w, i, j, k + 1));
+    lid_zero_flux_top(m, ft, nz - 1);
+    const double dsro = dsqr * Ro;
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(dw_loc, i, j, k) = -(dsro * ((A(fe, i, j, k) - A(fe, i - 1, j, k)) / dx +
+                                               (A(fn, i, j, k) - A(fn, i, j - 1, k)) / dy +
+                                               (A(ft, i, j, k) - A(ft, i, j, k - 1)) / dz));
+}
+
+/* ------------------------------------------------------------------------ */
+/* mixing.f90                                                                */
+/* ------------------------------------------------------------------------ */
+
+/* mixing.f90:225-274 */
+void orc_harmonic(orc_model *m, double *p1, double Hmix, double Vmix)
+{
+    const orc_params *p = &m->p;
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    double *fe = m->f[ORC_FLUX_EAST], *fn = m->f[ORC_FLUX_NORTH], *ft = m->f[ORC_FLUX_TOP];
+    const double dx = p->dx, dy = p->dy, dz = p->dz, dt = p->dt;
+    for (int k = 0; k <= nz + 1; ++k)
+        for (int j = 0; j <= ny + 1; ++j)
+            for (int i = -1; i <= nx + 1; ++i)
+                A(fe, i, j, k) = Hmix * (A(p1, i + 1, j, k) - A(p1, i, j, k)) / dx;
+    for (int k = 0; k <= nz + 1; ++k)
+        for (int j = -1; j <= ny + 1; ++j)
+            for (int i = 0; i <= nx + 1; ++i)
+                A(fn, i, j, k) = Hmix * (A(p1, i, j + 1, k) - A(p1, i, j, k)) / dy;
+    for (int k = -1; k <= nz + 1; ++k)
+        for (int j = 0; j <= ny + 1; ++j)
+            for (int i = 0; i <= nx + 1; ++i)
+                A(ft, i, j, k) = Vmix * (A(p1, i, j, k + 1) - A(p1, i, j, k)) / dz;
+    lid_zero_flux_top(m, ft, nz);
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(p1, i, j, k) = A(p1, i, j, k) +
+                                 dt * ((A(fe, i, j, k) - A(fe, i - 1, j, k)) / dx +
+                                       (A(fn, i, j, k) - A(fn, i, j - 1, k)) / dy +
+                                       (A(ft, i, j, k) - A(ft, i, j, k - 1)) / dz);
+}
+
+/* mixing.f90:96-158 */
+void orc_biharmonic_h(orc_model *m, double *p1, double Hmix)
+{
+    const orc_params *p = &m->p;
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    double *fe = m->f[ORC_FLUX_EAST], *fn = m->f[ORC_FLUX_NORTH];
+    double *del2 = m->del2;
+    const double dx = p->dx, dy = p->dy, dt = p->dt;
+    for (int k = 0; k <= nz + 1; ++k)
+        for (int j = 0; j <= ny + 1; ++j)
+            for (int i = -1; i <= nx + 1; ++i)
+                A(fe, i, j, k) = -(Hmix * (A(p1, i + 1, j, k) - A(p1, i, j, k)) / dx);
+    for (int k = 0; k <= nz + 1; ++k)
+        for (int j = -1; j <= ny + 1; ++j)
+            for (int i = 0; i <= nx + 1; ++i)
+                A(fn, i, j, k) = -(Hmix * (A(p1, i, j + 1, k) - A(p1, i, j, k)) / dy);
+    for (int k = 0; k <= nz + 1; ++k)
+        for (int j = 0; j <= ny + 1; ++j)
+            for (int i = 0; i <= nx + 1; ++i)
+                A(del2, i, j, k) = (A(fe, i, j, k) - A(fe, i - 1, j, k)) / dx +
+                                   (A(fn, i, j, k) - A(fn, i, j - 1, k)) / dy;
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 0; i <= nx; ++i)
+                A(fe, i, j, k) = (A(del2, i + 1, j, k) - A(del2, i, j, k)) / dx;
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(fn, i, j, k) = (A(del2, i, j + 1, k) - A(del2, i, j, k)) / dy;
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(p1, i, j, k) = A(p1, i, j, k) +
+                                 dt * ((A(fe, i, j, k) - A(fe, i - 1, j, k)) / dx +
+                                       (A(fn, i, j, k) - A(fn, i, j - 1, k)) / dy);
+}
+
+/* mixing.f90:164-217 */
+void orc_biharmonic_v(orc_model *m, double *p1, double Vmix)
+{
+    const orc_params *p = &m->p;
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    double *ft = m->f[ORC_FLUX_TOP];
+    double *del2 = m->del2;
+    const double dz = p->dz, dt = p->dt;
+    for (int k = -1; k <= nz + 1; ++k)
+        for (int j = 0; j <= ny + 1; ++j)
+            for (int i = 0; i <= nx + 1; ++i)
+                A(ft, i, j, k) = -(Vmix * (A(p1, i, j, k + 1) - A(p1, i, j, k)) / dz);
+    lid_zero_flux_top(m, ft, nz);
+    for (int k = 0; k <= nz + 1; ++k)
+        for (int j = 0; j <= ny + 1; ++j)
+            for (int i = 0; i <= nx + 1; ++i)
+                A(del2, i, j, k) = (A(ft, i, j, k) - A(ft, i, j, k - 1)) / dz;
+    for (int k = 0; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(ft, i, j, k) = (A(del2, i, j, k + 1) - A(del2, i, j, k)) / dz;
+    lid_zero_flux_top(m, ft, nz);
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(p1, i, j, k) = A(p1, i, j, k) + dt * ((A(ft, i, j, k) - A(ft, i, j, k - 1)) / dz);
+}
+
+/* ------------------------------------------------------------------------ */
+/* parallel_mpi.f90:70-211, single rank: periodic copies z, y, x              */
+/* ------------------------------------------------------------------------ */
+
+void orc_border_exchg_3d(orc_model *m, double *a)
+{
+    const int nx = m->p.nx, ny = m->p.ny, nz = m->p.nz;
+    const int64_t X = m->X, Y = m->Y;
+    /* border_exchg_in_z (:205-210): whole x-y planes */
+    for (int k = 1; k <= ONX; ++k) {
+        memcpy(&A(a, -1, -1, nz + k), &A(a, -1, -1, k), sizeof(double) * (size_t)(X * Y));
+        memcpy(&A(a, -1, -1, 1 - k), &A(a, -1, -1, nz - k + 1), sizeof(double) * (size_t)(X * Y));
+    }
+    /* border_exchg_in_y (:160-165): whole x-z planes */
+    for (int j = 1; j <= ONX; ++j) {
+        for (int k = -1; k <= nz + 2; ++k)
+            memcpy(&A(a, -1, ny + j, k), &A(a, -1, j, k), sizeof(double) * (size_t)X);
+        for (int k = -1; k <= nz + 2; ++k)
+            memcpy(&A(a, -1, 1 - j, k), &A(a, -1, ny - j + 1, k), sizeof(double) * (size_t)X);
+    }
+    /* border_exchg_in_x (:117-122): whole y-z planes */
+    for (int i = 1; i <= ONX; ++i) {
+        for (int k = -1; k <= nz + 2; ++k)
+            for (int j = -1; j <= ny + 2; ++j) A(a, nx + i, j, k) = A(a, i, j, k);
+        for (int k = -1; k <= nz + 2; ++k)
+            for (int j = -1; j <= ny + 2; ++j) A(a, 1 - i, j, k) = A(a, nx - i + 1, j, k);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* integrate.f90                                                             */
+/* ------------------------------------------------------------------------ */
+
+void orc_integrate_pre(orc_model *m, int itt, int tau, int taum1, int taum2)
+{
+    const orc_params *p = &m->p;
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    const int64_t H = m->H, X = m->X, Y = m->Y;
+    double *u = m->f[ORC_U], *v = m->f[ORC_V], *w = m->f[ORC_W], *b = m->f[ORC_B];
+    double *du = m->f[ORC_DU] + (tau - 1) * H, *dv = m->f[ORC_DV] + (tau - 1) * H;
+    double *dw = m->f[ORC_DW] + (tau - 1) * H, *db = m->f[ORC_DB] + (tau - 1) * H;
+
+    /* buoyancy advection (:12-17) */
+    if (p->enable_nonlinear) orc_buoyancy_advection_2nd(m, db);
+    else memset(db, 0, sizeof(double) * (size_t)H);
+
+    /* background stratification (:21-23), whole x-y planes */
+    const double N02 = p->N0 * p->N0;
+    for (int k = 1; k <= nz; ++k)
+        for (int j = -1; j <= ny + 2; ++j)
+            for (int i = -1; i <= nx + 2; ++i)
+                A(db, i, j, k) = A(db, i, j, k) - 0.5 * (A(w, i, j, k) + A(w, i, j, k - 1)) * N02;
+
+    /* momentum advection (:28-32) */
+    if (p->enable_nonlinear) orc_momentum_advection_2nd(m, du, dv, dw);
+    else {
+        memset(du, 0, sizeof(double) * (size_t)H);
+        memset(dv, 0, sizeof(double) * (size_t)H);
+        memset(dw, 0, sizeof(double) * (size_t)H);
+    }
+
+    /* Coriolis (:37-42) over all k including the halo */
+    const double f0 = p->f0;
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i)
+            for (int k = -1; k <= nz + 2; ++k) {
+                A(du, i, j, k) = A(du, i, j, k) +
+                    (A(v, i, j, k) + A(v, i, j - 1, k) + A(v, i + 1, j, k) + A(v, i + 1, j - 1, k)) * 0.25 * f0;
+                A(dv, i, j, k) = A(dv, i, j, k) -
+                    (A(u, i - 1, j, k) + A(u, i, j, k) + A(u, i - 1, j + 1, k) + A(u, i, j + 1, k)) * 0.25 * f0;
+            }
+
+    /* buoyancy force (:47-49), whole x-y planes */
+    for (int k = 1; k <= nz; ++k)
+        for (int j = -1; j <= ny + 2; ++j)
+            for (int i = -1; i <= nx + 2; ++i)
+                A(dw, i, j, k) = A(dw, i, j, k) + 0.5 * (A(b, i, j, k) + A(b, i, j, k + 1));
+
+    /* (:51-54) */
+    if (p->enable_vertical_boundaries) {
+        for (int k = 1 - ONX; k <= 0; ++k) memset(&A(dw, -1, -1, k), 0, sizeof(double) * (size_t)(X * Y));
+        for (int k = nz; k <= nz + ONX; ++k) memset(&A(dw, -1, -1, k), 0, sizeof(double) * (size_t)(X * Y));
+    }
+
+    /* friction (:59-78), in place, no halo exchange in between */
+    if (p->Ah > 0. || p->Av > 0.) orc_harmonic(m, u, p->Ah, p->Av);
+    if (p->Ah > 0. || p->Av > 0.) orc_harmonic(m, v, p->Ah, p->Av);
+    if (p->Ah > 0. || p->Av > 0.) {
+        orc_harmonic(m, w, p->Ah, p->Av);
+        if (p->enable_vertical_boundaries)
+            for (int k = nz; k <= nz + ONX; ++k) memset(&A(w, -1, -1, k), 0, sizeof(double) * (size_t)(X * Y));
+    }
+    if (p->Kh > 0. || p->Kv > 0.) orc_harmonic(m, b, p->Kh, p->Kv);
+    if (p->Ahbi > 0.) orc_biharmonic_h(m, u, p->Ahbi);
+    if (p->Ahbi > 0.) orc_biharmonic_h(m, v, p->Ahbi);
+    if (p->Ahbi > 0.) orc_biharmonic_h(m, w, p->Ahbi);
+    if (p->Avbi > 0.) orc_biharmonic_v(m, u, p->Avbi);
+    if (p->Avbi > 0.) orc_biharmonic_v(m, v, p->Avbi);
+    if (p->Avbi > 0.) {
+        orc_biharmonic_v(m, w, p->Avbi);
+        if (p->enable_vertical_boundaries)
+            for (int k = nz; k <= nz + ONX; ++k) memset(&A(w, -1, -1, k), 0, sizeof(double) * (size_t)(X * Y));
+    }
+    if (p->Khbi > 0.) orc_biharmonic_h(m, b, p->Khbi);
+    if (p->Kvbi > 0.) orc_biharmonic_v(m, b, p->Kvbi);
+
+    /* set_forcing (:82): the oracle runs configs with an empty hook */
+
+    /* Adams-Bashforth (:87-102), whole arrays */
+    const double dt = p->dt, dtw = p->dt / p->dsqr;
+    const double *du1 = m->f[ORC_DU] + (taum1 - 1) * H, *dv1 = m->f[ORC_DV] + (taum1 - 1) * H;
+    const double *dw1 = m->f[ORC_DW] + (taum1 - 1) * H, *db1 = m->f[ORC_DB] + (taum1 - 1) * H;
+    const double *du2 = m->f[ORC_DU] + (taum2 - 1) * H, *dv2 = m->f[ORC_DV] + (taum2 - 1) * H;
+    const double *dw2 = m->f[ORC_DW] + (taum2 - 1) * H, *db2 = m->f[ORC_DB] + (taum2 - 1) * H;
+    if (itt == 0) {
+        for (int64_t n = 0; n < H; ++n) {
+            u[n] = u[n] + dt * du[n];
+            v[n] = v[n] + dt * dv[n];
+            w[n] = w[n] + dtw * dw[n];
+            b[n] = b[n] + dt * db[n];
+        }
+    } else if (itt == 1 || !p->enable_AB_3_order) {
+        const double c1 = 1.5 + p->eps_ab, c2 = 0.5 + p->eps_ab;
+        for (int64_t n = 0; n < H; ++n) {
+            u[n] = u[n] + dt * (c1 * du[n] - c2 * du1[n]);
+            v[n] = v[n] + dt * (c1 * dv[n] - c2 * dv1[n]);
+            w[n] = w[n] + dtw * (c1 * dw[n] - c2 * dw1[n]);
+            b[n] = b[n] + dt * (c1 * db[n] - c2 * db1[n]);
+        }
+    } else {
+        const double a3 = 23.0 / 12.0, b3 = -16.0 / 12.0, c3 = 5.0 / 12.0;  /* main_module.f90:57 */
+        for (int64_t n = 0; n < H; ++n) {
+            u[n] = u[n] + dt * (a3 * du[n] + b3 * du1[n] + c3 * du2[n]);
+            v[n] = v[n] + dt * (a3 * dv[n] + b3 * dv1[n] + c3 * dv2[n]);
+            w[n] = w[n] + dtw * (a3 * dw[n] + b3 * dw1[n] + c3 * dw2[n]);
+            b[n] = b[n] + dt * (a3 * db[n] + b3 * db1[n] + c3 * db2[n]);
+        }
+    }
+
+    /* (:106-111) */
+    orc_border_exchg_3d(m, u);
+    orc_border_exchg_3d(m, v);
+    orc_border_exchg_3d(m, w);
+    orc_border_exchg_3d(m, b);
+}
+
+/* ------------------------------------------------------------------------ */
+/* my_fft.f90 transforms                                                     */
+/* ------------------------------------------------------------------------ */
+
+static void r2c_x(orc_model *m, const double *qp, cplx *qs)
+{
+    /* real (nx,ny,nz) -> complex (nx/2+1,ny,nz), along x */
+    const int nx = m->p.nx, ny = m->p.ny, nz = m->p.nz;
+    const int64_t nkx = m->nkx;
+    cplx *row = m->row;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j) {
+            const double *src = qp + (int64_t)nx * (j + (int64_t)ny * k);
+            for (int i = 0; i < nx; ++i) { row[i].re = src[i]; row[i].im = 0.0; }
+            cfft_strided(&m->px, row, 1, -1);
+            cplx *dst = qs + nkx * (j + (int64_t)ny * k);
+            for (int i = 0; i < nkx; ++i) dst[i] = row[i];
+        }
+}
+
+static void c2r_x(orc_model *m, const cplx *qs, double *qp)
+{
+    const int nx = m->p.nx, ny = m->p.ny, nz = m->p.nz;
+    const int64_t nkx = m->nkx;
+    cplx *row = m->row;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j) {
+            const cplx *src = qs + nkx * (j + (int64_t)ny * k);
+            for (int i = 0; i < nkx; ++i) row[i] = src[i];
+            row[0].im = 0.0;
+            if (nx % 2 == 0) row[nx / 2].im = 0.0;
+            for (int i = 1; i < (nx + 1) / 2; ++i) { row[nx - i].re = src[i].re; row[nx - i].im = -src[i].im; }
+            cfft_strided(&m->px, row, 1, +1);
+            double *dst = qp + (int64_t)nx * (j + (int64_t)ny * k);
+            for (int i = 0; i < nx; ++i) dst[i] = row[i].re;
+        }
+}
+
+static void c2c_y(orc_model *m, cplx *qs, int sign)
+{
+    const int ny = m->p.ny, nz = m->p.nz;
+    const int64_t nkx = m->nkx;
+    for (int k = 0; k < nz; ++k)
+        for (int i = 0; i < nkx; ++i)
+            cfft_strided(&m->py, qs + i + nkx * (int64_t)ny * k, nkx, sign);
+}
+
+static void c2c_z(orc_model *m, cplx *qs, int sign)
+{
+    const int ny = m->p.ny;
+    const int64_t nkx = m->nkx;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nkx; ++i)
+            cfft_strided(&m->pz, qs + i + nkx * (int64_t)j, nkx * ny, sign);
+}
+
+/* my_fft.f90:170-178 (p3dfft_ftran_r2c 'ffn', then QS = QS*dx*dy) */
+void orc_my_fft_2d(orc_model *m, const double *qp, double *qs_)
+{
+    cplx *qs = (cplx *)qs_;
+    r2c_x(m, qp, qs);
+    c2c_y(m, qs, -1);
+    const double dx = m->p.dx, dy = m->p.dy;
+    for (int64_t n = 0; n < m->S; ++n) {
+        qs[n].re = qs[n].re * dx * dy;
+        qs[n].im = qs[n].im * dx * dy;
+    }
+}
+
+/* my_fft.f90:159-167 (p3dfft_btran_c2r 'nff', then QP = QP/(Lx*Ly)) */
+void orc_my_ifft_2d(orc_model *m, const double *qs_, double *qp)
+{
+    cplx *tmp = (cplx *)malloc(sizeof(cplx) * (size_t)m->S);
+    memcpy(tmp, qs_, sizeof(cplx) * (size_t)m->S);
+    c2c_y(m, tmp, +1);
+    c2r_x(m, tmp, qp);
+    free(tmp);
+    const double s = m->p.Lx * m->p.Ly;
+    const int64_t N = (int64_t)m->p.nx * m->p.ny * m->p.nz;
+    for (int64_t n = 0; n < N; ++n) qp[n] = qp[n] / s;
+}
+
+/* my_fft.f90:197-205 ('fft', then QS = QS*dx*dy*dz) */
+void orc_my_fft_3d(orc_model *m, const double *qp, double *qs_)
+{
+    cplx *qs = (cplx *)qs_;
+    r2c_x(m, qp, qs);
+    c2c_y(m, qs, -1);
+    c2c_z(m, qs, -1);
+    const double dx = m->p.dx, dy = m->p.dy, dz = m->p.dz;
+    for (int64_t n = 0; n < m->S; ++n) {
+        qs[n].re = qs[n].re * dx * dy * dz;
+        qs[n].im = qs[n].im * dx * dy * dz;
+    }
+}
+
+/* my_fft.f90:184-192 ('fft' backward, then QP = QP/(Lx*Ly*Lz)) */
+void orc_my_ifft_3d(orc_model *m, const double *qs_, double *qp)
+{
+    cplx *tmp = (cplx *)malloc(sizeof(cplx) * (size_t)m->S);
+    memcpy(tmp, qs_, sizeof(cplx) * (size_t)m->S);
+    c2c_z(m, tmp, +1);
+    c2c_y(m, tmp, +1);
+    c2r_x(m, tmp, qp);
+    free(tmp);
+    const double s = m->p.Lx * m->p.Ly * m->p.Lz;
+    const int64_t N = (int64_t)m->p.nx * m->p.ny * m->p.nz;
+    for (int64_t n = 0; n < N; ++n) qp[n] = qp[n] / s;
+}
+
+/* ------------------------------------------------------------------------ */
+/* pressure_solver.f90                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* pressure_solver.f90:126-162 (Thomas algorithm; real a,b,c, complex d,x) */
+void orc_solve_tridiag(const double *a, const double *b, const double *c,
+                       const double *d_, double *x_, int n)
+{
+    const cplx *d = (const cplx *)d_;
+    cplx *x = (cplx *)x_;
+    double *cp = (double *)malloc(sizeof(double) * (size_t)n);
+    cplx *dp = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cp[0] = c[0] / b[0];
+    dp[0].re = d[0].re / b[0];
+    dp[0].im = d[0].im / b[0];
+    for (int i = 1; i < n; ++i) {
+        const double mm = b[i] - cp[i - 1] * a[i];
+        const double fxa = 1.0 / mm;
+        cp[i] = c[i] * fxa;
+        dp[i].re = (d[i].re - dp[i - 1].re * a[i]) * fxa;
+        dp[i].im = (d[i].im - dp[i - 1].im * a[i]) * fxa;
+    }
+    x[n - 1] = dp[n - 1];
+    for (int i = n - 2; i >= 0; --i) {
+        x[i].re = dp[i].re - cp[i] * x[i + 1].re;
+        x[i].im = dp[i].im - cp[i] * x[i + 1].im;
+    }
+    free(cp);
+    free(dp);
+}
+
+/* pressure_solver.f90:30-71, the part between the forward and inverse FFT */
+void orc_spectral_solve(orc_model *m, double *fs_)
+{
+    const orc_params *p = &m->p;
+    const int ny = p->ny, nz = p->nz;
+    const int64_t nkx = m->nkx, plane = nkx * ny;
+    cplx *FS = (cplx *)fs_;
+    const double *ksqr = m->f[ORC_KSQR];
+    if (p->enable_vertical_boundaries) {
+        const double fxa = 1.0 / (p->dz * p->dz) / p->dsqr;
+        double *a = (double *)malloc(sizeof(double) * (size_t)nz);
+        double *bb = (double *)malloc(sizeof(double) * (size_t)nz);
+        double *c = (double *)malloc(sizeof(double) * (size_t)nz);
+        cplx *col = (cplx *)malloc(sizeof(cplx) * (size_t)nz);
+        cplx *sol = (cplx *)malloc(sizeof(cplx) * (size_t)nz);
+        a[0] = 0.0;                 /* a_tri(1) is never read */
+        c[nz - 1] = 0.0;            /* c_tri(nz) is never used */
+        for (int k = 1; k < nz; ++k) a[k] = fxa;
+        for (int k = 0; k < nz - 1; ++k) c[k] = fxa;
+        const double dz2 = p->dz * p->dz;
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nkx; ++i) {
+                const double ks = ksqr[i + nkx * j];
+                for (int k = 0; k < nz; ++k) col[k] = FS[i + nkx * j + plane * k];
+                if (ks > 0) {
+                    bb[0] = -ks - fxa;
+                    for (int k = 1; k < nz - 1; ++k) bb[k] = -ks - 2 * fxa;
+                    bb[nz - 1] = -ks - fxa;
+                    orc_solve_tridiag(a, bb, c, (const double *)col, (double *)sol, nz);
+                } else {
+                    sol[0].re = 0.0; sol[0].im = 0.0;
+                    sol[1].re = col[0].re * dz2 * p->dsqr;
+                    sol[1].im = col[0].im * dz2 * p->dsqr;
+                    for (int k = 1; k < nz - 1; ++k) {
+                        sol[k + 1].re = 2 * sol[k].re - sol[k - 1].re + col[k].re * dz2 * p->dsqr;
+                        sol[k + 1].im = 2 * sol[k].im - sol[k - 1].im + col[k].im * dz2 * p->dsqr;
+                    }
+                }
+                for (int k = 0; k < nz; ++k) FS[i + nkx * j + plane * k] = sol[k];
+            }
+        free(a); free(bb); free(c); free(col); free(sol);
+    } else {
+        for (int64_t n = 0; n < m->S; ++n) {
+            const double ks = ksqr[n];
+            if (ks > 0.) {
+                FS[n].re = -(FS[n].re / ks);
+                FS[n].im = -(FS[n].im / ks);
+            } else {
+                FS[n].re = 0.0;
+                FS[n].im = 0.0;
+            }
+        }
+    }
+}
+
+/* pressure_solver.f90:2-120 */
+void orc_pressure_solver(orc_model *m)
+{
+    const orc_params *p = &m->p;
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    const int64_t X = m->X, Y = m->Y;
+    double *u = m->f[ORC_U], *v = m->f[ORC_V], *w = m->f[ORC_W], *pr = m->f[ORC_P];
+    double *aloc = m->aloc;
+    const double dx = p->dx, dy = p->dy, dz = p->dz, dt = p->dt;
+#define AL(i, j, k) aloc[((i) - 1) + (int64_t)nx * (((j) - 1) + (int64_t)ny * ((k) - 1))]
+    /* (:16-24) */
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                AL(i, j, k) = (A(u, i, j, k) - A(u, i - 1, j, k)) / dx +
+                              (A(v, i, j, k) - A(v, i, j - 1, k)) / dy +
+                              (A(w, i, j, k) - A(w, i, j, k - 1)) / dz;
+    const int64_t N = (int64_t)nx * ny * nz;
+    for (int64_t n = 0; n < N; ++n) aloc[n] = aloc[n] / dt;
+
+    /* (:30-72) */
+    double *FS = m->f[ORC_FS];
+    if (p->enable_vertical_boundaries) {
+        orc_my_fft_2d(m, aloc, FS);
+        orc_spectral_solve(m, FS);
+        orc_my_ifft_2d(m, FS, aloc);
+    } else {
+        orc_my_fft_3d(m, aloc, FS);
+        orc_spectral_solve(m, FS);
+        orc_my_ifft_3d(m, FS, aloc);
+    }
+
+    /* (:75-93) */
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) A(pr, i, j, k) = AL(i, j, k);
+    orc_border_exchg_3d(m, pr);
+    if (p->enable_vertical_boundaries) {
+        for (int k = 1 - ONX; k <= 0; ++k)
+            memcpy(&A(pr, -1, -1, k), &A(pr, -1, -1, 1), sizeof(double) * (size_t)(X * Y));
+        for (int k = nz + 1; k <= nz + ONX; ++k)
+            memcpy(&A(pr, -1, -1, k), &A(pr, -1, -1, nz), sizeof(double) * (size_t)(X * Y));
+    }
+
+    /* (:98-110) */
+    const double dtw = dt / p->dsqr;
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                A(u, i, j, k) = A(u, i, j, k) - dt * (A(pr, i + 1, j, k) - A(pr, i, j, k)) / dx;
+                A(v, i, j, k) = A(v, i, j, k) - dt * (A(pr, i, j + 1, k) - A(pr, i, j, k)) / dy;
+                A(w, i, j, k) = A(w, i, j, k) - dtw * (A(pr, i, j, k + 1) - A(pr, i, j, k)) / dz;
+            }
+    if (p->enable_vertical_boundaries) {
+        for (int k = 1 - ONX; k <= 0; ++k) memset(&A(w, -1, -1, k), 0, sizeof(double) * (size_t)(X * Y));
+        for (int k = nz; k <= nz + ONX; ++k) memset(&A(w, -1, -1, k), 0, sizeof(double) * (size_t)(X * Y));
+    }
+    /* (:114-118) */
+    orc_border_exchg_3d(m, u);
+    orc_border_exchg_3d(m, v);
+    orc_border_exchg_3d(m, w);
+#undef AL
+}
+
+void orc_integrate(orc_model *m, int itt, int tau, int taum1, int taum2)
+{
+    orc_integrate_pre(m, itt, tau, taum1, taum2);
+    orc_pressure_solver(m);
+}
+
+/* main.f90:139-161 without the diagnostics; tau rotation at :156-160 */
+void orc_run(orc_model *m, int nsteps, int32_t *st)
+{
+    for (int s = 0; s < nsteps; ++s) {
+        orc_integrate(m, st[0], st[1], st[2], st[3]);
+        st[1] = st[1] % 3 + 1;
+        st[2] = st[2] % 3 + 1;
+        st[3] = st[3] % 3 + 1;
+        st[0] += 1;
+    }
+}
