@@ -1,0 +1,165 @@
+"""Benchmark of the ps3D model time step on one MI355X.
+
+Workload (the config the metric is quoted on): two balanced zonal jets on a
+256 x 256 x 64 grid, rigid lid, FP64, AB3, harmonic (Ah, Av, Kh, Kv) and
+horizontal biharmonic (Ahbi, Khbi) friction -- 8 friction passes per step.
+A step is one call of integrate (integrate.f90:2-115) with the state already
+resident in HBM.  value = grid points x steps / seconds over exactly --steps
+steps, bracketed by device synchronisation.
+
+The JSON line also carries:
+* roofline: the kernel with the largest share of the step, timed with HIP
+  events around each of its launches on the model's stream during the timed
+  steps; achieved = its algorithmic bytes per launch / mean launch time.
+* cpu_baseline: the CPU oracle (a C port of the reference step, one core)
+  on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "grid-pt steps/s FP64 vs mixed @≤1e-5 rel L2; ensemble member-steps/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(name: str, prm, order: int = 3) -> float:
+    """Compulsory HBM bytes of one launch of a kernel (element size 8 B)."""
+    N = prm.nx * prm.ny * prm.nz
+    S = (prm.nx // 2 + 1) * prm.ny * prm.nz
+    e = 8
+    table = {
+        "tendency": 8 * e * N,                 # R u,v,w,b  W du,dv,dw,db(tau)
+        "harmonic": 2 * e * N,                 # R + W one field
+        "biharmonic_h": 2 * e * N,
+        "biharmonic_v": 2 * e * N,
+        "ab_update": 4 * (2 + order) * e * N,  # per field R u, q slots  W u
+        "fft_x_r2c": 3 * e * N + 2 * e * S,    # R u,v,w (divergence)  W FS
+        "fft_y_fwd": 4 * e * S,                # R + W complex FS
+        "fft_y_inv": 4 * e * S,
+        "thomas": 4 * e * S,
+        "z_solve": 4 * e * S,
+        "fft_x_c2r": 2 * e * S + e * N,        # R FS  W p
+        "project": 7 * e * N,                  # R u,v,w,p  W u,v,w
+        "halo": 0.0,
+    }
+    return float(table[name])
+
+
+def build_workload(precision=0):
+    from ps3d_amd import configs
+    return configs.jets(256, 256, 64, seed=2, precision=precision)
+
+
+def time_steps(model, steps: int) -> float:
+    model.synchronize()
+    t0 = time.perf_counter()
+    model.run(steps)
+    model.synchronize()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(prm, fields, budget_s: float = 15.0) -> dict:
+    import oracle
+    orc = oracle.OracleModel(prm)
+    for n, a in fields.items():
+        orc.f[n][...] = a
+    orc.run(1)   # untimed first step (Euler)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        orc.run(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 200:
+            break
+    orc.close()
+    return {"value": prm.npoints * steps / el, "unit": "grid-pt steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} steps of the same 256x256x64 jets workload (after 1 untimed step), "
+                      f"C oracle oracle/ps3d_oracle.c, 1 thread, {el:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--profile-all", action="store_true", help="print every kernel's share")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} is not "
+              "supported (members of an ensemble or z-slabs of one grid would be sharded across GPUs, "
+              "which is not part of this build)", file=sys.stderr)
+        return 2
+
+    from ps3d_amd.model import Model, kernel_names
+
+    prm, fields = build_workload()
+    model = Model(prm)
+    for n, a in fields.items():
+        model.f[n][...] = a
+    model.push()
+    model.run(max(args.warmup, 3))   # past Euler and AB2: the timed steps are AB3
+    model.synchronize()
+
+    # share of each kernel over a few untimed steps (one kernel at a time)
+    shares = {}
+    probe = 10
+    for name in kernel_names():
+        model.profile_kernel(name)
+        model.run(probe)
+        n, ms = model.profile_read()
+        if n:
+            shares[name] = (n / probe, ms / probe)
+    model.profile_kernel(None)
+    dominant = max(shares, key=lambda k: shares[k][1])
+
+    # timed region: exactly --steps steps, the dominant kernel under events
+    model.profile_kernel(dominant)
+    el = time_steps(model, args.steps)
+    launches, kms = model.profile_read()
+    model.profile_kernel(None)
+
+    value = prm.npoints * args.steps / el
+    per_launch_s = kms / launches / 1e3
+    traffic_bytes = algorithmic_bytes(dominant, prm)
+    achieved = traffic_bytes / per_launch_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "grid-pt steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "baroclinic zonal jets 256x256x64, rigid lid, FP64, AB3, "
+                               "harmonic + horizontal biharmonic friction",
+                   "grid": [prm.nx, prm.ny, prm.nz], "precision": "fp64", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "launches": launches, "mean_launch_us": per_launch_s * 1e6,
+                     "algorithmic_bytes_per_launch": traffic_bytes},
+    }
+    if args.profile_all:
+        out["kernel_shares_ms_per_step"] = {k: v[1] for k, v in sorted(shares.items(), key=lambda x: -x[1][1])}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(prm, fields)
+    model.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

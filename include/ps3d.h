@@ -1,0 +1,165 @@
+/*
+ * ps3d.h -- C-ABI of the MI355X (gfx950) implementation of the ps3D model
+ * time step.  Plain C types only; every function returns 0 on success or a
+ * PS3D_ERR_* code, with a message available from ps3d_last_error().  No C++
+ * exception crosses this boundary.
+ *
+ * Field layout (the contract with the Fortran driver's main_module,
+ * for_src/main_module.f90:118-131 on one rank): every 3-D field is the full
+ * array a(-1:nx+2, -1:ny+2, -1:nz+2) including the onx=2 halo, Fortran
+ * column-major, i fastest.  Tendencies du..db are (.., 1:3) arrays of three
+ * such slots.  FS is complex(8) (nx/2+1, ny, nz), kx fastest.  ksqr is
+ * (nx/2+1, ny, 1) with a rigid lid, else (nx/2+1, ny, nz).
+ *
+ * Ownership: the caller owns host arrays; the library owns device buffers
+ * and never frees caller memory.  One model = one HIP stream; calls on one
+ * model must not overlap from several host threads.
+ *
+ * Reference routines are cited as file:line of the ps3D sources (for_src/).
+ */
+#ifndef PS3D_H
+#define PS3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PS3D_OK            0
+#define PS3D_ERR_ARG       1   /* invalid argument / unsupported size */
+#define PS3D_ERR_HIP       2   /* HIP runtime error */
+#define PS3D_ERR_STATE     3   /* call not valid in the model's state */
+#define PS3D_ERR_INTERNAL  4
+
+/* precision of a model */
+#define PS3D_FP64   0   /* plain FP64 path: the drop-in */
+#define PS3D_MIXED  1   /* FP32 stencils/FFTs, FP64 solve/accumulation */
+
+/* field ids for ps3d_set_field / ps3d_get_field */
+#define PS3D_U           0
+#define PS3D_V           1
+#define PS3D_W           2
+#define PS3D_B           3
+#define PS3D_P           4
+#define PS3D_DU          5   /* (.., 3): all three tendency slots */
+#define PS3D_DV          6
+#define PS3D_DW          7
+#define PS3D_DB          8
+#define PS3D_FS          9   /* complex (nx/2+1, ny, nz) */
+#define PS3D_KSQR       10
+
+/* friction operators, mixing.f90 */
+#define PS3D_HARMONIC      0   /* harmonic(p1,Hmix,Vmix)   mixing.f90:225-274 */
+#define PS3D_BIHARMONIC_H  1   /* biharmonic_h(p1,Hmix)    mixing.f90:96-158  */
+#define PS3D_BIHARMONIC_V  2   /* biharmonic_v(p1,Vmix)    mixing.f90:164-217 */
+
+/* Parameters of main_module (main_module.f90:16-59) that the step reads.
+   Logicals are 0/1.  Every value is taken as given: the library derives
+   nothing the driver has already computed (so single-precision literals of
+   a config survive exactly). */
+typedef struct ps3d_params {
+    int32_t nx, ny, nz;
+    int32_t enable_nonlinear;
+    int32_t enable_vertical_boundaries;
+    int32_t enable_AB_3_order;
+    double Lx, Ly, Lz, dx, dy, dz, dt;
+    double f0, N0, Ro, dsqr, eps_ab;
+    double Ah, Av, Kh, Kv, Ahbi, Avbi, Khbi, Kvbi;
+    int32_t precision;            /* PS3D_FP64 or PS3D_MIXED */
+    int32_t reserved[7];          /* zero */
+} ps3d_params;
+
+typedef struct ps3d_model ps3d_model;
+
+/* Last error message of the calling thread ("" if none); copies at most
+   len-1 bytes plus a terminating NUL into buf; returns the message length. */
+int ps3d_last_error(char *buf, int len);
+
+/* Library version string, e.g. "ps3d-amd 0.1 gfx950". */
+const char *ps3d_version(void);
+
+/* Allocate a model and its device mirror.  Replaces allocate_main_module
+   (main_module.f90:112-144) and wavenumber_initialisation
+   (my_fft.f90:87-156) for the step.  ksqr: the driver's ksqr array
+   (main_module.f90:100) or NULL, in which case the library computes it with
+   hat_ksqr (eigenvectors.f90:224-229) exactly as my_fft.f90:130-155 does.
+   stream: a hipStream_t to run on, or NULL for a library-owned stream.
+   Sizes: nx, ny even and >= 4 (nz >= 2 with a rigid lid, even and >= 4
+   otherwise), every size a product of 2, 3 and 5 (the mixed-radix FFT),
+   main.f90:45-55 being the reference's own size check. */
+int ps3d_create(const ps3d_params *prm, const double *ksqr, void *stream,
+                ps3d_model **out);
+int ps3d_destroy(ps3d_model *m);
+
+/* Host <-> device copies of whole main_module arrays (halo included).
+   Synchronous with respect to the host.  ps3d_set_field of PS3D_KSQR
+   replaces the wavenumber table (and the solver factors derived from it). */
+int ps3d_set_field(ps3d_model *m, int field, const double *host);
+int ps3d_get_field(ps3d_model *m, int field, double *host);
+/* number of doubles of a field's host array (complex counts as 2) */
+int ps3d_field_size(const ps3d_model *m, int field, int64_t *n);
+
+/* ---- the step ------------------------------------------------------- */
+
+/* integrate.f90:2-115 for a config whose set_forcing is empty: tendencies,
+   friction, Adams-Bashforth, halo exchange, pressure_solver.  itt and the
+   tau slots (1..3) are the driver's (main.f90:112-113, 156-160). */
+int ps3d_integrate(ps3d_model *m, int itt, int tau, int taum1, int taum2);
+
+/* main.f90:139-161 without diagnostics: nsteps calls of integrate with the
+   tau rotation.  state = {itt, tau, taum1, taum2}, updated on return.
+   The steps run back to back on the model's stream (graph-captured per
+   tau phase when use_graph != 0); returns when they are queued. */
+int ps3d_run(ps3d_model *m, int nsteps, int32_t *state, int use_graph);
+
+/* wait for the model's stream */
+int ps3d_synchronize(ps3d_model *m);
+
+/* ---- phases, for drivers whose set_forcing does work ------------------
+   integrate.f90 in pieces: tendency (:12-54), friction (:59-78), then the
+   driver's set_forcing on host mirrors, then ab_update (:87-102) with the
+   halo exchange (:106-111), then pressure_solver (:113). */
+int ps3d_tendency(ps3d_model *m, int tau);
+/* advection.f90:2-48 into db(:,:,:,tau) interior only */
+int ps3d_buoyancy_advection(ps3d_model *m, int tau);
+/* advection.f90:53-184 into du,dv,dw(:,:,:,tau) interior only */
+int ps3d_momentum_advection(ps3d_model *m, int tau);
+/* the friction block integrate.f90:59-78 */
+int ps3d_friction(ps3d_model *m);
+/* one friction call on field (PS3D_U..PS3D_B), op PS3D_HARMONIC.. */
+int ps3d_mixing(ps3d_model *m, int op, int field, double hmix, double vmix);
+/* integrate.f90:87-111 */
+int ps3d_ab_update(ps3d_model *m, int itt, int tau, int taum1, int taum2);
+/* border_exchg_3D (parallel_mpi.f90:70-81), one rank, on a device field */
+int ps3d_border_exchg(ps3d_model *m, int field);
+/* pressure_solver.f90:2-120 */
+int ps3d_pressure_solver(ps3d_model *m);
+
+/* ---- building blocks, host buffers in and out (for tests and drivers) */
+
+/* my_fft_2D / my_fft_3D (my_fft.f90:170-178, 197-205): qp real
+   (nx,ny,nz) -> qs complex (nx/2+1,ny,nz) as interleaved doubles;
+   three_d = 0 transforms x,y ('ffn'), 1 transforms x,y,z ('fft'). */
+int ps3d_my_fft(ps3d_model *m, int three_d, const double *qp, double *qs);
+/* my_ifft_2D / my_ifft_3D (my_fft.f90:159-167, 184-192) */
+int ps3d_my_ifft(ps3d_model *m, int three_d, const double *qs, double *qp);
+/* the spectral step of pressure_solver.f90:30-71 on complex qs in place:
+   per-column solve_tridiag (pressure_solver.f90:126-162) and the zero-mode
+   recurrence with a rigid lid, else the division by -ksqr */
+int ps3d_spectral_solve(ps3d_model *m, double *qs);
+
+/* ---- measurement ----------------------------------------------------- */
+
+/* Time every launch of the named kernel with HIP events on the model's
+   stream (NULL turns it off).  Names are those of ps3d_kernel_names(). */
+int ps3d_profile_kernel(ps3d_model *m, const char *name);
+/* launches and total milliseconds recorded since the last call */
+int ps3d_profile_read(ps3d_model *m, int64_t *launches, double *total_ms);
+/* comma-separated names of the model's kernels */
+const char *ps3d_kernel_names(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PS3D_H */

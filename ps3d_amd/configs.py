@@ -1,0 +1,201 @@
+"""Synthetic scenario configs written against the reference's plug-in surface
+(``set_parameter`` + ``set_initial_conditions``, for_config/*.f90).
+
+Each builder returns ``(ModelParams, {"u","v","w","b": host arrays})`` with the
+main_module layout (halo included, already exchanged, lid applied as
+main.f90:119-122 does after set_initial_conditions).
+
+Noise is drawn from SplitMix64 keyed on (seed, field, i, j, k): integer-only
+and independent of evaluation order, so any implementation reproduces the
+same initial state bit for bit.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .params import PS3D_FP64, ModelParams
+
+FIELD_ID = {"u": 1, "v": 2, "w": 3, "b": 4}
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    """SplitMix64 finaliser over uint64 counters (wrapping arithmetic)."""
+    with np.errstate(over="ignore"):
+        z = x.astype(np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def _counters(seed: int, field: str, nx: int, ny: int, nz: int, stream: int = 0) -> np.ndarray:
+    i = np.arange(nx, dtype=np.uint64)[:, None, None]
+    j = np.arange(ny, dtype=np.uint64)[None, :, None]
+    k = np.arange(nz, dtype=np.uint64)[None, None, :]
+    lin = i + np.uint64(nx) * (j + np.uint64(ny) * k)
+    key = (np.uint64(seed) * np.uint64(64) + np.uint64(FIELD_ID[field] * 4 + stream)) << np.uint64(40)
+    return lin + key
+
+
+def uniform(seed: int, field: str, nx: int, ny: int, nz: int, stream: int = 0) -> np.ndarray:
+    """U(-1, 1) on the interior, shape (nx, ny, nz)."""
+    z = splitmix64(_counters(seed, field, nx, ny, nz, stream))
+    return (z >> np.uint64(11)).astype(np.float64) * (2.0 ** -52) - 1.0
+
+
+def normal(seed: int, field: str, nx: int, ny: int, nz: int) -> np.ndarray:
+    """N(0, 1) by Box-Muller from two uniform streams."""
+    u1 = (splitmix64(_counters(seed, field, nx, ny, nz, 1)) >> np.uint64(11)).astype(np.float64)
+    u2 = (splitmix64(_counters(seed, field, nx, ny, nz, 2)) >> np.uint64(11)).astype(np.float64)
+    u1 = (u1 + 1.0) * 2.0 ** -53          # (0, 1]
+    u2 = u2 * 2.0 ** -53
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
+
+
+def border_exchg(a: np.ndarray) -> None:
+    """parallel_mpi.f90:70-211 on one rank (z, then y, then x), in place."""
+    nx, ny, nz = a.shape[0] - 4, a.shape[1] - 4, a.shape[2] - 4
+    for k in (1, 2):
+        a[:, :, 1 + nz + k] = a[:, :, 1 + k]
+        a[:, :, 1 + 1 - k] = a[:, :, 1 + nz - k + 1]
+    for j in (1, 2):
+        a[:, 1 + ny + j, :] = a[:, 1 + j, :]
+        a[:, 1 + 1 - j, :] = a[:, 1 + ny - j + 1, :]
+    for i in (1, 2):
+        a[1 + nx + i, :, :] = a[1 + i, :, :]
+        a[1 + 1 - i, :, :] = a[1 + nx - i + 1, :, :]
+
+
+def _finish(prm: ModelParams, interior: dict) -> dict:
+    nx, ny, nz = prm.nx, prm.ny, prm.nz
+    out = {}
+    for n in ("u", "v", "w", "b"):
+        a = np.zeros(prm.shape, order="F")
+        a[2:2 + nx, 2:2 + ny, 2:2 + nz] = interior[n]
+        border_exchg(a)
+        out[n] = a
+    if prm.enable_vertical_boundaries:       # main.f90:119-122
+        out["w"][:, :, :2] = 0.0
+        out["w"][:, :, 1 + nz:] = 0.0
+    return out
+
+
+def eddy(nx=64, ny=64, nz=32, seed=1, radius_frac=1.0 / 6.0, noise=1e-3, cfl=0.3,
+         precision=PS3D_FP64, biharmonic=True):
+    """Baroclinic eddy on an f-plane with constant N^2 and a rigid lid.
+
+    Scales follow for_config/eddy1.f90:15-40 (f0 = 1e-4, N0 = 30 f0, H = 2000 m,
+    Lx = Ly = 20 N0 H / (pi f0), A_bi = dx^4 / 30 days).  Buoyancy anomaly
+    0.1 N0^2 H exp(-r^2/R^2) exp(-depth/H) with R = radius_frac*Lx, velocities
+    in thermal-wind balance (zero at the bottom), plus noise*U(-1,1) on u, v, b.
+    dt = cfl*dx/max|u|, capped at 0.5/N0 for the explicit AB3 gravity waves.
+    """
+    f0 = 1e-4
+    N0 = 30 * f0
+    H = 2000.0
+    Lz = H
+    L = 20 * N0 * H / (math.pi * f0)
+    dx, dy, dz = L / nx, L / ny, Lz / nz
+    A = 0.1 * N0 ** 2 * H
+    R = radius_frac * L
+    xc = yc = 0.5 * L
+
+    def G(x, y):
+        return np.exp(-((x - xc) ** 2 + (y - yc) ** 2) / R ** 2)
+
+    xi = (np.arange(1, nx + 1) - 0.5) * dx      # cell centres
+    yj = (np.arange(1, ny + 1) - 0.5) * dy
+    zk = (np.arange(1, nz + 1) - 0.5) * dz      # height above the bottom
+    E = np.exp(-(Lz - zk) / H)
+    E0 = math.exp(-Lz / H)
+    X, Y = np.meshgrid(xi, yj, indexing="ij")
+    b = A * G(X, Y)[:, :, None] * E[None, None, :]
+    # u at (i+1/2, j), v at (i, j+1/2): f u_z = -b_y, f v_z = b_x, zero at z = 0
+    Xu, Yu = np.meshgrid(xi + 0.5 * dx, yj, indexing="ij")
+    Xv, Yv = np.meshgrid(xi, yj + 0.5 * dy, indexing="ij")
+    gy = -2 * (Yu - yc) / R ** 2 * G(Xu, Yu)
+    gx = -2 * (Xv - xc) / R ** 2 * G(Xv, Yv)
+    u = -(A / f0) * gy[:, :, None] * (H * (E - E0))[None, None, :]
+    v = (A / f0) * gx[:, :, None] * (H * (E - E0))[None, None, :]
+    u = u + noise * uniform(seed, "u", nx, ny, nz)
+    v = v + noise * uniform(seed, "v", nx, ny, nz)
+    b = b + noise * uniform(seed, "b", nx, ny, nz)
+    umax = float(np.sqrt(u ** 2 + v ** 2).max())
+    dt = min(cfl * dx / umax, 0.5 / N0)
+    bih = dx ** 4 / (30 * 86400.0) if biharmonic else 0.0
+    prm = ModelParams(nx=nx, ny=ny, nz=nz, Lx=L, Ly=L, Lz=Lz, dx=dx, dy=dy, dz=dz, dt=dt,
+                      f0=f0, N0=N0, Ahbi=bih, Khbi=bih, enable_vertical_boundaries=True,
+                      enable_AB_3_order=True, eps_ab=0.0, precision=precision)
+    return prm, _finish(prm, {"u": u, "v": v, "w": np.zeros_like(u), "b": b})
+
+
+def jets(nx=256, ny=256, nz=64, seed=2, umax=0.5, noise=1e-3, friction=True, precision=PS3D_FP64):
+    """Two zonal jets of opposite sign, geostrophically balanced, rigid lid.
+
+    Scales follow for_config/jets_unscaled.f90:5-42 (Lx = Ly = 500 km,
+    Lz = 500 m with the lid, f0 = 1e-4, N0 = 50 f0, dt = 0.5/N0).  Jets
+    +-umax at y = Ly/4, 3Ly/4 with Gaussian width Ly/10, times cos(pi z/Lz);
+    b from the thermal-wind cumsum in y of jets_unscaled.f90:80-95; N(0,noise)
+    on u, v, w, b.  Harmonic (Ah, Av, Kh, Kv) and horizontal biharmonic
+    (Ahbi, Khbi) friction sized to the grid.
+    """
+    f0 = 1e-4
+    N0 = 50 * f0
+    Lx = Ly = 500e3
+    Lz = 500.0
+    dx, dy, dz = Lx / nx, Ly / ny, Lz / nz
+    dt = 0.5 / N0
+    width = Ly / 10
+    yj = (np.arange(1, ny + 1) - 1) * dy
+    zk = (np.arange(1, nz + 1) - 1) * dz
+    prof = (np.exp(-(yj - 3 * Ly / 4) ** 2 / width ** 2) - np.exp(-(yj - Ly / 4) ** 2 / width ** 2))
+    u = umax * prof[None, :, None] * np.cos(math.pi * zk / Lz)[None, None, :] * np.ones((nx, 1, 1))
+    # b_y = -f u_z by one-sided/centred differences and a cumulative sum in y
+    up = np.concatenate([u[:, :, :1], u, u[:, :, -1:]], axis=2)
+    b = np.empty_like(u)
+    for k in range(nz):
+        if k == 0:
+            b[:, :, k] = -dy * (up[:, :, k + 2] - up[:, :, k + 1]) / dz * f0
+        elif k == nz - 1:
+            b[:, :, k] = -dy * (up[:, :, k + 1] - up[:, :, k]) / dz * f0
+        else:
+            b[:, :, k] = -dy * (up[:, :, k + 2] - up[:, :, k]) / (2 * dz) * f0
+    b = np.cumsum(b, axis=1)
+    u = u + noise * normal(seed, "u", nx, ny, nz)
+    v = noise * normal(seed, "v", nx, ny, nz)
+    w = noise * normal(seed, "w", nx, ny, nz)
+    b = b + noise * normal(seed, "b", nx, ny, nz)
+    kw = {}
+    if friction:
+        kw = dict(Ah=1e-3 * dx ** 2 / dt, Av=1e-2 * dz ** 2 / dt, Kh=1e-3 * dx ** 2 / dt,
+                  Kv=1e-2 * dz ** 2 / dt, Ahbi=dx ** 4 / (30 * 86400.0), Khbi=dx ** 4 / (30 * 86400.0))
+    prm = ModelParams(nx=nx, ny=ny, nz=nz, Lx=Lx, Ly=Ly, Lz=Lz, dx=dx, dy=dy, dz=dz, dt=dt,
+                      f0=f0, N0=N0, enable_vertical_boundaries=True, enable_AB_3_order=True,
+                      eps_ab=0.0, precision=precision, **kw)
+    return prm, _finish(prm, {"u": u, "v": v, "w": w, "b": b})
+
+
+def random_state(nx, ny, nz, seed=7, lid=True, amp=0.1, **kw):
+    """A generic test state: smooth large-scale fields plus noise, with every
+    friction operator switched on (kw override any parameter)."""
+    L = 1.0e5
+    Lz = 1.0e3
+    base = dict(nx=nx, ny=ny, nz=nz, Lx=L, Ly=L * ny / nx, Lz=Lz, dt=60.0, f0=1e-4, N0=2e-3,
+                Ro=1.0, dsqr=1.0, eps_ab=0.01, Ah=20.0, Av=1e-3, Kh=10.0, Kv=5e-4,
+                Ahbi=1e7, Avbi=1e-2, Khbi=5e6, Kvbi=5e-3, enable_vertical_boundaries=lid,
+                enable_AB_3_order=True)
+    base.update(kw)
+    prm = ModelParams(**base)
+    xi = np.arange(nx)[:, None, None] / nx
+    yj = np.arange(ny)[None, :, None] / ny
+    zk = (np.arange(nz)[None, None, :] + 0.5) / nz
+    fields = {}
+    for n, ph in zip(("u", "v", "w", "b"), (0.1, 0.7, 1.3, 2.1)):
+        smooth = np.sin(2 * math.pi * (xi + ph)) * np.cos(2 * math.pi * (yj - ph)) * np.cos(math.pi * zk)
+        fields[n] = amp * (smooth + 0.1 * uniform(seed, n, nx, ny, nz))
+    fields["w"] *= 0.01
+    fields["b"] *= 1e-3
+    return prm, _finish(prm, fields)

@@ -1,0 +1,534 @@
+// api.hip -- the C-ABI of include/ps3d.h: model set-up, host <-> device
+// copies, the step and its phases, building blocks and measurement.
+#include "ps3d_internal.h"
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+namespace ps3d {
+void launch_divide_ksqr(Model &m, double2 *data);
+}
+
+using namespace ps3d;
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(F &&f)
+{
+    try {
+        f();
+        g_err.clear();
+        return PS3D_OK;
+    } catch (const Error &e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return PS3D_ERR_INTERNAL;
+    } catch (...) {
+        g_err = "unknown error";
+        return PS3D_ERR_INTERNAL;
+    }
+}
+
+bool is_235(int n)
+{
+    if (n < 1) return false;
+    for (int p : {2, 3, 5})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+template <class T>
+T *dalloc(Model &m, int64_t n)
+{
+    void *p = nullptr;
+    PS3D_HIP_CHECK(hipMalloc(&p, sizeof(T) * (size_t)(n > 0 ? n : 1)));
+    PS3D_HIP_CHECK(hipMemsetAsync(p, 0, sizeof(T) * (size_t)(n > 0 ? n : 1), m.stream));
+    m.allocations.push_back(p);
+    return static_cast<T *>(p);
+}
+
+// eigenvectors.f90:224-229
+double hat_ksqr(double k, double d) { return 2.0 * (1.0 - std::cos(k * d)) / (d * d); }
+
+// my_fft.f90:87-156, the ksqr part
+std::vector<double> make_ksqr(const ps3d_params &p)
+{
+    const double pi = std::acos(0.0) * 2.0;   // main_module.f90:116
+    const int nkx = p.nx / 2 + 1;
+    std::vector<double> kx(nkx), ky(p.ny), kz(p.nz);
+    for (int i = 1; i <= nkx; ++i) kx[i - 1] = (double)(2 * (i - 1)) * pi / p.Lx;
+    for (int j = 1; j <= (p.ny + 1) / 2; ++j) ky[j - 1] = (double)(2 * (j - 1)) * pi / p.Ly;
+    for (int j = (p.ny + 1) / 2 + 1; j <= p.ny; ++j) ky[j - 1] = -((double)(2 * (p.ny - j + 1)) * pi / p.Ly);
+    for (int k = 1; k <= (p.nz + 1) / 2; ++k) kz[k - 1] = (double)(2 * (k - 1)) * pi / p.Lz;
+    for (int k = (p.nz + 1) / 2 + 1; k <= p.nz; ++k) kz[k - 1] = -((double)(2 * (p.nz - k + 1)) * pi / p.Lz);
+    std::vector<double> ks;
+    if (p.enable_vertical_boundaries) {
+        ks.resize((size_t)nkx * p.ny);
+        for (int j = 0; j < p.ny; ++j)
+            for (int i = 0; i < nkx; ++i) ks[i + (size_t)nkx * j] = hat_ksqr(kx[i], p.dx) + hat_ksqr(ky[j], p.dy);
+    } else {
+        ks.resize((size_t)nkx * p.ny * p.nz);
+        for (int k = 0; k < p.nz; ++k)
+            for (int j = 0; j < p.ny; ++j)
+                for (int i = 0; i < nkx; ++i)
+                    ks[i + (size_t)nkx * (j + (size_t)p.ny * k)] =
+                        hat_ksqr(kx[i], p.dx) + hat_ksqr(ky[j], p.dy) + hat_ksqr(kz[k], p.dz) / p.dsqr;
+    }
+    return ks;
+}
+
+int64_t ksqr_len(const Model &m)
+{
+    return m.prm.enable_vertical_boundaries ? (int64_t)m.g.nkx * m.g.ny : m.g.S;
+}
+
+int64_t field_len(const Model &m, int field)
+{
+    switch (field) {
+    case PS3D_U: case PS3D_V: case PS3D_W: case PS3D_B: case PS3D_P: return m.g.H;
+    case PS3D_DU: case PS3D_DV: case PS3D_DW: case PS3D_DB: return 3 * m.g.H;
+    case PS3D_FS: return 2 * m.g.S;
+    case PS3D_KSQR: return ksqr_len(m);
+    default: return -1;
+    }
+}
+
+void *field_dev(Model &m, int field)
+{
+    switch (field) {
+    case PS3D_U: case PS3D_V: case PS3D_W: case PS3D_B: return m.cur_buf(field);
+    case PS3D_P: return m.p;
+    case PS3D_DU: return m.du;
+    case PS3D_DV: return m.dv;
+    case PS3D_DW: return m.dw;
+    case PS3D_DB: return m.db;
+    case PS3D_FS: return m.FS;
+    case PS3D_KSQR: return m.ksqr;
+    default: return nullptr;
+    }
+}
+
+void check_model(const ps3d_model *h)
+{
+    if (!h) throw Error(PS3D_ERR_ARG, "null model");
+}
+
+void check_tau(int tau, int taum1, int taum2)
+{
+    auto ok = [](int t) { return t >= 1 && t <= 3; };
+    PS3D_REQUIRE(ok(tau) && ok(taum1) && ok(taum2), "tau slots must be 1..3");
+    PS3D_REQUIRE(tau != taum1 && tau != taum2 && taum1 != taum2, "tau slots must differ");
+}
+
+void pressure(Model &m)
+{
+    const ps3d_params &p = m.prm;
+    fft_r2c_x(m, nullptr, 0, 1, m.FS);
+    if (p.enable_vertical_boundaries) {
+        fft_c2c_y(m, m.FS, -1, p.dx, p.dy);
+        launch_thomas(m, m.FS);
+        fft_c2c_y(m, m.FS, +1, 0.0, 0.0);
+        fft_c2r_x(m, m.FS, m.p, 1, p.Lx * p.Ly);
+    } else {
+        fft_c2c_y(m, m.FS, -1, 0.0, 0.0);
+        fft_z_periodic_solve(m, m.FS);
+        fft_c2c_y(m, m.FS, +1, 0.0, 0.0);
+        fft_c2r_x(m, m.FS, m.p, 1, p.Lx * p.Ly * p.Lz);
+    }
+    launch_project_exchange(m);
+}
+
+void integrate(Model &m, int itt, int tau, int taum1, int taum2)
+{
+    launch_tendency(m, tau);
+    launch_friction_block(m);
+    launch_ab_exchange(m, itt, tau, taum1, taum2);
+    pressure(m);
+}
+
+const char *kKernelNames =
+    "tendency,harmonic,biharmonic_h,biharmonic_v,ab_update,halo,fft_x_r2c,fft_y_fwd,thomas,"
+    "fft_y_inv,fft_x_c2r,z_solve,fft_z_fwd,fft_z_inv,project";
+
+}  // namespace
+
+namespace ps3d {
+
+ProfScope::ProfScope(Model &mm, const char *name) : m(mm), on(false)
+{
+    if (!m.prof_name.empty() && m.prof_name == name) {
+        on = true;
+        PS3D_HIP_CHECK(hipEventCreate(&e1));
+        PS3D_HIP_CHECK(hipEventCreate(&e2));
+        PS3D_HIP_CHECK(hipEventRecord(e1, m.stream));
+    }
+}
+
+ProfScope::~ProfScope()
+{
+    if (on) {
+        (void)hipEventRecord(e2, m.stream);
+        m.prof_ev.push_back(e1);
+        m.prof_ev.push_back(e2);
+    }
+}
+
+}  // namespace ps3d
+
+struct ps3d_model : public ps3d::Model {};
+
+extern "C" {
+
+int ps3d_last_error(char *buf, int len)
+{
+    const int n = (int)g_err.size();
+    if (buf && len > 0) {
+        const int c = n < len - 1 ? n : len - 1;
+        std::memcpy(buf, g_err.data(), (size_t)c);
+        buf[c] = '\0';
+    }
+    return n;
+}
+
+const char *ps3d_version(void) { return "ps3d-amd 0.1 gfx950"; }
+
+const char *ps3d_kernel_names(void) { return kKernelNames; }
+
+int ps3d_create(const ps3d_params *prm, const double *ksqr, void *stream, ps3d_model **out)
+{
+    return guard([&] {
+        PS3D_REQUIRE(prm && out, "null argument");
+        *out = nullptr;
+        const ps3d_params &p = *prm;
+        PS3D_REQUIRE(p.precision == PS3D_FP64, "precision: only PS3D_FP64 is available in this build");
+        PS3D_REQUIRE(p.nx >= 2 && p.nx % 2 == 0, "nx must be even and >= 2 (main.f90:45-55)");
+        PS3D_REQUIRE(p.ny >= 2 && p.ny % 2 == 0, "ny must be even and >= 2 (main.f90:45-55)");
+        PS3D_REQUIRE(p.nz >= 2, "nz must be >= 2");
+        PS3D_REQUIRE(p.enable_vertical_boundaries || p.nz % 2 == 0, "nz must be even without a rigid lid");
+        PS3D_REQUIRE(is_235(p.nx / 2) && is_235(p.ny) && (p.enable_vertical_boundaries || is_235(p.nz)),
+                     "FFT sizes (nx/2, ny, nz) must be products of 2, 3 and 5");
+        PS3D_REQUIRE(p.ny <= 1024 && p.nz <= 1024 && p.nx <= 8192, "grid too large for the FFT kernels");
+        PS3D_REQUIRE(p.dx > 0 && p.dy > 0 && p.dz > 0 && p.dt != 0 && p.dsqr > 0, "bad grid spacing or dt");
+        auto *m = new ps3d_model();
+        try {
+            m->prm = p;
+            Grid &g = m->g;
+            g.nx = p.nx; g.ny = p.ny; g.nz = p.nz;
+            g.X = p.nx + 2 * ONX; g.Y = p.ny + 2 * ONX; g.Z = p.nz + 2 * ONX;
+            g.XY = g.X * g.Y;
+            g.H = g.XY * g.Z;
+            g.N = (int64_t)p.nx * p.ny * p.nz;
+            g.nkx = p.nx / 2 + 1;
+            g.S = (int64_t)g.nkx * p.ny * p.nz;
+            StepConsts &c = m->c;
+            c.nx = p.nx; c.ny = p.ny; c.nz = p.nz;
+            c.X = g.X; c.XY = g.XY;
+            c.lid = p.enable_vertical_boundaries;
+            c.nonlinear = p.enable_nonlinear;
+            c.dx = p.dx; c.dy = p.dy; c.dz = p.dz; c.dt = p.dt;
+            c.Ro = p.Ro; c.dsqr = p.dsqr; c.f0 = p.f0; c.N0 = p.N0;
+            if (stream) {
+                m->stream = static_cast<hipStream_t>(stream);
+            } else {
+                PS3D_HIP_CHECK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+                m->own_stream = true;
+            }
+            m->u = dalloc<double>(*m, g.H);
+            m->v = dalloc<double>(*m, g.H);
+            m->w = dalloc<double>(*m, g.H);
+            m->b = dalloc<double>(*m, g.H);
+            m->p = dalloc<double>(*m, g.H);
+            for (int f = 0; f < 4; ++f) m->alt[f] = dalloc<double>(*m, g.H);
+            m->du = dalloc<double>(*m, 3 * g.H);
+            m->dv = dalloc<double>(*m, 3 * g.H);
+            m->dw = dalloc<double>(*m, 3 * g.H);
+            m->db = dalloc<double>(*m, 3 * g.H);
+            m->FS = dalloc<double2>(*m, g.S);
+            m->aloc = dalloc<double>(*m, g.N);
+            m->ksqr = dalloc<double>(*m, ksqr_len(*m));
+            if (p.enable_vertical_boundaries) {
+                m->tri_cp = dalloc<double>(*m, g.S);
+                m->tri_rm = dalloc<double>(*m, g.S);
+            }
+            fft_plan_init(*m, m->px, p.nx / 2);
+            fft_plan_init(*m, m->py, p.ny);
+            if (!p.enable_vertical_boundaries) fft_plan_init(*m, m->pz, p.nz);
+            {
+                std::vector<double2> wk(g.nkx);
+                for (int k = 0; k < g.nkx; ++k) {
+                    const double ang = 2.0 * M_PI * (double)k / (double)p.nx;
+                    wk[k] = make_double2(std::cos(ang), -std::sin(ang));
+                }
+                m->tw_r2c = dalloc<double2>(*m, g.nkx);
+                PS3D_HIP_CHECK(hipMemcpyAsync(m->tw_r2c, wk.data(), sizeof(double2) * g.nkx,
+                                              hipMemcpyHostToDevice, m->stream));
+                PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
+            }
+            std::vector<double> ks;
+            const double *ksrc = ksqr;
+            if (!ksrc) {
+                ks = make_ksqr(p);
+                ksrc = ks.data();
+            }
+            PS3D_HIP_CHECK(hipMemcpyAsync(m->ksqr, ksrc, sizeof(double) * ksqr_len(*m), hipMemcpyHostToDevice,
+                                          m->stream));
+            tridiag_factor(*m);
+            PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
+        } catch (...) {
+            ps3d_destroy(m);
+            throw;
+        }
+        *out = m;
+    });
+}
+
+int ps3d_destroy(ps3d_model *m)
+{
+    return guard([&] {
+        if (!m) return;
+        if (m->stream) (void)hipStreamSynchronize(m->stream);
+        for (auto &ge : m->graphs) (void)hipGraphExecDestroy(ge.exec);
+        for (auto ev : m->prof_ev) (void)hipEventDestroy(ev);
+        for (void *p : m->allocations) (void)hipFree(p);
+        if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);
+        delete m;
+    });
+}
+
+int ps3d_field_size(const ps3d_model *m, int field, int64_t *n)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(n, "null argument");
+        const int64_t l = field_len(*m, field);
+        PS3D_REQUIRE(l >= 0, "unknown field id");
+        *n = l;
+    });
+}
+
+int ps3d_set_field(ps3d_model *m, int field, const double *host)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(host, "null host pointer");
+        const int64_t l = field_len(*m, field);
+        PS3D_REQUIRE(l >= 0, "unknown field id");
+        PS3D_HIP_CHECK(hipMemcpyAsync(field_dev(*m, field), host, sizeof(double) * l, hipMemcpyHostToDevice,
+                                      m->stream));
+        if (field == PS3D_KSQR) tridiag_factor(*m);
+        PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
+    });
+}
+
+int ps3d_get_field(ps3d_model *m, int field, double *host)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(host, "null host pointer");
+        const int64_t l = field_len(*m, field);
+        PS3D_REQUIRE(l >= 0, "unknown field id");
+        PS3D_HIP_CHECK(hipMemcpyAsync(host, field_dev(*m, field), sizeof(double) * l, hipMemcpyDeviceToHost,
+                                      m->stream));
+        PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
+    });
+}
+
+int ps3d_integrate(ps3d_model *m, int itt, int tau, int taum1, int taum2)
+{
+    return guard([&] {
+        check_model(m);
+        check_tau(tau, taum1, taum2);
+        PS3D_REQUIRE(itt >= 0, "itt must be >= 0");
+        integrate(*m, itt, tau, taum1, taum2);
+    });
+}
+
+int ps3d_run(ps3d_model *m, int nsteps, int32_t *st, int use_graph)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(st && nsteps >= 0, "bad arguments");
+        (void)use_graph;
+        for (int s = 0; s < nsteps; ++s) {
+            check_tau(st[1], st[2], st[3]);
+            integrate(*m, st[0], st[1], st[2], st[3]);
+            st[1] = st[1] % 3 + 1;
+            st[2] = st[2] % 3 + 1;
+            st[3] = st[3] % 3 + 1;
+            st[0] += 1;
+        }
+    });
+}
+
+int ps3d_synchronize(ps3d_model *m)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
+    });
+}
+
+int ps3d_tendency(ps3d_model *m, int tau)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(tau >= 1 && tau <= 3, "tau must be 1..3");
+        launch_tendency(*m, tau);
+    });
+}
+
+int ps3d_buoyancy_advection(ps3d_model *m, int tau)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(tau >= 1 && tau <= 3, "tau must be 1..3");
+        launch_buoyancy_advection(*m, tau);
+    });
+}
+
+int ps3d_momentum_advection(ps3d_model *m, int tau)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(tau >= 1 && tau <= 3, "tau must be 1..3");
+        launch_momentum_advection(*m, tau);
+    });
+}
+
+int ps3d_friction(ps3d_model *m)
+{
+    return guard([&] {
+        check_model(m);
+        launch_friction_block(*m);
+    });
+}
+
+int ps3d_mixing(ps3d_model *m, int op, int field, double hmix, double vmix)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(field >= PS3D_U && field <= PS3D_B, "mixing field must be u, v, w or b");
+        PS3D_REQUIRE(op >= PS3D_HARMONIC && op <= PS3D_BIHARMONIC_V, "unknown mixing operator");
+        launch_mixing(*m, op, field, hmix, vmix, 0);
+    });
+}
+
+int ps3d_ab_update(ps3d_model *m, int itt, int tau, int taum1, int taum2)
+{
+    return guard([&] {
+        check_model(m);
+        check_tau(tau, taum1, taum2);
+        launch_ab_exchange(*m, itt, tau, taum1, taum2);
+    });
+}
+
+int ps3d_border_exchg(ps3d_model *m, int field)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(field >= PS3D_U && field <= PS3D_P, "border_exchg field must be u, v, w, b or p");
+        launch_border_exchg(*m, field == PS3D_P ? m->p : m->cur_buf(field), 0);
+    });
+}
+
+int ps3d_pressure_solver(ps3d_model *m)
+{
+    return guard([&] {
+        check_model(m);
+        for (int f = 0; f < 4; ++f)
+            PS3D_REQUIRE(m->cur[f] == 0, "pressure_solver: friction results not yet folded in by ab_update");
+        pressure(*m);
+    });
+}
+
+int ps3d_my_fft(ps3d_model *m, int three_d, const double *qp, double *qs)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(qp && qs, "null buffer");
+        const ps3d_params &p = m->prm;
+        PS3D_REQUIRE(!three_d || !p.enable_vertical_boundaries,
+                     "3-D transform needs a periodic model (no z plan with a rigid lid)");
+        PS3D_HIP_CHECK(hipMemcpyAsync(m->aloc, qp, sizeof(double) * m->g.N, hipMemcpyHostToDevice, m->stream));
+        fft_r2c_x(*m, m->aloc, 0, 0, m->FS);
+        if (three_d) {
+            fft_c2c_y(*m, m->FS, -1, 0.0, 0.0);
+            fft_c2c_z(*m, m->FS, -1, p.dx, p.dy, p.dz);
+        } else {
+            fft_c2c_y(*m, m->FS, -1, p.dx, p.dy);
+        }
+        PS3D_HIP_CHECK(hipMemcpyAsync(qs, m->FS, sizeof(double2) * m->g.S, hipMemcpyDeviceToHost, m->stream));
+        PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
+    });
+}
+
+int ps3d_my_ifft(ps3d_model *m, int three_d, const double *qs, double *qp)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(qp && qs, "null buffer");
+        const ps3d_params &p = m->prm;
+        PS3D_REQUIRE(!three_d || !p.enable_vertical_boundaries,
+                     "3-D transform needs a periodic model (no z plan with a rigid lid)");
+        PS3D_HIP_CHECK(hipMemcpyAsync(m->FS, qs, sizeof(double2) * m->g.S, hipMemcpyHostToDevice, m->stream));
+        if (three_d) fft_c2c_z(*m, m->FS, +1, 0.0, 0.0, 0.0);
+        fft_c2c_y(*m, m->FS, +1, 0.0, 0.0);
+        fft_c2r_x(*m, m->FS, m->aloc, 0, three_d ? p.Lx * p.Ly * p.Lz : p.Lx * p.Ly);
+        PS3D_HIP_CHECK(hipMemcpyAsync(qp, m->aloc, sizeof(double) * m->g.N, hipMemcpyDeviceToHost, m->stream));
+        PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
+    });
+}
+
+int ps3d_spectral_solve(ps3d_model *m, double *qs)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(qs, "null buffer");
+        PS3D_HIP_CHECK(hipMemcpyAsync(m->FS, qs, sizeof(double2) * m->g.S, hipMemcpyHostToDevice, m->stream));
+        if (m->prm.enable_vertical_boundaries) launch_thomas(*m, m->FS);
+        else launch_divide_ksqr(*m, m->FS);
+        PS3D_HIP_CHECK(hipMemcpyAsync(qs, m->FS, sizeof(double2) * m->g.S, hipMemcpyDeviceToHost, m->stream));
+        PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
+    });
+}
+
+int ps3d_profile_kernel(ps3d_model *m, const char *name)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
+        for (auto ev : m->prof_ev) (void)hipEventDestroy(ev);
+        m->prof_ev.clear();
+        m->prof_launches = 0;
+        m->prof_ms = 0.0;
+        m->prof_name = name ? name : "";
+    });
+}
+
+int ps3d_profile_read(ps3d_model *m, int64_t *launches, double *total_ms)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
+        for (size_t i = 0; i + 1 < m->prof_ev.size(); i += 2) {
+            float ms = 0.f;
+            PS3D_HIP_CHECK(hipEventElapsedTime(&ms, m->prof_ev[i], m->prof_ev[i + 1]));
+            m->prof_ms += ms;
+            m->prof_launches += 1;
+        }
+        for (auto ev : m->prof_ev) (void)hipEventDestroy(ev);
+        m->prof_ev.clear();
+        if (launches) *launches = m->prof_launches;
+        if (total_ms) *total_ms = m->prof_ms;
+        m->prof_launches = 0;
+        m->prof_ms = 0.0;
+    });
+}
+
+}  // extern "C"

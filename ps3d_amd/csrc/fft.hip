@@ -1,0 +1,472 @@
+// fft.hip -- mixed-radix (4, 2, 3, 5) FP64 FFTs for the spectral Poisson
+// solve of pressure_solver.f90, replacing P3DFFT (my_fft.f90:159-205).
+//
+// Conventions of the transforms they replace: forward r2c unnormalised with
+// e^{-i k x}; inverse c2r unnormalised with e^{+i k x}, the imaginary parts of
+// the kx = 0 and kx = nx/2 modes ignored; the spectral array is (nx/2+1, ny, nz)
+// with kx fastest.  The scalings of my_fft_2D/3D (QS*dx*dy(*dz)) and of
+// my_ifft_2D/3D (QP/(Lx*Ly(*Lz))) are applied inside the kernels.
+//
+// Layout on the device: every 1-D transform of a pass is staged in LDS,
+// batched so that global loads and stores are coalesced along the unit-stride
+// axis.  x-pass: 1 row per transform, rows of one k-level side by side.
+// y- and z-passes: TX consecutive kx columns per work-group, so each global
+// access touches TX*16 contiguous bytes; the transposition needed to run the
+// y (or z) transform on those columns happens in LDS, which is what the
+// pencil transposes of P3DFFT become on one device.
+//
+// Each pass is a Stockham autosort step on one LDS buffer: a thread reads the
+// R inputs of its butterflies into registers, the work-group synchronises,
+// and the outputs are written back in natural order.
+#include "ps3d_internal.h"
+
+#include <cmath>
+
+namespace ps3d {
+
+namespace {
+
+constexpr int MAXE = 16;   // max complex values one thread stages per pass
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+// i*a for sign=+1, -i*a for sign=-1
+__device__ __forceinline__ double2 mul_isign(double2 a, int sign)
+{
+    return sign > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+template <int R>
+__device__ __forceinline__ void dft(double2 *v, int sign)
+{
+    if constexpr (R == 2) {
+        const double2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        const double c = -0.5, s = 0.86602540378443864676;   // cos, sin(2 pi/3)
+        const double2 t = cadd(v[1], v[2]);
+        const double2 d = csub(v[1], v[2]);
+        const double2 m = make_double2(v[0].x + c * t.x, v[0].y + c * t.y);
+        const double2 q = mul_isign(cscale(d, s), sign);
+        v[0] = cadd(v[0], t);
+        v[1] = cadd(m, q);
+        v[2] = csub(m, q);
+    } else if constexpr (R == 4) {
+        const double2 a = cadd(v[0], v[2]), bq = csub(v[0], v[2]);
+        const double2 c = cadd(v[1], v[3]), d = mul_isign(csub(v[1], v[3]), sign);
+        v[0] = cadd(a, c);
+        v[2] = csub(a, c);
+        v[1] = cadd(bq, d);
+        v[3] = csub(bq, d);
+    } else if constexpr (R == 5) {
+        const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+        const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+        const double2 a14 = cadd(v[1], v[4]), d14 = csub(v[1], v[4]);
+        const double2 a23 = cadd(v[2], v[3]), d23 = csub(v[2], v[3]);
+        const double2 m1 = make_double2(v[0].x + c1 * a14.x + c2 * a23.x, v[0].y + c1 * a14.y + c2 * a23.y);
+        const double2 m2 = make_double2(v[0].x + c2 * a14.x + c1 * a23.x, v[0].y + c2 * a14.y + c1 * a23.y);
+        const double2 q1 = mul_isign(make_double2(s1 * d14.x + s2 * d23.x, s1 * d14.y + s2 * d23.y), sign);
+        const double2 q2 = mul_isign(make_double2(s2 * d14.x - s1 * d23.x, s2 * d14.y - s1 * d23.y), sign);
+        v[0] = make_double2(v[0].x + a14.x + a23.x, v[0].y + a14.y + a23.y);
+        v[1] = cadd(m1, q1);
+        v[4] = csub(m1, q1);
+        v[2] = cadd(m2, q2);
+        v[3] = csub(m2, q2);
+    }
+}
+
+// One Stockham pass of radix R on ncol sequences of length N in LDS.
+// Element e of column col lives at buf[e*es + col*cs].  COLFAST maps
+// consecutive threads to consecutive columns (for es > 1 layouts).
+template <int R, bool COLFAST>
+__device__ __forceinline__ void stockham_pass(double2 *buf, int N, int Ns, int ncol, int es, int cs,
+                                              const double2 *__restrict__ tw, int sign)
+{
+    constexpr int MAXB = MAXE / R > 0 ? MAXE / R + (MAXE % R ? 1 : 0) : 1;
+    const int nb = N / R;
+    const int total = nb * ncol;
+    const int T = blockDim.x;
+    const int twstride = N / (Ns * R);
+    double2 v[MAXB][R];
+#pragma unroll
+    for (int s = 0; s < MAXB; ++s) {
+        const int t = threadIdx.x + s * T;
+        if (t < total) {
+            const int col = COLFAST ? t % ncol : t / nb;
+            const int jb = COLFAST ? t / ncol : t % nb;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[s][r] = buf[(jb + r * nb) * es + col * cs];
+            if (Ns > 1) {
+                const int jm = jb % Ns;
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    double2 w = tw[r * jm * twstride];
+                    if (sign > 0) w.y = -w.y;
+                    v[s][r] = cmul(v[s][r], w);
+                }
+            }
+            dft<R>(v[s], sign);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < MAXB; ++s) {
+        const int t = threadIdx.x + s * T;
+        if (t < total) {
+            const int col = COLFAST ? t % ncol : t / nb;
+            const int jb = COLFAST ? t / ncol : t % nb;
+            const int jm = jb % Ns;
+            const int base = (jb / Ns) * Ns * R + jm;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[(base + r * Ns) * es + col * cs] = v[s][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <bool COLFAST>
+__device__ void lds_fft(double2 *buf, const FftPlan &pl, int ncol, int es, int cs, int sign)
+{
+    int Ns = 1;
+    for (int f = 0; f < pl.nf; ++f) {
+        const int R = pl.fac[f];
+        switch (R) {
+        case 4: stockham_pass<4, COLFAST>(buf, pl.n, Ns, ncol, es, cs, pl.tw, sign); break;
+        case 2: stockham_pass<2, COLFAST>(buf, pl.n, Ns, ncol, es, cs, pl.tw, sign); break;
+        case 3: stockham_pass<3, COLFAST>(buf, pl.n, Ns, ncol, es, cs, pl.tw, sign); break;
+        case 5: stockham_pass<5, COLFAST>(buf, pl.n, Ns, ncol, es, cs, pl.tw, sign); break;
+        default: break;
+        }
+        Ns *= R;
+    }
+}
+
+__device__ __forceinline__ int64_t fidx(const StepConsts &c, int i, int j, int k)
+{
+    return (int64_t)(i + 1) + c.X * (int64_t)(j + 1) + c.XY * (int64_t)(k + 1);
+}
+
+// ---------------------------------------------------------------------------
+// x-pass, forward r2c.  Rows r = j + ny*k (0-based).  Source: a dense real
+// (nx,ny,nz) array, or (FUSE_DIV) the divergence of u, v, w divided by dt,
+// pressure_solver.f90:16-24, computed while loading.
+// ---------------------------------------------------------------------------
+template <bool FUSE_DIV>
+__global__ __launch_bounds__(256) void k_r2c_x(StepConsts c, FftPlan pl, const double2 *__restrict__ wk,
+                                               const double *__restrict__ src, const double *__restrict__ u,
+                                               const double *__restrict__ v, const double *__restrict__ w,
+                                               double2 *__restrict__ dst, int rows_per_block, int64_t nrows)
+{
+    extern __shared__ double2 lds[];
+    const int n2 = pl.n;
+    const int nkx = n2 + 1;
+    const int cs = n2 + 1;   // padded row stride in LDS
+    const int64_t row0 = (int64_t)blockIdx.x * rows_per_block;
+    const int nrow = (int)min((int64_t)rows_per_block, nrows - row0);
+    // load: z[m] = x[2m] + i x[2m+1]
+    for (int t = threadIdx.x; t < nrow * n2; t += blockDim.x) {
+        const int rr = t / n2, m = t % n2;
+        const int64_t row = row0 + rr;
+        double a0, a1;
+        if (FUSE_DIV) {
+            const int j = (int)(row % c.ny) + 1, k = (int)(row / c.ny) + 1;
+            const int64_t X = c.X, XY = c.XY;
+            int64_t o = fidx(c, 2 * m + 1, j, k);
+            a0 = (u[o] - u[o - 1]) / c.dx + (v[o] - v[o - X]) / c.dy + (w[o] - w[o - XY]) / c.dz;
+            o += 1;
+            a1 = (u[o] - u[o - 1]) / c.dx + (v[o] - v[o - X]) / c.dy + (w[o] - w[o - XY]) / c.dz;
+            a0 = a0 / c.dt;
+            a1 = a1 / c.dt;
+        } else {
+            const double2 pr = reinterpret_cast<const double2 *>(src + row * (2 * (int64_t)n2))[m];
+            a0 = pr.x;
+            a1 = pr.y;
+        }
+        lds[rr * cs + m] = make_double2(a0, a1);
+    }
+    __syncthreads();
+    lds_fft<false>(lds, pl, nrow, 1, cs, -1);
+    // post-process: X[k] = E[k] + W^k O[k]
+    for (int t = threadIdx.x; t < nrow * nkx; t += blockDim.x) {
+        const int rr = t / nkx, kk = t % nkx;
+        const double2 zk = lds[rr * cs + (kk == n2 ? 0 : kk)];
+        const double2 zm = lds[rr * cs + (kk == 0 ? 0 : n2 - kk)];
+        const double2 e = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+        const double2 bq = make_double2(zk.x - zm.x, zk.y + zm.y);   // zk - conj(zm)
+        const double2 o = make_double2(0.5 * bq.y, -0.5 * bq.x);     // bq / (2i)
+        const double2 xk = cadd(e, cmul(wk[kk], o));
+        dst[(row0 + rr) * nkx + kk] = xk;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x-pass, inverse c2r, scaled by 1/inv_scale (my_fft.f90:166, :191).
+// Destination: the interior of a field (halo layout) or a dense array.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_c2r_x(StepConsts c, FftPlan pl, const double2 *__restrict__ wk,
+                                               const double2 *__restrict__ src, double *__restrict__ dst,
+                                               int dst_is_field, double inv_scale, int rows_per_block,
+                                               int64_t nrows)
+{
+    extern __shared__ double2 lds[];
+    const int n2 = pl.n;
+    const int nkx = n2 + 1;
+    const int cs = n2 + 1;
+    double2 *raw = lds + rows_per_block * cs;
+    const int64_t row0 = (int64_t)blockIdx.x * rows_per_block;
+    const int nrow = (int)min((int64_t)rows_per_block, nrows - row0);
+    for (int t = threadIdx.x; t < nrow * nkx; t += blockDim.x) {
+        const int rr = t / nkx, kk = t % nkx;
+        double2 x = src[(row0 + rr) * nkx + kk];
+        if (kk == 0 || kk == n2) x.y = 0.0;
+        raw[rr * cs + kk] = x;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nrow * n2; t += blockDim.x) {
+        const int rr = t / n2, kk = t % n2;
+        const double2 xk = raw[rr * cs + kk];
+        const double2 xm = raw[rr * cs + n2 - kk];
+        const double2 a = make_double2(xk.x + xm.x, xk.y - xm.y);   // xk + conj(xm)
+        const double2 d = make_double2(xk.x - xm.x, xk.y + xm.y);   // xk - conj(xm)
+        const double2 wc = make_double2(wk[kk].x, -wk[kk].y);       // conj(W^k)
+        const double2 q = cmul(wc, d);
+        lds[rr * cs + kk] = make_double2(a.x - q.y, a.y + q.x);     // a + i q
+    }
+    __syncthreads();
+    lds_fft<false>(lds, pl, nrow, 1, cs, +1);
+    for (int t = threadIdx.x; t < nrow * n2; t += blockDim.x) {
+        const int rr = t / n2, m = t % n2;
+        const int64_t row = row0 + rr;
+        const double2 z = lds[rr * cs + m];
+        const double a0 = z.x / inv_scale, a1 = z.y / inv_scale;
+        if (dst_is_field) {
+            const int j = (int)(row % c.ny) + 1, k = (int)(row / c.ny) + 1;
+            const int64_t o = fidx(c, 2 * m + 1, j, k);
+            dst[o] = a0;
+            dst[o + 1] = a1;
+        } else {
+            reinterpret_cast<double2 *>(dst + row * (2 * (int64_t)n2))[m] = make_double2(a0, a1);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// strided c2c pass over TX consecutive kx columns.  Element e of column col
+// is at data[kx0+col + e*estride + lane_off]; optional scaling of the result
+// by s1 then s2 (my_fft.f90:177: QS*dx*dy).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(512) void k_c2c_cols(FftPlan pl, double2 *__restrict__ data, int nkx, int TX,
+                                                  int64_t estride, int64_t lstride, int sign, double s1,
+                                                  double s2, double s3, int nscale)
+{
+    extern __shared__ double2 lds[];
+    const int N = pl.n;
+    const int kx0 = blockIdx.x * TX;
+    const int ncol = min(TX, nkx - kx0);
+    double2 *base = data + kx0 + (int64_t)blockIdx.y * lstride;
+    for (int t = threadIdx.x; t < N * TX; t += blockDim.x) {
+        const int col = t % TX, e = t / TX;
+        if (col < ncol) lds[e * TX + col] = base[col + e * estride];
+    }
+    __syncthreads();
+    lds_fft<true>(lds, pl, TX, TX, 1, sign);
+    for (int t = threadIdx.x; t < N * TX; t += blockDim.x) {
+        const int col = t % TX, e = t / TX;
+        if (col < ncol) {
+            double2 x = lds[e * TX + col];
+            if (nscale >= 1) x = cscale(x, s1);
+            if (nscale >= 2) x = cscale(x, s2);
+            if (nscale >= 3) x = cscale(x, s3);
+            base[col + e * estride] = x;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// periodic solve along z, pressure_solver.f90:58-70 with my_fft.f90's z
+// transforms: forward z FFT, QS*dx*dy*dz, division by -ksqr (0 where
+// ksqr <= 0), inverse z FFT -- one kernel, the column block stays in LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(512) void k_z_periodic_solve(FftPlan pl, double2 *__restrict__ data,
+                                                          const double *__restrict__ ksqr, int nkx, int ny,
+                                                          int TX, double dx, double dy, double dz)
+{
+    extern __shared__ double2 lds[];
+    const int N = pl.n;
+    const int kx0 = blockIdx.x * TX;
+    const int j = blockIdx.y;
+    const int ncol = min(TX, nkx - kx0);
+    const int64_t estride = (int64_t)nkx * ny;
+    double2 *base = data + kx0 + (int64_t)j * nkx;
+    const double *kb = ksqr + kx0 + (int64_t)j * nkx;
+    for (int t = threadIdx.x; t < N * TX; t += blockDim.x) {
+        const int col = t % TX, e = t / TX;
+        if (col < ncol) lds[e * TX + col] = base[col + e * estride];
+    }
+    __syncthreads();
+    lds_fft<true>(lds, pl, TX, TX, 1, -1);
+    for (int t = threadIdx.x; t < N * TX; t += blockDim.x) {
+        const int col = t % TX, e = t / TX;
+        if (col < ncol) {
+            double2 x = lds[e * TX + col];
+            x = cscale(cscale(cscale(x, dx), dy), dz);
+            const double ks = kb[col + e * estride];
+            if (ks > 0.0) x = make_double2(-(x.x / ks), -(x.y / ks));
+            else x = make_double2(0.0, 0.0);
+            lds[e * TX + col] = x;
+        }
+    }
+    __syncthreads();
+    lds_fft<true>(lds, pl, TX, TX, 1, +1);
+    for (int t = threadIdx.x; t < N * TX; t += blockDim.x) {
+        const int col = t % TX, e = t / TX;
+        if (col < ncol) base[col + e * estride] = lds[e * TX + col];
+    }
+}
+
+// rows per block of an x-pass so that every radix's butterflies fit in
+// MAXE staged values per thread (256 threads)
+int rows_for(const FftPlan &pl, int threads)
+{
+    int rows = 64;
+    for (; rows > 1; --rows) {
+        bool ok = true;
+        for (int f = 0; f < pl.nf; ++f) {
+            const int R = pl.fac[f];
+            const int maxb = MAXE / R + (MAXE % R ? 1 : 0);
+            if ((int64_t)(pl.n / R) * rows > (int64_t)maxb * threads) ok = false;
+        }
+        if (ok && (int64_t)rows * (pl.n + 1) * 2 * 16 <= 64 * 1024) break;
+    }
+    return rows;
+}
+
+int threads_for_cols(const FftPlan &pl, int TX)
+{
+    for (int T = 64; T <= 512; T *= 2) {
+        bool ok = true;
+        for (int f = 0; f < pl.nf; ++f) {
+            const int R = pl.fac[f];
+            const int maxb = MAXE / R + (MAXE % R ? 1 : 0);
+            if ((int64_t)(pl.n / R) * TX > (int64_t)maxb * T) ok = false;
+        }
+        if (ok) return T;
+    }
+    return -1;
+}
+
+constexpr int TXC = 8;   // kx columns per work-group in the y and z passes
+
+// dynamic LDS above 64 KiB must be requested per kernel
+template <class K>
+void allow_lds(K kernel, size_t bytes)
+{
+    if (bytes > 64 * 1024)
+        PS3D_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+}  // namespace
+
+void fft_plan_init(Model &m, FftPlan &pl, int n)
+{
+    pl.n = n;
+    pl.nf = 0;
+    int r = n;
+    while (r % 4 == 0) { pl.fac[pl.nf++] = 4; r /= 4; }
+    while (r % 2 == 0) { pl.fac[pl.nf++] = 2; r /= 2; }
+    while (r % 3 == 0) { pl.fac[pl.nf++] = 3; r /= 3; }
+    while (r % 5 == 0) { pl.fac[pl.nf++] = 5; r /= 5; }
+    PS3D_REQUIRE(r == 1, "FFT length " + std::to_string(n) + " has a prime factor other than 2, 3, 5");
+    PS3D_REQUIRE(pl.nf <= MAX_FAC, "FFT length too long");
+    std::vector<double2> tw(n);
+    for (int q = 0; q < n; ++q) {
+        const double ang = 2.0 * M_PI * (double)q / (double)n;
+        tw[q] = make_double2(std::cos(ang), -std::sin(ang));
+    }
+    double2 *d = nullptr;
+    PS3D_HIP_CHECK(hipMalloc(&d, sizeof(double2) * n));
+    m.allocations.push_back(d);
+    PS3D_HIP_CHECK(hipMemcpy(d, tw.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+    pl.tw = d;
+    PS3D_REQUIRE(threads_for_cols(pl, TXC) > 0, "FFT length too long for one work-group");
+}
+
+void fft_r2c_x(Model &m, const double *src, int /*src_is_field*/, int fuse_div, double2 *dst)
+{
+    const Grid &g = m.g;
+    const int rows = rows_for(m.px, 256);
+    const int64_t nrows = (int64_t)g.ny * g.nz;
+    const size_t lds = (size_t)rows * (m.px.n + 1) * sizeof(double2);
+    const dim3 grid((unsigned)((nrows + rows - 1) / rows));
+    ProfScope ps(m, "fft_x_r2c");
+    if (fuse_div)
+        hipLaunchKernelGGL(k_r2c_x<true>, grid, dim3(256), lds, m.stream, m.c, m.px, m.tw_r2c, nullptr, m.u,
+                           m.v, m.w, dst, rows, nrows);
+    else
+        hipLaunchKernelGGL(k_r2c_x<false>, grid, dim3(256), lds, m.stream, m.c, m.px, m.tw_r2c, src, nullptr,
+                           nullptr, nullptr, dst, rows, nrows);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void fft_c2r_x(Model &m, const double2 *src, double *dst, int dst_is_field, double inv_scale)
+{
+    const Grid &g = m.g;
+    const int rows = rows_for(m.px, 256);
+    const int64_t nrows = (int64_t)g.ny * g.nz;
+    const size_t lds = (size_t)2 * rows * (m.px.n + 1) * sizeof(double2);
+    const dim3 grid((unsigned)((nrows + rows - 1) / rows));
+    ProfScope ps(m, "fft_x_c2r");
+    hipLaunchKernelGGL(k_c2r_x, grid, dim3(256), lds, m.stream, m.c, m.px, m.tw_r2c, src, dst, dst_is_field,
+                       inv_scale, rows, nrows);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void fft_c2c_y(Model &m, double2 *data, int sign, double scale1, double scale2)
+{
+    const Grid &g = m.g;
+    const int T = threads_for_cols(m.py, TXC);
+    const size_t lds = (size_t)m.py.n * TXC * sizeof(double2);
+    const dim3 grid((unsigned)((g.nkx + TXC - 1) / TXC), (unsigned)g.nz);
+    const int nscale = (scale1 != 0.0) ? 2 : 0;
+    allow_lds(k_c2c_cols, lds);
+    ProfScope ps(m, sign < 0 ? "fft_y_fwd" : "fft_y_inv");
+    hipLaunchKernelGGL(k_c2c_cols, grid, dim3(T), lds, m.stream, m.py, data, g.nkx, TXC, (int64_t)g.nkx,
+                       (int64_t)g.nkx * g.ny, sign, scale1, scale2, 0.0, nscale);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void fft_c2c_z(Model &m, double2 *data, int sign, double s1, double s2, double s3)
+{
+    const Grid &g = m.g;
+    const int T = threads_for_cols(m.pz, TXC);
+    const size_t lds = (size_t)m.pz.n * TXC * sizeof(double2);
+    const dim3 grid((unsigned)((g.nkx + TXC - 1) / TXC), (unsigned)g.ny);
+    const int nscale = (s1 != 0.0) ? 3 : 0;
+    allow_lds(k_c2c_cols, lds);
+    ProfScope ps(m, sign < 0 ? "fft_z_fwd" : "fft_z_inv");
+    hipLaunchKernelGGL(k_c2c_cols, grid, dim3(T), lds, m.stream, m.pz, data, g.nkx, TXC,
+                       (int64_t)g.nkx * g.ny, (int64_t)g.nkx, sign, s1, s2, s3, nscale);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void fft_z_periodic_solve(Model &m, double2 *data)
+{
+    const Grid &g = m.g;
+    const int T = threads_for_cols(m.pz, TXC);
+    const size_t lds = (size_t)m.pz.n * TXC * sizeof(double2);
+    const dim3 grid((unsigned)((g.nkx + TXC - 1) / TXC), (unsigned)g.ny);
+    allow_lds(k_z_periodic_solve, lds);
+    ProfScope ps(m, "z_solve");
+    hipLaunchKernelGGL(k_z_periodic_solve, grid, dim3(T), lds, m.stream, m.pz, data, m.ksqr, g.nkx, g.ny, TXC,
+                       m.prm.dx, m.prm.dy, m.prm.dz);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace ps3d

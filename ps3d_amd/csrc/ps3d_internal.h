@@ -1,0 +1,136 @@
+// ps3d_internal.h -- device-side data structures of the ps3D step on gfx950.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ps3d.h"
+
+namespace ps3d {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define PS3D_HIP_CHECK(x)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess)                                                          \
+            throw ::ps3d::Error(PS3D_ERR_HIP, std::string(#x) + " failed: " +          \
+                                                  hipGetErrorString(e_));              \
+    } while (0)
+
+#define PS3D_REQUIRE(cond, msg)                                                        \
+    do {                                                                               \
+        if (!(cond)) throw ::ps3d::Error(PS3D_ERR_ARG, (msg));                         \
+    } while (0)
+
+constexpr int ONX = 2;   // halo width, main_module.f90:93
+
+// Extents of one model.  Physical fields are (X, Y, Z) = (nx+4, ny+4, nz+4)
+// with Fortran index i in -1..nx+2 at offset i+1 (i fastest).
+struct Grid {
+    int nx, ny, nz;
+    int64_t X, Y, Z, XY, H;   // padded extents, plane and field sizes
+    int64_t N;                // nx*ny*nz
+    int nkx;                  // nx/2+1
+    int64_t S;                // nkx*ny*nz complex spectral points
+};
+
+// Scalars the kernels read, in the form they use them.  Every value comes
+// from the driver's main_module unchanged (no re-derivation).
+struct StepConsts {
+    int nx, ny, nz;
+    int64_t X, XY;
+    int lid;           // enable_vertical_boundaries
+    int nonlinear;     // enable_nonlinear
+    double dx, dy, dz, dt, Ro, dsqr, f0, N0;
+};
+
+// One mixed-radix FFT length: factors (radix 4, 2, 3, 5) and its table
+// tw[q] = exp(-2 pi i q / n), q = 0..n-1, in device memory.
+constexpr int MAX_FAC = 16;
+struct FftPlan {
+    int n;
+    int nf;
+    int fac[MAX_FAC];
+    const double2 *tw;
+};
+
+struct Model {
+    ps3d_params prm{};
+    Grid g{};
+    StepConsts c{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+
+    // prognostic fields (canonical buffers), halo included
+    double *u = nullptr, *v = nullptr, *w = nullptr, *b = nullptr, *p = nullptr;
+    // second buffers for the out-of-place friction passes
+    double *alt[4] = {nullptr, nullptr, nullptr, nullptr};
+    // tendencies, 3 slots of H each (main_module.f90:124-127)
+    double *du = nullptr, *dv = nullptr, *dw = nullptr, *db = nullptr;
+    double2 *FS = nullptr;       // spectral array (nkx, ny, nz)
+    double *ksqr = nullptr;      // (nkx, ny) rigid lid, (nkx, ny, nz) periodic
+    double *tri_cp = nullptr;    // Thomas factors c'(k) per column (rigid lid)
+    double *tri_rm = nullptr;    // 1/m(k) per column (1/b(1) at k=1)
+    double *aloc = nullptr;      // real (nx, ny, nz) scratch for the building-block API
+
+    FftPlan px{}, py{}, pz{};    // lengths nx/2, ny, nz
+    double2 *tw_r2c = nullptr;   // exp(-2 pi i k / nx), k = 0..nx/2
+    std::vector<void *> allocations;
+
+    // per-field "current buffer" during the friction block (0 = canonical)
+    int cur[4] = {0, 0, 0, 0};
+
+    // profiling of one kernel by name
+    std::string prof_name;
+    std::vector<hipEvent_t> prof_ev;   // pairs (start, stop)
+    int64_t prof_launches = 0;
+    double prof_ms = 0.0;
+
+    // graphs of a whole step, keyed by (AB order kind, tau)
+    struct GraphEntry { int key; hipGraphExec_t exec; };
+    std::vector<GraphEntry> graphs;
+
+    double *field_buf(int id) const;
+    double *cur_buf(int f) const { return cur[f] ? alt[f] : field_buf(f); }
+};
+
+// ---- launchers (stencil.hip) -------------------------------------------
+void launch_tendency(Model &m, int tau);
+void launch_buoyancy_advection(Model &m, int tau);
+void launch_momentum_advection(Model &m, int tau);
+void launch_friction_block(Model &m);
+void launch_mixing(Model &m, int op, int field, double hmix, double vmix, int zero_top);
+void launch_ab_exchange(Model &m, int itt, int tau, int taum1, int taum2);
+void launch_border_exchg(Model &m, double *a, int neumann_z);
+void launch_project_exchange(Model &m);
+
+// ---- launchers (fft.hip) ------------------------------------------------
+void fft_plan_init(Model &m, FftPlan &pl, int n);
+void fft_r2c_x(Model &m, const double *src, int src_is_field, int fuse_div, double2 *dst);
+void fft_c2c_y(Model &m, double2 *data, int sign, double scale1, double scale2);
+void fft_c2r_x(Model &m, const double2 *src, double *dst, int dst_is_field, double inv_scale);
+void fft_z_periodic_solve(Model &m, double2 *data);
+void fft_c2c_z(Model &m, double2 *data, int sign, double s1, double s2, double s3);
+
+// ---- launchers (tridiag.hip) --------------------------------------------
+void tridiag_factor(Model &m);
+void launch_thomas(Model &m, double2 *data);
+
+// ---- profiling helpers (api.hip) ----------------------------------------
+struct ProfScope {
+    Model &m;
+    bool on;
+    hipEvent_t e1{}, e2{};
+    ProfScope(Model &mm, const char *name);
+    ~ProfScope();
+};
+
+}  // namespace ps3d

@@ -1,0 +1,477 @@
+// stencil.hip -- finite-volume stencils of the ps3D step on gfx950, FP64.
+//
+// Every kernel evaluates the reference's expressions with the same operands
+// and the same association (contraction off), so results are bit-identical
+// to the reference loops: advection.f90, mixing.f90, integrate.f90:21-54 and
+// :87-102, pressure_solver.f90:16-24 and :98-110, parallel_mpi.f90:70-211.
+//
+// Fluxes are recomputed per point rather than staged in the flux_east /
+// flux_north / flux_top scratch arrays: one pass over HBM per stencil
+// instead of one per flux array.
+#include "ps3d_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace ps3d {
+
+namespace {
+
+constexpr int BX = 64;   // one wavefront along i (unit stride)
+constexpr int BY = 4;
+
+__device__ __forceinline__ int64_t idx(const StepConsts &c, int i, int j, int k)
+{
+    return (int64_t)(i + 1) + c.X * (int64_t)(j + 1) + c.XY * (int64_t)(k + 1);
+}
+
+__device__ __forceinline__ int wrap(int i, int n)
+{
+    return i < 1 ? i + n : (i > n ? i - n : i);
+}
+
+// ---------------------------------------------------------------------------
+// tendencies: advection.f90:2-184 + integrate.f90:21-54 (interior points)
+// ---------------------------------------------------------------------------
+template <bool DO_B, bool DO_M, bool DO_INLINE>
+__global__ __launch_bounds__(BX * BY) void k_tendency(StepConsts c, const double *__restrict__ u,
+                                                       const double *__restrict__ v,
+                                                       const double *__restrict__ w,
+                                                       const double *__restrict__ b,
+                                                       double *__restrict__ du, double *__restrict__ dv,
+                                                       double *__restrict__ dw, double *__restrict__ db)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > c.nx || j > c.ny) return;
+    const int64_t X = c.X, XY = c.XY;
+    const int64_t o = idx(c, i, j, k);
+    const bool lid = c.lid != 0;
+    const bool top0 = lid && (k >= c.nz);       // flux_top(k) zeroed at the lid
+    const bool bot0 = lid && (k - 1 <= 0);      // flux_top(k-1) zeroed at the bottom
+    const bool top0w = lid && (k >= c.nz - 1);  // w flux_top zeroed from nz-1 (advection.f90:169)
+    const double Ro = c.Ro, dx = c.dx, dy = c.dy, dz = c.dz;
+
+    if (DO_B) {
+        double adv = 0.0;
+        if (c.nonlinear) {
+            const double fe1 = 0.5 * (b[o] + b[o + 1]) * u[o];
+            const double fe0 = 0.5 * (b[o - 1] + b[o]) * u[o - 1];
+            const double fn1 = 0.5 * (b[o] + b[o + X]) * v[o];
+            const double fn0 = 0.5 * (b[o - X] + b[o]) * v[o - X];
+            const double ft1 = top0 ? 0.0 : 0.5 * (b[o] + b[o + XY]) * w[o];
+            const double ft0 = bot0 ? 0.0 : 0.5 * (b[o - XY] + b[o]) * w[o - XY];
+            adv = -(Ro * ((fe1 - fe0) / dx + (fn1 - fn0) / dy + (ft1 - ft0) / dz));
+        }
+        if (DO_INLINE) adv = adv - 0.5 * (w[o] + w[o - XY]) * (c.N0 * c.N0);
+        db[o] = adv;
+    }
+    if (DO_M) {
+        double au = 0.0, av = 0.0, aw = 0.0;
+        if (c.nonlinear) {
+            {   // zonal momentum, advection.f90:65-100
+                const double fe1 = 0.25 * (u[o] + u[o + 1]) * (u[o + 1] + u[o]);
+                const double fe0 = 0.25 * (u[o - 1] + u[o]) * (u[o] + u[o - 1]);
+                const double fn1 = 0.25 * (u[o] + u[o + X]) * (v[o + 1] + v[o]);
+                const double fn0 = 0.25 * (u[o - X] + u[o]) * (v[o + 1 - X] + v[o - X]);
+                const double ft1 = top0 ? 0.0 : 0.25 * (u[o + XY] + u[o]) * (w[o] + w[o + 1]);
+                const double ft0 = bot0 ? 0.0 : 0.25 * (u[o] + u[o - XY]) * (w[o - XY] + w[o + 1 - XY]);
+                au = -(Ro * ((fe1 - fe0) / dx + (fn1 - fn0) / dy + (ft1 - ft0) / dz));
+            }
+            {   // meridional momentum, :104-140
+                const double fe1 = 0.25 * (v[o] + v[o + 1]) * (u[o + X] + u[o]);
+                const double fe0 = 0.25 * (v[o - 1] + v[o]) * (u[o - 1 + X] + u[o - 1]);
+                const double fn1 = 0.25 * (v[o] + v[o + X]) * (v[o + X] + v[o]);
+                const double fn0 = 0.25 * (v[o - X] + v[o]) * (v[o] + v[o - X]);
+                const double ft1 = top0 ? 0.0 : 0.25 * (v[o + XY] + v[o]) * (w[o] + w[o + X]);
+                const double ft0 = bot0 ? 0.0 : 0.25 * (v[o] + v[o - XY]) * (w[o - XY] + w[o + X - XY]);
+                av = -(Ro * ((fe1 - fe0) / dx + (fn1 - fn0) / dy + (ft1 - ft0) / dz));
+            }
+            {   // vertical momentum, :145-181
+                const double fe1 = 0.25 * (w[o] + w[o + 1]) * (u[o] + u[o + XY]);
+                const double fe0 = 0.25 * (w[o - 1] + w[o]) * (u[o - 1] + u[o - 1 + XY]);
+                const double fn1 = 0.25 * (w[o] + w[o + X]) * (v[o] + v[o + XY]);
+                const double fn0 = 0.25 * (w[o - X] + w[o]) * (v[o - X] + v[o - X + XY]);
+                const double ft1 = top0w ? 0.0 : 0.25 * (w[o + XY] + w[o]) * (w[o] + w[o + XY]);
+                const double ft0 = (bot0 || (lid && k - 1 >= c.nz - 1))
+                                       ? 0.0
+                                       : 0.25 * (w[o] + w[o - XY]) * (w[o - XY] + w[o]);
+                aw = -((c.dsqr * Ro) * ((fe1 - fe0) / dx + (fn1 - fn0) / dy + (ft1 - ft0) / dz));
+            }
+        }
+        if (DO_INLINE) {
+            // Coriolis, integrate.f90:37-42
+            au = au + (v[o] + v[o - X] + v[o + 1] + v[o + 1 - X]) * 0.25 * c.f0;
+            av = av - (u[o - 1] + u[o] + u[o - 1 + X] + u[o + X]) * 0.25 * c.f0;
+            // buoyancy force, :47-49, and the lid, :51-54
+            aw = aw + 0.5 * (b[o] + b[o + XY]);
+            if (lid && k >= c.nz) aw = 0.0;
+        }
+        du[o] = au;
+        dv[o] = av;
+        dw[o] = aw;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// friction, mixing.f90 -- out of place: interior from `in`, halo copied,
+// so later passes see the stale halo exactly as the in-place reference does
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BX * BY) void k_harmonic(StepConsts c, const double *__restrict__ in,
+                                                       double *__restrict__ out, double Hm, double Vm,
+                                                       int zero_top)
+{
+    const int i = blockIdx.x * BX + threadIdx.x - 1;
+    const int j = blockIdx.y * BY + threadIdx.y - 1;
+    const int k = blockIdx.z - 1;
+    if (i > c.nx + 2 || j > c.ny + 2) return;
+    const int64_t o = idx(c, i, j, k);
+    double r;
+    if (i >= 1 && i <= c.nx && j >= 1 && j <= c.ny && k >= 1 && k <= c.nz) {
+        const int64_t X = c.X, XY = c.XY;
+        const bool lid = c.lid != 0;
+        const double fe1 = Hm * (in[o + 1] - in[o]) / c.dx;
+        const double fe0 = Hm * (in[o] - in[o - 1]) / c.dx;
+        const double fn1 = Hm * (in[o + X] - in[o]) / c.dy;
+        const double fn0 = Hm * (in[o] - in[o - X]) / c.dy;
+        const double ft1 = (lid && k >= c.nz) ? 0.0 : Vm * (in[o + XY] - in[o]) / c.dz;
+        const double ft0 = (lid && k - 1 <= 0) ? 0.0 : Vm * (in[o] - in[o - XY]) / c.dz;
+        r = in[o] + c.dt * ((fe1 - fe0) / c.dx + (fn1 - fn0) / c.dy + (ft1 - ft0) / c.dz);
+    } else {
+        r = in[o];
+    }
+    if (zero_top && k >= c.nz) r = 0.0;
+    out[o] = r;
+}
+
+__global__ __launch_bounds__(BX * BY) void k_biharmonic_h(StepConsts c, const double *__restrict__ in,
+                                                           double *__restrict__ out, double Hm)
+{
+    const int i = blockIdx.x * BX + threadIdx.x - 1;
+    const int j = blockIdx.y * BY + threadIdx.y - 1;
+    const int k = blockIdx.z - 1;
+    if (i > c.nx + 2 || j > c.ny + 2) return;
+    const int64_t o = idx(c, i, j, k);
+    double r;
+    if (i >= 1 && i <= c.nx && j >= 1 && j <= c.ny && k >= 1 && k <= c.nz) {
+        const int64_t X = c.X;
+        const double dx = c.dx, dy = c.dy;
+        auto f1e = [&](int64_t q) { return -(Hm * (in[q + 1] - in[q]) / dx); };
+        auto f1n = [&](int64_t q) { return -(Hm * (in[q + X] - in[q]) / dy); };
+        auto d2 = [&](int64_t q) { return (f1e(q) - f1e(q - 1)) / dx + (f1n(q) - f1n(q - X)) / dy; };
+        const double dc = d2(o), de = d2(o + 1), dwv = d2(o - 1), dn = d2(o + X), ds = d2(o - X);
+        const double fe1 = (de - dc) / dx, fe0 = (dc - dwv) / dx;
+        const double fn1 = (dn - dc) / dy, fn0 = (dc - ds) / dy;
+        r = in[o] + c.dt * ((fe1 - fe0) / dx + (fn1 - fn0) / dy);
+    } else {
+        r = in[o];
+    }
+    out[o] = r;
+}
+
+__global__ __launch_bounds__(BX * BY) void k_biharmonic_v(StepConsts c, const double *__restrict__ in,
+                                                           double *__restrict__ out, double Vm,
+                                                           int zero_top)
+{
+    const int i = blockIdx.x * BX + threadIdx.x - 1;
+    const int j = blockIdx.y * BY + threadIdx.y - 1;
+    const int k = blockIdx.z - 1;
+    if (i > c.nx + 2 || j > c.ny + 2) return;
+    const int64_t o = idx(c, i, j, k);
+    double r;
+    if (i >= 1 && i <= c.nx && j >= 1 && j <= c.ny && k >= 1 && k <= c.nz) {
+        const int64_t XY = c.XY;
+        const bool lid = c.lid != 0;
+        const int nz = c.nz;
+        const double dz = c.dz;
+        auto f1 = [&](int kk, int64_t q) {
+            return (lid && (kk <= 0 || kk >= nz)) ? 0.0 : -(Vm * (in[q + XY] - in[q]) / dz);
+        };
+        auto d2 = [&](int kk, int64_t q) { return (f1(kk, q) - f1(kk - 1, q - XY)) / dz; };
+        const double dm = d2(k - 1, o - XY), dc = d2(k, o), dp = d2(k + 1, o + XY);
+        const double f21 = (lid && k >= nz) ? 0.0 : (dp - dc) / dz;
+        const double f20 = (lid && k - 1 <= 0) ? 0.0 : (dc - dm) / dz;
+        r = in[o] + c.dt * ((f21 - f20) / dz);
+    } else {
+        r = in[o];
+    }
+    if (zero_top && k >= c.nz) r = 0.0;
+    out[o] = r;
+}
+
+// ---------------------------------------------------------------------------
+// Adams-Bashforth, integrate.f90:87-102 (interior; the halo is refilled by
+// the exchange that follows at :106-111)
+// ---------------------------------------------------------------------------
+struct AbArgs {
+    const double *src[4];
+    double *dst[4];
+    const double *t0[4], *t1[4], *t2[4];   // tau, taum1, taum2 slots
+    double scale[4];                        // dt, dt, dt/dsqr, dt
+    double ca, cb, cc;                      // coefficients
+    int order;                              // 1, 2, 3
+};
+
+__global__ __launch_bounds__(BX * BY) void k_ab(StepConsts c, AbArgs a)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = (int)(blockIdx.z % (unsigned)c.nz) + 1;
+    const int f = (int)(blockIdx.z / (unsigned)c.nz);
+    if (i > c.nx || j > c.ny) return;
+    const int64_t o = idx(c, i, j, k);
+    const double s = a.scale[f];
+    double val;
+    if (a.order == 1) {
+        val = a.src[f][o] + s * a.t0[f][o];
+    } else if (a.order == 2) {
+        val = a.src[f][o] + s * (a.ca * a.t0[f][o] - a.cb * a.t1[f][o]);
+    } else {
+        val = a.src[f][o] + s * (a.ca * a.t0[f][o] + a.cb * a.t1[f][o] + a.cc * a.t2[f][o]);
+    }
+    a.dst[f][o] = val;
+}
+
+// ---------------------------------------------------------------------------
+// halo exchange, parallel_mpi.f90:70-211 on one rank.  z, then y, then x
+// periodic copies of whole planes leave every halo point holding the value
+// at its fully wrapped interior index, which is what each thread copies.
+// Only halo points are enumerated: 4 full z-planes, 4 y-rows per interior
+// level, 4 x-points per interior row.  neumann_z: pressure_solver.f90:82-93
+// (copies of levels 1 and nz into the z-halo instead of periodic images).
+// ---------------------------------------------------------------------------
+struct HaloArgs {
+    double *f[5];
+    int neumann[5];
+};
+
+__global__ __launch_bounds__(256) void k_halo(StepConsts c, HaloArgs h, int64_t nhalo)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= nhalo) return;
+    double *a = h.f[blockIdx.y];
+    const int neu = h.neumann[blockIdx.y];
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const int64_t X = c.X, Y = ny + 4;
+    const int64_t nzp = 4 * X * Y, nyr = 4 * X * (int64_t)nz;
+    int i, j, k;
+    if (t < nzp) {
+        const int64_t q = t / (X * Y), r = t % (X * Y);
+        k = q < 2 ? (int)q - 1 : nz + (int)q - 1;   // -1, 0, nz+1, nz+2
+        j = (int)(r / X) - 1;
+        i = (int)(r % X) - 1;
+    } else if (t < nzp + nyr) {
+        const int64_t s = t - nzp;
+        const int64_t q = s / (X * nz), r = s % (X * nz);
+        j = q < 2 ? (int)q - 1 : ny + (int)q - 1;
+        k = (int)(r / X) + 1;
+        i = (int)(r % X) - 1;
+    } else {
+        const int64_t s = t - nzp - nyr;
+        const int64_t q = s % 4, r = s / 4;
+        i = q < 2 ? (int)q - 1 : nx + (int)q - 1;
+        j = (int)(r % ny) + 1;
+        k = (int)(r / ny) + 1;
+    }
+    const int si = wrap(i, nx), sj = wrap(j, ny);
+    const int sk = neu ? (k < 1 ? 1 : (k > nz ? nz : k)) : wrap(k, nz);
+    a[idx(c, i, j, k)] = a[idx(c, si, sj, sk)];
+}
+
+// ---------------------------------------------------------------------------
+// projection, pressure_solver.f90:98-110 (interior).  p is read at wrapped
+// indices (periodic in x, y; periodic or Neumann in z), which is what its
+// exchanged halo holds, so p's own halo can be filled afterwards.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BX * BY) void k_project(StepConsts c, double *__restrict__ u,
+                                                      double *__restrict__ v, double *__restrict__ w,
+                                                      const double *__restrict__ p)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > c.nx || j > c.ny) return;
+    const int64_t o = idx(c, i, j, k);
+    const double pc = p[o];
+    const double pe = p[idx(c, i == c.nx ? 1 : i + 1, j, k)];
+    const double pn = p[idx(c, i, j == c.ny ? 1 : j + 1, k)];
+    const double pt = p[idx(c, i, j, k == c.nz ? (c.lid ? c.nz : 1) : k + 1)];
+    u[o] = u[o] - c.dt * (pe - pc) / c.dx;
+    v[o] = v[o] - c.dt * (pn - pc) / c.dy;
+    if (c.lid && k >= c.nz) w[o] = 0.0;
+    else w[o] = w[o] - (c.dt / c.dsqr) * (pt - pc) / c.dz;
+}
+
+inline dim3 interior_grid(const Grid &g, int zmult = 1)
+{
+    return dim3((unsigned)((g.nx + BX - 1) / BX), (unsigned)((g.ny + BY - 1) / BY), (unsigned)(g.nz * zmult));
+}
+
+inline dim3 full_grid(const Grid &g)
+{
+    return dim3((unsigned)((g.X + BX - 1) / BX), (unsigned)((g.Y + BY - 1) / BY), (unsigned)g.Z);
+}
+
+}  // namespace
+
+double *Model::field_buf(int id) const
+{
+    switch (id) {
+    case PS3D_U: return u;
+    case PS3D_V: return v;
+    case PS3D_W: return w;
+    case PS3D_B: return b;
+    case PS3D_P: return p;
+    default: return nullptr;
+    }
+}
+
+void launch_tendency(Model &m, int tau)
+{
+    ProfScope ps(m, "tendency");
+    const int64_t off = (int64_t)(tau - 1) * m.g.H;
+    hipLaunchKernelGGL((k_tendency<true, true, true>), interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c,
+                       m.u, m.v, m.w, m.b, m.du + off, m.dv + off, m.dw + off, m.db + off);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void launch_buoyancy_advection(Model &m, int tau)
+{
+    const int64_t off = (int64_t)(tau - 1) * m.g.H;
+    hipLaunchKernelGGL((k_tendency<true, false, false>), interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c,
+                       m.u, m.v, m.w, m.b, nullptr, nullptr, nullptr, m.db + off);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void launch_momentum_advection(Model &m, int tau)
+{
+    const int64_t off = (int64_t)(tau - 1) * m.g.H;
+    hipLaunchKernelGGL((k_tendency<false, true, false>), interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c,
+                       m.u, m.v, m.w, m.b, m.du + off, m.dv + off, m.dw + off, nullptr);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void launch_mixing(Model &m, int op, int f, double hmix, double vmix, int zero_top)
+{
+    const double *in = m.cur_buf(f);
+    double *out = m.cur[f] ? m.field_buf(f) : m.alt[f];
+    switch (op) {
+    case PS3D_HARMONIC: {
+        ProfScope ps(m, "harmonic");
+        hipLaunchKernelGGL(k_harmonic, full_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, in, out, hmix, vmix,
+                           zero_top);
+        break;
+    }
+    case PS3D_BIHARMONIC_H: {
+        ProfScope ps(m, "biharmonic_h");
+        hipLaunchKernelGGL(k_biharmonic_h, full_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, in, out, hmix);
+        break;
+    }
+    case PS3D_BIHARMONIC_V: {
+        ProfScope ps(m, "biharmonic_v");
+        hipLaunchKernelGGL(k_biharmonic_v, full_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, in, out, vmix,
+                           zero_top);
+        break;
+    }
+    default: throw Error(PS3D_ERR_ARG, "unknown mixing operator");
+    }
+    PS3D_HIP_CHECK(hipGetLastError());
+    m.cur[f] ^= 1;
+}
+
+// integrate.f90:59-78
+void launch_friction_block(Model &m)
+{
+    const ps3d_params &p = m.prm;
+    const int lid = p.enable_vertical_boundaries;
+    if (p.Ah > 0. || p.Av > 0.) {
+        launch_mixing(m, PS3D_HARMONIC, PS3D_U, p.Ah, p.Av, 0);
+        launch_mixing(m, PS3D_HARMONIC, PS3D_V, p.Ah, p.Av, 0);
+        launch_mixing(m, PS3D_HARMONIC, PS3D_W, p.Ah, p.Av, lid);
+    }
+    if (p.Kh > 0. || p.Kv > 0.) launch_mixing(m, PS3D_HARMONIC, PS3D_B, p.Kh, p.Kv, 0);
+    if (p.Ahbi > 0.) {
+        launch_mixing(m, PS3D_BIHARMONIC_H, PS3D_U, p.Ahbi, 0.0, 0);
+        launch_mixing(m, PS3D_BIHARMONIC_H, PS3D_V, p.Ahbi, 0.0, 0);
+        launch_mixing(m, PS3D_BIHARMONIC_H, PS3D_W, p.Ahbi, 0.0, 0);
+    }
+    if (p.Avbi > 0.) {
+        launch_mixing(m, PS3D_BIHARMONIC_V, PS3D_U, 0.0, p.Avbi, 0);
+        launch_mixing(m, PS3D_BIHARMONIC_V, PS3D_V, 0.0, p.Avbi, 0);
+        launch_mixing(m, PS3D_BIHARMONIC_V, PS3D_W, 0.0, p.Avbi, lid);
+    }
+    if (p.Khbi > 0.) launch_mixing(m, PS3D_BIHARMONIC_H, PS3D_B, p.Khbi, 0.0, 0);
+    if (p.Kvbi > 0.) launch_mixing(m, PS3D_BIHARMONIC_V, PS3D_B, 0.0, p.Kvbi, 0);
+}
+
+// integrate.f90:87-111
+void launch_ab_exchange(Model &m, int itt, int tau, int taum1, int taum2)
+{
+    AbArgs a{};
+    const int64_t H = m.g.H;
+    double *tend[4] = {m.du, m.dv, m.dw, m.db};
+    for (int f = 0; f < 4; ++f) {
+        a.src[f] = m.cur_buf(f);
+        a.dst[f] = m.field_buf(f);
+        a.t0[f] = tend[f] + (int64_t)(tau - 1) * H;
+        a.t1[f] = tend[f] + (int64_t)(taum1 - 1) * H;
+        a.t2[f] = tend[f] + (int64_t)(taum2 - 1) * H;
+        a.scale[f] = m.prm.dt;
+    }
+    a.scale[2] = m.prm.dt / m.prm.dsqr;
+    if (itt == 0) {
+        a.order = 1;
+    } else if (itt == 1 || !m.prm.enable_AB_3_order) {
+        a.order = 2;
+        a.ca = 1.5 + m.prm.eps_ab;
+        a.cb = 0.5 + m.prm.eps_ab;
+    } else {
+        a.order = 3;
+        a.ca = 23.0 / 12.0;   // main_module.f90:57
+        a.cb = -16.0 / 12.0;
+        a.cc = 5.0 / 12.0;
+    }
+    {
+        ProfScope ps(m, "ab_update");
+        hipLaunchKernelGGL(k_ab, interior_grid(m.g, 4), dim3(BX, BY), 0, m.stream, m.c, a);
+        PS3D_HIP_CHECK(hipGetLastError());
+    }
+    for (int f = 0; f < 4; ++f) m.cur[f] = 0;
+    HaloArgs h{};
+    for (int f = 0; f < 4; ++f) { h.f[f] = m.field_buf(f); h.neumann[f] = 0; }
+    const Grid &g = m.g;
+    const int64_t nhalo = g.H - g.N;
+    ProfScope ps(m, "halo");
+    hipLaunchKernelGGL(k_halo, dim3((unsigned)((nhalo + 255) / 256), 4), dim3(256), 0, m.stream, m.c, h, nhalo);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void launch_border_exchg(Model &m, double *a, int neumann_z)
+{
+    HaloArgs h{};
+    h.f[0] = a;
+    h.neumann[0] = neumann_z;
+    const int64_t nhalo = m.g.H - m.g.N;
+    ProfScope ps(m, "halo");
+    hipLaunchKernelGGL(k_halo, dim3((unsigned)((nhalo + 255) / 256), 1), dim3(256), 0, m.stream, m.c, h, nhalo);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+// pressure_solver.f90:75-118 after p's interior is in place
+void launch_project_exchange(Model &m)
+{
+    {
+        ProfScope ps(m, "project");
+        hipLaunchKernelGGL(k_project, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, m.u, m.v, m.w, m.p);
+        PS3D_HIP_CHECK(hipGetLastError());
+    }
+    HaloArgs h{};
+    h.f[0] = m.u; h.f[1] = m.v; h.f[2] = m.w; h.f[3] = m.p;
+    h.neumann[3] = m.prm.enable_vertical_boundaries;
+    const int64_t nhalo = m.g.H - m.g.N;
+    ProfScope ps(m, "halo");
+    hipLaunchKernelGGL(k_halo, dim3((unsigned)((nhalo + 255) / 256), 4), dim3(256), 0, m.stream, m.c, h, nhalo);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace ps3d

@@ -1,0 +1,140 @@
+// tridiag.hip -- rigid-lid vertical solve of pressure_solver.f90:30-54.
+//
+// For every horizontal wavenumber column (kx, ky) with ksqr > 0 the system
+// is tridiagonal with a = c = 1/dz^2/dsqr and b = -ksqr - 2a (-ksqr - a at
+// k = 1 and k = nz); solve_tridiag (pressure_solver.f90:126-162) eliminates
+// it with the Thomas algorithm.  The elimination factors c'(k) and 1/m(k)
+// depend only on ksqr, so they are computed once per model, with the same
+// operations in the same order as the reference, and each step only runs
+// the two sweeps over the complex right-hand side.  The (0,0) column takes
+// the reference's double-cumulative-sum recurrence (:44-50).
+//
+// One thread per column; a wavefront covers 64 consecutive kx, so every
+// level of the sweep is one coalesced 1 KiB access.  FS is updated in place:
+// the forward sweep leaves d'(k) in FS, the backward sweep overwrites it
+// with the solution.
+#include "ps3d_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace ps3d {
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_tri_factor(const double *__restrict__ ksqr, double *__restrict__ cp,
+                                                    double *__restrict__ rm, int64_t ncols, int nz, double fxa)
+{
+    const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (col >= ncols) return;
+    const double ks = ksqr[col];
+    if (!(ks > 0)) return;
+    // b(1) = -ksqr - fxa ; cp(1) = c(1)/b(1)
+    const double b1 = -ks - fxa;
+    double cprev = fxa / b1;
+    cp[col] = cprev;
+    rm[col] = b1;                 // level 1 keeps b(1): d'(1) = d(1)/b(1) is a division
+    for (int k = 2; k <= nz; ++k) {
+        const double bk = (k == nz) ? -ks - fxa : -ks - 2 * fxa;
+        const double mm = bk - cprev * fxa;
+        const double r = 1.0 / mm;
+        const double ck = (k == nz) ? 0.0 : fxa * r;   // c(nz) is never used
+        cp[col + (int64_t)(k - 1) * ncols] = ck;
+        rm[col + (int64_t)(k - 1) * ncols] = r;
+        cprev = ck;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_thomas(double2 *__restrict__ fs, const double *__restrict__ ksqr,
+                                                const double *__restrict__ cp, const double *__restrict__ rm,
+                                                int64_t ncols, int nz, double fxa, double dz2, double dsqr)
+{
+    const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (col >= ncols) return;
+    double2 *d = fs + col;
+    const double ks = ksqr[col];
+    if (ks > 0) {
+        double2 dp = d[0];
+        const double b1 = rm[col];
+        dp = make_double2(dp.x / b1, dp.y / b1);
+        d[0] = dp;
+        for (int k = 1; k < nz; ++k) {
+            const int64_t o = (int64_t)k * ncols;
+            const double2 dk = d[o];
+            const double r = rm[col + o];
+            dp = make_double2((dk.x - dp.x * fxa) * r, (dk.y - dp.y * fxa) * r);
+            d[o] = dp;
+        }
+        double2 x = dp;
+        for (int k = nz - 2; k >= 0; --k) {
+            const int64_t o = (int64_t)k * ncols;
+            const double c = cp[col + o];
+            const double2 dk = d[o];
+            x = make_double2(dk.x - c * x.x, dk.y - c * x.y);
+            d[o] = x;
+        }
+    } else {
+        // zero mode: p(1) = 0, p(2) = f(1) dz^2 dsqr, p(k+1) = 2p(k) - p(k-1) + f(k) dz^2 dsqr
+        double2 bk = d[0];
+        double2 pm1 = make_double2(0.0, 0.0);
+        d[0] = pm1;
+        if (nz < 2) return;
+        double2 nxt = d[ncols];
+        double2 pk = make_double2(bk.x * dz2 * dsqr, bk.y * dz2 * dsqr);
+        d[ncols] = pk;
+        bk = nxt;
+        for (int k = 2; k < nz; ++k) {
+            const int64_t o = (int64_t)k * ncols;
+            nxt = d[o];
+            const double2 pn = make_double2(2 * pk.x - pm1.x + bk.x * dz2 * dsqr,
+                                            2 * pk.y - pm1.y + bk.y * dz2 * dsqr);
+            d[o] = pn;
+            pm1 = pk;
+            pk = pn;
+            bk = nxt;
+        }
+    }
+}
+
+// periodic spectral step without the z transforms (building-block API)
+__global__ __launch_bounds__(256) void k_divide_ksqr(double2 *__restrict__ fs, const double *__restrict__ ksqr,
+                                                     int64_t S)
+{
+    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (n >= S) return;
+    const double ks = ksqr[n];
+    double2 x = fs[n];
+    if (ks > 0.) x = make_double2(-(x.x / ks), -(x.y / ks));
+    else x = make_double2(0.0, 0.0);
+    fs[n] = x;
+}
+
+}  // namespace
+
+void tridiag_factor(Model &m)
+{
+    if (!m.prm.enable_vertical_boundaries) return;
+    const int64_t ncols = (int64_t)m.g.nkx * m.g.ny;
+    const double fxa = 1.0 / (m.prm.dz * m.prm.dz) / m.prm.dsqr;
+    hipLaunchKernelGGL(k_tri_factor, dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, m.stream, m.ksqr,
+                       m.tri_cp, m.tri_rm, ncols, m.g.nz, fxa);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void launch_thomas(Model &m, double2 *data)
+{
+    const int64_t ncols = (int64_t)m.g.nkx * m.g.ny;
+    const double fxa = 1.0 / (m.prm.dz * m.prm.dz) / m.prm.dsqr;
+    ProfScope ps(m, "thomas");
+    hipLaunchKernelGGL(k_thomas, dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, m.stream, data, m.ksqr,
+                       m.tri_cp, m.tri_rm, ncols, m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void launch_divide_ksqr(Model &m, double2 *data)
+{
+    hipLaunchKernelGGL(k_divide_ksqr, dim3((unsigned)((m.g.S + 255) / 256)), dim3(256), 0, m.stream, data,
+                       m.ksqr, m.g.S);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace ps3d

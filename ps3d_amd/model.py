@@ -1,0 +1,199 @@
+"""Host-side mirror of the reference's step interface, over the C-ABI.
+
+``Model`` holds host copies of the ``main_module`` fields in their reference
+layout (Fortran order, halo onx=2 included: numpy index = Fortran index + 1)
+and drives the device model through libps3d.so:
+
+=====================  ===============================================
+Model method           reference routine it stands for
+=====================  ===============================================
+``integrate()``        integrate.f90:2-115 (empty set_forcing)
+``run(n)``             main.f90:139-161 (loop without diagnostics)
+``tendency()``         integrate.f90:12-54
+``friction()``         integrate.f90:59-78
+``ab_update()``        integrate.f90:87-111
+``pressure_solver()``  pressure_solver.f90:2-120
+``buoyancy_advection`` advection.f90:2-48
+``momentum_advection`` advection.f90:53-184
+``harmonic`` ...       mixing.f90:225-274, :96-158, :164-217
+``border_exchg``       parallel_mpi.f90:70-81
+``my_fft/my_ifft``     my_fft.f90:159-205
+``spectral_solve``     pressure_solver.f90:30-71
+=====================  ===============================================
+
+Device state is authoritative between ``push()`` and ``pull()``.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+from .params import CParams, ModelParams
+
+FIELDS = {"u": _lib.PS3D_U, "v": _lib.PS3D_V, "w": _lib.PS3D_W, "b": _lib.PS3D_B,
+          "p": _lib.PS3D_P, "du": _lib.PS3D_DU, "dv": _lib.PS3D_DV, "dw": _lib.PS3D_DW,
+          "db": _lib.PS3D_DB}
+PROGNOSTIC = ("u", "v", "w", "b")
+
+
+def _dp(a: np.ndarray):
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.flags.c_contiguous)
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Model:
+    """One ps3D model instance on the GPU (``cuda:0`` / HIP device 0)."""
+
+    def __init__(self, prm: ModelParams, ksqr: np.ndarray | None = None, stream: int | None = None):
+        self.lib = _lib.load()
+        self.prm = prm
+        self._cprm = CParams.from_params(prm)
+        h = ctypes.c_void_p()
+        kp = None
+        if ksqr is not None:
+            self._ksqr = np.asfortranarray(ksqr, dtype=np.float64)
+            kp = _dp(self._ksqr)
+        check(self.lib.ps3d_create(ctypes.byref(self._cprm), kp,
+                                   ctypes.c_void_p(stream) if stream else None, ctypes.byref(h)))
+        self.h = h
+        shp = prm.shape
+        self.f = {n: np.zeros(shp, order="F") for n in ("u", "v", "w", "b", "p")}
+        for n in ("du", "dv", "dw", "db"):
+            self.f[n] = np.zeros(shp + (3,), order="F")
+        # driver time-step state, main.f90:112-113
+        self.itt, self.tau, self.taum1, self.taum2 = 0, 3, 2, 1
+
+    # ---- lifetime ------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            check(self.lib.ps3d_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __getattr__(self, name):
+        f = self.__dict__.get("f")
+        if f is not None and name in f:
+            return f[name]
+        raise AttributeError(name)
+
+    # ---- host <-> device -----------------------------------------------
+    def push(self, names=None):
+        for n in names or FIELDS:
+            check(self.lib.ps3d_set_field(self.h, FIELDS[n], _dp(self.f[n])))
+
+    def pull(self, names=None):
+        for n in names or FIELDS:
+            check(self.lib.ps3d_get_field(self.h, FIELDS[n], _dp(self.f[n])))
+
+    def get_spectral(self) -> np.ndarray:
+        prm = self.prm
+        buf = np.zeros((2, prm.nx // 2 + 1, prm.ny, prm.nz), order="F")
+        check(self.lib.ps3d_get_field(self.h, _lib.PS3D_FS, _dp(buf)))
+        return buf[0] + 1j * buf[1]
+
+    def get_ksqr(self) -> np.ndarray:
+        n = ctypes.c_int64()
+        check(self.lib.ps3d_field_size(self.h, _lib.PS3D_KSQR, ctypes.byref(n)))
+        buf = np.zeros(n.value)
+        check(self.lib.ps3d_get_field(self.h, _lib.PS3D_KSQR, _dp(buf)))
+        return buf
+
+    def synchronize(self):
+        check(self.lib.ps3d_synchronize(self.h))
+
+    # ---- the step --------------------------------------------------------
+    def rotate(self):
+        """main.f90:156-160"""
+        self.tau = self.tau % 3 + 1
+        self.taum1 = self.taum1 % 3 + 1
+        self.taum2 = self.taum2 % 3 + 1
+        self.itt += 1
+
+    def integrate(self):
+        check(self.lib.ps3d_integrate(self.h, self.itt, self.tau, self.taum1, self.taum2))
+
+    def run(self, nsteps: int, use_graph: bool = False):
+        st = (ctypes.c_int32 * 4)(self.itt, self.tau, self.taum1, self.taum2)
+        check(self.lib.ps3d_run(self.h, nsteps, st, int(use_graph)))
+        self.itt, self.tau, self.taum1, self.taum2 = list(st)
+
+    def tendency(self):
+        check(self.lib.ps3d_tendency(self.h, self.tau))
+
+    def buoyancy_advection(self):
+        check(self.lib.ps3d_buoyancy_advection(self.h, self.tau))
+
+    def momentum_advection(self):
+        check(self.lib.ps3d_momentum_advection(self.h, self.tau))
+
+    def friction(self):
+        check(self.lib.ps3d_friction(self.h))
+
+    def harmonic(self, name, hmix, vmix):
+        check(self.lib.ps3d_mixing(self.h, _lib.PS3D_HARMONIC, FIELDS[name], hmix, vmix))
+
+    def biharmonic_h(self, name, hmix):
+        check(self.lib.ps3d_mixing(self.h, _lib.PS3D_BIHARMONIC_H, FIELDS[name], hmix, 0.0))
+
+    def biharmonic_v(self, name, vmix):
+        check(self.lib.ps3d_mixing(self.h, _lib.PS3D_BIHARMONIC_V, FIELDS[name], 0.0, vmix))
+
+    def ab_update(self):
+        check(self.lib.ps3d_ab_update(self.h, self.itt, self.tau, self.taum1, self.taum2))
+
+    def border_exchg(self, name):
+        check(self.lib.ps3d_border_exchg(self.h, FIELDS[name]))
+
+    def pressure_solver(self):
+        check(self.lib.ps3d_pressure_solver(self.h))
+
+    # ---- building blocks ---------------------------------------------------
+    def my_fft(self, qp: np.ndarray, three_d: bool) -> np.ndarray:
+        prm = self.prm
+        qp = np.asfortranarray(qp, dtype=np.float64)
+        out = np.zeros((2, prm.nx // 2 + 1, prm.ny, prm.nz), order="F")
+        check(self.lib.ps3d_my_fft(self.h, int(three_d), _dp(qp), _dp(out)))
+        return out[0] + 1j * out[1]
+
+    def my_ifft(self, qs: np.ndarray, three_d: bool) -> np.ndarray:
+        prm = self.prm
+        buf = np.zeros((2, prm.nx // 2 + 1, prm.ny, prm.nz), order="F")
+        buf[0], buf[1] = qs.real, qs.imag
+        out = np.zeros((prm.nx, prm.ny, prm.nz), order="F")
+        check(self.lib.ps3d_my_ifft(self.h, int(three_d), _dp(buf), _dp(out)))
+        return out
+
+    def spectral_solve(self, qs: np.ndarray) -> np.ndarray:
+        prm = self.prm
+        buf = np.zeros((2, prm.nx // 2 + 1, prm.ny, prm.nz), order="F")
+        buf[0], buf[1] = qs.real, qs.imag
+        check(self.lib.ps3d_spectral_solve(self.h, _dp(buf)))
+        return buf[0] + 1j * buf[1]
+
+    # ---- measurement ---------------------------------------------------------
+    def profile_kernel(self, name: str | None):
+        check(self.lib.ps3d_profile_kernel(self.h, name.encode() if name else None))
+
+    def profile_read(self):
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        check(self.lib.ps3d_profile_read(self.h, ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+
+def kernel_names() -> list[str]:
+    return _lib.load().ps3d_kernel_names().decode().split(",")
