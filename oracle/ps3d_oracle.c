@@ -10,10 +10,16 @@
  * last bit where the reference is compiled without FMA contraction.  Build
  * with -ffp-contract=off.
  *
- * The FFT is not the reference's (P3DFFT over FFTW, absent here): it is a
- * plain mixed-radix Cooley-Tukey DFT with exact table twiddles, unnormalised,
- * e^{-i} forward, c2r ignoring the imaginary parts of the kx=0 and kx=nx/2
- * modes (the FFTW/P3DFFT conventions used at my_fft.f90:159-205).
+ * The FFT is not the reference's (P3DFFT over FFTW, absent here; FFTW picks
+ * its factorisation at run time, so there is no fixed operation order to
+ * follow).  It is a Stockham autosort mixed-radix DFT: radix-4 passes first,
+ * then 2, 3, 5, with the twiddle table tw[q] = exp(-2 pi i q/n) and the
+ * butterfly formulas written out below; real transforms pack x[2m] + i x[2m+1]
+ * into a half-length complex DFT.  Conventions are those of my_fft.f90:159-205:
+ * unnormalised, e^{-i} forward, c2r ignoring the imaginary parts of the kx=0
+ * and kx=nx/2 modes.  The operation order is the one the GPU kernels use, so
+ * the oracle and the GPU agree bit for bit; the transform itself is checked
+ * against numpy.fft and the DFT definition.
  */
 #include "ps3d_oracle.h"
 
@@ -39,8 +45,9 @@ struct orc_model {
     int64_t len[ORC_NFIELDS];
     double *aloc;             /* nx*ny*nz */
     double *del2;             /* H, the automatic array of mixing.f90 */
-    plan1d px, py, pz;
-    cplx *row;                /* max(nx,ny,nz) */
+    plan1d px, py, pz;        /* lengths nx/2, ny, nz */
+    cplx *wk;                 /* exp(-2 pi i k/nx), k = 0..nx/2 */
+    cplx *row;                /* 2*max(nx,ny,nz) + 2 */
 };
 
 #define ONX 2
@@ -60,17 +67,17 @@ static void plan_init(plan1d *pl, int n)
     pl->n = n;
     pl->nfac = 0;
     int r = n;
-    const int pref[3] = {4, 2, 3};
-    (void)pref;
-    for (int f = 2; r > 1;) {
-        if (r % f == 0) { pl->fac[pl->nfac++] = f; r /= f; }
-        else f = (f == 2) ? 3 : f + 2;
-    }
+    while (r % 4 == 0) { pl->fac[pl->nfac++] = 4; r /= 4; }
+    while (r % 2 == 0) { pl->fac[pl->nfac++] = 2; r /= 2; }
+    while (r % 3 == 0) { pl->fac[pl->nfac++] = 3; r /= 3; }
+    while (r % 5 == 0) { pl->fac[pl->nfac++] = 5; r /= 5; }
+    for (int f = 7; r > 1; f += 2)           /* other primes: plain DFT radices */
+        while (r % f == 0) { pl->fac[pl->nfac++] = f; r /= f; }
     pl->tw = (cplx *)malloc(sizeof(cplx) * (size_t)(n > 0 ? n : 1));
-    for (int t = 0; t < n; ++t) {
-        double ang = 2.0 * M_PI * (double)t / (double)n;
-        pl->tw[t].re = cos(ang);
-        pl->tw[t].im = -sin(ang);
+    for (int q = 0; q < n; ++q) {
+        const double ang = 2.0 * M_PI * (double)q / (double)n;
+        pl->tw[q].re = cos(ang);
+        pl->tw[q].im = -sin(ang);
     }
     pl->scratch = (cplx *)malloc(sizeof(cplx) * (size_t)(2 * n + 2));
 }
@@ -81,47 +88,120 @@ static void plan_free(plan1d *pl)
     free(pl->scratch);
 }
 
-/* Recursive decimation in time.  in: n values at stride s; out: n contiguous.
-   W_n^{q} = tw[(q * (N/n)) mod N] of the top-level table (size N). */
-static void dft_rec(const plan1d *pl, int fi, int n, const cplx *in, int64_t s,
-                    cplx *out, int sign)
+static inline cplx c_mul(cplx a, cplx b)
 {
-    if (n == 1) { out[0] = in[0]; return; }
-    const int N = pl->n;
-    const int p = pl->fac[fi];
-    const int mlen = n / p;
-    for (int r = 0; r < p; ++r)
-        dft_rec(pl, fi + 1, mlen, in + r * s, s * p, out + r * mlen, sign);
-    /* combine: X[k] = sum_r W_n^{r k} Y_r[k mod m] */
-    cplx tmp[64];
-    cplx *acc = (n <= 64) ? tmp : (cplx *)malloc(sizeof(cplx) * (size_t)n);
-    const int stride = N / n;
-    for (int k = 0; k < n; ++k) {
-        const int km = k % mlen;
-        double re = out[km].re, im = out[km].im;
-        for (int r = 1; r < p; ++r) {
-            int q = (int)(((int64_t)r * k) % n) * stride;
-            cplx w = pl->tw[q];
-            if (sign > 0) w.im = -w.im;
-            const cplx y = out[r * mlen + km];
-            re += y.re * w.re - y.im * w.im;
-            im += y.re * w.im + y.im * w.re;
+    cplx r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    return r;
+}
+static inline cplx c_add(cplx a, cplx b) { cplx r = {a.re + b.re, a.im + b.im}; return r; }
+static inline cplx c_sub(cplx a, cplx b) { cplx r = {a.re - b.re, a.im - b.im}; return r; }
+static inline cplx c_scale(cplx a, double s) { cplx r = {a.re * s, a.im * s}; return r; }
+/* i*a for sign = +1, -i*a for sign = -1 */
+static inline cplx c_isign(cplx a, int sign)
+{
+    cplx r;
+    if (sign > 0) { r.re = -a.im; r.im = a.re; }
+    else { r.re = a.im; r.im = -a.re; }
+    return r;
+}
+
+/* R-point DFT in place, v[k] = sum_r v[r] exp(sign 2 pi i r k / R) */
+static void dft_small(cplx *v, int R, int sign, const plan1d *pl)
+{
+    if (R == 2) {
+        const cplx a = v[0], b = v[1];
+        v[0] = c_add(a, b);
+        v[1] = c_sub(a, b);
+    } else if (R == 3) {
+        const double c = -0.5, s = 0.86602540378443864676;
+        const cplx t = c_add(v[1], v[2]);
+        const cplx d = c_sub(v[1], v[2]);
+        const cplx mm = {v[0].re + c * t.re, v[0].im + c * t.im};
+        const cplx q = c_isign(c_scale(d, s), sign);
+        v[0] = c_add(v[0], t);
+        v[1] = c_add(mm, q);
+        v[2] = c_sub(mm, q);
+    } else if (R == 4) {
+        const cplx a = c_add(v[0], v[2]), bq = c_sub(v[0], v[2]);
+        const cplx c = c_add(v[1], v[3]), d = c_isign(c_sub(v[1], v[3]), sign);
+        v[0] = c_add(a, c);
+        v[2] = c_sub(a, c);
+        v[1] = c_add(bq, d);
+        v[3] = c_sub(bq, d);
+    } else if (R == 5) {
+        const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+        const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+        const cplx a14 = c_add(v[1], v[4]), d14 = c_sub(v[1], v[4]);
+        const cplx a23 = c_add(v[2], v[3]), d23 = c_sub(v[2], v[3]);
+        const cplx m1 = {v[0].re + c1 * a14.re + c2 * a23.re, v[0].im + c1 * a14.im + c2 * a23.im};
+        const cplx m2 = {v[0].re + c2 * a14.re + c1 * a23.re, v[0].im + c2 * a14.im + c1 * a23.im};
+        const cplx t1 = {s1 * d14.re + s2 * d23.re, s1 * d14.im + s2 * d23.im};
+        const cplx t2 = {s2 * d14.re - s1 * d23.re, s2 * d14.im - s1 * d23.im};
+        const cplx q1 = c_isign(t1, sign), q2 = c_isign(t2, sign);
+        v[0].re = v[0].re + a14.re + a23.re;
+        v[0].im = v[0].im + a14.im + a23.im;
+        v[1] = c_add(m1, q1);
+        v[4] = c_sub(m1, q1);
+        v[2] = c_add(m2, q2);
+        v[3] = c_sub(m2, q2);
+    } else {
+        /* other prime radices: direct sum with table twiddles */
+        cplx out[64];
+        const int st = pl->n / R;
+        for (int k = 0; k < R; ++k) {
+            cplx acc = v[0];
+            for (int r = 1; r < R; ++r) {
+                cplx w = pl->tw[((r * k) % R) * st];
+                if (sign > 0) w.im = -w.im;
+                acc = c_add(acc, c_mul(v[r], w));
+            }
+            out[k] = acc;
         }
-        acc[k].re = re;
-        acc[k].im = im;
+        for (int k = 0; k < R; ++k) v[k] = out[k];
     }
-    memcpy(out, acc, sizeof(cplx) * (size_t)n);
-    if (acc != tmp) free(acc);
+}
+
+/* Stockham autosort DFT of n contiguous values: pass f with radix R and
+   Ns = product of the earlier radices reads v[r] = a[j + r n/R] for butterfly j,
+   multiplies v[r] by W^{r (j mod Ns) n/(Ns R)}, runs the R-point DFT and
+   writes b[(j/Ns) Ns R + j mod Ns + r Ns]. */
+static void stockham(const plan1d *pl, cplx *a, cplx *b, int sign)
+{
+    const int N = pl->n;
+    int Ns = 1;
+    cplx *src = a, *dst = b;
+    for (int f = 0; f < pl->nfac; ++f) {
+        const int R = pl->fac[f];
+        const int nb = N / R;
+        const int twstride = N / (Ns * R);
+        cplx v[64];
+        for (int j = 0; j < nb; ++j) {
+            for (int r = 0; r < R; ++r) v[r] = src[j + r * nb];
+            const int jm = j % Ns;
+            if (Ns > 1)
+                for (int r = 1; r < R; ++r) {
+                    cplx w = pl->tw[r * jm * twstride];
+                    if (sign > 0) w.im = -w.im;
+                    v[r] = c_mul(v[r], w);
+                }
+            dft_small(v, R, sign, pl);
+            const int base = (j / Ns) * Ns * R + jm;
+            for (int r = 0; r < R; ++r) dst[base + r * Ns] = v[r];
+        }
+        cplx *t = src; src = dst; dst = t;
+        Ns *= R;
+    }
+    if (src != a) memcpy(a, src, sizeof(cplx) * (size_t)N);
 }
 
 static void cfft_strided(const plan1d *pl, cplx *data, int64_t stride, int sign)
 {
     const int n = pl->n;
     cplx *in = pl->scratch;
-    cplx *out = pl->scratch + n;
+    cplx *tmp = pl->scratch + n;
     for (int t = 0; t < n; ++t) in[t] = data[t * stride];
-    dft_rec(pl, 0, n, in, 1, out, sign);
-    for (int t = 0; t < n; ++t) data[t * stride] = out[t];
+    stockham(pl, in, tmp, sign);
+    for (int t = 0; t < n; ++t) data[t * stride] = in[t];
 }
 
 void orc_cfft(int n, double *data, int sign)
@@ -159,12 +239,18 @@ orc_model *orc_create(const orc_params *prm)
         m->f[id] = (double *)calloc((size_t)m->len[id], sizeof(double));
     m->aloc = (double *)calloc((size_t)nx * ny * nz, sizeof(double));
     m->del2 = (double *)calloc((size_t)m->H, sizeof(double));
-    plan_init(&m->px, nx);
+    plan_init(&m->px, nx / 2);
     plan_init(&m->py, ny);
     plan_init(&m->pz, nz);
+    m->wk = (cplx *)malloc(sizeof(cplx) * (size_t)m->nkx);
+    for (int k = 0; k < m->nkx; ++k) {
+        const double ang = 2.0 * M_PI * (double)k / (double)nx;
+        m->wk[k].re = cos(ang);
+        m->wk[k].im = -sin(ang);
+    }
     int nmax = nx > ny ? nx : ny;
     nmax = nmax > nz ? nmax : nz;
-    m->row = (cplx *)malloc(sizeof(cplx) * (size_t)nmax);
+    m->row = (cplx *)malloc(sizeof(cplx) * (size_t)(2 * nmax + 2));
     orc_wavenumbers(m);
     return m;
 }
@@ -178,6 +264,7 @@ void orc_destroy(orc_model *m)
     plan_free(&m->px);
     plan_free(&m->py);
     plan_free(&m->pz);
+    free(m->wk);
     free(m->row);
     free(m);
 }
@@ -609,37 +696,60 @@ void orc_integrate_pre(orc_model *m, int itt, int tau, int taum1, int taum2)
 /* my_fft.f90 transforms                                                     */
 /* ------------------------------------------------------------------------ */
 
+/* real (nx,ny,nz) -> complex (nx/2+1,ny,nz) along x: z[m] = x[2m] + i x[2m+1],
+   Z = DFT_{nx/2}(z), X[k] = E[k] + W^k O[k] with E = (Z[k] + conj Z[n2-k])/2,
+   O = (Z[k] - conj Z[n2-k])/(2i), W = exp(-2 pi i/nx) */
 static void r2c_x(orc_model *m, const double *qp, cplx *qs)
 {
-    /* real (nx,ny,nz) -> complex (nx/2+1,ny,nz), along x */
     const int nx = m->p.nx, ny = m->p.ny, nz = m->p.nz;
+    const int n2 = nx / 2;
     const int64_t nkx = m->nkx;
     cplx *row = m->row;
     for (int k = 0; k < nz; ++k)
         for (int j = 0; j < ny; ++j) {
             const double *src = qp + (int64_t)nx * (j + (int64_t)ny * k);
-            for (int i = 0; i < nx; ++i) { row[i].re = src[i]; row[i].im = 0.0; }
+            for (int i = 0; i < n2; ++i) { row[i].re = src[2 * i]; row[i].im = src[2 * i + 1]; }
             cfft_strided(&m->px, row, 1, -1);
             cplx *dst = qs + nkx * (j + (int64_t)ny * k);
-            for (int i = 0; i < nkx; ++i) dst[i] = row[i];
+            for (int kk = 0; kk < nkx; ++kk) {
+                const cplx zk = row[kk == n2 ? 0 : kk];
+                const cplx zm = row[kk == 0 ? 0 : n2 - kk];
+                const cplx e = {0.5 * (zk.re + zm.re), 0.5 * (zk.im - zm.im)};
+                const cplx bq = {zk.re - zm.re, zk.im + zm.im};
+                const cplx o = {0.5 * bq.im, -0.5 * bq.re};
+                dst[kk] = c_add(e, c_mul(m->wk[kk], o));
+            }
         }
 }
 
+/* inverse of r2c_x (unnormalised), imaginary parts of X[0], X[nx/2] ignored:
+   Z'[k] = (X[k] + conj X[n2-k]) + i conj(W^k) (X[k] - conj X[n2-k]),
+   z = IDFT_{nx/2}(Z'), x[2m] = Re z[m], x[2m+1] = Im z[m] */
 static void c2r_x(orc_model *m, const cplx *qs, double *qp)
 {
     const int nx = m->p.nx, ny = m->p.ny, nz = m->p.nz;
+    const int n2 = nx / 2;
     const int64_t nkx = m->nkx;
     cplx *row = m->row;
+    cplx *raw = m->row + nkx;
     for (int k = 0; k < nz; ++k)
         for (int j = 0; j < ny; ++j) {
             const cplx *src = qs + nkx * (j + (int64_t)ny * k);
-            for (int i = 0; i < nkx; ++i) row[i] = src[i];
-            row[0].im = 0.0;
-            if (nx % 2 == 0) row[nx / 2].im = 0.0;
-            for (int i = 1; i < (nx + 1) / 2; ++i) { row[nx - i].re = src[i].re; row[nx - i].im = -src[i].im; }
+            for (int i = 0; i < nkx; ++i) raw[i] = src[i];
+            raw[0].im = 0.0;
+            raw[n2].im = 0.0;
+            for (int kk = 0; kk < n2; ++kk) {
+                const cplx xk = raw[kk], xm = raw[n2 - kk];
+                const cplx a = {xk.re + xm.re, xk.im - xm.im};
+                const cplx d = {xk.re - xm.re, xk.im + xm.im};
+                const cplx wc = {m->wk[kk].re, -m->wk[kk].im};
+                const cplx q = c_mul(wc, d);
+                row[kk].re = a.re - q.im;
+                row[kk].im = a.im + q.re;
+            }
             cfft_strided(&m->px, row, 1, +1);
             double *dst = qp + (int64_t)nx * (j + (int64_t)ny * k);
-            for (int i = 0; i < nx; ++i) dst[i] = row[i].re;
+            for (int i = 0; i < n2; ++i) { dst[2 * i] = row[i].re; dst[2 * i + 1] = row[i].im; }
         }
 }
 

@@ -233,6 +233,7 @@ int ps3d_create(const ps3d_params *prm, const double *ksqr, void *stream, ps3d_m
             c.nonlinear = p.enable_nonlinear;
             c.dx = p.dx; c.dy = p.dy; c.dz = p.dz; c.dt = p.dt;
             c.Ro = p.Ro; c.dsqr = p.dsqr; c.f0 = p.f0; c.N0 = p.N0;
+            c.rdx = 1.0 / p.dx; c.rdy = 1.0 / p.dy; c.rdz = 1.0 / p.dz; c.rdt = 1.0 / p.dt;
             if (stream) {
                 m->stream = static_cast<hipStream_t>(stream);
             } else {

@@ -22,6 +22,11 @@
 
 #include <cmath>
 
+// the factorisation (radix 4 passes, then 2, 3, 5), the twiddle table and the
+// butterfly formulas fix the operation order; with contraction off the
+// transforms are reproducible bit for bit by a CPU restatement of them
+#pragma clang fp contract(off)
+
 namespace ps3d {
 
 namespace {
@@ -178,11 +183,13 @@ __global__ __launch_bounds__(256) void k_r2c_x(StepConsts c, FftPlan pl, const d
             const int j = (int)(row % c.ny) + 1, k = (int)(row / c.ny) + 1;
             const int64_t X = c.X, XY = c.XY;
             int64_t o = fidx(c, 2 * m + 1, j, k);
-            a0 = (u[o] - u[o - 1]) / c.dx + (v[o] - v[o - X]) / c.dy + (w[o] - w[o - XY]) / c.dz;
+            a0 = div_rn(u[o] - u[o - 1], c.dx, c.rdx) + div_rn(v[o] - v[o - X], c.dy, c.rdy) +
+                 div_rn(w[o] - w[o - XY], c.dz, c.rdz);
             o += 1;
-            a1 = (u[o] - u[o - 1]) / c.dx + (v[o] - v[o - X]) / c.dy + (w[o] - w[o - XY]) / c.dz;
-            a0 = a0 / c.dt;
-            a1 = a1 / c.dt;
+            a1 = div_rn(u[o] - u[o - 1], c.dx, c.rdx) + div_rn(v[o] - v[o - X], c.dy, c.rdy) +
+                 div_rn(w[o] - w[o - XY], c.dz, c.rdz);
+            a0 = div_rn(a0, c.dt, c.rdt);
+            a1 = div_rn(a1, c.dt, c.rdt);
         } else {
             const double2 pr = reinterpret_cast<const double2 *>(src + row * (2 * (int64_t)n2))[m];
             a0 = pr.x;
@@ -211,8 +218,8 @@ __global__ __launch_bounds__(256) void k_r2c_x(StepConsts c, FftPlan pl, const d
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_c2r_x(StepConsts c, FftPlan pl, const double2 *__restrict__ wk,
                                                const double2 *__restrict__ src, double *__restrict__ dst,
-                                               int dst_is_field, double inv_scale, int rows_per_block,
-                                               int64_t nrows)
+                                               int dst_is_field, double inv_scale, double rinv_scale,
+                                               int rows_per_block, int64_t nrows)
 {
     extern __shared__ double2 lds[];
     const int n2 = pl.n;
@@ -244,7 +251,7 @@ __global__ __launch_bounds__(256) void k_c2r_x(StepConsts c, FftPlan pl, const d
         const int rr = t / n2, m = t % n2;
         const int64_t row = row0 + rr;
         const double2 z = lds[rr * cs + m];
-        const double a0 = z.x / inv_scale, a1 = z.y / inv_scale;
+        const double a0 = div_rn(z.x, inv_scale, rinv_scale), a1 = div_rn(z.y, inv_scale, rinv_scale);
         if (dst_is_field) {
             const int j = (int)(row % c.ny) + 1, k = (int)(row / c.ny) + 1;
             const int64_t o = fidx(c, 2 * m + 1, j, k);
@@ -424,7 +431,7 @@ void fft_c2r_x(Model &m, const double2 *src, double *dst, int dst_is_field, doub
     const dim3 grid((unsigned)((nrows + rows - 1) / rows));
     ProfScope ps(m, "fft_x_c2r");
     hipLaunchKernelGGL(k_c2r_x, grid, dim3(256), lds, m.stream, m.c, m.px, m.tw_r2c, src, dst, dst_is_field,
-                       inv_scale, rows, nrows);
+                       inv_scale, 1.0 / inv_scale, rows, nrows);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 

@@ -50,7 +50,20 @@ struct StepConsts {
     int lid;           // enable_vertical_boundaries
     int nonlinear;     // enable_nonlinear
     double dx, dy, dz, dt, Ro, dsqr, f0, N0;
+    double rdx, rdy, rdz, rdt;   // 1/dx ... rounded to nearest, for div_rn
 };
+
+// a / b, correctly rounded, from rb = RN(1/b): q = RN(a*rb) is within one ulp
+// of a/b, the residual a - q*b is exact in one FMA, and one correction step
+// q + residual*rb rounds to RN(a/b) (Markstein's theorem).  Three
+// instructions instead of the ~10 of a general IEEE division, same bits as
+// a / b for all finite normal results (a zero quotient comes out as +0).
+__device__ __forceinline__ double div_rn(double a, double b, double rb)
+{
+    const double q = a * rb;
+    const double e = __builtin_fma(-q, b, a);
+    return __builtin_fma(e, rb, q);
+}
 
 // One mixed-radix FFT length: factors (radix 4, 2, 3, 5) and its table
 // tw[q] = exp(-2 pi i q / n), q = 0..n-1, in device memory.

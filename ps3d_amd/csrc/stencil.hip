@@ -61,7 +61,7 @@ __global__ __launch_bounds__(BX * BY) void k_tendency(StepConsts c, const double
             const double fn0 = 0.5 * (b[o - X] + b[o]) * v[o - X];
             const double ft1 = top0 ? 0.0 : 0.5 * (b[o] + b[o + XY]) * w[o];
             const double ft0 = bot0 ? 0.0 : 0.5 * (b[o - XY] + b[o]) * w[o - XY];
-            adv = -(Ro * ((fe1 - fe0) / dx + (fn1 - fn0) / dy + (ft1 - ft0) / dz));
+            adv = -(Ro * (div_rn(fe1 - fe0, dx, c.rdx) + div_rn(fn1 - fn0, dy, c.rdy) + div_rn(ft1 - ft0, dz, c.rdz)));
         }
         if (DO_INLINE) adv = adv - 0.5 * (w[o] + w[o - XY]) * (c.N0 * c.N0);
         db[o] = adv;
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(BX * BY) void k_tendency(StepConsts c, const double
                 const double fn0 = 0.25 * (u[o - X] + u[o]) * (v[o + 1 - X] + v[o - X]);
                 const double ft1 = top0 ? 0.0 : 0.25 * (u[o + XY] + u[o]) * (w[o] + w[o + 1]);
                 const double ft0 = bot0 ? 0.0 : 0.25 * (u[o] + u[o - XY]) * (w[o - XY] + w[o + 1 - XY]);
-                au = -(Ro * ((fe1 - fe0) / dx + (fn1 - fn0) / dy + (ft1 - ft0) / dz));
+                au = -(Ro * (div_rn(fe1 - fe0, dx, c.rdx) + div_rn(fn1 - fn0, dy, c.rdy) + div_rn(ft1 - ft0, dz, c.rdz)));
             }
             {   // meridional momentum, :104-140
                 const double fe1 = 0.25 * (v[o] + v[o + 1]) * (u[o + X] + u[o]);
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(BX * BY) void k_tendency(StepConsts c, const double
                 const double fn0 = 0.25 * (v[o - X] + v[o]) * (v[o] + v[o - X]);
                 const double ft1 = top0 ? 0.0 : 0.25 * (v[o + XY] + v[o]) * (w[o] + w[o + X]);
                 const double ft0 = bot0 ? 0.0 : 0.25 * (v[o] + v[o - XY]) * (w[o - XY] + w[o + X - XY]);
-                av = -(Ro * ((fe1 - fe0) / dx + (fn1 - fn0) / dy + (ft1 - ft0) / dz));
+                av = -(Ro * (div_rn(fe1 - fe0, dx, c.rdx) + div_rn(fn1 - fn0, dy, c.rdy) + div_rn(ft1 - ft0, dz, c.rdz)));
             }
             {   // vertical momentum, :145-181
                 const double fe1 = 0.25 * (w[o] + w[o + 1]) * (u[o] + u[o + XY]);
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(BX * BY) void k_tendency(StepConsts c, const double
                 const double ft0 = (bot0 || (lid && k - 1 >= c.nz - 1))
                                        ? 0.0
                                        : 0.25 * (w[o] + w[o - XY]) * (w[o - XY] + w[o]);
-                aw = -((c.dsqr * Ro) * ((fe1 - fe0) / dx + (fn1 - fn0) / dy + (ft1 - ft0) / dz));
+                aw = -((c.dsqr * Ro) * (div_rn(fe1 - fe0, dx, c.rdx) + div_rn(fn1 - fn0, dy, c.rdy) + div_rn(ft1 - ft0, dz, c.rdz)));
             }
         }
         if (DO_INLINE) {
@@ -130,13 +130,14 @@ __global__ __launch_bounds__(BX * BY) void k_harmonic(StepConsts c, const double
     if (i >= 1 && i <= c.nx && j >= 1 && j <= c.ny && k >= 1 && k <= c.nz) {
         const int64_t X = c.X, XY = c.XY;
         const bool lid = c.lid != 0;
-        const double fe1 = Hm * (in[o + 1] - in[o]) / c.dx;
-        const double fe0 = Hm * (in[o] - in[o - 1]) / c.dx;
-        const double fn1 = Hm * (in[o + X] - in[o]) / c.dy;
-        const double fn0 = Hm * (in[o] - in[o - X]) / c.dy;
-        const double ft1 = (lid && k >= c.nz) ? 0.0 : Vm * (in[o + XY] - in[o]) / c.dz;
-        const double ft0 = (lid && k - 1 <= 0) ? 0.0 : Vm * (in[o] - in[o - XY]) / c.dz;
-        r = in[o] + c.dt * ((fe1 - fe0) / c.dx + (fn1 - fn0) / c.dy + (ft1 - ft0) / c.dz);
+        const double fe1 = div_rn(Hm * (in[o + 1] - in[o]), c.dx, c.rdx);
+        const double fe0 = div_rn(Hm * (in[o] - in[o - 1]), c.dx, c.rdx);
+        const double fn1 = div_rn(Hm * (in[o + X] - in[o]), c.dy, c.rdy);
+        const double fn0 = div_rn(Hm * (in[o] - in[o - X]), c.dy, c.rdy);
+        const double ft1 = (lid && k >= c.nz) ? 0.0 : div_rn(Vm * (in[o + XY] - in[o]), c.dz, c.rdz);
+        const double ft0 = (lid && k - 1 <= 0) ? 0.0 : div_rn(Vm * (in[o] - in[o - XY]), c.dz, c.rdz);
+        r = in[o] + c.dt * (div_rn(fe1 - fe0, c.dx, c.rdx) + div_rn(fn1 - fn0, c.dy, c.rdy) +
+                            div_rn(ft1 - ft0, c.dz, c.rdz));
     } else {
         r = in[o];
     }
@@ -155,14 +156,16 @@ __global__ __launch_bounds__(BX * BY) void k_biharmonic_h(StepConsts c, const do
     double r;
     if (i >= 1 && i <= c.nx && j >= 1 && j <= c.ny && k >= 1 && k <= c.nz) {
         const int64_t X = c.X;
-        const double dx = c.dx, dy = c.dy;
-        auto f1e = [&](int64_t q) { return -(Hm * (in[q + 1] - in[q]) / dx); };
-        auto f1n = [&](int64_t q) { return -(Hm * (in[q + X] - in[q]) / dy); };
-        auto d2 = [&](int64_t q) { return (f1e(q) - f1e(q - 1)) / dx + (f1n(q) - f1n(q - X)) / dy; };
+        const double dx = c.dx, dy = c.dy, rdx = c.rdx, rdy = c.rdy;
+        auto f1e = [&](int64_t q) { return -div_rn(Hm * (in[q + 1] - in[q]), dx, rdx); };
+        auto f1n = [&](int64_t q) { return -div_rn(Hm * (in[q + X] - in[q]), dy, rdy); };
+        auto d2 = [&](int64_t q) {
+            return div_rn(f1e(q) - f1e(q - 1), dx, rdx) + div_rn(f1n(q) - f1n(q - X), dy, rdy);
+        };
         const double dc = d2(o), de = d2(o + 1), dwv = d2(o - 1), dn = d2(o + X), ds = d2(o - X);
-        const double fe1 = (de - dc) / dx, fe0 = (dc - dwv) / dx;
-        const double fn1 = (dn - dc) / dy, fn0 = (dc - ds) / dy;
-        r = in[o] + c.dt * ((fe1 - fe0) / dx + (fn1 - fn0) / dy);
+        const double fe1 = div_rn(de - dc, dx, rdx), fe0 = div_rn(dc - dwv, dx, rdx);
+        const double fn1 = div_rn(dn - dc, dy, rdy), fn0 = div_rn(dc - ds, dy, rdy);
+        r = in[o] + c.dt * (div_rn(fe1 - fe0, dx, rdx) + div_rn(fn1 - fn0, dy, rdy));
     } else {
         r = in[o];
     }
@@ -183,15 +186,15 @@ __global__ __launch_bounds__(BX * BY) void k_biharmonic_v(StepConsts c, const do
         const int64_t XY = c.XY;
         const bool lid = c.lid != 0;
         const int nz = c.nz;
-        const double dz = c.dz;
+        const double dz = c.dz, rdz = c.rdz;
         auto f1 = [&](int kk, int64_t q) {
-            return (lid && (kk <= 0 || kk >= nz)) ? 0.0 : -(Vm * (in[q + XY] - in[q]) / dz);
+            return (lid && (kk <= 0 || kk >= nz)) ? 0.0 : -div_rn(Vm * (in[q + XY] - in[q]), dz, rdz);
         };
-        auto d2 = [&](int kk, int64_t q) { return (f1(kk, q) - f1(kk - 1, q - XY)) / dz; };
+        auto d2 = [&](int kk, int64_t q) { return div_rn(f1(kk, q) - f1(kk - 1, q - XY), dz, rdz); };
         const double dm = d2(k - 1, o - XY), dc = d2(k, o), dp = d2(k + 1, o + XY);
-        const double f21 = (lid && k >= nz) ? 0.0 : (dp - dc) / dz;
-        const double f20 = (lid && k - 1 <= 0) ? 0.0 : (dc - dm) / dz;
-        r = in[o] + c.dt * ((f21 - f20) / dz);
+        const double f21 = (lid && k >= nz) ? 0.0 : div_rn(dp - dc, dz, rdz);
+        const double f20 = (lid && k - 1 <= 0) ? 0.0 : div_rn(dc - dm, dz, rdz);
+        r = in[o] + c.dt * div_rn(f21 - f20, dz, rdz);
     } else {
         r = in[o];
     }
@@ -296,10 +299,10 @@ __global__ __launch_bounds__(BX * BY) void k_project(StepConsts c, double *__res
     const double pe = p[idx(c, i == c.nx ? 1 : i + 1, j, k)];
     const double pn = p[idx(c, i, j == c.ny ? 1 : j + 1, k)];
     const double pt = p[idx(c, i, j, k == c.nz ? (c.lid ? c.nz : 1) : k + 1)];
-    u[o] = u[o] - c.dt * (pe - pc) / c.dx;
-    v[o] = v[o] - c.dt * (pn - pc) / c.dy;
+    u[o] = u[o] - div_rn(c.dt * (pe - pc), c.dx, c.rdx);
+    v[o] = v[o] - div_rn(c.dt * (pn - pc), c.dy, c.rdy);
     if (c.lid && k >= c.nz) w[o] = 0.0;
-    else w[o] = w[o] - (c.dt / c.dsqr) * (pt - pc) / c.dz;
+    else w[o] = w[o] - div_rn((c.dt / c.dsqr) * (pt - pc), c.dz, c.rdz);
 }
 
 inline dim3 interior_grid(const Grid &g, int zmult = 1)

@@ -1,11 +1,12 @@
 """Parity of the HIP path (through the C-ABI) with the CPU oracle.
 
-Stencil, halo, Adams-Bashforth and projection kernels evaluate the
-reference's expressions in the same order with FMA contraction off, so they
-must agree BIT FOR BIT.  FFT-based results differ from the oracle only by
-the rounding of two different FFT implementations: tolerance 1e-13 relative
-per call, and <= 1e-12 relative L2 per prognostic field after 100 steps (the
-drop-in criterion).
+Every kernel evaluates the same operations in the same order as the oracle
+(stencils: the reference's own expressions; FFT: the same Stockham
+factorisation, twiddles and butterflies), with FMA contraction off and
+divisions correctly rounded, so the GPU must agree with the oracle BIT FOR
+BIT -- per kernel, per transform and over 100 whole steps.  The drop-in
+criterion (<= 1e-12 relative L2 per prognostic field after 100 steps) is
+asserted as well, and the measured error is printed.
 """
 import numpy as np
 import pytest
@@ -123,9 +124,9 @@ def test_my_fft_matches_oracle(case):
     for td in modes:
         qs = gpu.my_fft(x, td)
         ref = orc.my_fft(x, td)
-        assert rel_l2(qs, ref) < 1e-14, td
+        assert np.array_equal(qs, ref), td
         back = gpu.my_ifft(ref, td)
-        assert rel_l2(back, orc.my_ifft(ref, td)) < 1e-14, td
+        assert np.array_equal(back, orc.my_ifft(ref, td)), td
         assert np.abs(back - x).max() < 1e-12, td
     gpu.close()
 
@@ -150,7 +151,7 @@ def test_pressure_solver_matches_oracle(case):
     orc.pressure_solver()
     gpu.pull(["u", "v", "w", "p"])
     for n in ("u", "v", "w", "p"):
-        assert rel_l2(gpu.f[n], orc.f[n]) < 1e-13, n
+        assert np.array_equal(gpu.f[n], orc.f[n]), n
     gpu.close()
 
 
@@ -166,6 +167,7 @@ def test_integrate_100_steps(case):
         e = rel_l2(interior(gpu.f[n], prm), interior(orc.f[n], prm))
         print(f"  {n}: rel L2 = {e:.2e}", flush=True)
         assert e <= 1e-12, n
+        assert np.array_equal(gpu.f[n], orc.f[n]), n
     gpu.close()
 
 
@@ -188,12 +190,15 @@ def test_eddy_100_steps_c1_shape():
         e = rel_l2(interior(gpu.f[n], prm), interior(orc.f[n], prm))
         print(f"eddy64 {n}: rel L2 = {e:.2e}", flush=True)
         assert e <= 1e-12, n
+        assert np.array_equal(gpu.f[n], orc.f[n]), n
     gpu.close()
 
 
 def test_divergence_roundoff_c2_size():
-    """256x256x64 jets: after each step the discrete divergence
-    (pressure_solver.f90:16-23) is at round-off relative to its terms."""
+    """256x256x64 jets: after the projection the discrete divergence
+    (pressure_solver.f90:16-23) is at round-off relative to the terms that
+    cancel in it -- the largest being the pressure-gradient update of w,
+    dt/dsqr * delta_z p / dz, differenced once more in z."""
     from ps3d_amd import configs
     from ps3d_amd.model import Model
     prm, fields = configs.jets(256, 256, 64, seed=2)
@@ -202,14 +207,16 @@ def test_divergence_roundoff_c2_size():
         gpu.f[n][...] = a
     gpu.push()
     gpu.run(5)
-    gpu.pull(["u", "v", "w"])
-    u, v, w = gpu.f["u"], gpu.f["v"], gpu.f["w"]
+    gpu.pull(["u", "v", "w", "p"])
+    u, v, w, p = gpu.f["u"], gpu.f["v"], gpu.f["w"], gpu.f["p"]
     I, J, K = slice(2, 2 + prm.nx), slice(2, 2 + prm.ny), slice(2, 2 + prm.nz)
     du = (u[I, J, K] - u[1:1 + prm.nx, J, K]) / prm.dx
     dv = (v[I, J, K] - v[I, 1:1 + prm.ny, K]) / prm.dy
     dw = (w[I, J, K] - w[I, J, 1:1 + prm.nz]) / prm.dz
     div = du + dv + dw
-    scale = max(np.abs(du).max(), np.abs(dv).max(), np.abs(dw).max())
+    dpz = np.abs(p[I, J, 3:3 + prm.nz] - p[I, J, K]).max()
+    scale = max(np.abs(du).max(), np.abs(dv).max(), np.abs(dw).max(),
+                prm.dt / prm.dsqr * dpz / prm.dz ** 2)
     print(f"max|div| = {np.abs(div).max():.3e}, term scale = {scale:.3e}", flush=True)
     assert np.abs(div).max() <= 1e-12 * scale
     assert np.isfinite(u).all() and np.isfinite(w).all()
