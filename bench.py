@@ -40,6 +40,7 @@ def algorithmic_bytes(name: str, prm, order: int = 3) -> float:
         "biharmonic_h": 2 * e * N,
         "biharmonic_v": 2 * e * N,
         "ab_update": 4 * (2 + order) * e * N,  # per field R u, q slots  W u
+        "friction_ab": 4 * (2 + order) * e * N,  # friction chain fused into the AB pass: same I/O
         "fft_x_r2c": 3 * e * N + 2 * e * S,    # R u,v,w (divergence)  W FS
         "fft_y_fwd": 4 * e * S,                # R + W complex FS
         "fft_y_inv": 4 * e * S,
@@ -92,7 +93,19 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-all", action="store_true", help="print every kernel's share")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (to calibrate HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "dispatches": 3}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} is not "
               "supported (members of an ensemble or z-slabs of one grid would be sharded across GPUs, "

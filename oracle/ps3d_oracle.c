@@ -62,6 +62,18 @@ static inline int64_t IX(const orc_model *m, int i, int j, int k)
 /* 1-D complex DFT                                                           */
 /* ------------------------------------------------------------------------ */
 
+/* exp(-2 pi i q/n).  sincos() is called explicitly: compilers fuse a cos()
+   and sin() of one argument into sincos() or not as they please, and the two
+   can differ in the last bit, so the GPU library calls the same routine. */
+static void twiddle(int q, int n, cplx *w)
+{
+    const double ang = 2.0 * M_PI * (double)q / (double)n;
+    double s, c;
+    sincos(ang, &s, &c);
+    w->re = c;
+    w->im = -s;
+}
+
 static void plan_init(plan1d *pl, int n)
 {
     pl->n = n;
@@ -74,11 +86,7 @@ static void plan_init(plan1d *pl, int n)
     for (int f = 7; r > 1; f += 2)           /* other primes: plain DFT radices */
         while (r % f == 0) { pl->fac[pl->nfac++] = f; r /= f; }
     pl->tw = (cplx *)malloc(sizeof(cplx) * (size_t)(n > 0 ? n : 1));
-    for (int q = 0; q < n; ++q) {
-        const double ang = 2.0 * M_PI * (double)q / (double)n;
-        pl->tw[q].re = cos(ang);
-        pl->tw[q].im = -sin(ang);
-    }
+    for (int q = 0; q < n; ++q) twiddle(q, n, &pl->tw[q]);
     pl->scratch = (cplx *)malloc(sizeof(cplx) * (size_t)(2 * n + 2));
 }
 
@@ -243,11 +251,7 @@ orc_model *orc_create(const orc_params *prm)
     plan_init(&m->py, ny);
     plan_init(&m->pz, nz);
     m->wk = (cplx *)malloc(sizeof(cplx) * (size_t)m->nkx);
-    for (int k = 0; k < m->nkx; ++k) {
-        const double ang = 2.0 * M_PI * (double)k / (double)nx;
-        m->wk[k].re = cos(ang);
-        m->wk[k].im = -sin(ang);
-    }
+    for (int k = 0; k < m->nkx; ++k) twiddle(k, nx, &m->wk[k]);
     int nmax = nx > ny ? nx : ny;
     nmax = nmax > nz ? nmax : nz;
     m->row = (cplx *)malloc(sizeof(cplx) * (size_t)(2 * nmax + 2));

@@ -147,13 +147,17 @@ void pressure(Model &m)
 void integrate(Model &m, int itt, int tau, int taum1, int taum2)
 {
     launch_tendency(m, tau);
-    launch_friction_block(m);
-    launch_ab_exchange(m, itt, tau, taum1, taum2);
+    if (fused_friction_ok(m) && m.fuse_friction) {
+        launch_friction_ab(m, itt, tau, taum1, taum2);
+    } else {
+        launch_friction_block(m);
+        launch_ab_exchange(m, itt, tau, taum1, taum2);
+    }
     pressure(m);
 }
 
 const char *kKernelNames =
-    "tendency,harmonic,biharmonic_h,biharmonic_v,ab_update,halo,fft_x_r2c,fft_y_fwd,thomas,"
+    "tendency,friction_ab,harmonic,biharmonic_h,biharmonic_v,ab_update,halo,fft_x_r2c,fft_y_fwd,thomas,"
     "fft_y_inv,fft_x_c2r,z_solve,fft_z_fwd,fft_z_inv,project";
 
 }  // namespace
@@ -262,10 +266,7 @@ int ps3d_create(const ps3d_params *prm, const double *ksqr, void *stream, ps3d_m
             if (!p.enable_vertical_boundaries) fft_plan_init(*m, m->pz, p.nz);
             {
                 std::vector<double2> wk(g.nkx);
-                for (int k = 0; k < g.nkx; ++k) {
-                    const double ang = 2.0 * M_PI * (double)k / (double)p.nx;
-                    wk[k] = make_double2(std::cos(ang), -std::sin(ang));
-                }
+                for (int k = 0; k < g.nkx; ++k) wk[k] = host_twiddle(k, p.nx);
                 m->tw_r2c = dalloc<double2>(*m, g.nkx);
                 PS3D_HIP_CHECK(hipMemcpyAsync(m->tw_r2c, wk.data(), sizeof(double2) * g.nkx,
                                               hipMemcpyHostToDevice, m->stream));

@@ -337,6 +337,418 @@ __global__ __launch_bounds__(512) void k_z_periodic_solve(FftPlan pl, double2 *_
     }
 }
 
+// ===========================================================================
+// Compile-time-sized transforms.  Same factorisation, twiddles, butterflies
+// and pre/post-processing as the generic kernels above (so the same bits),
+// with the length, batch, strides and thread count known to the compiler:
+// every index of every pass is a constant expression of the thread id.
+// ===========================================================================
+namespace ft {
+
+struct Fac {
+    int n;
+    int f[MAX_FAC];
+};
+
+constexpr Fac factorize(int N)
+{
+    Fac F{0, {}};
+    int r = N;
+    while (r % 4 == 0) { F.f[F.n++] = 4; r /= 4; }
+    while (r % 2 == 0) { F.f[F.n++] = 2; r /= 2; }
+    while (r % 3 == 0) { F.f[F.n++] = 3; r /= 3; }
+    while (r % 5 == 0) { F.f[F.n++] = 5; r /= 5; }
+    return F;
+}
+
+// one Stockham pass; element e of column col at buf[e*ES + col*CS]
+template <int R, int N, int NS, int NCOL, int T, int ES, int CS, bool COLFAST, int SIGN>
+__device__ __forceinline__ void pass(double2 *buf, const double2 *__restrict__ tw)
+{
+    constexpr int NB = N / R;
+    constexpr int TOTAL = NB * NCOL;
+    constexpr int PER = (TOTAL + T - 1) / T;
+    constexpr int TWS = N / (NS * R);
+    double2 v[PER][R];
+#pragma unroll
+    for (int s = 0; s < PER; ++s) {
+        const int t = (int)threadIdx.x + s * T;
+        if (TOTAL % T == 0 || t < TOTAL) {
+            const int col = COLFAST ? t % NCOL : t / NB;
+            const int jb = COLFAST ? t / NCOL : t % NB;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[s][r] = buf[(jb + r * NB) * ES + col * CS];
+            if constexpr (NS > 1) {
+                const int jm = jb % NS;
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    double2 w = tw[r * jm * TWS];
+                    if constexpr (SIGN > 0) w.y = -w.y;
+                    v[s][r] = cmul(v[s][r], w);
+                }
+            }
+            dft<R>(v[s], SIGN);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < PER; ++s) {
+        const int t = (int)threadIdx.x + s * T;
+        if (TOTAL % T == 0 || t < TOTAL) {
+            const int col = COLFAST ? t % NCOL : t / NB;
+            const int jb = COLFAST ? t / NCOL : t % NB;
+            const int jm = jb % NS;
+            const int base = (jb / NS) * NS * R + jm;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[(base + r * NS) * ES + col * CS] = v[s][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int P, int NS, int NCOL, int T, int ES, int CS, bool COLFAST, int SIGN>
+__device__ __forceinline__ void passes(double2 *buf, const double2 *__restrict__ tw)
+{
+    constexpr Fac F = factorize(N);
+    if constexpr (P < F.n) {
+        constexpr int R = F.f[P];
+        pass<R, N, NS, NCOL, T, ES, CS, COLFAST, SIGN>(buf, tw);
+        passes<N, P + 1, NS * R, NCOL, T, ES, CS, COLFAST, SIGN>(buf, tw);
+    }
+}
+
+template <int N, int NCOL, int T, int ES, int CS, bool COLFAST, int SIGN>
+__device__ __forceinline__ void fft(double2 *buf, const double2 *__restrict__ tw)
+{
+    passes<N, 0, 1, NCOL, T, ES, CS, COLFAST, SIGN>(buf, tw);
+}
+
+// x-pass, forward r2c of ROWS rows (see k_r2c_x); the post-processing works
+// on the pair (k, n2-k) in place
+template <int N2, int ROWS, int T, bool FUSE_DIV>
+__global__ __launch_bounds__(T) void r2c_x(StepConsts c, const double2 *__restrict__ tw,
+                                           const double2 *__restrict__ wk, const double *__restrict__ src,
+                                           const double *__restrict__ u, const double *__restrict__ v,
+                                           const double *__restrict__ w, double2 *__restrict__ dst,
+                                           int64_t nrows)
+{
+    constexpr int CS = N2 + 1;
+    constexpr int NKX = N2 + 1;
+    __shared__ double2 lds[ROWS * CS];
+    const int64_t row0 = (int64_t)blockIdx.x * ROWS;
+#pragma unroll
+    for (int s = 0; s < (ROWS * N2 + T - 1) / T; ++s) {
+        const int t = (int)threadIdx.x + s * T;
+        if ((ROWS * N2) % T == 0 || t < ROWS * N2) {
+            const int rr = t / N2, m = t % N2;
+            const int64_t row = row0 + rr;
+            double a0 = 0.0, a1 = 0.0;
+            if (row < nrows) {
+                if (FUSE_DIV) {
+                    const int j = (int)(row % c.ny) + 1, k = (int)(row / c.ny) + 1;
+                    const int64_t X = c.X, XY = c.XY;
+                    int64_t o = fidx(c, 2 * m + 1, j, k);
+                    a0 = div_rn(u[o] - u[o - 1], c.dx, c.rdx) + div_rn(v[o] - v[o - X], c.dy, c.rdy) +
+                         div_rn(w[o] - w[o - XY], c.dz, c.rdz);
+                    o += 1;
+                    a1 = div_rn(u[o] - u[o - 1], c.dx, c.rdx) + div_rn(v[o] - v[o - X], c.dy, c.rdy) +
+                         div_rn(w[o] - w[o - XY], c.dz, c.rdz);
+                    a0 = div_rn(a0, c.dt, c.rdt);
+                    a1 = div_rn(a1, c.dt, c.rdt);
+                } else {
+                    const double2 pr = reinterpret_cast<const double2 *>(src + row * (2 * (int64_t)N2))[m];
+                    a0 = pr.x;
+                    a1 = pr.y;
+                }
+            }
+            lds[rr * CS + m] = make_double2(a0, a1);
+        }
+    }
+    __syncthreads();
+    fft<N2, ROWS, T, 1, CS, false, -1>(lds, tw);
+    constexpr int HALF = N2 / 2 + 1;   // k = 0..N2/2 handles (k, N2-k)
+#pragma unroll
+    for (int s = 0; s < (ROWS * HALF + T - 1) / T; ++s) {
+        const int t = (int)threadIdx.x + s * T;
+        if ((ROWS * HALF) % T == 0 || t < ROWS * HALF) {
+            const int rr = t / HALF, kk = t % HALF;
+            const int64_t row = row0 + rr;
+            if (row < nrows) {
+                const double2 z0 = lds[rr * CS + kk];
+                const double2 z1 = lds[rr * CS + (kk == 0 ? 0 : N2 - kk)];
+                double2 *d = dst + row * NKX;
+                {   // X[kk] from zk = z0, zm = z1
+                    const double2 e = make_double2(0.5 * (z0.x + z1.x), 0.5 * (z0.y - z1.y));
+                    const double2 bq = make_double2(z0.x - z1.x, z0.y + z1.y);
+                    const double2 o = make_double2(0.5 * bq.y, -0.5 * bq.x);
+                    d[kk] = cadd(e, cmul(wk[kk], o));
+                }
+                const int km = (kk == 0) ? N2 : N2 - kk;   // its partner
+                if (km != kk) {                             // X[km] from zk = z1 (or Z[0]), zm = z0
+                    const double2 zk = (kk == 0) ? z0 : z1;
+                    const double2 zm = z0;
+                    const double2 e = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+                    const double2 bq = make_double2(zk.x - zm.x, zk.y + zm.y);
+                    const double2 o = make_double2(0.5 * bq.y, -0.5 * bq.x);
+                    d[km] = cadd(e, cmul(wk[km], o));
+                }
+            }
+        }
+    }
+}
+
+// x-pass, inverse c2r of ROWS rows (see k_c2r_x), pre-processing in place
+template <int N2, int ROWS, int T>
+__global__ __launch_bounds__(T) void c2r_x(StepConsts c, const double2 *__restrict__ tw,
+                                           const double2 *__restrict__ wk, const double2 *__restrict__ src,
+                                           double *__restrict__ dst, int dst_is_field, double inv_scale,
+                                           double rinv_scale, int64_t nrows)
+{
+    constexpr int CS = N2 + 1;
+    constexpr int NKX = N2 + 1;
+    constexpr int HALF = N2 / 2 + 1;
+    __shared__ double2 lds[ROWS * CS];
+#pragma unroll
+    for (int s = 0; s < (ROWS * HALF + T - 1) / T; ++s) {
+        const int t = (int)threadIdx.x + s * T;
+        if ((ROWS * HALF) % T == 0 || t < ROWS * HALF) {
+            const int rr = t / HALF, kk = t % HALF;
+            const int64_t row = (int64_t)blockIdx.x * ROWS + rr;
+            if (row < nrows) {
+                const double2 *sr = src + row * NKX;
+                double2 xk = sr[kk];
+                const int km = N2 - kk;
+                double2 xm = sr[km];
+                if (kk == 0) { xk.y = 0.0; xm.y = 0.0; }   // kx = 0 and kx = nx/2
+                {   // Z'[kk] from xk, xm
+                    const double2 a = make_double2(xk.x + xm.x, xk.y - xm.y);
+                    const double2 dd = make_double2(xk.x - xm.x, xk.y + xm.y);
+                    const double2 wc = make_double2(wk[kk].x, -wk[kk].y);
+                    const double2 q = cmul(wc, dd);
+                    lds[rr * CS + kk] = make_double2(a.x - q.y, a.y + q.x);
+                }
+                if (kk != 0 && km != kk) {   // Z'[km] from xk' = xm, xm' = xk
+                    const double2 a = make_double2(xm.x + xk.x, xm.y - xk.y);
+                    const double2 dd = make_double2(xm.x - xk.x, xm.y + xk.y);
+                    const double2 wc = make_double2(wk[km].x, -wk[km].y);
+                    const double2 q = cmul(wc, dd);
+                    lds[rr * CS + km] = make_double2(a.x - q.y, a.y + q.x);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    fft<N2, ROWS, T, 1, CS, false, +1>(lds, tw);
+#pragma unroll
+    for (int s = 0; s < (ROWS * N2 + T - 1) / T; ++s) {
+        const int t = (int)threadIdx.x + s * T;
+        if ((ROWS * N2) % T == 0 || t < ROWS * N2) {
+            const int rr = t / N2, m = t % N2;
+            const int64_t row = (int64_t)blockIdx.x * ROWS + rr;
+            if (row < nrows) {
+                const double2 z = lds[rr * CS + m];
+                const double a0 = div_rn(z.x, inv_scale, rinv_scale), a1 = div_rn(z.y, inv_scale, rinv_scale);
+                if (dst_is_field) {
+                    const int j = (int)(row % c.ny) + 1, k = (int)(row / c.ny) + 1;
+                    const int64_t o = fidx(c, 2 * m + 1, j, k);
+                    dst[o] = a0;
+                    dst[o + 1] = a1;
+                } else {
+                    reinterpret_cast<double2 *>(dst + row * (2 * (int64_t)N2))[m] = make_double2(a0, a1);
+                }
+            }
+        }
+    }
+}
+
+// strided c2c pass over TX consecutive kx columns (see k_c2c_cols)
+template <int N, int TX, int T, int SIGN>
+__global__ __launch_bounds__(T) void c2c_cols(const double2 *__restrict__ tw, double2 *__restrict__ data,
+                                              int nkx, int64_t estride, int64_t lstride, double s1, double s2,
+                                              double s3, int nscale)
+{
+    __shared__ double2 lds[N * TX];
+    const int kx0 = blockIdx.x * TX;
+    const int ncol = min(TX, nkx - kx0);
+    double2 *base = data + kx0 + (int64_t)blockIdx.y * lstride;
+#pragma unroll
+    for (int s = 0; s < (N * TX + T - 1) / T; ++s) {
+        const int t = (int)threadIdx.x + s * T;
+        if ((N * TX) % T == 0 || t < N * TX) {
+            const int col = t % TX, e = t / TX;
+            lds[e * TX + col] = (col < ncol) ? base[col + e * estride] : make_double2(0.0, 0.0);
+        }
+    }
+    __syncthreads();
+    fft<N, TX, T, TX, 1, true, SIGN>(lds, tw);
+#pragma unroll
+    for (int s = 0; s < (N * TX + T - 1) / T; ++s) {
+        const int t = (int)threadIdx.x + s * T;
+        if ((N * TX) % T == 0 || t < N * TX) {
+            const int col = t % TX, e = t / TX;
+            if (col < ncol) {
+                double2 x = lds[e * TX + col];
+                if (nscale >= 1) x = cscale(x, s1);
+                if (nscale >= 2) x = cscale(x, s2);
+                if (nscale >= 3) x = cscale(x, s3);
+                base[col + e * estride] = x;
+            }
+        }
+    }
+}
+
+// periodic z solve (see k_z_periodic_solve)
+template <int N, int TX, int T>
+__global__ __launch_bounds__(T) void z_periodic_solve(const double2 *__restrict__ tw, double2 *__restrict__ data,
+                                                      const double *__restrict__ ksqr, int nkx, int ny, double dx,
+                                                      double dy, double dz)
+{
+    __shared__ double2 lds[N * TX];
+    const int kx0 = blockIdx.x * TX;
+    const int j = blockIdx.y;
+    const int ncol = min(TX, nkx - kx0);
+    const int64_t estride = (int64_t)nkx * ny;
+    double2 *base = data + kx0 + (int64_t)j * nkx;
+    const double *kb = ksqr + kx0 + (int64_t)j * nkx;
+#pragma unroll
+    for (int s = 0; s < (N * TX + T - 1) / T; ++s) {
+        const int t = (int)threadIdx.x + s * T;
+        if ((N * TX) % T == 0 || t < N * TX) {
+            const int col = t % TX, e = t / TX;
+            lds[e * TX + col] = (col < ncol) ? base[col + e * estride] : make_double2(0.0, 0.0);
+        }
+    }
+    __syncthreads();
+    fft<N, TX, T, TX, 1, true, -1>(lds, tw);
+#pragma unroll
+    for (int s = 0; s < (N * TX + T - 1) / T; ++s) {
+        const int t = (int)threadIdx.x + s * T;
+        if ((N * TX) % T == 0 || t < N * TX) {
+            const int col = t % TX, e = t / TX;
+            if (col < ncol) {
+                double2 x = lds[e * TX + col];
+                x = cscale(cscale(cscale(x, dx), dy), dz);
+                const double ks = kb[col + e * estride];
+                if (ks > 0.0) x = make_double2(-(x.x / ks), -(x.y / ks));
+                else x = make_double2(0.0, 0.0);
+                lds[e * TX + col] = x;
+            }
+        }
+    }
+    __syncthreads();
+    fft<N, TX, T, TX, 1, true, +1>(lds, tw);
+#pragma unroll
+    for (int s = 0; s < (N * TX + T - 1) / T; ++s) {
+        const int t = (int)threadIdx.x + s * T;
+        if ((N * TX) % T == 0 || t < N * TX) {
+            const int col = t % TX, e = t / TX;
+            if (col < ncol) base[col + e * estride] = lds[e * TX + col];
+        }
+    }
+}
+
+// launch configurations: x-pass rows so that ROWS*N2 = 1024 values, y/z
+// passes 8 columns; 256 threads
+constexpr int XROWS(int n2) { return n2 >= 1024 ? 1 : 1024 / n2; }
+constexpr int TCOL = 8;
+constexpr int TTHREADS = 256;
+constexpr int TCOLS(int n) { return n * TCOL / 16 > 256 ? n * TCOL / 16 : 256; }
+
+template <int N2>
+bool launch_r2c(Model &m, const double *src, int fuse_div, double2 *dst, int64_t nrows)
+{
+    constexpr int ROWS = XROWS(N2);
+    const dim3 grid((unsigned)((nrows + ROWS - 1) / ROWS));
+    if (fuse_div)
+        hipLaunchKernelGGL((r2c_x<N2, ROWS, TTHREADS, true>), grid, dim3(TTHREADS), 0, m.stream, m.c, m.px.tw,
+                           m.tw_r2c, nullptr, m.u, m.v, m.w, dst, nrows);
+    else
+        hipLaunchKernelGGL((r2c_x<N2, ROWS, TTHREADS, false>), grid, dim3(TTHREADS), 0, m.stream, m.c, m.px.tw,
+                           m.tw_r2c, src, nullptr, nullptr, nullptr, dst, nrows);
+    return true;
+}
+
+template <int N2>
+bool launch_c2r(Model &m, const double2 *src, double *dst, int dst_is_field, double inv_scale, int64_t nrows)
+{
+    constexpr int ROWS = XROWS(N2);
+    const dim3 grid((unsigned)((nrows + ROWS - 1) / ROWS));
+    hipLaunchKernelGGL((c2r_x<N2, ROWS, TTHREADS>), grid, dim3(TTHREADS), 0, m.stream, m.c, m.px.tw, m.tw_r2c,
+                       src, dst, dst_is_field, inv_scale, 1.0 / inv_scale, nrows);
+    return true;
+}
+
+template <int N>
+bool launch_cols(const FftPlan &pl, hipStream_t st, double2 *data, int nkx, int64_t estride, int64_t lstride,
+                 unsigned ny_or_nz, int sign, double s1, double s2, double s3, int nscale)
+{
+    const dim3 grid((unsigned)((nkx + TCOL - 1) / TCOL), ny_or_nz);
+    constexpr int T = TCOLS(N);
+    if (sign < 0)
+        hipLaunchKernelGGL((c2c_cols<N, TCOL, T, -1>), grid, dim3(T), 0, st, pl.tw, data, nkx, estride, lstride, s1,
+                           s2, s3, nscale);
+    else
+        hipLaunchKernelGGL((c2c_cols<N, TCOL, T, +1>), grid, dim3(T), 0, st, pl.tw, data, nkx, estride, lstride, s1,
+                           s2, s3, nscale);
+    return true;
+}
+
+template <int N>
+bool launch_zsolve(Model &m, double2 *data)
+{
+    const dim3 grid((unsigned)((m.g.nkx + TCOL - 1) / TCOL), (unsigned)m.g.ny);
+    constexpr int T = TCOLS(N);
+    hipLaunchKernelGGL((z_periodic_solve<N, TCOL, T>), grid, dim3(T), 0, m.stream, m.pz.tw, data,
+                       m.ksqr, m.g.nkx, m.g.ny, m.prm.dx, m.prm.dy, m.prm.dz);
+    return true;
+}
+
+// the sizes compiled in; others take the generic kernels
+#define PS3D_FFT_SIZES(X) X(8) X(16) X(32) X(48) X(64) X(96) X(128) X(256) X(512) X(1024)
+
+bool dispatch_r2c(Model &m, const double *src, int fuse_div, double2 *dst, int64_t nrows)
+{
+    switch (m.px.n) {
+#define CASE(n) case n: return launch_r2c<n>(m, src, fuse_div, dst, nrows);
+        PS3D_FFT_SIZES(CASE)
+#undef CASE
+    default: return false;
+    }
+}
+
+bool dispatch_c2r(Model &m, const double2 *src, double *dst, int dst_is_field, double inv_scale, int64_t nrows)
+{
+    switch (m.px.n) {
+#define CASE(n) case n: return launch_c2r<n>(m, src, dst, dst_is_field, inv_scale, nrows);
+        PS3D_FFT_SIZES(CASE)
+#undef CASE
+    default: return false;
+    }
+}
+
+bool dispatch_cols(const FftPlan &pl, hipStream_t st, double2 *data, int nkx, int64_t estride, int64_t lstride,
+                   unsigned ny_or_nz, int sign, double s1, double s2, double s3, int nscale)
+{
+    switch (pl.n) {
+#define CASE(n) case n: return launch_cols<n>(pl, st, data, nkx, estride, lstride, ny_or_nz, sign, s1, s2, s3, nscale);
+        PS3D_FFT_SIZES(CASE)
+#undef CASE
+    default: return false;
+    }
+}
+
+bool dispatch_zsolve(Model &m, double2 *data)
+{
+    switch (m.pz.n) {
+#define CASE(n) case n: return launch_zsolve<n>(m, data);
+        PS3D_FFT_SIZES(CASE)
+#undef CASE
+    default: return false;
+    }
+}
+
+}  // namespace ft
+
 // rows per block of an x-pass so that every radix's butterflies fit in
 // MAXE staged values per thread (256 threads)
 int rows_for(const FftPlan &pl, int threads)
@@ -393,10 +805,7 @@ void fft_plan_init(Model &m, FftPlan &pl, int n)
     PS3D_REQUIRE(r == 1, "FFT length " + std::to_string(n) + " has a prime factor other than 2, 3, 5");
     PS3D_REQUIRE(pl.nf <= MAX_FAC, "FFT length too long");
     std::vector<double2> tw(n);
-    for (int q = 0; q < n; ++q) {
-        const double ang = 2.0 * M_PI * (double)q / (double)n;
-        tw[q] = make_double2(std::cos(ang), -std::sin(ang));
-    }
+    for (int q = 0; q < n; ++q) tw[q] = host_twiddle(q, n);
     double2 *d = nullptr;
     PS3D_HIP_CHECK(hipMalloc(&d, sizeof(double2) * n));
     m.allocations.push_back(d);
@@ -413,6 +822,10 @@ void fft_r2c_x(Model &m, const double *src, int /*src_is_field*/, int fuse_div, 
     const size_t lds = (size_t)rows * (m.px.n + 1) * sizeof(double2);
     const dim3 grid((unsigned)((nrows + rows - 1) / rows));
     ProfScope ps(m, "fft_x_r2c");
+    if (ft::dispatch_r2c(m, src, fuse_div, dst, nrows)) {
+        PS3D_HIP_CHECK(hipGetLastError());
+        return;
+    }
     if (fuse_div)
         hipLaunchKernelGGL(k_r2c_x<true>, grid, dim3(256), lds, m.stream, m.c, m.px, m.tw_r2c, nullptr, m.u,
                            m.v, m.w, dst, rows, nrows);
@@ -430,6 +843,10 @@ void fft_c2r_x(Model &m, const double2 *src, double *dst, int dst_is_field, doub
     const size_t lds = (size_t)2 * rows * (m.px.n + 1) * sizeof(double2);
     const dim3 grid((unsigned)((nrows + rows - 1) / rows));
     ProfScope ps(m, "fft_x_c2r");
+    if (ft::dispatch_c2r(m, src, dst, dst_is_field, inv_scale, nrows)) {
+        PS3D_HIP_CHECK(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(k_c2r_x, grid, dim3(256), lds, m.stream, m.c, m.px, m.tw_r2c, src, dst, dst_is_field,
                        inv_scale, 1.0 / inv_scale, rows, nrows);
     PS3D_HIP_CHECK(hipGetLastError());
@@ -444,6 +861,11 @@ void fft_c2c_y(Model &m, double2 *data, int sign, double scale1, double scale2)
     const int nscale = (scale1 != 0.0) ? 2 : 0;
     allow_lds(k_c2c_cols, lds);
     ProfScope ps(m, sign < 0 ? "fft_y_fwd" : "fft_y_inv");
+    if (ft::dispatch_cols(m.py, m.stream, data, g.nkx, (int64_t)g.nkx, (int64_t)g.nkx * g.ny, (unsigned)g.nz, sign,
+                          scale1, scale2, 0.0, nscale)) {
+        PS3D_HIP_CHECK(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(k_c2c_cols, grid, dim3(T), lds, m.stream, m.py, data, g.nkx, TXC, (int64_t)g.nkx,
                        (int64_t)g.nkx * g.ny, sign, scale1, scale2, 0.0, nscale);
     PS3D_HIP_CHECK(hipGetLastError());
@@ -458,6 +880,11 @@ void fft_c2c_z(Model &m, double2 *data, int sign, double s1, double s2, double s
     const int nscale = (s1 != 0.0) ? 3 : 0;
     allow_lds(k_c2c_cols, lds);
     ProfScope ps(m, sign < 0 ? "fft_z_fwd" : "fft_z_inv");
+    if (ft::dispatch_cols(m.pz, m.stream, data, g.nkx, (int64_t)g.nkx * g.ny, (int64_t)g.nkx, (unsigned)g.ny, sign,
+                          s1, s2, s3, nscale)) {
+        PS3D_HIP_CHECK(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(k_c2c_cols, grid, dim3(T), lds, m.stream, m.pz, data, g.nkx, TXC,
                        (int64_t)g.nkx * g.ny, (int64_t)g.nkx, sign, s1, s2, s3, nscale);
     PS3D_HIP_CHECK(hipGetLastError());
@@ -471,6 +898,10 @@ void fft_z_periodic_solve(Model &m, double2 *data)
     const dim3 grid((unsigned)((g.nkx + TXC - 1) / TXC), (unsigned)g.ny);
     allow_lds(k_z_periodic_solve, lds);
     ProfScope ps(m, "z_solve");
+    if (ft::dispatch_zsolve(m, data)) {
+        PS3D_HIP_CHECK(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(k_z_periodic_solve, grid, dim3(T), lds, m.stream, m.pz, data, m.ksqr, g.nkx, g.ny, TXC,
                        m.prm.dx, m.prm.dy, m.prm.dz);
     PS3D_HIP_CHECK(hipGetLastError());

@@ -53,6 +53,17 @@ struct StepConsts {
     double rdx, rdy, rdz, rdt;   // 1/dx ... rounded to nearest, for div_rn
 };
 
+// exp(-2 pi i q/n) for the FFT tables, computed on the host.  sincos() is
+// called explicitly: a compiler may or may not fuse separate cos() and sin()
+// calls into it, and the two paths can differ in the last bit.
+inline double2 host_twiddle(int q, int n)
+{
+    const double ang = 2.0 * M_PI * (double)q / (double)n;
+    double s, c;
+    ::sincos(ang, &s, &c);
+    return make_double2(c, -s);
+}
+
 // a / b, correctly rounded, from rb = RN(1/b): q = RN(a*rb) is within one ulp
 // of a/b, the residual a - q*b is exact in one FMA, and one correction step
 // q + residual*rb rounds to RN(a/b) (Markstein's theorem).  Three
@@ -100,6 +111,8 @@ struct Model {
 
     // per-field "current buffer" during the friction block (0 = canonical)
     int cur[4] = {0, 0, 0, 0};
+    // use the fused friction + Adams-Bashforth kernel where it applies
+    bool fuse_friction = true;
 
     // profiling of one kernel by name
     std::string prof_name;
@@ -122,6 +135,8 @@ void launch_momentum_advection(Model &m, int tau);
 void launch_friction_block(Model &m);
 void launch_mixing(Model &m, int op, int field, double hmix, double vmix, int zero_top);
 void launch_ab_exchange(Model &m, int itt, int tau, int taum1, int taum2);
+bool fused_friction_ok(const Model &m);
+void launch_friction_ab(Model &m, int itt, int tau, int taum1, int taum2);
 void launch_border_exchg(Model &m, double *a, int neumann_z);
 void launch_project_exchange(Model &m);
 

@@ -38,12 +38,17 @@ __global__ __launch_bounds__(BX * BY) void k_tendency(StepConsts c, const double
                                                        const double *__restrict__ w,
                                                        const double *__restrict__ b,
                                                        double *__restrict__ du, double *__restrict__ dv,
-                                                       double *__restrict__ dw, double *__restrict__ db)
+                                                       double *__restrict__ dw, double *__restrict__ db, int kc)
 {
     const int i = blockIdx.x * BX + threadIdx.x + 1;
     const int j = blockIdx.y * BY + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
     if (i > c.nx || j > c.ny) return;
+    // each block marches up kc levels: planes k-1, k, k+1 are re-read from
+    // this XCD's L2 instead of HBM
+    const int kb = blockIdx.z * kc + 1;
+    const int ke = min(c.nz, kb + kc - 1);
+    for (int k = kb; k <= ke; ++k)
+    {
     const int64_t X = c.X, XY = c.XY;
     const int64_t o = idx(c, i, j, k);
     const bool lid = c.lid != 0;
@@ -110,6 +115,7 @@ __global__ __launch_bounds__(BX * BY) void k_tendency(StepConsts c, const double
         du[o] = au;
         dv[o] = av;
         dw[o] = aw;
+    }
     }
 }
 
@@ -236,6 +242,144 @@ __global__ __launch_bounds__(BX * BY) void k_ab(StepConsts c, AbArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// friction chain and Adams-Bashforth of one field in one pass:
+// integrate.f90:59-70, :77 (harmonic, then biharmonic_h, each over the stale
+// halo that the reference's in-place passes leave) and :87-102.
+// A work-group owns an FTX x FTY column tile and marches up a chunk of
+// levels.  Planes of f0 over the tile plus a 3-point apron rotate through
+// LDS; f1 = harmonic(f0) is formed on the tile plus a 2-point apron (halo
+// points keep f0, as in the reference); each tile point then applies
+// biharmonic_h to f1 and the AB update, writing a second buffer (neighbouring
+// tiles still read f0).
+// ---------------------------------------------------------------------------
+constexpr int FTX = 64, FTY = 8;
+constexpr int R0X = FTX + 6, R0Y = FTY + 6;
+constexpr int R1X = FTX + 4, R1Y = FTY + 4;
+constexpr int FTHREADS = FTX * FTY;
+
+struct FricField {
+    const double *in;
+    double *out;
+    const double *t0, *t1, *t2;   // tendency slots tau, taum1, taum2
+    double Hm, Vm, Hbi;           // harmonic (H, V) and biharmonic_h coefficients
+    double scale;                 // dt, or dt/dsqr for w
+    int harm, bih, zero_top;
+};
+
+struct FricArgs {
+    FricField f[4];
+    double ca, cb, cc;
+    int order;
+    int nchunk, kc, tiles_x, tiles_y, nblocks;
+};
+
+__global__ __launch_bounds__(FTHREADS) void k_fric_ab(StepConsts c, FricArgs a)
+{
+    __shared__ double f0p[3][R0Y][R0X];
+    __shared__ double f1p[R1Y][R1X];
+    __shared__ double d2p[FTY + 2][FTX + 2];   // biharmonic_h's del2 on the tile + 1
+    // blocks are dealt round-robin to the 8 XCDs; renumber so that each XCD
+    // walks a contiguous run of tiles (neighbours share apron rows in its L2)
+    int bid = (int)blockIdx.x;
+    {
+        const int per = a.nblocks / 8;
+        if (bid < per * 8) bid = (bid % 8) * per + bid / 8;
+    }
+    const int ntile = a.tiles_x * a.tiles_y;
+    const int tile = bid % ntile;
+    const int rest = bid / ntile;
+    const int ch = rest % a.nchunk;
+    const int fi = rest / a.nchunk;
+    const FricField &F = a.f[fi];
+    const int i0 = (tile / a.tiles_y) * FTX + 1;
+    const int j0 = (tile % a.tiles_y) * FTY + 1;
+    const int kb = 1 + ch * a.kc;
+    const int ke = min(c.nz, kb + a.kc - 1);
+    const int t = threadIdx.x;
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const bool lid = c.lid != 0;
+    const double dx = c.dx, dy = c.dy, dz = c.dz, rdx = c.rdx, rdy = c.rdy, rdz = c.rdz, dt = c.dt;
+
+    auto slot = [&](int k) { return (k - kb + 1) % 3; };
+    auto load_plane = [&](int k) {
+        const int s = slot(k);
+        for (int q = t; q < R0X * R0Y; q += FTHREADS) {
+            const int rx = q % R0X, ry = q / R0X;
+            const int ii = i0 - 3 + rx, jj = j0 - 3 + ry;
+            double val = 0.0;
+            if (ii >= -1 && ii <= nx + 2 && jj >= -1 && jj <= ny + 2) val = F.in[idx(c, ii, jj, k)];
+            f0p[s][ry][rx] = val;
+        }
+    };
+
+    load_plane(kb - 1);
+    load_plane(kb);
+    for (int k = kb; k <= ke; ++k) {
+        load_plane(k + 1);
+        __syncthreads();
+        const int sm = slot(k - 1), sc = slot(k), sp = slot(k + 1);
+        for (int q = t; q < R1X * R1Y; q += FTHREADS) {
+            const int rx = q % R1X, ry = q / R1X;
+            const int ii = i0 - 2 + rx, jj = j0 - 2 + ry;
+            const int ox = rx + 1, oy = ry + 1;
+            const double v0 = f0p[sc][oy][ox];
+            double f1 = v0;
+            if (F.harm && ii >= 1 && ii <= nx && jj >= 1 && jj <= ny) {
+                const double Hm = F.Hm, Vm = F.Vm;
+                const double fe1 = div_rn(Hm * (f0p[sc][oy][ox + 1] - v0), dx, rdx);
+                const double fe0 = div_rn(Hm * (v0 - f0p[sc][oy][ox - 1]), dx, rdx);
+                const double fn1 = div_rn(Hm * (f0p[sc][oy + 1][ox] - v0), dy, rdy);
+                const double fn0 = div_rn(Hm * (v0 - f0p[sc][oy - 1][ox]), dy, rdy);
+                const double ft1 = (lid && k >= nz) ? 0.0 : div_rn(Vm * (f0p[sp][oy][ox] - v0), dz, rdz);
+                const double ft0 = (lid && k - 1 <= 0) ? 0.0 : div_rn(Vm * (v0 - f0p[sm][oy][ox]), dz, rdz);
+                f1 = v0 + dt * (div_rn(fe1 - fe0, dx, rdx) + div_rn(fn1 - fn0, dy, rdy) +
+                                div_rn(ft1 - ft0, dz, rdz));
+            }
+            if (F.zero_top && k >= nz) f1 = 0.0;
+            f1p[ry][rx] = f1;
+        }
+        __syncthreads();
+        if (F.bih) {
+            // del2 = (f1e(i) - f1e(i-1))/dx + (f1n(j) - f1n(j-1))/dy, each point once
+            const double Hb = F.Hbi;
+            for (int q = t; q < (FTX + 2) * (FTY + 2); q += FTHREADS) {
+                const int xx = q % (FTX + 2) + 1, yy = q / (FTX + 2) + 1;   // f1p coordinates
+                const double f1e1 = -div_rn(Hb * (f1p[yy][xx + 1] - f1p[yy][xx]), dx, rdx);
+                const double f1e0 = -div_rn(Hb * (f1p[yy][xx] - f1p[yy][xx - 1]), dx, rdx);
+                const double f1n1 = -div_rn(Hb * (f1p[yy + 1][xx] - f1p[yy][xx]), dy, rdy);
+                const double f1n0 = -div_rn(Hb * (f1p[yy][xx] - f1p[yy - 1][xx]), dy, rdy);
+                d2p[yy - 1][xx - 1] = div_rn(f1e1 - f1e0, dx, rdx) + div_rn(f1n1 - f1n0, dy, rdy);
+            }
+            __syncthreads();
+        }
+        {
+            const int px = t % FTX, py = t / FTX;
+            const int i = i0 + px, j = j0 + py;
+            if (i <= nx && j <= ny) {
+                const int cx = px + 2, cy = py + 2;
+                double f2 = f1p[cy][cx];
+                if (F.bih) {
+                    const int ex = px + 1, ey = py + 1;   // d2p coordinates
+                    const double dc = d2p[ey][ex], de = d2p[ey][ex + 1], dwv = d2p[ey][ex - 1];
+                    const double dn = d2p[ey + 1][ex], ds = d2p[ey - 1][ex];
+                    const double fe1 = div_rn(de - dc, dx, rdx), fe0 = div_rn(dc - dwv, dx, rdx);
+                    const double fn1 = div_rn(dn - dc, dy, rdy), fn0 = div_rn(dc - ds, dy, rdy);
+                    f2 = f2 + dt * (div_rn(fe1 - fe0, dx, rdx) + div_rn(fn1 - fn0, dy, rdy));
+                }
+                const int64_t o = idx(c, i, j, k);
+                const double s = F.scale;
+                double val;
+                if (a.order == 1) val = f2 + s * F.t0[o];
+                else if (a.order == 2) val = f2 + s * (a.ca * F.t0[o] - a.cb * F.t1[o]);
+                else val = f2 + s * (a.ca * F.t0[o] + a.cb * F.t1[o] + a.cc * F.t2[o]);
+                F.out[o] = val;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // halo exchange, parallel_mpi.f90:70-211 on one rank.  z, then y, then x
 // periodic copies of whole planes leave every halo point holding the value
 // at its fully wrapped interior index, which is what each thread copies.
@@ -329,28 +473,47 @@ double *Model::field_buf(int id) const
     }
 }
 
+// grid of column tiles x level chunks for the k-marching stencils: enough
+// blocks to fill the device (~2048), chunks of at least 4 levels
+inline dim3 march_grid(const Grid &g, int &kc)
+{
+    const int tiles = ((g.nx + BX - 1) / BX) * ((g.ny + BY - 1) / BY);
+    int nch = (2048 + tiles - 1) / tiles;
+    nch = nch < 1 ? 1 : nch;
+    kc = (g.nz + nch - 1) / nch;
+    kc = kc < 4 ? (g.nz < 4 ? g.nz : 4) : kc;
+    nch = (g.nz + kc - 1) / kc;
+    return dim3((unsigned)((g.nx + BX - 1) / BX), (unsigned)((g.ny + BY - 1) / BY), (unsigned)nch);
+}
+
 void launch_tendency(Model &m, int tau)
 {
     ProfScope ps(m, "tendency");
     const int64_t off = (int64_t)(tau - 1) * m.g.H;
-    hipLaunchKernelGGL((k_tendency<true, true, true>), interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c,
-                       m.u, m.v, m.w, m.b, m.du + off, m.dv + off, m.dw + off, m.db + off);
+    int kc = 1;
+    const dim3 grid = march_grid(m.g, kc);
+    hipLaunchKernelGGL((k_tendency<true, true, true>), grid, dim3(BX, BY), 0, m.stream, m.c, m.u, m.v, m.w, m.b,
+                       m.du + off, m.dv + off, m.dw + off, m.db + off, kc);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
 void launch_buoyancy_advection(Model &m, int tau)
 {
     const int64_t off = (int64_t)(tau - 1) * m.g.H;
-    hipLaunchKernelGGL((k_tendency<true, false, false>), interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c,
-                       m.u, m.v, m.w, m.b, nullptr, nullptr, nullptr, m.db + off);
+    int kc = 1;
+    const dim3 grid = march_grid(m.g, kc);
+    hipLaunchKernelGGL((k_tendency<true, false, false>), grid, dim3(BX, BY), 0, m.stream, m.c, m.u, m.v, m.w,
+                       m.b, nullptr, nullptr, nullptr, m.db + off, kc);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
 void launch_momentum_advection(Model &m, int tau)
 {
     const int64_t off = (int64_t)(tau - 1) * m.g.H;
-    hipLaunchKernelGGL((k_tendency<false, true, false>), interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c,
-                       m.u, m.v, m.w, m.b, m.du + off, m.dv + off, m.dw + off, nullptr);
+    int kc = 1;
+    const dim3 grid = march_grid(m.g, kc);
+    hipLaunchKernelGGL((k_tendency<false, true, false>), grid, dim3(BX, BY), 0, m.stream, m.c, m.u, m.v, m.w,
+                       m.b, m.du + off, m.dv + off, m.dw + off, nullptr, kc);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
@@ -443,6 +606,81 @@ void launch_ab_exchange(Model &m, int itt, int tau, int taum1, int taum2)
     HaloArgs h{};
     for (int f = 0; f < 4; ++f) { h.f[f] = m.field_buf(f); h.neumann[f] = 0; }
     const Grid &g = m.g;
+    const int64_t nhalo = g.H - g.N;
+    ProfScope ps(m, "halo");
+    hipLaunchKernelGGL(k_halo, dim3((unsigned)((nhalo + 255) / 256), 4), dim3(256), 0, m.stream, m.c, h, nhalo);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+bool fused_friction_ok(const Model &m)
+{
+    return !(m.prm.Avbi > 0.) && !(m.prm.Kvbi > 0.);
+}
+
+// integrate.f90:59-111 for configs without vertical biharmonic friction:
+// the friction chain and AB of all four fields in one launch, then the halo
+// exchange.  The new fields are written to the second buffers, which then
+// become the canonical ones.
+void launch_friction_ab(Model &m, int itt, int tau, int taum1, int taum2)
+{
+    const ps3d_params &p = m.prm;
+    const Grid &g = m.g;
+    FricArgs a{};
+    double *tend[4] = {m.du, m.dv, m.dw, m.db};
+    for (int f = 0; f < 4; ++f) {
+        FricField &F = a.f[f];
+        F.in = m.field_buf(f);
+        F.out = m.alt[f];
+        F.t0 = tend[f] + (int64_t)(tau - 1) * g.H;
+        F.t1 = tend[f] + (int64_t)(taum1 - 1) * g.H;
+        F.t2 = tend[f] + (int64_t)(taum2 - 1) * g.H;
+        F.scale = (f == 2) ? p.dt / p.dsqr : p.dt;
+        if (f < 3) {
+            F.harm = (p.Ah > 0. || p.Av > 0.);
+            F.Hm = p.Ah;
+            F.Vm = p.Av;
+            F.bih = p.Ahbi > 0.;
+            F.Hbi = p.Ahbi;
+        } else {
+            F.harm = (p.Kh > 0. || p.Kv > 0.);
+            F.Hm = p.Kh;
+            F.Vm = p.Kv;
+            F.bih = p.Khbi > 0.;
+            F.Hbi = p.Khbi;
+        }
+        F.zero_top = (f == 2 && F.harm && p.enable_vertical_boundaries) ? 1 : 0;
+    }
+    if (itt == 0) {
+        a.order = 1;
+    } else if (itt == 1 || !p.enable_AB_3_order) {
+        a.order = 2;
+        a.ca = 1.5 + p.eps_ab;
+        a.cb = 0.5 + p.eps_ab;
+    } else {
+        a.order = 3;
+        a.ca = 23.0 / 12.0;
+        a.cb = -16.0 / 12.0;
+        a.cc = 5.0 / 12.0;
+    }
+    a.tiles_x = (g.nx + FTX - 1) / FTX;
+    a.tiles_y = (g.ny + FTY - 1) / FTY;
+    const int tiles = a.tiles_x * a.tiles_y;
+    int nchunk = (2048 + tiles * 4 - 1) / (tiles * 4);
+    nchunk = nchunk < 1 ? 1 : (nchunk > g.nz ? g.nz : nchunk);
+    a.kc = (g.nz + nchunk - 1) / nchunk;
+    a.nchunk = (g.nz + a.kc - 1) / a.kc;
+    a.nblocks = tiles * a.nchunk * 4;
+    {
+        ProfScope ps(m, "friction_ab");
+        hipLaunchKernelGGL(k_fric_ab, dim3((unsigned)a.nblocks), dim3(FTHREADS), 0, m.stream, m.c, a);
+        PS3D_HIP_CHECK(hipGetLastError());
+    }
+    std::swap(m.u, m.alt[0]);
+    std::swap(m.v, m.alt[1]);
+    std::swap(m.w, m.alt[2]);
+    std::swap(m.b, m.alt[3]);
+    HaloArgs h{};
+    for (int f = 0; f < 4; ++f) { h.f[f] = m.field_buf(f); h.neumann[f] = 0; }
     const int64_t nhalo = g.H - g.N;
     ProfScope ps(m, "halo");
     hipLaunchKernelGGL(k_halo, dim3((unsigned)((nhalo + 255) / 256), 4), dim3(256), 0, m.stream, m.c, h, nhalo);

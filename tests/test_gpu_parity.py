@@ -18,6 +18,13 @@ pytestmark = pytest.mark.gpu
 CASES = {
     "lid": dict(nx=32, ny=24, nz=16, lid=True),
     "periodic": dict(nx=24, ny=20, nz=10, lid=False),
+    # no vertical biharmonic friction: integrate takes the fused
+    # friction + Adams-Bashforth kernel
+    "lid_fused": dict(nx=96, ny=40, nz=12, lid=True, Avbi=0.0, Kvbi=0.0),
+    "periodic_fused": dict(nx=30, ny=18, nz=20, lid=False, Avbi=0.0, Kvbi=0.0),
+    # every FFT length compiled in (x: 16, y: 32, z: 16); the cases above mix
+    # compiled-in and generic lengths
+    "periodic_pow2": dict(nx=32, ny=32, nz=16, lid=False, Avbi=0.0, Kvbi=0.0),
 }
 
 
@@ -25,8 +32,10 @@ def _pair(case, seed=5, **kw):
     import oracle
     from ps3d_amd import configs
     from ps3d_amd.model import Model
-    c = CASES[case]
-    prm, fields = configs.random_state(c["nx"], c["ny"], c["nz"], seed=seed, lid=c["lid"], **kw)
+    c = dict(CASES[case])
+    nx, ny, nz, lid = c.pop("nx"), c.pop("ny"), c.pop("nz"), c.pop("lid")
+    c.update(kw)
+    prm, fields = configs.random_state(nx, ny, nz, seed=seed, lid=lid, **c)
     gpu = Model(prm)
     orc = oracle.OracleModel(prm)
     rng = np.random.default_rng(seed)
