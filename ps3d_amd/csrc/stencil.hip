@@ -120,6 +120,156 @@ __global__ __launch_bounds__(BX * BY) void k_tendency(StepConsts c, const double
 }
 
 // ---------------------------------------------------------------------------
+// the same tendencies, LDS-tiled: a work-group owns a TTX x TTY column tile
+// and marches up a chunk of levels with planes k-1, k, k+1 of u, v, w, b
+// (tile plus a one-point apron) in an LDS ring; the next plane is fetched
+// into registers while the current level is computed, so every value is
+// read from memory once per work-group.
+// ---------------------------------------------------------------------------
+constexpr int TTX = 64, TTY = 8;
+constexpr int TPX = TTX + 2, TPY = TTY + 2;
+constexpr int TTHR = TTX * TTY;
+constexpr int TPER = (TPX * TPY + TTHR - 1) / TTHR;   // plane values per thread
+
+struct TendArgs {
+    const double *u, *v, *w, *b;
+    double *du, *dv, *dw, *db;
+    int kc, nchunk, tiles_x, tiles_y, nblocks;
+};
+
+__global__ __launch_bounds__(TTHR) void k_tendency_lds(StepConsts c, TendArgs a)
+{
+    __shared__ double sp[4][3][TPY][TPX];
+    int bid = (int)blockIdx.x;
+    {
+        const int per = a.nblocks / 8;   // XCD-contiguous runs of (chunk, tile)
+        if (bid < per * 8) bid = (bid % 8) * per + bid / 8;
+    }
+    const int ch = bid % a.nchunk;
+    const int tile = bid / a.nchunk;
+    const int i0 = (tile / a.tiles_y) * TTX + 1;
+    const int j0 = (tile % a.tiles_y) * TTY + 1;
+    const int kb = 1 + ch * a.kc;
+    const int ke = min(c.nz, kb + a.kc - 1);
+    const int t = threadIdx.x;
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const double *fld[4] = {a.u, a.v, a.w, a.b};
+
+    double pre[4][TPER];
+    auto fetch = [&](int k) {
+#pragma unroll
+        for (int s = 0; s < TPER; ++s) {
+            const int q = t + s * TTHR;
+            const int rx = q % TPX, ry = q / TPX;
+            const int ii = i0 - 1 + rx, jj = j0 - 1 + ry;
+            const bool ok = q < TPX * TPY && ii <= nx + 2 && jj <= ny + 2;
+            const int64_t o = idx(c, ii, jj, k);
+#pragma unroll
+            for (int f = 0; f < 4; ++f) pre[f][s] = ok ? fld[f][o] : 0.0;
+        }
+    };
+    auto store = [&](int k) {
+        const int sl = (k - kb + 1) % 3;
+#pragma unroll
+        for (int s = 0; s < TPER; ++s) {
+            const int q = t + s * TTHR;
+            if (q < TPX * TPY) {
+                const int rx = q % TPX, ry = q / TPX;
+#pragma unroll
+                for (int f = 0; f < 4; ++f) sp[f][sl][ry][rx] = pre[f][s];
+            }
+        }
+    };
+
+    fetch(kb - 1);
+    store(kb - 1);
+    fetch(kb);
+    store(kb);
+    fetch(kb + 1);
+    const int px = t % TTX, py = t / TTX;
+    const int i = i0 + px, j = j0 + py;
+    const int rx = px + 1, ry = py + 1;
+    const bool lid = c.lid != 0;
+    const double Ro = c.Ro, dx = c.dx, dy = c.dy, dz = c.dz;
+    for (int k = kb; k <= ke; ++k) {
+        store(k + 1);
+        if (k + 2 <= nz + 2 && k + 1 <= ke) fetch(k + 2);
+        __syncthreads();
+        const int sm = (k - kb) % 3, sc = (k - kb + 1) % 3, sx = (k - kb + 2) % 3;
+        if (i <= nx && j <= ny) {
+            // value of field f at (i+di, j+dj, k+dk)
+            auto F = [&](int f, int dk, int dj, int di) -> double {
+                const int sl = dk < 0 ? sm : (dk > 0 ? sx : sc);
+                return sp[f][sl][ry + dj][rx + di];
+            };
+            auto U = [&](int dk, int dj, int di) { return F(0, dk, dj, di); };
+            auto V = [&](int dk, int dj, int di) { return F(1, dk, dj, di); };
+            auto W = [&](int dk, int dj, int di) { return F(2, dk, dj, di); };
+            auto B = [&](int dk, int dj, int di) { return F(3, dk, dj, di); };
+            const bool top0 = lid && (k >= nz);
+            const bool bot0 = lid && (k - 1 <= 0);
+            const bool top0w = lid && (k >= nz - 1);
+            const int64_t o = idx(c, i, j, k);
+            double adv = 0.0, au = 0.0, av = 0.0, aw = 0.0;
+            if (c.nonlinear) {
+                {
+                    const double fe1 = 0.5 * (B(0, 0, 0) + B(0, 0, 1)) * U(0, 0, 0);
+                    const double fe0 = 0.5 * (B(0, 0, -1) + B(0, 0, 0)) * U(0, 0, -1);
+                    const double fn1 = 0.5 * (B(0, 0, 0) + B(0, 1, 0)) * V(0, 0, 0);
+                    const double fn0 = 0.5 * (B(0, -1, 0) + B(0, 0, 0)) * V(0, -1, 0);
+                    const double ft1 = top0 ? 0.0 : 0.5 * (B(0, 0, 0) + B(1, 0, 0)) * W(0, 0, 0);
+                    const double ft0 = bot0 ? 0.0 : 0.5 * (B(-1, 0, 0) + B(0, 0, 0)) * W(-1, 0, 0);
+                    adv = -(Ro * (div_rn(fe1 - fe0, dx, c.rdx) + div_rn(fn1 - fn0, dy, c.rdy) +
+                                  div_rn(ft1 - ft0, dz, c.rdz)));
+                }
+                {
+                    const double fe1 = 0.25 * (U(0, 0, 0) + U(0, 0, 1)) * (U(0, 0, 1) + U(0, 0, 0));
+                    const double fe0 = 0.25 * (U(0, 0, -1) + U(0, 0, 0)) * (U(0, 0, 0) + U(0, 0, -1));
+                    const double fn1 = 0.25 * (U(0, 0, 0) + U(0, 1, 0)) * (V(0, 0, 1) + V(0, 0, 0));
+                    const double fn0 = 0.25 * (U(0, -1, 0) + U(0, 0, 0)) * (V(0, -1, 1) + V(0, -1, 0));
+                    const double ft1 = top0 ? 0.0 : 0.25 * (U(1, 0, 0) + U(0, 0, 0)) * (W(0, 0, 0) + W(0, 0, 1));
+                    const double ft0 = bot0 ? 0.0 : 0.25 * (U(0, 0, 0) + U(-1, 0, 0)) * (W(-1, 0, 0) + W(-1, 0, 1));
+                    au = -(Ro * (div_rn(fe1 - fe0, dx, c.rdx) + div_rn(fn1 - fn0, dy, c.rdy) +
+                                 div_rn(ft1 - ft0, dz, c.rdz)));
+                }
+                {
+                    const double fe1 = 0.25 * (V(0, 0, 0) + V(0, 0, 1)) * (U(0, 1, 0) + U(0, 0, 0));
+                    const double fe0 = 0.25 * (V(0, 0, -1) + V(0, 0, 0)) * (U(0, 1, -1) + U(0, 0, -1));
+                    const double fn1 = 0.25 * (V(0, 0, 0) + V(0, 1, 0)) * (V(0, 1, 0) + V(0, 0, 0));
+                    const double fn0 = 0.25 * (V(0, -1, 0) + V(0, 0, 0)) * (V(0, 0, 0) + V(0, -1, 0));
+                    const double ft1 = top0 ? 0.0 : 0.25 * (V(1, 0, 0) + V(0, 0, 0)) * (W(0, 0, 0) + W(0, 1, 0));
+                    const double ft0 = bot0 ? 0.0 : 0.25 * (V(0, 0, 0) + V(-1, 0, 0)) * (W(-1, 0, 0) + W(-1, 1, 0));
+                    av = -(Ro * (div_rn(fe1 - fe0, dx, c.rdx) + div_rn(fn1 - fn0, dy, c.rdy) +
+                                 div_rn(ft1 - ft0, dz, c.rdz)));
+                }
+                {
+                    const double fe1 = 0.25 * (W(0, 0, 0) + W(0, 0, 1)) * (U(0, 0, 0) + U(1, 0, 0));
+                    const double fe0 = 0.25 * (W(0, 0, -1) + W(0, 0, 0)) * (U(0, 0, -1) + U(1, 0, -1));
+                    const double fn1 = 0.25 * (W(0, 0, 0) + W(0, 1, 0)) * (V(0, 0, 0) + V(1, 0, 0));
+                    const double fn0 = 0.25 * (W(0, -1, 0) + W(0, 0, 0)) * (V(0, -1, 0) + V(1, -1, 0));
+                    const double ft1 = top0w ? 0.0 : 0.25 * (W(1, 0, 0) + W(0, 0, 0)) * (W(0, 0, 0) + W(1, 0, 0));
+                    const double ft0 = (bot0 || (lid && k - 1 >= nz - 1))
+                                           ? 0.0
+                                           : 0.25 * (W(0, 0, 0) + W(-1, 0, 0)) * (W(-1, 0, 0) + W(0, 0, 0));
+                    aw = -((c.dsqr * Ro) * (div_rn(fe1 - fe0, dx, c.rdx) + div_rn(fn1 - fn0, dy, c.rdy) +
+                                            div_rn(ft1 - ft0, dz, c.rdz)));
+                }
+            }
+            adv = adv - 0.5 * (W(0, 0, 0) + W(-1, 0, 0)) * (c.N0 * c.N0);
+            au = au + (V(0, 0, 0) + V(0, -1, 0) + V(0, 0, 1) + V(0, -1, 1)) * 0.25 * c.f0;
+            av = av - (U(0, 0, -1) + U(0, 0, 0) + U(0, 1, -1) + U(0, 1, 0)) * 0.25 * c.f0;
+            aw = aw + 0.5 * (B(0, 0, 0) + B(1, 0, 0));
+            if (lid && k >= nz) aw = 0.0;
+            a.db[o] = adv;
+            a.du[o] = au;
+            a.dv[o] = av;
+            a.dw[o] = aw;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // friction, mixing.f90 -- out of place: interior from `in`, halo copied,
 // so later passes see the stale halo exactly as the in-place reference does
 // ---------------------------------------------------------------------------
@@ -301,21 +451,45 @@ __global__ __launch_bounds__(FTHREADS) void k_fric_ab(StepConsts c, FricArgs a)
     const double dx = c.dx, dy = c.dy, dz = c.dz, rdx = c.rdx, rdy = c.rdy, rdz = c.rdz, dt = c.dt;
 
     auto slot = [&](int k) { return (k - kb + 1) % 3; };
-    auto load_plane = [&](int k) {
-        const int s = slot(k);
-        for (int q = t; q < R0X * R0Y; q += FTHREADS) {
+    constexpr int PER = (R0X * R0Y + FTHREADS - 1) / FTHREADS;
+    double pre[PER];
+    auto fetch = [&](int k) {
+#pragma unroll
+        for (int s = 0; s < PER; ++s) {
+            const int q = t + s * FTHREADS;
             const int rx = q % R0X, ry = q / R0X;
             const int ii = i0 - 3 + rx, jj = j0 - 3 + ry;
-            double val = 0.0;
-            if (ii >= -1 && ii <= nx + 2 && jj >= -1 && jj <= ny + 2) val = F.in[idx(c, ii, jj, k)];
-            f0p[s][ry][rx] = val;
+            const bool ok = q < R0X * R0Y && ii >= -1 && ii <= nx + 2 && jj >= -1 && jj <= ny + 2;
+            pre[s] = ok ? F.in[idx(c, ii, jj, k)] : 0.0;
         }
     };
+    auto store = [&](int k) {
+        const int sl = slot(k);
+#pragma unroll
+        for (int s = 0; s < PER; ++s) {
+            const int q = t + s * FTHREADS;
+            if (q < R0X * R0Y) f0p[sl][q / R0X][q % R0X] = pre[s];
+        }
+    };
+    const int tpx = t % FTX, tpy = t / FTX;
+    const bool mine = (i0 + tpx <= nx) && (j0 + tpy <= ny);
 
-    load_plane(kb - 1);
-    load_plane(kb);
+    fetch(kb - 1);
+    store(kb - 1);
+    fetch(kb);
+    store(kb);
+    fetch(kb + 1);
     for (int k = kb; k <= ke; ++k) {
-        load_plane(k + 1);
+        store(k + 1);
+        if (k + 1 <= ke) fetch(k + 2);
+        // this level's tendency slots, issued before the LDS phases
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        if (mine) {
+            const int64_t o = idx(c, i0 + tpx, j0 + tpy, k);
+            s0 = F.t0[o];
+            if (a.order >= 2) s1 = F.t1[o];
+            if (a.order >= 3) s2 = F.t2[o];
+        }
         __syncthreads();
         const int sm = slot(k - 1), sc = slot(k), sp = slot(k + 1);
         for (int q = t; q < R1X * R1Y; q += FTHREADS) {
@@ -369,9 +543,9 @@ __global__ __launch_bounds__(FTHREADS) void k_fric_ab(StepConsts c, FricArgs a)
                 const int64_t o = idx(c, i, j, k);
                 const double s = F.scale;
                 double val;
-                if (a.order == 1) val = f2 + s * F.t0[o];
-                else if (a.order == 2) val = f2 + s * (a.ca * F.t0[o] - a.cb * F.t1[o]);
-                else val = f2 + s * (a.ca * F.t0[o] + a.cb * F.t1[o] + a.cc * F.t2[o]);
+                if (a.order == 1) val = f2 + s * s0;
+                else if (a.order == 2) val = f2 + s * (a.ca * s0 - a.cb * s1);
+                else val = f2 + s * (a.ca * s0 + a.cb * s1 + a.cc * s2);
                 F.out[o] = val;
             }
         }
@@ -489,11 +663,22 @@ inline dim3 march_grid(const Grid &g, int &kc)
 void launch_tendency(Model &m, int tau)
 {
     ProfScope ps(m, "tendency");
-    const int64_t off = (int64_t)(tau - 1) * m.g.H;
-    int kc = 1;
-    const dim3 grid = march_grid(m.g, kc);
-    hipLaunchKernelGGL((k_tendency<true, true, true>), grid, dim3(BX, BY), 0, m.stream, m.c, m.u, m.v, m.w, m.b,
-                       m.du + off, m.dv + off, m.dw + off, m.db + off, kc);
+    const Grid &g = m.g;
+    const int64_t off = (int64_t)(tau - 1) * g.H;
+    TendArgs a{};
+    a.u = m.u; a.v = m.v; a.w = m.w; a.b = m.b;
+    a.du = m.du + off; a.dv = m.dv + off; a.dw = m.dw + off; a.db = m.db + off;
+    a.tiles_x = (g.nx + TTX - 1) / TTX;
+    a.tiles_y = (g.ny + TTY - 1) / TTY;
+    const int tiles = a.tiles_x * a.tiles_y;
+    // about 4 work-groups per CU over 256 CUs, chunks of at least 8 levels
+    int nchunk = (1024 + tiles - 1) / tiles;
+    const int maxch = g.nz / 8 > 1 ? g.nz / 8 : 1;
+    nchunk = nchunk > maxch ? maxch : nchunk;
+    a.kc = (g.nz + nchunk - 1) / nchunk;
+    a.nchunk = (g.nz + a.kc - 1) / a.kc;
+    a.nblocks = tiles * a.nchunk;
+    hipLaunchKernelGGL(k_tendency_lds, dim3((unsigned)a.nblocks), dim3(TTHR), 0, m.stream, m.c, a);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 

@@ -44,33 +44,62 @@ __global__ __launch_bounds__(256) void k_tri_factor(const double *__restrict__ k
     }
 }
 
-__global__ __launch_bounds__(256) void k_thomas(double2 *__restrict__ fs, const double *__restrict__ ksqr,
-                                                const double *__restrict__ cp, const double *__restrict__ rm,
-                                                int64_t ncols, int nz, double fxa, double dz2, double dsqr)
+// one wavefront per work-group: the (nx/2+1)*ny columns are few (33K at
+// 256x256), and 64-lane groups spread them over every CU
+__global__ __launch_bounds__(64) void k_thomas(double2 *__restrict__ fs, const double *__restrict__ ksqr,
+                                               const double *__restrict__ cp, const double *__restrict__ rm,
+                                               int64_t ncols, int nz, double fxa, double dz2, double dsqr)
 {
-    const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t col = (int64_t)blockIdx.x * 64 + threadIdx.x;
     if (col >= ncols) return;
     double2 *d = fs + col;
     const double ks = ksqr[col];
     if (ks > 0) {
+        // the sweeps are sequential in k, but their loads are not: fetch
+        // LOOK levels at a time so each lane keeps that many requests in flight
+        constexpr int LOOK = 16;
         double2 dp = d[0];
         const double b1 = rm[col];
         dp = make_double2(dp.x / b1, dp.y / b1);
         d[0] = dp;
-        for (int k = 1; k < nz; ++k) {
-            const int64_t o = (int64_t)k * ncols;
-            const double2 dk = d[o];
-            const double r = rm[col + o];
-            dp = make_double2((dk.x - dp.x * fxa) * r, (dk.y - dp.y * fxa) * r);
-            d[o] = dp;
+        for (int k0 = 1; k0 < nz; k0 += LOOK) {
+            double2 dd[LOOK];
+            double rr[LOOK];
+#pragma unroll
+            for (int q = 0; q < LOOK; ++q) {
+                if (k0 + q < nz) {
+                    const int64_t o = (int64_t)(k0 + q) * ncols;
+                    dd[q] = d[o];
+                    rr[q] = rm[col + o];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < LOOK; ++q) {
+                if (k0 + q < nz) {
+                    dp = make_double2((dd[q].x - dp.x * fxa) * rr[q], (dd[q].y - dp.y * fxa) * rr[q]);
+                    d[(int64_t)(k0 + q) * ncols] = dp;
+                }
+            }
         }
         double2 x = dp;
-        for (int k = nz - 2; k >= 0; --k) {
-            const int64_t o = (int64_t)k * ncols;
-            const double c = cp[col + o];
-            const double2 dk = d[o];
-            x = make_double2(dk.x - c * x.x, dk.y - c * x.y);
-            d[o] = x;
+        for (int k0 = nz - 2; k0 >= 0; k0 -= LOOK) {
+            double2 dd[LOOK];
+            double cc[LOOK];
+#pragma unroll
+            for (int q = 0; q < LOOK; ++q) {
+                if (k0 - q >= 0) {
+                    const int64_t o = (int64_t)(k0 - q) * ncols;
+                    dd[q] = d[o];
+                    cc[q] = cp[col + o];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < LOOK; ++q) {
+                if (k0 - q >= 0) {
+                    x = make_double2(dd[q].x - cc[q] * x.x, dd[q].y - cc[q] * x.y);
+                    d[(int64_t)(k0 - q) * ncols] = x;
+                }
+            }
         }
     } else {
         // zero mode: p(1) = 0, p(2) = f(1) dz^2 dsqr, p(k+1) = 2p(k) - p(k-1) + f(k) dz^2 dsqr
@@ -125,7 +154,7 @@ void launch_thomas(Model &m, double2 *data)
     const int64_t ncols = (int64_t)m.g.nkx * m.g.ny;
     const double fxa = 1.0 / (m.prm.dz * m.prm.dz) / m.prm.dsqr;
     ProfScope ps(m, "thomas");
-    hipLaunchKernelGGL(k_thomas, dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, m.stream, data, m.ksqr,
+    hipLaunchKernelGGL(k_thomas, dim3((unsigned)((ncols + 63) / 64)), dim3(64), 0, m.stream, data, m.ksqr,
                        m.tri_cp, m.tri_rm, ncols, m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr);
     PS3D_HIP_CHECK(hipGetLastError());
 }
