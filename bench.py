@@ -13,6 +13,11 @@ The JSON line also carries:
   steps; achieved = its algorithmic bytes per launch / mean launch time.
 * cpu_baseline: the CPU oracle (a C port of the reference step, one core)
   on a bounded sample of the same workload.
+* mixed: the mixed-precision form (FP32 stencils and transforms, FP64
+  spectral solve and (hi, lo) accumulation) on the same workload: its
+  grid-pt steps/s, the speed-up over FP64, and its relative L2 error per
+  field and of the total energy (diagnose.F90:14-24) against the FP64 path
+  at steps 1, 100 and 1000 from the same initial state.
 """
 from __future__ import annotations
 
@@ -66,6 +71,44 @@ def time_steps(model, steps: int) -> float:
     return time.perf_counter() - t0
 
 
+def mixed_leg(prm, fields, steps: int, warmup: int, fp64_value: float, check_steps=(1, 100, 1000)) -> dict:
+    from ps3d_amd import diagnose
+    from ps3d_amd.model import Model
+    from ps3d_amd.params import PS3D_FP64, PS3D_MIXED
+    pm = prm.replace(precision=PS3D_MIXED)
+    m = Model(pm)
+    for n, a in fields.items():
+        m.f[n][...] = a
+    m.push()
+    m.run(max(warmup, 3))
+    el = time_steps(m, steps)
+    value = prm.npoints * steps / el
+    m.close()
+    # accuracy: both forms from the same initial state
+    g64 = Model(prm.replace(precision=PS3D_FP64))
+    gmx = Model(pm)
+    for n, a in fields.items():
+        g64.f[n][...] = a
+        gmx.f[n][...] = a
+    g64.push()
+    gmx.push()
+    errs = {}
+    done = 0
+    for upto in check_steps:
+        g64.run(upto - done)
+        gmx.run(upto - done)
+        done = upto
+        g64.pull()
+        gmx.pull()
+        errs[str(upto)] = diagnose.compare(gmx.f, g64.f, prm)
+    g64.close()
+    gmx.close()
+    worst = max(max(e.values()) for e in errs.values())
+    return {"value": value, "unit": "grid-pt steps/s", "ms_per_step": 1e3 * el / steps,
+            "speedup_vs_fp64": value / fp64_value, "rel_l2_vs_fp64": errs, "max_rel_l2": worst,
+            "dtype": "f32 stencils/FFT, f64 solve + accumulation"}
+
+
 def cpu_baseline(prm, fields, budget_s: float = 15.0) -> dict:
     import oracle
     orc = oracle.OracleModel(prm)
@@ -93,6 +136,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-all", action="store_true", help="print every kernel's share")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (to calibrate HBM byte counters)")
     args = ap.parse_args()
@@ -167,9 +211,11 @@ def main() -> int:
     }
     if args.profile_all:
         out["kernel_shares_ms_per_step"] = {k: v[1] for k, v in sorted(shares.items(), key=lambda x: -x[1][1])}
+    model.close()
+    if not args.no_mixed:
+        out["mixed"] = mixed_leg(prm, fields, args.steps, args.warmup, value)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prm, fields)
-    model.close()
     print(json.dumps(out), flush=True)
     return 0
 

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libps3d.so")
 
 # include/ps3d.h
 PS3D_OK = 0
+PS3D_ERR_ARG, PS3D_ERR_HIP, PS3D_ERR_STATE, PS3D_ERR_INTERNAL = 1, 2, 3, 4
 PS3D_U, PS3D_V, PS3D_W, PS3D_B, PS3D_P = 0, 1, 2, 3, 4
 PS3D_DU, PS3D_DV, PS3D_DW, PS3D_DB = 5, 6, 7, 8
 PS3D_FS, PS3D_KSQR = 9, 10

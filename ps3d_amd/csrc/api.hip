@@ -119,6 +119,41 @@ void check_model(const ps3d_model *h)
     if (!h) throw Error(PS3D_ERR_ARG, "null model");
 }
 
+// the building-block entry points work on the FP64 buffers of the drop-in
+void require_fp64(const Model &m, const char *what)
+{
+    if (m.mixed)
+        throw Error(PS3D_ERR_STATE, std::string(what) + ": not available on a PS3D_MIXED model (its step runs "
+                                                       "only as a whole: ps3d_integrate / ps3d_run)");
+}
+
+// set / get of a mixed model through the FP64 staging buffer
+void set_field_mixed(Model &m, int field, const double *host, int64_t l)
+{
+    PS3D_HIP_CHECK(hipMemcpyAsync(m.stage, host, sizeof(double) * l, hipMemcpyHostToDevice, m.stream));
+    if (field <= PS3D_B) launch_convert(m, 0, l, m.stage, nullptr, nullptr, m.fh[field], nullptr, m.fl[field]);
+    else if (field == PS3D_P) launch_convert(m, 2, l, m.stage, nullptr, nullptr, m.pf, nullptr, nullptr);
+    else if (field <= PS3D_DB) launch_convert(m, 2, l, m.stage, nullptr, nullptr, m.tf[field - PS3D_DU], nullptr, nullptr);
+    else launch_convert(m, 2, l, m.stage, nullptr, nullptr, reinterpret_cast<float *>(m.FSf), nullptr, nullptr);
+}
+
+void get_field_mixed(Model &m, int field, double *host, int64_t l)
+{
+    if (field <= PS3D_B) {
+        launch_convert(m, 1, l, nullptr, m.stage, m.fh[field], nullptr, m.fl[field], nullptr);
+        // after a step the lo parts' halos are stale (the stencils read the
+        // hi parts only): rebuild the halo of the joined field
+        if (m.lo_stale[field]) launch_border_exchg(m, m.stage, 0);
+    } else if (field == PS3D_P) {
+        launch_convert(m, 3, l, nullptr, m.stage, m.pf, nullptr, nullptr, nullptr);
+    } else if (field <= PS3D_DB) {
+        launch_convert(m, 3, l, nullptr, m.stage, m.tf[field - PS3D_DU], nullptr, nullptr, nullptr);
+    } else {
+        launch_convert(m, 3, l, nullptr, m.stage, reinterpret_cast<const float *>(m.FSf), nullptr, nullptr, nullptr);
+    }
+    PS3D_HIP_CHECK(hipMemcpyAsync(host, m.stage, sizeof(double) * l, hipMemcpyDeviceToHost, m.stream));
+}
+
 void check_tau(int tau, int taum1, int taum2)
 {
     auto ok = [](int t) { return t >= 1 && t <= 3; };
@@ -126,9 +161,34 @@ void check_tau(int tau, int taum1, int taum2)
     PS3D_REQUIRE(tau != taum1 && tau != taum2 && taum1 != taum2, "tau slots must differ");
 }
 
+// pressure_solver.f90:5-118 in the mixed form: FP32 transforms of the hi
+// parts, FP64 spectral division / Thomas elimination, FP64-accumulated
+// projection of (hi, lo)
+void pressure_mixed(Model &m)
+{
+    const ps3d_params &p = m.prm;
+    fft_r2c_x_f(m, m.fh[0], m.fh[1], m.fh[2], m.FSf);
+    if (p.enable_vertical_boundaries) {
+        fft_c2c_y_f(m, m.FSf, -1, p.dx, p.dy);
+        launch_thomas_f(m, m.FSf);
+        fft_c2c_y_f(m, m.FSf, +1, 0.0, 0.0);
+        fft_c2r_x_f(m, m.FSf, m.pf, p.Lx * p.Ly);
+    } else {
+        fft_c2c_y_f(m, m.FSf, -1, 0.0, 0.0);
+        fft_z_periodic_solve_f(m, m.FSf);
+        fft_c2c_y_f(m, m.FSf, +1, 0.0, 0.0);
+        fft_c2r_x_f(m, m.FSf, m.pf, p.Lx * p.Ly * p.Lz);
+    }
+    launch_project_exchange_f(m);
+}
+
 void pressure(Model &m)
 {
     const ps3d_params &p = m.prm;
+    if (m.mixed) {
+        pressure_mixed(m);
+        return;
+    }
     fft_r2c_x(m, nullptr, 0, 1, m.FS);
     if (p.enable_vertical_boundaries) {
         fft_c2c_y(m, m.FS, -1, p.dx, p.dy);
@@ -147,6 +207,12 @@ void pressure(Model &m)
 void integrate(Model &m, int itt, int tau, int taum1, int taum2)
 {
     launch_tendency(m, tau);
+    if (m.mixed) {
+        launch_friction_ab(m, itt, tau, taum1, taum2);
+        pressure(m);
+        for (int f = 0; f < 4; ++f) m.lo_stale[f] = true;
+        return;
+    }
     if (fused_friction_ok(m) && m.fuse_friction) {
         launch_friction_ab(m, itt, tau, taum1, taum2);
     } else {
@@ -210,7 +276,9 @@ int ps3d_create(const ps3d_params *prm, const double *ksqr, void *stream, ps3d_m
         PS3D_REQUIRE(prm && out, "null argument");
         *out = nullptr;
         const ps3d_params &p = *prm;
-        PS3D_REQUIRE(p.precision == PS3D_FP64, "precision: only PS3D_FP64 is available in this build");
+        PS3D_REQUIRE(p.precision == PS3D_FP64 || p.precision == PS3D_MIXED,
+                     "precision must be PS3D_FP64 or PS3D_MIXED");
+        const bool mixed = p.precision == PS3D_MIXED;
         PS3D_REQUIRE(p.nx >= 2 && p.nx % 2 == 0, "nx must be even and >= 2 (main.f90:45-55)");
         PS3D_REQUIRE(p.ny >= 2 && p.ny % 2 == 0, "ny must be even and >= 2 (main.f90:45-55)");
         PS3D_REQUIRE(p.nz >= 2, "nz must be >= 2");
@@ -219,6 +287,14 @@ int ps3d_create(const ps3d_params *prm, const double *ksqr, void *stream, ps3d_m
                      "FFT sizes (nx/2, ny, nz) must be products of 2, 3 and 5");
         PS3D_REQUIRE(p.ny <= 1024 && p.nz <= 1024 && p.nx <= 8192, "grid too large for the FFT kernels");
         PS3D_REQUIRE(p.dx > 0 && p.dy > 0 && p.dz > 0 && p.dt != 0 && p.dsqr > 0, "bad grid spacing or dt");
+        if (mixed) {
+            PS3D_REQUIRE(!(p.Avbi > 0.) && !(p.Kvbi > 0.),
+                         "PS3D_MIXED: vertical biharmonic friction (Avbi, Kvbi > 0) is only in the FP64 form");
+            PS3D_REQUIRE(fft_sizes_compiled(p.nx / 2) && fft_sizes_compiled(p.ny) &&
+                             (p.enable_vertical_boundaries || fft_sizes_compiled(p.nz)),
+                         "PS3D_MIXED: nx/2, ny (and nz without a rigid lid) must be compiled-in FFT lengths (" +
+                             fft_sizes_text() + ")");
+        }
         auto *m = new ps3d_model();
         try {
             m->prm = p;
@@ -244,18 +320,33 @@ int ps3d_create(const ps3d_params *prm, const double *ksqr, void *stream, ps3d_m
                 PS3D_HIP_CHECK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
                 m->own_stream = true;
             }
-            m->u = dalloc<double>(*m, g.H);
-            m->v = dalloc<double>(*m, g.H);
-            m->w = dalloc<double>(*m, g.H);
-            m->b = dalloc<double>(*m, g.H);
-            m->p = dalloc<double>(*m, g.H);
-            for (int f = 0; f < 4; ++f) m->alt[f] = dalloc<double>(*m, g.H);
-            m->du = dalloc<double>(*m, 3 * g.H);
-            m->dv = dalloc<double>(*m, 3 * g.H);
-            m->dw = dalloc<double>(*m, 3 * g.H);
-            m->db = dalloc<double>(*m, 3 * g.H);
-            m->FS = dalloc<double2>(*m, g.S);
-            m->aloc = dalloc<double>(*m, g.N);
+            if (mixed) {
+                m->mixed = true;
+                for (int f = 0; f < 4; ++f) {
+                    m->fh[f] = dalloc<float>(*m, g.H);
+                    m->fl[f] = dalloc<float>(*m, g.H);
+                    m->ah[f] = dalloc<float>(*m, g.H);
+                    m->al[f] = dalloc<float>(*m, g.H);
+                    m->tf[f] = dalloc<float>(*m, 3 * g.H);
+                }
+                m->pf = dalloc<float>(*m, g.H);
+                m->FSf = dalloc<float2>(*m, g.S);
+                m->stage = dalloc<double>(*m, 3 * g.H);
+                if (p.enable_vertical_boundaries && p.nz > 64) m->dp64 = dalloc<double2>(*m, g.S);
+            } else {
+                m->u = dalloc<double>(*m, g.H);
+                m->v = dalloc<double>(*m, g.H);
+                m->w = dalloc<double>(*m, g.H);
+                m->b = dalloc<double>(*m, g.H);
+                m->p = dalloc<double>(*m, g.H);
+                for (int f = 0; f < 4; ++f) m->alt[f] = dalloc<double>(*m, g.H);
+                m->du = dalloc<double>(*m, 3 * g.H);
+                m->dv = dalloc<double>(*m, 3 * g.H);
+                m->dw = dalloc<double>(*m, 3 * g.H);
+                m->db = dalloc<double>(*m, 3 * g.H);
+                m->FS = dalloc<double2>(*m, g.S);
+                m->aloc = dalloc<double>(*m, g.N);
+            }
             m->ksqr = dalloc<double>(*m, ksqr_len(*m));
             if (p.enable_vertical_boundaries) {
                 m->tri_cp = dalloc<double>(*m, g.S);
@@ -269,6 +360,11 @@ int ps3d_create(const ps3d_params *prm, const double *ksqr, void *stream, ps3d_m
                 for (int k = 0; k < g.nkx; ++k) wk[k] = host_twiddle(k, p.nx);
                 m->tw_r2c = dalloc<double2>(*m, g.nkx);
                 PS3D_HIP_CHECK(hipMemcpyAsync(m->tw_r2c, wk.data(), sizeof(double2) * g.nkx,
+                                              hipMemcpyHostToDevice, m->stream));
+                std::vector<float2> wkf(g.nkx);
+                for (int k = 0; k < g.nkx; ++k) wkf[k] = make_float2((float)wk[k].x, (float)wk[k].y);
+                m->twf_r2c = dalloc<float2>(*m, g.nkx);
+                PS3D_HIP_CHECK(hipMemcpyAsync(m->twf_r2c, wkf.data(), sizeof(float2) * g.nkx,
                                               hipMemcpyHostToDevice, m->stream));
                 PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
             }
@@ -321,8 +417,13 @@ int ps3d_set_field(ps3d_model *m, int field, const double *host)
         PS3D_REQUIRE(host, "null host pointer");
         const int64_t l = field_len(*m, field);
         PS3D_REQUIRE(l >= 0, "unknown field id");
-        PS3D_HIP_CHECK(hipMemcpyAsync(field_dev(*m, field), host, sizeof(double) * l, hipMemcpyHostToDevice,
-                                      m->stream));
+        if (m->mixed && field != PS3D_KSQR) {
+            set_field_mixed(*m, field, host, l);
+            if (field <= PS3D_B) m->lo_stale[field] = false;
+        } else {
+            PS3D_HIP_CHECK(hipMemcpyAsync(field_dev(*m, field), host, sizeof(double) * l, hipMemcpyHostToDevice,
+                                          m->stream));
+        }
         if (field == PS3D_KSQR) tridiag_factor(*m);
         PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
     });
@@ -335,8 +436,11 @@ int ps3d_get_field(ps3d_model *m, int field, double *host)
         PS3D_REQUIRE(host, "null host pointer");
         const int64_t l = field_len(*m, field);
         PS3D_REQUIRE(l >= 0, "unknown field id");
-        PS3D_HIP_CHECK(hipMemcpyAsync(host, field_dev(*m, field), sizeof(double) * l, hipMemcpyDeviceToHost,
-                                      m->stream));
+        if (m->mixed && field != PS3D_KSQR)
+            get_field_mixed(*m, field, host, l);
+        else
+            PS3D_HIP_CHECK(hipMemcpyAsync(host, field_dev(*m, field), sizeof(double) * l, hipMemcpyDeviceToHost,
+                                          m->stream));
         PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
     });
 }
@@ -380,6 +484,7 @@ int ps3d_tendency(ps3d_model *m, int tau)
 {
     return guard([&] {
         check_model(m);
+        require_fp64(*m, "ps3d_tendency");
         PS3D_REQUIRE(tau >= 1 && tau <= 3, "tau must be 1..3");
         launch_tendency(*m, tau);
     });
@@ -389,6 +494,7 @@ int ps3d_buoyancy_advection(ps3d_model *m, int tau)
 {
     return guard([&] {
         check_model(m);
+        require_fp64(*m, "ps3d_buoyancy_advection");
         PS3D_REQUIRE(tau >= 1 && tau <= 3, "tau must be 1..3");
         launch_buoyancy_advection(*m, tau);
     });
@@ -398,6 +504,7 @@ int ps3d_momentum_advection(ps3d_model *m, int tau)
 {
     return guard([&] {
         check_model(m);
+        require_fp64(*m, "ps3d_momentum_advection");
         PS3D_REQUIRE(tau >= 1 && tau <= 3, "tau must be 1..3");
         launch_momentum_advection(*m, tau);
     });
@@ -407,6 +514,7 @@ int ps3d_friction(ps3d_model *m)
 {
     return guard([&] {
         check_model(m);
+        require_fp64(*m, "ps3d_friction");
         launch_friction_block(*m);
     });
 }
@@ -415,6 +523,7 @@ int ps3d_mixing(ps3d_model *m, int op, int field, double hmix, double vmix)
 {
     return guard([&] {
         check_model(m);
+        require_fp64(*m, "ps3d_mixing");
         PS3D_REQUIRE(field >= PS3D_U && field <= PS3D_B, "mixing field must be u, v, w or b");
         PS3D_REQUIRE(op >= PS3D_HARMONIC && op <= PS3D_BIHARMONIC_V, "unknown mixing operator");
         launch_mixing(*m, op, field, hmix, vmix, 0);
@@ -425,6 +534,7 @@ int ps3d_ab_update(ps3d_model *m, int itt, int tau, int taum1, int taum2)
 {
     return guard([&] {
         check_model(m);
+        require_fp64(*m, "ps3d_ab_update");
         check_tau(tau, taum1, taum2);
         launch_ab_exchange(*m, itt, tau, taum1, taum2);
     });
@@ -434,6 +544,7 @@ int ps3d_border_exchg(ps3d_model *m, int field)
 {
     return guard([&] {
         check_model(m);
+        require_fp64(*m, "ps3d_border_exchg");
         PS3D_REQUIRE(field >= PS3D_U && field <= PS3D_P, "border_exchg field must be u, v, w, b or p");
         launch_border_exchg(*m, field == PS3D_P ? m->p : m->cur_buf(field), 0);
     });
@@ -453,6 +564,7 @@ int ps3d_my_fft(ps3d_model *m, int three_d, const double *qp, double *qs)
 {
     return guard([&] {
         check_model(m);
+        require_fp64(*m, "ps3d_my_fft");
         PS3D_REQUIRE(qp && qs, "null buffer");
         const ps3d_params &p = m->prm;
         PS3D_REQUIRE(!three_d || !p.enable_vertical_boundaries,
@@ -474,6 +586,7 @@ int ps3d_my_ifft(ps3d_model *m, int three_d, const double *qs, double *qp)
 {
     return guard([&] {
         check_model(m);
+        require_fp64(*m, "ps3d_my_ifft");
         PS3D_REQUIRE(qp && qs, "null buffer");
         const ps3d_params &p = m->prm;
         PS3D_REQUIRE(!three_d || !p.enable_vertical_boundaries,
@@ -491,6 +604,7 @@ int ps3d_spectral_solve(ps3d_model *m, double *qs)
 {
     return guard([&] {
         check_model(m);
+        require_fp64(*m, "ps3d_spectral_solve");
         PS3D_REQUIRE(qs, "null buffer");
         PS3D_HIP_CHECK(hipMemcpyAsync(m->FS, qs, sizeof(double2) * m->g.S, hipMemcpyHostToDevice, m->stream));
         if (m->prm.enable_vertical_boundaries) launch_thomas(*m, m->FS);

@@ -345,6 +345,66 @@ __global__ __launch_bounds__(512) void k_z_periodic_solve(FftPlan pl, double2 *_
 // ===========================================================================
 namespace ft {
 
+// complex arithmetic over float2 / double2, same formulas as above
+template <class C> struct real_of;
+template <> struct real_of<double2> { using type = double; };
+template <> struct real_of<float2> { using type = float; };
+
+template <class C>
+__device__ __forceinline__ C mk(typename real_of<C>::type x, typename real_of<C>::type y)
+{
+    C r;
+    r.x = x;
+    r.y = y;
+    return r;
+}
+template <class C> __device__ __forceinline__ C mul(C a, C b) { return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+template <class C> __device__ __forceinline__ C add(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
+template <class C> __device__ __forceinline__ C sub(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y); }
+template <class C> __device__ __forceinline__ C scl(C a, typename real_of<C>::type s) { return mk<C>(a.x * s, a.y * s); }
+template <class C, int SIGN> __device__ __forceinline__ C isg(C a) { return SIGN > 0 ? mk<C>(-a.y, a.x) : mk<C>(a.y, -a.x); }
+
+template <int R, int SIGN, class C>
+__device__ __forceinline__ void dftc(C *v)
+{
+    using T = typename real_of<C>::type;
+    if constexpr (R == 2) {
+        const C a = v[0], b = v[1];
+        v[0] = add(a, b);
+        v[1] = sub(a, b);
+    } else if constexpr (R == 3) {
+        const T c = T(-0.5), s = T(0.86602540378443864676);
+        const C t = add(v[1], v[2]);
+        const C d = sub(v[1], v[2]);
+        const C m = mk<C>(v[0].x + c * t.x, v[0].y + c * t.y);
+        const C q = isg<C, SIGN>(scl(d, s));
+        v[0] = add(v[0], t);
+        v[1] = add(m, q);
+        v[2] = sub(m, q);
+    } else if constexpr (R == 4) {
+        const C a = add(v[0], v[2]), bq = sub(v[0], v[2]);
+        const C c = add(v[1], v[3]), d = isg<C, SIGN>(sub(v[1], v[3]));
+        v[0] = add(a, c);
+        v[2] = sub(a, c);
+        v[1] = add(bq, d);
+        v[3] = sub(bq, d);
+    } else if constexpr (R == 5) {
+        const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+        const T s1 = T(0.95105651629515357212), s2 = T(0.58778525229247312917);
+        const C a14 = add(v[1], v[4]), d14 = sub(v[1], v[4]);
+        const C a23 = add(v[2], v[3]), d23 = sub(v[2], v[3]);
+        const C m1 = mk<C>(v[0].x + c1 * a14.x + c2 * a23.x, v[0].y + c1 * a14.y + c2 * a23.y);
+        const C m2 = mk<C>(v[0].x + c2 * a14.x + c1 * a23.x, v[0].y + c2 * a14.y + c1 * a23.y);
+        const C q1 = isg<C, SIGN>(mk<C>(s1 * d14.x + s2 * d23.x, s1 * d14.y + s2 * d23.y));
+        const C q2 = isg<C, SIGN>(mk<C>(s2 * d14.x - s1 * d23.x, s2 * d14.y - s1 * d23.y));
+        v[0] = mk<C>(v[0].x + a14.x + a23.x, v[0].y + a14.y + a23.y);
+        v[1] = add(m1, q1);
+        v[4] = sub(m1, q1);
+        v[2] = add(m2, q2);
+        v[3] = sub(m2, q2);
+    }
+}
+
 struct Fac {
     int n;
     int f[MAX_FAC];
@@ -362,14 +422,14 @@ constexpr Fac factorize(int N)
 }
 
 // one Stockham pass; element e of column col at buf[e*ES + col*CS]
-template <int R, int N, int NS, int NCOL, int T, int ES, int CS, bool COLFAST, int SIGN>
-__device__ __forceinline__ void pass(double2 *buf, const double2 *__restrict__ tw)
+template <int R, int N, int NS, int NCOL, int T, int ES, int CS, bool COLFAST, int SIGN, class C>
+__device__ __forceinline__ void pass(C *buf, const C *__restrict__ tw)
 {
     constexpr int NB = N / R;
     constexpr int TOTAL = NB * NCOL;
     constexpr int PER = (TOTAL + T - 1) / T;
     constexpr int TWS = N / (NS * R);
-    double2 v[PER][R];
+    C v[PER][R];
 #pragma unroll
     for (int s = 0; s < PER; ++s) {
         const int t = (int)threadIdx.x + s * T;
@@ -382,12 +442,12 @@ __device__ __forceinline__ void pass(double2 *buf, const double2 *__restrict__ t
                 const int jm = jb % NS;
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    double2 w = tw[r * jm * TWS];
+                    C w = tw[r * jm * TWS];
                     if constexpr (SIGN > 0) w.y = -w.y;
-                    v[s][r] = cmul(v[s][r], w);
+                    v[s][r] = mul(v[s][r], w);
                 }
             }
-            dft<R>(v[s], SIGN);
+            dftc<R, SIGN>(v[s]);
         }
     }
     __syncthreads();
@@ -406,8 +466,8 @@ __device__ __forceinline__ void pass(double2 *buf, const double2 *__restrict__ t
     __syncthreads();
 }
 
-template <int N, int P, int NS, int NCOL, int T, int ES, int CS, bool COLFAST, int SIGN>
-__device__ __forceinline__ void passes(double2 *buf, const double2 *__restrict__ tw)
+template <int N, int P, int NS, int NCOL, int T, int ES, int CS, bool COLFAST, int SIGN, class C>
+__device__ __forceinline__ void passes(C *buf, const C *__restrict__ tw)
 {
     constexpr Fac F = factorize(N);
     if constexpr (P < F.n) {
@@ -417,51 +477,51 @@ __device__ __forceinline__ void passes(double2 *buf, const double2 *__restrict__
     }
 }
 
-template <int N, int NCOL, int T, int ES, int CS, bool COLFAST, int SIGN>
-__device__ __forceinline__ void fft(double2 *buf, const double2 *__restrict__ tw)
+template <int N, int NCOL, int T, int ES, int CS, bool COLFAST, int SIGN, class C>
+__device__ __forceinline__ void fft(C *buf, const C *__restrict__ tw)
 {
     passes<N, 0, 1, NCOL, T, ES, CS, COLFAST, SIGN>(buf, tw);
 }
 
 // x-pass, forward r2c of ROWS rows (see k_r2c_x); the post-processing works
-// on the pair (k, n2-k) in place
-template <int N2, int ROWS, int T, bool FUSE_DIV>
-__global__ __launch_bounds__(T) void r2c_x(StepConsts c, const double2 *__restrict__ tw,
-                                           const double2 *__restrict__ wk, const double *__restrict__ src,
-                                           const double *__restrict__ u, const double *__restrict__ v,
-                                           const double *__restrict__ w, double2 *__restrict__ dst,
-                                           int64_t nrows)
+// on the pair (k, n2-k) in place.  RT = double (drop-in) or float (mixed).
+template <int N2, int ROWS, int T, bool FUSE_DIV, class RT>
+__global__ __launch_bounds__(T) void r2c_x(StepConsts c, const typename cplx_of<RT>::type *__restrict__ tw,
+                                           const typename cplx_of<RT>::type *__restrict__ wk,
+                                           const RT *__restrict__ src, const RT *__restrict__ u,
+                                           const RT *__restrict__ v, const RT *__restrict__ w,
+                                           typename cplx_of<RT>::type *__restrict__ dst, int64_t nrows, int dstride)
 {
+    using C = typename cplx_of<RT>::type;
     constexpr int CS = N2 + 1;
-    constexpr int NKX = N2 + 1;
-    __shared__ double2 lds[ROWS * CS];
+    __shared__ C lds[ROWS * CS];
     const int64_t row0 = (int64_t)blockIdx.x * ROWS;
+    const RT h = RT(0.5);
 #pragma unroll
     for (int s = 0; s < (ROWS * N2 + T - 1) / T; ++s) {
         const int t = (int)threadIdx.x + s * T;
         if ((ROWS * N2) % T == 0 || t < ROWS * N2) {
             const int rr = t / N2, m = t % N2;
             const int64_t row = row0 + rr;
-            double a0 = 0.0, a1 = 0.0;
+            RT a0 = RT(0), a1 = RT(0);
             if (row < nrows) {
                 if (FUSE_DIV) {
                     const int j = (int)(row % c.ny) + 1, k = (int)(row / c.ny) + 1;
                     const int64_t X = c.X, XY = c.XY;
                     int64_t o = fidx(c, 2 * m + 1, j, k);
-                    a0 = div_rn(u[o] - u[o - 1], c.dx, c.rdx) + div_rn(v[o] - v[o - X], c.dy, c.rdy) +
-                         div_rn(w[o] - w[o - XY], c.dz, c.rdz);
+                    a0 = sdiv<RT>(u[o] - u[o - 1], c.dx, c.rdx) + sdiv<RT>(v[o] - v[o - X], c.dy, c.rdy) +
+                         sdiv<RT>(w[o] - w[o - XY], c.dz, c.rdz);
                     o += 1;
-                    a1 = div_rn(u[o] - u[o - 1], c.dx, c.rdx) + div_rn(v[o] - v[o - X], c.dy, c.rdy) +
-                         div_rn(w[o] - w[o - XY], c.dz, c.rdz);
-                    a0 = div_rn(a0, c.dt, c.rdt);
-                    a1 = div_rn(a1, c.dt, c.rdt);
+                    a1 = sdiv<RT>(u[o] - u[o - 1], c.dx, c.rdx) + sdiv<RT>(v[o] - v[o - X], c.dy, c.rdy) +
+                         sdiv<RT>(w[o] - w[o - XY], c.dz, c.rdz);
+                    a0 = sdiv<RT>(a0, c.dt, c.rdt);
+                    a1 = sdiv<RT>(a1, c.dt, c.rdt);
                 } else {
-                    const double2 pr = reinterpret_cast<const double2 *>(src + row * (2 * (int64_t)N2))[m];
-                    a0 = pr.x;
-                    a1 = pr.y;
+                    a0 = src[row * (2 * (int64_t)N2) + 2 * m];
+                    a1 = src[row * (2 * (int64_t)N2) + 2 * m + 1];
                 }
             }
-            lds[rr * CS + m] = make_double2(a0, a1);
+            lds[rr * CS + m] = mk<C>(a0, a1);
         }
     }
     __syncthreads();
@@ -474,23 +534,23 @@ __global__ __launch_bounds__(T) void r2c_x(StepConsts c, const double2 *__restri
             const int rr = t / HALF, kk = t % HALF;
             const int64_t row = row0 + rr;
             if (row < nrows) {
-                const double2 z0 = lds[rr * CS + kk];
-                const double2 z1 = lds[rr * CS + (kk == 0 ? 0 : N2 - kk)];
-                double2 *d = dst + row * NKX;
+                const C z0 = lds[rr * CS + kk];
+                const C z1 = lds[rr * CS + (kk == 0 ? 0 : N2 - kk)];
+                C *d = dst + row * dstride;
                 {   // X[kk] from zk = z0, zm = z1
-                    const double2 e = make_double2(0.5 * (z0.x + z1.x), 0.5 * (z0.y - z1.y));
-                    const double2 bq = make_double2(z0.x - z1.x, z0.y + z1.y);
-                    const double2 o = make_double2(0.5 * bq.y, -0.5 * bq.x);
-                    d[kk] = cadd(e, cmul(wk[kk], o));
+                    const C e = mk<C>(h * (z0.x + z1.x), h * (z0.y - z1.y));
+                    const C bq = mk<C>(z0.x - z1.x, z0.y + z1.y);
+                    const C o = mk<C>(h * bq.y, -h * bq.x);
+                    d[kk] = add(e, mul(wk[kk], o));
                 }
                 const int km = (kk == 0) ? N2 : N2 - kk;   // its partner
                 if (km != kk) {                             // X[km] from zk = z1 (or Z[0]), zm = z0
-                    const double2 zk = (kk == 0) ? z0 : z1;
-                    const double2 zm = z0;
-                    const double2 e = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
-                    const double2 bq = make_double2(zk.x - zm.x, zk.y + zm.y);
-                    const double2 o = make_double2(0.5 * bq.y, -0.5 * bq.x);
-                    d[km] = cadd(e, cmul(wk[km], o));
+                    const C zk = (kk == 0) ? z0 : z1;
+                    const C zm = z0;
+                    const C e = mk<C>(h * (zk.x + zm.x), h * (zk.y - zm.y));
+                    const C bq = mk<C>(zk.x - zm.x, zk.y + zm.y);
+                    const C o = mk<C>(h * bq.y, -h * bq.x);
+                    d[km] = add(e, mul(wk[km], o));
                 }
             }
         }
@@ -498,16 +558,17 @@ __global__ __launch_bounds__(T) void r2c_x(StepConsts c, const double2 *__restri
 }
 
 // x-pass, inverse c2r of ROWS rows (see k_c2r_x), pre-processing in place
-template <int N2, int ROWS, int T>
-__global__ __launch_bounds__(T) void c2r_x(StepConsts c, const double2 *__restrict__ tw,
-                                           const double2 *__restrict__ wk, const double2 *__restrict__ src,
-                                           double *__restrict__ dst, int dst_is_field, double inv_scale,
-                                           double rinv_scale, int64_t nrows)
+template <int N2, int ROWS, int T, class RT>
+__global__ __launch_bounds__(T) void c2r_x(StepConsts c, const typename cplx_of<RT>::type *__restrict__ tw,
+                                           const typename cplx_of<RT>::type *__restrict__ wk,
+                                           const typename cplx_of<RT>::type *__restrict__ src,
+                                           RT *__restrict__ dst, int dst_is_field, double inv_scale,
+                                           double rinv_scale, int64_t nrows, int sstride)
 {
+    using C = typename cplx_of<RT>::type;
     constexpr int CS = N2 + 1;
-    constexpr int NKX = N2 + 1;
     constexpr int HALF = N2 / 2 + 1;
-    __shared__ double2 lds[ROWS * CS];
+    __shared__ C lds[ROWS * CS];
 #pragma unroll
     for (int s = 0; s < (ROWS * HALF + T - 1) / T; ++s) {
         const int t = (int)threadIdx.x + s * T;
@@ -515,24 +576,24 @@ __global__ __launch_bounds__(T) void c2r_x(StepConsts c, const double2 *__restri
             const int rr = t / HALF, kk = t % HALF;
             const int64_t row = (int64_t)blockIdx.x * ROWS + rr;
             if (row < nrows) {
-                const double2 *sr = src + row * NKX;
-                double2 xk = sr[kk];
+                const C *sr = src + row * sstride;
+                C xk = sr[kk];
                 const int km = N2 - kk;
-                double2 xm = sr[km];
-                if (kk == 0) { xk.y = 0.0; xm.y = 0.0; }   // kx = 0 and kx = nx/2
+                C xm = sr[km];
+                if (kk == 0) { xk.y = RT(0); xm.y = RT(0); }   // kx = 0 and kx = nx/2
                 {   // Z'[kk] from xk, xm
-                    const double2 a = make_double2(xk.x + xm.x, xk.y - xm.y);
-                    const double2 dd = make_double2(xk.x - xm.x, xk.y + xm.y);
-                    const double2 wc = make_double2(wk[kk].x, -wk[kk].y);
-                    const double2 q = cmul(wc, dd);
-                    lds[rr * CS + kk] = make_double2(a.x - q.y, a.y + q.x);
+                    const C a = mk<C>(xk.x + xm.x, xk.y - xm.y);
+                    const C dd = mk<C>(xk.x - xm.x, xk.y + xm.y);
+                    const C wc = mk<C>(wk[kk].x, -wk[kk].y);
+                    const C q = mul(wc, dd);
+                    lds[rr * CS + kk] = mk<C>(a.x - q.y, a.y + q.x);
                 }
                 if (kk != 0 && km != kk) {   // Z'[km] from xk' = xm, xm' = xk
-                    const double2 a = make_double2(xm.x + xk.x, xm.y - xk.y);
-                    const double2 dd = make_double2(xm.x - xk.x, xm.y + xk.y);
-                    const double2 wc = make_double2(wk[km].x, -wk[km].y);
-                    const double2 q = cmul(wc, dd);
-                    lds[rr * CS + km] = make_double2(a.x - q.y, a.y + q.x);
+                    const C a = mk<C>(xm.x + xk.x, xm.y - xk.y);
+                    const C dd = mk<C>(xm.x - xk.x, xm.y + xk.y);
+                    const C wc = mk<C>(wk[km].x, -wk[km].y);
+                    const C q = mul(wc, dd);
+                    lds[rr * CS + km] = mk<C>(a.x - q.y, a.y + q.x);
                 }
             }
         }
@@ -546,15 +607,16 @@ __global__ __launch_bounds__(T) void c2r_x(StepConsts c, const double2 *__restri
             const int rr = t / N2, m = t % N2;
             const int64_t row = (int64_t)blockIdx.x * ROWS + rr;
             if (row < nrows) {
-                const double2 z = lds[rr * CS + m];
-                const double a0 = div_rn(z.x, inv_scale, rinv_scale), a1 = div_rn(z.y, inv_scale, rinv_scale);
+                const C z = lds[rr * CS + m];
+                const RT a0 = sdiv<RT>(z.x, inv_scale, rinv_scale), a1 = sdiv<RT>(z.y, inv_scale, rinv_scale);
                 if (dst_is_field) {
                     const int j = (int)(row % c.ny) + 1, k = (int)(row / c.ny) + 1;
                     const int64_t o = fidx(c, 2 * m + 1, j, k);
                     dst[o] = a0;
                     dst[o + 1] = a1;
                 } else {
-                    reinterpret_cast<double2 *>(dst + row * (2 * (int64_t)N2))[m] = make_double2(a0, a1);
+                    dst[row * (2 * (int64_t)N2) + 2 * m] = a0;
+                    dst[row * (2 * (int64_t)N2) + 2 * m + 1] = a1;
                 }
             }
         }
@@ -562,21 +624,22 @@ __global__ __launch_bounds__(T) void c2r_x(StepConsts c, const double2 *__restri
 }
 
 // strided c2c pass over TX consecutive kx columns (see k_c2c_cols)
-template <int N, int TX, int T, int SIGN>
-__global__ __launch_bounds__(T) void c2c_cols(const double2 *__restrict__ tw, double2 *__restrict__ data,
-                                              int nkx, int64_t estride, int64_t lstride, double s1, double s2,
+template <int N, int TX, int T, int SIGN, class C>
+__global__ __launch_bounds__(T) void c2c_cols(const C *__restrict__ tw, C *__restrict__ data, int nkx,
+                                              int64_t estride, int64_t lstride, double s1, double s2,
                                               double s3, int nscale)
 {
-    __shared__ double2 lds[N * TX];
+    using RT = typename real_of<C>::type;
+    __shared__ C lds[N * TX];
     const int kx0 = blockIdx.x * TX;
     const int ncol = min(TX, nkx - kx0);
-    double2 *base = data + kx0 + (int64_t)blockIdx.y * lstride;
+    C *base = data + kx0 + (int64_t)blockIdx.y * lstride;
 #pragma unroll
     for (int s = 0; s < (N * TX + T - 1) / T; ++s) {
         const int t = (int)threadIdx.x + s * T;
         if ((N * TX) % T == 0 || t < N * TX) {
             const int col = t % TX, e = t / TX;
-            lds[e * TX + col] = (col < ncol) ? base[col + e * estride] : make_double2(0.0, 0.0);
+            lds[e * TX + col] = (col < ncol) ? base[col + e * estride] : mk<C>(RT(0), RT(0));
         }
     }
     __syncthreads();
@@ -587,35 +650,37 @@ __global__ __launch_bounds__(T) void c2c_cols(const double2 *__restrict__ tw, do
         if ((N * TX) % T == 0 || t < N * TX) {
             const int col = t % TX, e = t / TX;
             if (col < ncol) {
-                double2 x = lds[e * TX + col];
-                if (nscale >= 1) x = cscale(x, s1);
-                if (nscale >= 2) x = cscale(x, s2);
-                if (nscale >= 3) x = cscale(x, s3);
+                C x = lds[e * TX + col];
+                if (nscale >= 1) x = scl(x, (RT)s1);
+                if (nscale >= 2) x = scl(x, (RT)s2);
+                if (nscale >= 3) x = scl(x, (RT)s3);
                 base[col + e * estride] = x;
             }
         }
     }
 }
 
-// periodic z solve (see k_z_periodic_solve)
-template <int N, int TX, int T>
-__global__ __launch_bounds__(T) void z_periodic_solve(const double2 *__restrict__ tw, double2 *__restrict__ data,
-                                                      const double *__restrict__ ksqr, int nkx, int ny, double dx,
-                                                      double dy, double dz)
+// periodic z solve (see k_z_periodic_solve); the scaling and the division by
+// ksqr are done in FP64 whatever the storage type
+template <int N, int TX, int T, class C>
+__global__ __launch_bounds__(T) void z_periodic_solve(const C *__restrict__ tw, C *__restrict__ data,
+                                                      const double *__restrict__ ksqr, int nkx, int nkxs, int ny,
+                                                      double dx, double dy, double dz)
 {
-    __shared__ double2 lds[N * TX];
+    using RT = typename real_of<C>::type;
+    __shared__ C lds[N * TX];
     const int kx0 = blockIdx.x * TX;
     const int j = blockIdx.y;
     const int ncol = min(TX, nkx - kx0);
-    const int64_t estride = (int64_t)nkx * ny;
-    double2 *base = data + kx0 + (int64_t)j * nkx;
+    const int64_t estride = (int64_t)nkxs * ny;
+    C *base = data + kx0 + (int64_t)j * nkxs;
     const double *kb = ksqr + kx0 + (int64_t)j * nkx;
 #pragma unroll
     for (int s = 0; s < (N * TX + T - 1) / T; ++s) {
         const int t = (int)threadIdx.x + s * T;
         if ((N * TX) % T == 0 || t < N * TX) {
             const int col = t % TX, e = t / TX;
-            lds[e * TX + col] = (col < ncol) ? base[col + e * estride] : make_double2(0.0, 0.0);
+            lds[e * TX + col] = (col < ncol) ? base[col + e * estride] : mk<C>(RT(0), RT(0));
         }
     }
     __syncthreads();
@@ -626,12 +691,12 @@ __global__ __launch_bounds__(T) void z_periodic_solve(const double2 *__restrict_
         if ((N * TX) % T == 0 || t < N * TX) {
             const int col = t % TX, e = t / TX;
             if (col < ncol) {
-                double2 x = lds[e * TX + col];
-                x = cscale(cscale(cscale(x, dx), dy), dz);
-                const double ks = kb[col + e * estride];
-                if (ks > 0.0) x = make_double2(-(x.x / ks), -(x.y / ks));
-                else x = make_double2(0.0, 0.0);
-                lds[e * TX + col] = x;
+                const C xs = lds[e * TX + col];
+                double xr = (double)xs.x * dx * dy * dz, xi = (double)xs.y * dx * dy * dz;
+                const double ks = kb[col + e * (int64_t)nkx * ny];
+                if (ks > 0.0) { xr = -(xr / ks); xi = -(xi / ks); }
+                else { xr = 0.0; xi = 0.0; }
+                lds[e * TX + col] = mk<C>((RT)xr, (RT)xi);
             }
         }
     }
@@ -648,99 +713,111 @@ __global__ __launch_bounds__(T) void z_periodic_solve(const double2 *__restrict_
 }
 
 // launch configurations: x-pass rows so that ROWS*N2 = 1024 values, y/z
-// passes 8 columns; 256 threads
+// passes 8 columns; 256 threads (more for the longest columns)
 constexpr int XROWS(int n2) { return n2 >= 1024 ? 1 : 1024 / n2; }
 constexpr int TCOL = 8;
 constexpr int TTHREADS = 256;
 constexpr int TCOLS(int n) { return n * TCOL / 16 > 256 ? n * TCOL / 16 : 256; }
 
-template <int N2>
-bool launch_r2c(Model &m, const double *src, int fuse_div, double2 *dst, int64_t nrows)
+template <int N2, class RT>
+bool launch_r2c(Model &m, const RT *src, const RT *u, const RT *v, const RT *w, int fuse_div,
+                typename cplx_of<RT>::type *dst, const typename cplx_of<RT>::type *tw,
+                const typename cplx_of<RT>::type *wk, int64_t nrows, int dstride)
 {
     constexpr int ROWS = XROWS(N2);
     const dim3 grid((unsigned)((nrows + ROWS - 1) / ROWS));
     if (fuse_div)
-        hipLaunchKernelGGL((r2c_x<N2, ROWS, TTHREADS, true>), grid, dim3(TTHREADS), 0, m.stream, m.c, m.px.tw,
-                           m.tw_r2c, nullptr, m.u, m.v, m.w, dst, nrows);
+        hipLaunchKernelGGL((r2c_x<N2, ROWS, TTHREADS, true, RT>), grid, dim3(TTHREADS), 0, m.stream, m.c, tw, wk,
+                           nullptr, u, v, w, dst, nrows, dstride);
     else
-        hipLaunchKernelGGL((r2c_x<N2, ROWS, TTHREADS, false>), grid, dim3(TTHREADS), 0, m.stream, m.c, m.px.tw,
-                           m.tw_r2c, src, nullptr, nullptr, nullptr, dst, nrows);
+        hipLaunchKernelGGL((r2c_x<N2, ROWS, TTHREADS, false, RT>), grid, dim3(TTHREADS), 0, m.stream, m.c, tw, wk,
+                           src, nullptr, nullptr, nullptr, dst, nrows, dstride);
     return true;
 }
 
-template <int N2>
-bool launch_c2r(Model &m, const double2 *src, double *dst, int dst_is_field, double inv_scale, int64_t nrows)
+template <int N2, class RT>
+bool launch_c2r(Model &m, const typename cplx_of<RT>::type *src, RT *dst, int dst_is_field, double inv_scale,
+                const typename cplx_of<RT>::type *tw, const typename cplx_of<RT>::type *wk, int64_t nrows,
+                int sstride)
 {
     constexpr int ROWS = XROWS(N2);
     const dim3 grid((unsigned)((nrows + ROWS - 1) / ROWS));
-    hipLaunchKernelGGL((c2r_x<N2, ROWS, TTHREADS>), grid, dim3(TTHREADS), 0, m.stream, m.c, m.px.tw, m.tw_r2c,
-                       src, dst, dst_is_field, inv_scale, 1.0 / inv_scale, nrows);
+    hipLaunchKernelGGL((c2r_x<N2, ROWS, TTHREADS, RT>), grid, dim3(TTHREADS), 0, m.stream, m.c, tw, wk, src, dst,
+                       dst_is_field, inv_scale, 1.0 / inv_scale, nrows, sstride);
     return true;
 }
 
-template <int N>
-bool launch_cols(const FftPlan &pl, hipStream_t st, double2 *data, int nkx, int64_t estride, int64_t lstride,
+template <int N, class C>
+bool launch_cols(const C *tw, hipStream_t st, C *data, int nkx, int64_t estride, int64_t lstride,
                  unsigned ny_or_nz, int sign, double s1, double s2, double s3, int nscale)
 {
     const dim3 grid((unsigned)((nkx + TCOL - 1) / TCOL), ny_or_nz);
     constexpr int T = TCOLS(N);
     if (sign < 0)
-        hipLaunchKernelGGL((c2c_cols<N, TCOL, T, -1>), grid, dim3(T), 0, st, pl.tw, data, nkx, estride, lstride, s1,
-                           s2, s3, nscale);
+        hipLaunchKernelGGL((c2c_cols<N, TCOL, T, -1, C>), grid, dim3(T), 0, st, tw, data, nkx, estride, lstride,
+                           s1, s2, s3, nscale);
     else
-        hipLaunchKernelGGL((c2c_cols<N, TCOL, T, +1>), grid, dim3(T), 0, st, pl.tw, data, nkx, estride, lstride, s1,
-                           s2, s3, nscale);
+        hipLaunchKernelGGL((c2c_cols<N, TCOL, T, +1, C>), grid, dim3(T), 0, st, tw, data, nkx, estride, lstride,
+                           s1, s2, s3, nscale);
     return true;
 }
 
-template <int N>
-bool launch_zsolve(Model &m, double2 *data)
+template <int N, class C>
+bool launch_zsolve(Model &m, C *data, const C *tw, int nkxs)
 {
     const dim3 grid((unsigned)((m.g.nkx + TCOL - 1) / TCOL), (unsigned)m.g.ny);
     constexpr int T = TCOLS(N);
-    hipLaunchKernelGGL((z_periodic_solve<N, TCOL, T>), grid, dim3(T), 0, m.stream, m.pz.tw, data,
-                       m.ksqr, m.g.nkx, m.g.ny, m.prm.dx, m.prm.dy, m.prm.dz);
+    hipLaunchKernelGGL((z_periodic_solve<N, TCOL, T, C>), grid, dim3(T), 0, m.stream, tw, data, m.ksqr, m.g.nkx,
+                       nkxs, m.g.ny, m.prm.dx, m.prm.dy, m.prm.dz);
     return true;
 }
 
-// the sizes compiled in; others take the generic kernels
+// the sizes compiled in; others take the generic FP64 kernels
 #define PS3D_FFT_SIZES(X) X(8) X(16) X(32) X(48) X(64) X(96) X(128) X(256) X(512) X(1024)
 
-bool dispatch_r2c(Model &m, const double *src, int fuse_div, double2 *dst, int64_t nrows)
+template <class RT>
+bool dispatch_r2c(Model &m, const RT *src, const RT *u, const RT *v, const RT *w, int fuse_div,
+                  typename cplx_of<RT>::type *dst, const typename cplx_of<RT>::type *tw,
+                  const typename cplx_of<RT>::type *wk, int64_t nrows, int dstride)
 {
     switch (m.px.n) {
-#define CASE(n) case n: return launch_r2c<n>(m, src, fuse_div, dst, nrows);
+#define CASE(n) case n: return launch_r2c<n, RT>(m, src, u, v, w, fuse_div, dst, tw, wk, nrows, dstride);
         PS3D_FFT_SIZES(CASE)
 #undef CASE
     default: return false;
     }
 }
 
-bool dispatch_c2r(Model &m, const double2 *src, double *dst, int dst_is_field, double inv_scale, int64_t nrows)
+template <class RT>
+bool dispatch_c2r(Model &m, const typename cplx_of<RT>::type *src, RT *dst, int dst_is_field, double inv_scale,
+                  const typename cplx_of<RT>::type *tw, const typename cplx_of<RT>::type *wk, int64_t nrows,
+                  int sstride)
 {
     switch (m.px.n) {
-#define CASE(n) case n: return launch_c2r<n>(m, src, dst, dst_is_field, inv_scale, nrows);
+#define CASE(n) case n: return launch_c2r<n, RT>(m, src, dst, dst_is_field, inv_scale, tw, wk, nrows, sstride);
         PS3D_FFT_SIZES(CASE)
 #undef CASE
     default: return false;
     }
 }
 
-bool dispatch_cols(const FftPlan &pl, hipStream_t st, double2 *data, int nkx, int64_t estride, int64_t lstride,
+template <class C>
+bool dispatch_cols(int n, const C *tw, hipStream_t st, C *data, int nkx, int64_t estride, int64_t lstride,
                    unsigned ny_or_nz, int sign, double s1, double s2, double s3, int nscale)
 {
-    switch (pl.n) {
-#define CASE(n) case n: return launch_cols<n>(pl, st, data, nkx, estride, lstride, ny_or_nz, sign, s1, s2, s3, nscale);
+    switch (n) {
+#define CASE(n) case n: return launch_cols<n, C>(tw, st, data, nkx, estride, lstride, ny_or_nz, sign, s1, s2, s3, nscale);
         PS3D_FFT_SIZES(CASE)
 #undef CASE
     default: return false;
     }
 }
 
-bool dispatch_zsolve(Model &m, double2 *data)
+template <class C>
+bool dispatch_zsolve(Model &m, C *data, const C *tw, int nkxs)
 {
     switch (m.pz.n) {
-#define CASE(n) case n: return launch_zsolve<n>(m, data);
+#define CASE(n) case n: return launch_zsolve<n, C>(m, data, tw, nkxs);
         PS3D_FFT_SIZES(CASE)
 #undef CASE
     default: return false;
@@ -811,6 +888,13 @@ void fft_plan_init(Model &m, FftPlan &pl, int n)
     m.allocations.push_back(d);
     PS3D_HIP_CHECK(hipMemcpy(d, tw.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
     pl.tw = d;
+    std::vector<float2> twf(n);
+    for (int q = 0; q < n; ++q) twf[q] = make_float2((float)tw[q].x, (float)tw[q].y);
+    float2 *df = nullptr;
+    PS3D_HIP_CHECK(hipMalloc(&df, sizeof(float2) * n));
+    m.allocations.push_back(df);
+    PS3D_HIP_CHECK(hipMemcpy(df, twf.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    pl.twf = df;
     PS3D_REQUIRE(threads_for_cols(pl, TXC) > 0, "FFT length too long for one work-group");
 }
 
@@ -822,7 +906,7 @@ void fft_r2c_x(Model &m, const double *src, int /*src_is_field*/, int fuse_div, 
     const size_t lds = (size_t)rows * (m.px.n + 1) * sizeof(double2);
     const dim3 grid((unsigned)((nrows + rows - 1) / rows));
     ProfScope ps(m, "fft_x_r2c");
-    if (ft::dispatch_r2c(m, src, fuse_div, dst, nrows)) {
+    if (ft::dispatch_r2c<double>(m, src, m.u, m.v, m.w, fuse_div, dst, m.px.tw, m.tw_r2c, nrows, g.nkx)) {
         PS3D_HIP_CHECK(hipGetLastError());
         return;
     }
@@ -843,7 +927,7 @@ void fft_c2r_x(Model &m, const double2 *src, double *dst, int dst_is_field, doub
     const size_t lds = (size_t)2 * rows * (m.px.n + 1) * sizeof(double2);
     const dim3 grid((unsigned)((nrows + rows - 1) / rows));
     ProfScope ps(m, "fft_x_c2r");
-    if (ft::dispatch_c2r(m, src, dst, dst_is_field, inv_scale, nrows)) {
+    if (ft::dispatch_c2r<double>(m, src, dst, dst_is_field, inv_scale, m.px.tw, m.tw_r2c, nrows, g.nkx)) {
         PS3D_HIP_CHECK(hipGetLastError());
         return;
     }
@@ -861,8 +945,8 @@ void fft_c2c_y(Model &m, double2 *data, int sign, double scale1, double scale2)
     const int nscale = (scale1 != 0.0) ? 2 : 0;
     allow_lds(k_c2c_cols, lds);
     ProfScope ps(m, sign < 0 ? "fft_y_fwd" : "fft_y_inv");
-    if (ft::dispatch_cols(m.py, m.stream, data, g.nkx, (int64_t)g.nkx, (int64_t)g.nkx * g.ny, (unsigned)g.nz, sign,
-                          scale1, scale2, 0.0, nscale)) {
+    if (ft::dispatch_cols<double2>(m.py.n, m.py.tw, m.stream, data, g.nkx, (int64_t)g.nkx, (int64_t)g.nkx * g.ny,
+                                   (unsigned)g.nz, sign, scale1, scale2, 0.0, nscale)) {
         PS3D_HIP_CHECK(hipGetLastError());
         return;
     }
@@ -880,8 +964,8 @@ void fft_c2c_z(Model &m, double2 *data, int sign, double s1, double s2, double s
     const int nscale = (s1 != 0.0) ? 3 : 0;
     allow_lds(k_c2c_cols, lds);
     ProfScope ps(m, sign < 0 ? "fft_z_fwd" : "fft_z_inv");
-    if (ft::dispatch_cols(m.pz, m.stream, data, g.nkx, (int64_t)g.nkx * g.ny, (int64_t)g.nkx, (unsigned)g.ny, sign,
-                          s1, s2, s3, nscale)) {
+    if (ft::dispatch_cols<double2>(m.pz.n, m.pz.tw, m.stream, data, g.nkx, (int64_t)g.nkx * g.ny, (int64_t)g.nkx,
+                                   (unsigned)g.ny, sign, s1, s2, s3, nscale)) {
         PS3D_HIP_CHECK(hipGetLastError());
         return;
     }
@@ -898,13 +982,72 @@ void fft_z_periodic_solve(Model &m, double2 *data)
     const dim3 grid((unsigned)((g.nkx + TXC - 1) / TXC), (unsigned)g.ny);
     allow_lds(k_z_periodic_solve, lds);
     ProfScope ps(m, "z_solve");
-    if (ft::dispatch_zsolve(m, data)) {
+    if (ft::dispatch_zsolve<double2>(m, data, m.pz.tw, g.nkx)) {
         PS3D_HIP_CHECK(hipGetLastError());
         return;
     }
     hipLaunchKernelGGL(k_z_periodic_solve, grid, dim3(T), lds, m.stream, m.pz, data, m.ksqr, g.nkx, g.ny, TXC,
                        m.prm.dx, m.prm.dy, m.prm.dz);
     PS3D_HIP_CHECK(hipGetLastError());
+}
+
+// ---- FP32 transforms of the mixed form (compiled-in lengths only) ----------
+
+void fft_r2c_x_f(Model &m, const float *u, const float *v, const float *w, float2 *dst)
+{
+    const Grid &g = m.g;
+    ProfScope ps(m, "fft_x_r2c");
+    PS3D_REQUIRE(ft::dispatch_r2c<float>(m, nullptr, u, v, w, 1, dst, m.px.twf, m.twf_r2c, (int64_t)g.ny * g.nz,
+                                         g.nkx),
+                 "mixed precision: nx/2 must be one of the compiled-in FFT lengths");
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void fft_c2r_x_f(Model &m, const float2 *src, float *dst, double inv_scale)
+{
+    const Grid &g = m.g;
+    ProfScope ps(m, "fft_x_c2r");
+    PS3D_REQUIRE(ft::dispatch_c2r<float>(m, src, dst, 1, inv_scale, m.px.twf, m.twf_r2c, (int64_t)g.ny * g.nz, g.nkx),
+                 "mixed precision: nx/2 must be one of the compiled-in FFT lengths");
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void fft_c2c_y_f(Model &m, float2 *data, int sign, double scale1, double scale2)
+{
+    const Grid &g = m.g;
+    const int nscale = (scale1 != 0.0) ? 2 : 0;
+    ProfScope ps(m, sign < 0 ? "fft_y_fwd" : "fft_y_inv");
+    PS3D_REQUIRE(ft::dispatch_cols<float2>(m.py.n, m.py.twf, m.stream, data, g.nkx, (int64_t)g.nkx,
+                                           (int64_t)g.nkx * g.ny, (unsigned)g.nz, sign, scale1, scale2, 0.0, nscale),
+                 "mixed precision: ny must be one of the compiled-in FFT lengths");
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void fft_z_periodic_solve_f(Model &m, float2 *data)
+{
+    ProfScope ps(m, "z_solve");
+    PS3D_REQUIRE(ft::dispatch_zsolve<float2>(m, data, m.pz.twf, m.g.nkx),
+                 "mixed precision: nz must be one of the compiled-in FFT lengths");
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+bool fft_sizes_compiled(int n)
+{
+    switch (n) {
+#define CASE(n) case n: return true;
+        PS3D_FFT_SIZES(CASE)
+#undef CASE
+    default: return false;
+    }
+}
+
+std::string fft_sizes_text()
+{
+    std::string t;
+#define ITEM(n) t += (t.empty() ? "" : ", ") + std::to_string(n);
+    PS3D_FFT_SIZES(ITEM)
+#undef ITEM
+    return t;
 }
 
 }  // namespace ps3d

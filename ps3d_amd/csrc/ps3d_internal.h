@@ -76,6 +76,21 @@ __device__ __forceinline__ double div_rn(double a, double b, double rb)
     return __builtin_fma(e, rb, q);
 }
 
+// Division by a grid constant in a stencil of element type T: the FP64 path
+// keeps the reference's correctly rounded quotient, the FP32 path of the
+// mixed form multiplies by the reciprocal.
+template <class T>
+__device__ __forceinline__ T sdiv(T a, double b, double rb);
+template <>
+__device__ __forceinline__ double sdiv<double>(double a, double b, double rb) { return div_rn(a, b, rb); }
+template <>
+__device__ __forceinline__ float sdiv<float>(float a, double, double rb) { return a * (float)rb; }
+
+// complex type of a real type
+template <class T> struct cplx_of;
+template <> struct cplx_of<double> { using type = double2; };
+template <> struct cplx_of<float> { using type = float2; };
+
 // One mixed-radix FFT length: factors (radix 4, 2, 3, 5) and its table
 // tw[q] = exp(-2 pi i q / n), q = 0..n-1, in device memory.
 constexpr int MAX_FAC = 16;
@@ -84,6 +99,7 @@ struct FftPlan {
     int nf;
     int fac[MAX_FAC];
     const double2 *tw;
+    const float2 *twf;   // the same table rounded to FP32 (mixed form)
 };
 
 struct Model {
@@ -113,6 +129,20 @@ struct Model {
     int cur[4] = {0, 0, 0, 0};
     // use the fused friction + Adams-Bashforth kernel where it applies
     bool fuse_friction = true;
+
+    // mixed form (PS3D_MIXED): prognostic fields as FP32 (hi, lo) pairs whose
+    // sum carries the FP64-accumulated state; FP32 tendencies, pressure and
+    // spectral array; FP64 Thomas elimination through dp64
+    bool mixed = false;
+    float *fh[4] = {}, *fl[4] = {};   // canonical (hi, lo) of u, v, w, b
+    float *ah[4] = {}, *al[4] = {};   // second buffers
+    float *tf[4] = {};                // tendencies du..db, 3 slots each
+    float *pf = nullptr;              // pressure
+    float2 *FSf = nullptr;            // spectral array
+    double2 *dp64 = nullptr;          // forward-sweep values of the Thomas solve
+    double *stage = nullptr;          // 3H doubles for host conversions
+    float2 *twf_r2c = nullptr;
+    bool lo_stale[4] = {};            // lo halo not exchanged since the last step
 
     // profiling of one kernel by name
     std::string prof_name;
@@ -148,9 +178,23 @@ void fft_c2r_x(Model &m, const double2 *src, double *dst, int dst_is_field, doub
 void fft_z_periodic_solve(Model &m, double2 *data);
 void fft_c2c_z(Model &m, double2 *data, int sign, double s1, double s2, double s3);
 
+void fft_r2c_x_f(Model &m, const float *u, const float *v, const float *w, float2 *dst);
+void fft_c2r_x_f(Model &m, const float2 *src, float *dst, double inv_scale);
+void fft_c2c_y_f(Model &m, float2 *data, int sign, double scale1, double scale2);
+void fft_z_periodic_solve_f(Model &m, float2 *data);
+bool fft_sizes_compiled(int n);
+std::string fft_sizes_text();
+
 // ---- launchers (tridiag.hip) --------------------------------------------
 void tridiag_factor(Model &m);
 void launch_thomas(Model &m, double2 *data);
+void launch_thomas_f(Model &m, float2 *data);
+
+// ---- mixed form (stencil.hip) -------------------------------------------
+void launch_project_exchange_f(Model &m);
+void launch_halo_f(Model &m, float *const *arrays, const int *neumann, int n);
+void launch_convert(Model &m, int dir, int64_t n, const double *src_d, double *dst_d, const float *src_f,
+                    float *dst_f, const float *src_lo, float *dst_lo);
 
 // ---- profiling helpers (api.hip) ----------------------------------------
 struct ProfScope {

@@ -10,6 +10,8 @@
 // instead of one per flux array.
 #include "ps3d_internal.h"
 
+#include <type_traits>
+
 #pragma clang fp contract(off)
 
 namespace ps3d {
@@ -131,15 +133,19 @@ constexpr int TPX = TTX + 2, TPY = TTY + 2;
 constexpr int TTHR = TTX * TTY;
 constexpr int TPER = (TPX * TPY + TTHR - 1) / TTHR;   // plane values per thread
 
+template <class T>
 struct TendArgs {
-    const double *u, *v, *w, *b;
-    double *du, *dv, *dw, *db;
+    const T *u, *v, *w, *b;
+    T *du, *dv, *dw, *db;
     int kc, nchunk, tiles_x, tiles_y, nblocks;
 };
 
-__global__ __launch_bounds__(TTHR) void k_tendency_lds(StepConsts c, TendArgs a)
+// T = double: the drop-in (bit-exact); T = float: the FP32 stencils of the
+// mixed form (same expressions, reciprocal multiplies instead of divisions)
+template <class T>
+__global__ __launch_bounds__(TTHR) void k_tendency_lds(StepConsts c, TendArgs<T> a)
 {
-    __shared__ double sp[4][3][TPY][TPX];
+    __shared__ T sp[4][3][TPY][TPX];
     int bid = (int)blockIdx.x;
     {
         const int per = a.nblocks / 8;   // XCD-contiguous runs of (chunk, tile)
@@ -153,9 +159,9 @@ __global__ __launch_bounds__(TTHR) void k_tendency_lds(StepConsts c, TendArgs a)
     const int ke = min(c.nz, kb + a.kc - 1);
     const int t = threadIdx.x;
     const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const double *fld[4] = {a.u, a.v, a.w, a.b};
+    const T *fld[4] = {a.u, a.v, a.w, a.b};
 
-    double pre[4][TPER];
+    T pre[4][TPER];
     auto fetch = [&](int k) {
 #pragma unroll
         for (int s = 0; s < TPER; ++s) {
@@ -165,7 +171,7 @@ __global__ __launch_bounds__(TTHR) void k_tendency_lds(StepConsts c, TendArgs a)
             const bool ok = q < TPX * TPY && ii <= nx + 2 && jj <= ny + 2;
             const int64_t o = idx(c, ii, jj, k);
 #pragma unroll
-            for (int f = 0; f < 4; ++f) pre[f][s] = ok ? fld[f][o] : 0.0;
+            for (int f = 0; f < 4; ++f) pre[f][s] = ok ? fld[f][o] : T(0);
         }
     };
     auto store = [&](int k) {
@@ -190,7 +196,9 @@ __global__ __launch_bounds__(TTHR) void k_tendency_lds(StepConsts c, TendArgs a)
     const int i = i0 + px, j = j0 + py;
     const int rx = px + 1, ry = py + 1;
     const bool lid = c.lid != 0;
-    const double Ro = c.Ro, dx = c.dx, dy = c.dy, dz = c.dz;
+    const T Ro = (T)c.Ro;
+    const double dx = c.dx, dy = c.dy, dz = c.dz;
+    const T h = T(0.5), q4 = T(0.25), zero = T(0);
     for (int k = kb; k <= ke; ++k) {
         store(k + 1);
         if (k + 2 <= nz + 2 && k + 1 <= ke) fetch(k + 2);
@@ -198,7 +206,7 @@ __global__ __launch_bounds__(TTHR) void k_tendency_lds(StepConsts c, TendArgs a)
         const int sm = (k - kb) % 3, sc = (k - kb + 1) % 3, sx = (k - kb + 2) % 3;
         if (i <= nx && j <= ny) {
             // value of field f at (i+di, j+dj, k+dk)
-            auto F = [&](int f, int dk, int dj, int di) -> double {
+            auto F = [&](int f, int dk, int dj, int di) -> T {
                 const int sl = dk < 0 ? sm : (dk > 0 ? sx : sc);
                 return sp[f][sl][ry + dj][rx + di];
             };
@@ -210,56 +218,56 @@ __global__ __launch_bounds__(TTHR) void k_tendency_lds(StepConsts c, TendArgs a)
             const bool bot0 = lid && (k - 1 <= 0);
             const bool top0w = lid && (k >= nz - 1);
             const int64_t o = idx(c, i, j, k);
-            double adv = 0.0, au = 0.0, av = 0.0, aw = 0.0;
+            T adv = zero, au = zero, av = zero, aw = zero;
             if (c.nonlinear) {
                 {
-                    const double fe1 = 0.5 * (B(0, 0, 0) + B(0, 0, 1)) * U(0, 0, 0);
-                    const double fe0 = 0.5 * (B(0, 0, -1) + B(0, 0, 0)) * U(0, 0, -1);
-                    const double fn1 = 0.5 * (B(0, 0, 0) + B(0, 1, 0)) * V(0, 0, 0);
-                    const double fn0 = 0.5 * (B(0, -1, 0) + B(0, 0, 0)) * V(0, -1, 0);
-                    const double ft1 = top0 ? 0.0 : 0.5 * (B(0, 0, 0) + B(1, 0, 0)) * W(0, 0, 0);
-                    const double ft0 = bot0 ? 0.0 : 0.5 * (B(-1, 0, 0) + B(0, 0, 0)) * W(-1, 0, 0);
-                    adv = -(Ro * (div_rn(fe1 - fe0, dx, c.rdx) + div_rn(fn1 - fn0, dy, c.rdy) +
-                                  div_rn(ft1 - ft0, dz, c.rdz)));
+                    const T fe1 = h * (B(0, 0, 0) + B(0, 0, 1)) * U(0, 0, 0);
+                    const T fe0 = h * (B(0, 0, -1) + B(0, 0, 0)) * U(0, 0, -1);
+                    const T fn1 = h * (B(0, 0, 0) + B(0, 1, 0)) * V(0, 0, 0);
+                    const T fn0 = h * (B(0, -1, 0) + B(0, 0, 0)) * V(0, -1, 0);
+                    const T ft1 = top0 ? zero : h * (B(0, 0, 0) + B(1, 0, 0)) * W(0, 0, 0);
+                    const T ft0 = bot0 ? zero : h * (B(-1, 0, 0) + B(0, 0, 0)) * W(-1, 0, 0);
+                    adv = -(Ro * (sdiv<T>(fe1 - fe0, dx, c.rdx) + sdiv<T>(fn1 - fn0, dy, c.rdy) +
+                                  sdiv<T>(ft1 - ft0, dz, c.rdz)));
                 }
                 {
-                    const double fe1 = 0.25 * (U(0, 0, 0) + U(0, 0, 1)) * (U(0, 0, 1) + U(0, 0, 0));
-                    const double fe0 = 0.25 * (U(0, 0, -1) + U(0, 0, 0)) * (U(0, 0, 0) + U(0, 0, -1));
-                    const double fn1 = 0.25 * (U(0, 0, 0) + U(0, 1, 0)) * (V(0, 0, 1) + V(0, 0, 0));
-                    const double fn0 = 0.25 * (U(0, -1, 0) + U(0, 0, 0)) * (V(0, -1, 1) + V(0, -1, 0));
-                    const double ft1 = top0 ? 0.0 : 0.25 * (U(1, 0, 0) + U(0, 0, 0)) * (W(0, 0, 0) + W(0, 0, 1));
-                    const double ft0 = bot0 ? 0.0 : 0.25 * (U(0, 0, 0) + U(-1, 0, 0)) * (W(-1, 0, 0) + W(-1, 0, 1));
-                    au = -(Ro * (div_rn(fe1 - fe0, dx, c.rdx) + div_rn(fn1 - fn0, dy, c.rdy) +
-                                 div_rn(ft1 - ft0, dz, c.rdz)));
+                    const T fe1 = q4 * (U(0, 0, 0) + U(0, 0, 1)) * (U(0, 0, 1) + U(0, 0, 0));
+                    const T fe0 = q4 * (U(0, 0, -1) + U(0, 0, 0)) * (U(0, 0, 0) + U(0, 0, -1));
+                    const T fn1 = q4 * (U(0, 0, 0) + U(0, 1, 0)) * (V(0, 0, 1) + V(0, 0, 0));
+                    const T fn0 = q4 * (U(0, -1, 0) + U(0, 0, 0)) * (V(0, -1, 1) + V(0, -1, 0));
+                    const T ft1 = top0 ? zero : q4 * (U(1, 0, 0) + U(0, 0, 0)) * (W(0, 0, 0) + W(0, 0, 1));
+                    const T ft0 = bot0 ? zero : q4 * (U(0, 0, 0) + U(-1, 0, 0)) * (W(-1, 0, 0) + W(-1, 0, 1));
+                    au = -(Ro * (sdiv<T>(fe1 - fe0, dx, c.rdx) + sdiv<T>(fn1 - fn0, dy, c.rdy) +
+                                 sdiv<T>(ft1 - ft0, dz, c.rdz)));
                 }
                 {
-                    const double fe1 = 0.25 * (V(0, 0, 0) + V(0, 0, 1)) * (U(0, 1, 0) + U(0, 0, 0));
-                    const double fe0 = 0.25 * (V(0, 0, -1) + V(0, 0, 0)) * (U(0, 1, -1) + U(0, 0, -1));
-                    const double fn1 = 0.25 * (V(0, 0, 0) + V(0, 1, 0)) * (V(0, 1, 0) + V(0, 0, 0));
-                    const double fn0 = 0.25 * (V(0, -1, 0) + V(0, 0, 0)) * (V(0, 0, 0) + V(0, -1, 0));
-                    const double ft1 = top0 ? 0.0 : 0.25 * (V(1, 0, 0) + V(0, 0, 0)) * (W(0, 0, 0) + W(0, 1, 0));
-                    const double ft0 = bot0 ? 0.0 : 0.25 * (V(0, 0, 0) + V(-1, 0, 0)) * (W(-1, 0, 0) + W(-1, 1, 0));
-                    av = -(Ro * (div_rn(fe1 - fe0, dx, c.rdx) + div_rn(fn1 - fn0, dy, c.rdy) +
-                                 div_rn(ft1 - ft0, dz, c.rdz)));
+                    const T fe1 = q4 * (V(0, 0, 0) + V(0, 0, 1)) * (U(0, 1, 0) + U(0, 0, 0));
+                    const T fe0 = q4 * (V(0, 0, -1) + V(0, 0, 0)) * (U(0, 1, -1) + U(0, 0, -1));
+                    const T fn1 = q4 * (V(0, 0, 0) + V(0, 1, 0)) * (V(0, 1, 0) + V(0, 0, 0));
+                    const T fn0 = q4 * (V(0, -1, 0) + V(0, 0, 0)) * (V(0, 0, 0) + V(0, -1, 0));
+                    const T ft1 = top0 ? zero : q4 * (V(1, 0, 0) + V(0, 0, 0)) * (W(0, 0, 0) + W(0, 1, 0));
+                    const T ft0 = bot0 ? zero : q4 * (V(0, 0, 0) + V(-1, 0, 0)) * (W(-1, 0, 0) + W(-1, 1, 0));
+                    av = -(Ro * (sdiv<T>(fe1 - fe0, dx, c.rdx) + sdiv<T>(fn1 - fn0, dy, c.rdy) +
+                                 sdiv<T>(ft1 - ft0, dz, c.rdz)));
                 }
                 {
-                    const double fe1 = 0.25 * (W(0, 0, 0) + W(0, 0, 1)) * (U(0, 0, 0) + U(1, 0, 0));
-                    const double fe0 = 0.25 * (W(0, 0, -1) + W(0, 0, 0)) * (U(0, 0, -1) + U(1, 0, -1));
-                    const double fn1 = 0.25 * (W(0, 0, 0) + W(0, 1, 0)) * (V(0, 0, 0) + V(1, 0, 0));
-                    const double fn0 = 0.25 * (W(0, -1, 0) + W(0, 0, 0)) * (V(0, -1, 0) + V(1, -1, 0));
-                    const double ft1 = top0w ? 0.0 : 0.25 * (W(1, 0, 0) + W(0, 0, 0)) * (W(0, 0, 0) + W(1, 0, 0));
-                    const double ft0 = (bot0 || (lid && k - 1 >= nz - 1))
-                                           ? 0.0
-                                           : 0.25 * (W(0, 0, 0) + W(-1, 0, 0)) * (W(-1, 0, 0) + W(0, 0, 0));
-                    aw = -((c.dsqr * Ro) * (div_rn(fe1 - fe0, dx, c.rdx) + div_rn(fn1 - fn0, dy, c.rdy) +
-                                            div_rn(ft1 - ft0, dz, c.rdz)));
+                    const T fe1 = q4 * (W(0, 0, 0) + W(0, 0, 1)) * (U(0, 0, 0) + U(1, 0, 0));
+                    const T fe0 = q4 * (W(0, 0, -1) + W(0, 0, 0)) * (U(0, 0, -1) + U(1, 0, -1));
+                    const T fn1 = q4 * (W(0, 0, 0) + W(0, 1, 0)) * (V(0, 0, 0) + V(1, 0, 0));
+                    const T fn0 = q4 * (W(0, -1, 0) + W(0, 0, 0)) * (V(0, -1, 0) + V(1, -1, 0));
+                    const T ft1 = top0w ? zero : q4 * (W(1, 0, 0) + W(0, 0, 0)) * (W(0, 0, 0) + W(1, 0, 0));
+                    const T ft0 = (bot0 || (lid && k - 1 >= nz - 1))
+                                           ? zero
+                                           : q4 * (W(0, 0, 0) + W(-1, 0, 0)) * (W(-1, 0, 0) + W(0, 0, 0));
+                    aw = -(((T)c.dsqr * Ro) * (sdiv<T>(fe1 - fe0, dx, c.rdx) + sdiv<T>(fn1 - fn0, dy, c.rdy) +
+                                            sdiv<T>(ft1 - ft0, dz, c.rdz)));
                 }
             }
-            adv = adv - 0.5 * (W(0, 0, 0) + W(-1, 0, 0)) * (c.N0 * c.N0);
-            au = au + (V(0, 0, 0) + V(0, -1, 0) + V(0, 0, 1) + V(0, -1, 1)) * 0.25 * c.f0;
-            av = av - (U(0, 0, -1) + U(0, 0, 0) + U(0, 1, -1) + U(0, 1, 0)) * 0.25 * c.f0;
-            aw = aw + 0.5 * (B(0, 0, 0) + B(1, 0, 0));
-            if (lid && k >= nz) aw = 0.0;
+            adv = adv - h * (W(0, 0, 0) + W(-1, 0, 0)) * (T)(c.N0 * c.N0);
+            au = au + (V(0, 0, 0) + V(0, -1, 0) + V(0, 0, 1) + V(0, -1, 1)) * q4 * (T)c.f0;
+            av = av - (U(0, 0, -1) + U(0, 0, 0) + U(0, 1, -1) + U(0, 1, 0)) * q4 * (T)c.f0;
+            aw = aw + h * (B(0, 0, 0) + B(1, 0, 0));
+            if (lid && k >= nz) aw = zero;
             a.db[o] = adv;
             a.du[o] = au;
             a.dv[o] = av;
@@ -407,27 +415,36 @@ constexpr int R0X = FTX + 6, R0Y = FTY + 6;
 constexpr int R1X = FTX + 4, R1Y = FTY + 4;
 constexpr int FTHREADS = FTX * FTY;
 
+template <class T>
 struct FricField {
-    const double *in;
-    double *out;
-    const double *t0, *t1, *t2;   // tendency slots tau, taum1, taum2
+    const T *in;                  // f0: the FP64 field, or the FP32 hi part (mixed form)
+    const T *in_lo;               // FP32 lo part (mixed form), else null
+    T *out, *out_lo;
+    const T *t0, *t1, *t2;        // tendency slots tau, taum1, taum2
     double Hm, Vm, Hbi;           // harmonic (H, V) and biharmonic_h coefficients
     double scale;                 // dt, or dt/dsqr for w
     int harm, bih, zero_top;
 };
 
+template <class T>
 struct FricArgs {
-    FricField f[4];
+    FricField<T> f[4];
     double ca, cb, cc;
     int order;
     int nchunk, kc, tiles_x, tiles_y, nblocks;
 };
 
-__global__ __launch_bounds__(FTHREADS) void k_fric_ab(StepConsts c, FricArgs a)
+// T = double: the drop-in, bit-exact.  T = float: the mixed form -- stencils
+// in FP32, and the new state (hi + lo) + harmonic increment + biharmonic
+// increment + AB increment summed in FP64 and split back into (hi, lo).
+template <class T>
+__global__ __launch_bounds__(FTHREADS) void k_fric_ab(StepConsts c, FricArgs<T> a)
 {
-    __shared__ double f0p[3][R0Y][R0X];
-    __shared__ double f1p[R1Y][R1X];
-    __shared__ double d2p[FTY + 2][FTX + 2];   // biharmonic_h's del2 on the tile + 1
+    constexpr bool MIXED = std::is_same<T, float>::value;
+    __shared__ T f0p[3][R0Y][R0X];
+    __shared__ T f1p[R1Y][R1X];
+    __shared__ T d2p[FTY + 2][FTX + 2];   // biharmonic_h's del2 on the tile + 1
+    __shared__ T hinc[MIXED ? FTY : 1][MIXED ? FTX : 1];   // harmonic increment at tile points
     // blocks are dealt round-robin to the 8 XCDs; renumber so that each XCD
     // walks a contiguous run of tiles (neighbours share apron rows in its L2)
     int bid = (int)blockIdx.x;
@@ -440,7 +457,7 @@ __global__ __launch_bounds__(FTHREADS) void k_fric_ab(StepConsts c, FricArgs a)
     const int rest = bid / ntile;
     const int ch = rest % a.nchunk;
     const int fi = rest / a.nchunk;
-    const FricField &F = a.f[fi];
+    const FricField<T> &F = a.f[fi];
     const int i0 = (tile / a.tiles_y) * FTX + 1;
     const int j0 = (tile % a.tiles_y) * FTY + 1;
     const int kb = 1 + ch * a.kc;
@@ -448,11 +465,13 @@ __global__ __launch_bounds__(FTHREADS) void k_fric_ab(StepConsts c, FricArgs a)
     const int t = threadIdx.x;
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const bool lid = c.lid != 0;
-    const double dx = c.dx, dy = c.dy, dz = c.dz, rdx = c.rdx, rdy = c.rdy, rdz = c.rdz, dt = c.dt;
+    const double dx = c.dx, dy = c.dy, dz = c.dz, rdx = c.rdx, rdy = c.rdy, rdz = c.rdz;
+    const T dt = (T)c.dt;
+    const T zero = T(0);
 
     auto slot = [&](int k) { return (k - kb + 1) % 3; };
     constexpr int PER = (R0X * R0Y + FTHREADS - 1) / FTHREADS;
-    double pre[PER];
+    T pre[PER];
     auto fetch = [&](int k) {
 #pragma unroll
         for (int s = 0; s < PER; ++s) {
@@ -460,7 +479,7 @@ __global__ __launch_bounds__(FTHREADS) void k_fric_ab(StepConsts c, FricArgs a)
             const int rx = q % R0X, ry = q / R0X;
             const int ii = i0 - 3 + rx, jj = j0 - 3 + ry;
             const bool ok = q < R0X * R0Y && ii >= -1 && ii <= nx + 2 && jj >= -1 && jj <= ny + 2;
-            pre[s] = ok ? F.in[idx(c, ii, jj, k)] : 0.0;
+            pre[s] = ok ? F.in[idx(c, ii, jj, k)] : zero;
         }
     };
     auto store = [&](int k) {
@@ -482,13 +501,14 @@ __global__ __launch_bounds__(FTHREADS) void k_fric_ab(StepConsts c, FricArgs a)
     for (int k = kb; k <= ke; ++k) {
         store(k + 1);
         if (k + 1 <= ke) fetch(k + 2);
-        // this level's tendency slots, issued before the LDS phases
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        // this level's tendency slots (and lo part), issued before the LDS phases
+        T s0 = zero, s1 = zero, s2 = zero, lo0 = zero;
         if (mine) {
             const int64_t o = idx(c, i0 + tpx, j0 + tpy, k);
             s0 = F.t0[o];
             if (a.order >= 2) s1 = F.t1[o];
             if (a.order >= 3) s2 = F.t2[o];
+            if (MIXED) lo0 = F.in_lo[o];
         }
         __syncthreads();
         const int sm = slot(k - 1), sc = slot(k), sp = slot(k + 1);
@@ -496,57 +516,73 @@ __global__ __launch_bounds__(FTHREADS) void k_fric_ab(StepConsts c, FricArgs a)
             const int rx = q % R1X, ry = q / R1X;
             const int ii = i0 - 2 + rx, jj = j0 - 2 + ry;
             const int ox = rx + 1, oy = ry + 1;
-            const double v0 = f0p[sc][oy][ox];
-            double f1 = v0;
+            const T v0 = f0p[sc][oy][ox];
+            T f1 = v0, inc = zero;
             if (F.harm && ii >= 1 && ii <= nx && jj >= 1 && jj <= ny) {
-                const double Hm = F.Hm, Vm = F.Vm;
-                const double fe1 = div_rn(Hm * (f0p[sc][oy][ox + 1] - v0), dx, rdx);
-                const double fe0 = div_rn(Hm * (v0 - f0p[sc][oy][ox - 1]), dx, rdx);
-                const double fn1 = div_rn(Hm * (f0p[sc][oy + 1][ox] - v0), dy, rdy);
-                const double fn0 = div_rn(Hm * (v0 - f0p[sc][oy - 1][ox]), dy, rdy);
-                const double ft1 = (lid && k >= nz) ? 0.0 : div_rn(Vm * (f0p[sp][oy][ox] - v0), dz, rdz);
-                const double ft0 = (lid && k - 1 <= 0) ? 0.0 : div_rn(Vm * (v0 - f0p[sm][oy][ox]), dz, rdz);
-                f1 = v0 + dt * (div_rn(fe1 - fe0, dx, rdx) + div_rn(fn1 - fn0, dy, rdy) +
-                                div_rn(ft1 - ft0, dz, rdz));
+                const T Hm = (T)F.Hm, Vm = (T)F.Vm;
+                const T fe1 = sdiv<T>(Hm * (f0p[sc][oy][ox + 1] - v0), dx, rdx);
+                const T fe0 = sdiv<T>(Hm * (v0 - f0p[sc][oy][ox - 1]), dx, rdx);
+                const T fn1 = sdiv<T>(Hm * (f0p[sc][oy + 1][ox] - v0), dy, rdy);
+                const T fn0 = sdiv<T>(Hm * (v0 - f0p[sc][oy - 1][ox]), dy, rdy);
+                const T ft1 = (lid && k >= nz) ? zero : sdiv<T>(Vm * (f0p[sp][oy][ox] - v0), dz, rdz);
+                const T ft0 = (lid && k - 1 <= 0) ? zero : sdiv<T>(Vm * (v0 - f0p[sm][oy][ox]), dz, rdz);
+                inc = dt * (sdiv<T>(fe1 - fe0, dx, rdx) + sdiv<T>(fn1 - fn0, dy, rdy) + sdiv<T>(ft1 - ft0, dz, rdz));
+                f1 = v0 + inc;
             }
-            if (F.zero_top && k >= nz) f1 = 0.0;
+            if (F.zero_top && k >= nz) f1 = zero;
             f1p[ry][rx] = f1;
+            if (MIXED && rx >= 2 && rx < FTX + 2 && ry >= 2 && ry < FTY + 2) hinc[ry - 2][rx - 2] = inc;
         }
         __syncthreads();
         if (F.bih) {
             // del2 = (f1e(i) - f1e(i-1))/dx + (f1n(j) - f1n(j-1))/dy, each point once
-            const double Hb = F.Hbi;
+            const T Hb = (T)F.Hbi;
             for (int q = t; q < (FTX + 2) * (FTY + 2); q += FTHREADS) {
                 const int xx = q % (FTX + 2) + 1, yy = q / (FTX + 2) + 1;   // f1p coordinates
-                const double f1e1 = -div_rn(Hb * (f1p[yy][xx + 1] - f1p[yy][xx]), dx, rdx);
-                const double f1e0 = -div_rn(Hb * (f1p[yy][xx] - f1p[yy][xx - 1]), dx, rdx);
-                const double f1n1 = -div_rn(Hb * (f1p[yy + 1][xx] - f1p[yy][xx]), dy, rdy);
-                const double f1n0 = -div_rn(Hb * (f1p[yy][xx] - f1p[yy - 1][xx]), dy, rdy);
-                d2p[yy - 1][xx - 1] = div_rn(f1e1 - f1e0, dx, rdx) + div_rn(f1n1 - f1n0, dy, rdy);
+                const T f1e1 = -sdiv<T>(Hb * (f1p[yy][xx + 1] - f1p[yy][xx]), dx, rdx);
+                const T f1e0 = -sdiv<T>(Hb * (f1p[yy][xx] - f1p[yy][xx - 1]), dx, rdx);
+                const T f1n1 = -sdiv<T>(Hb * (f1p[yy + 1][xx] - f1p[yy][xx]), dy, rdy);
+                const T f1n0 = -sdiv<T>(Hb * (f1p[yy][xx] - f1p[yy - 1][xx]), dy, rdy);
+                d2p[yy - 1][xx - 1] = sdiv<T>(f1e1 - f1e0, dx, rdx) + sdiv<T>(f1n1 - f1n0, dy, rdy);
             }
             __syncthreads();
         }
-        {
-            const int px = t % FTX, py = t / FTX;
+        if (mine) {
+            const int px = tpx, py = tpy;
             const int i = i0 + px, j = j0 + py;
-            if (i <= nx && j <= ny) {
-                const int cx = px + 2, cy = py + 2;
-                double f2 = f1p[cy][cx];
-                if (F.bih) {
-                    const int ex = px + 1, ey = py + 1;   // d2p coordinates
-                    const double dc = d2p[ey][ex], de = d2p[ey][ex + 1], dwv = d2p[ey][ex - 1];
-                    const double dn = d2p[ey + 1][ex], ds = d2p[ey - 1][ex];
-                    const double fe1 = div_rn(de - dc, dx, rdx), fe0 = div_rn(dc - dwv, dx, rdx);
-                    const double fn1 = div_rn(dn - dc, dy, rdy), fn0 = div_rn(dc - ds, dy, rdy);
-                    f2 = f2 + dt * (div_rn(fe1 - fe0, dx, rdx) + div_rn(fn1 - fn0, dy, rdy));
-                }
-                const int64_t o = idx(c, i, j, k);
+            const int cx = px + 2, cy = py + 2;
+            T f2 = f1p[cy][cx], binc = zero;
+            if (F.bih) {
+                const int ex = px + 1, ey = py + 1;   // d2p coordinates
+                const T dc = d2p[ey][ex], de = d2p[ey][ex + 1], dwv = d2p[ey][ex - 1];
+                const T dn = d2p[ey + 1][ex], ds = d2p[ey - 1][ex];
+                const T fe1 = sdiv<T>(de - dc, dx, rdx), fe0 = sdiv<T>(dc - dwv, dx, rdx);
+                const T fn1 = sdiv<T>(dn - dc, dy, rdy), fn0 = sdiv<T>(dc - ds, dy, rdy);
+                binc = dt * (sdiv<T>(fe1 - fe0, dx, rdx) + sdiv<T>(fn1 - fn0, dy, rdy));
+                f2 = f2 + binc;
+            }
+            const int64_t o = idx(c, i, j, k);
+            if constexpr (!MIXED) {
                 const double s = F.scale;
                 double val;
                 if (a.order == 1) val = f2 + s * s0;
                 else if (a.order == 2) val = f2 + s * (a.ca * s0 - a.cb * s1);
                 else val = f2 + s * (a.ca * s0 + a.cb * s1 + a.cc * s2);
                 F.out[o] = val;
+            } else {
+                const double s = F.scale;
+                double ab;
+                if (a.order == 1) ab = s * (double)s0;
+                else if (a.order == 2) ab = s * (a.ca * (double)s0 - a.cb * (double)s1);
+                else ab = s * (a.ca * (double)s0 + a.cb * (double)s1 + a.cc * (double)s2);
+                // the harmonic pass zeroes w(nz) under a rigid lid before the
+                // biharmonic and AB increments are added
+                const bool zt = F.zero_top && k >= nz;
+                const double base = zt ? 0.0 : (double)f0p[sc][cy + 1][cx + 1] + (double)lo0;
+                const double x = base + ((zt ? 0.0 : (double)hinc[py][px]) + (double)binc + ab);
+                const float hi = (float)x;
+                F.out[o] = hi;
+                F.out_lo[o] = (float)(x - (double)hi);
             }
         }
         __syncthreads();
@@ -561,16 +597,19 @@ __global__ __launch_bounds__(FTHREADS) void k_fric_ab(StepConsts c, FricArgs a)
 // level, 4 x-points per interior row.  neumann_z: pressure_solver.f90:82-93
 // (copies of levels 1 and nz into the z-halo instead of periodic images).
 // ---------------------------------------------------------------------------
-struct HaloArgs {
-    double *f[5];
-    int neumann[5];
+template <class T>
+struct HaloArgsT {
+    T *f[8];
+    int neumann[8];
 };
+using HaloArgs = HaloArgsT<double>;
 
-__global__ __launch_bounds__(256) void k_halo(StepConsts c, HaloArgs h, int64_t nhalo)
+template <class T>
+__global__ __launch_bounds__(256) void k_halo(StepConsts c, HaloArgsT<T> h, int64_t nhalo)
 {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= nhalo) return;
-    double *a = h.f[blockIdx.y];
+    T *a = h.f[blockIdx.y];
     const int neu = h.neumann[blockIdx.y];
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const int64_t X = c.X, Y = ny + 4;
@@ -623,6 +662,56 @@ __global__ __launch_bounds__(BX * BY) void k_project(StepConsts c, double *__res
     else w[o] = w[o] - div_rn((c.dt / c.dsqr) * (pt - pc), c.dz, c.rdz);
 }
 
+// projection of the mixed form: p in FP32, the state update accumulated in
+// FP64 on (hi + lo) and split back
+__global__ __launch_bounds__(BX * BY) void k_project_f(StepConsts c, float *__restrict__ uh, float *__restrict__ ul,
+                                                        float *__restrict__ vh, float *__restrict__ vl,
+                                                        float *__restrict__ wh, float *__restrict__ wl,
+                                                        const float *__restrict__ p)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > c.nx || j > c.ny) return;
+    const int64_t o = idx(c, i, j, k);
+    const double pc = p[o];
+    const double pe = p[idx(c, i == c.nx ? 1 : i + 1, j, k)];
+    const double pn = p[idx(c, i, j == c.ny ? 1 : j + 1, k)];
+    const double pt = p[idx(c, i, j, k == c.nz ? (c.lid ? c.nz : 1) : k + 1)];
+    auto put = [&](float *h, float *l, double x) {
+        const float hi = (float)x;
+        h[o] = hi;
+        l[o] = (float)(x - (double)hi);
+    };
+    put(uh, ul, ((double)uh[o] + (double)ul[o]) - c.dt * (pe - pc) * c.rdx);
+    put(vh, vl, ((double)vh[o] + (double)vl[o]) - c.dt * (pn - pc) * c.rdy);
+    if (c.lid && k >= c.nz) put(wh, wl, 0.0);
+    else put(wh, wl, ((double)wh[o] + (double)wl[o]) - (c.dt / c.dsqr) * (pt - pc) * c.rdz);
+}
+
+// host-layout conversions of the mixed form: 0 split FP64 -> (hi, lo),
+// 1 join (hi, lo) -> FP64, 2 FP64 -> FP32, 3 FP32 -> FP64
+__global__ __launch_bounds__(256) void k_convert(int dir, int64_t n, const double *__restrict__ sd,
+                                                 double *__restrict__ dd, const float *__restrict__ sf,
+                                                 float *__restrict__ df, const float *__restrict__ slo,
+                                                 float *__restrict__ dlo)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    if (dir == 0) {
+        const double x = sd[t];
+        const float hi = (float)x;
+        df[t] = hi;
+        dlo[t] = (float)(x - (double)hi);
+    } else if (dir == 1) {
+        dd[t] = (double)sf[t] + (double)slo[t];
+    } else if (dir == 2) {
+        df[t] = (float)sd[t];
+    } else {
+        dd[t] = (double)sf[t];
+    }
+}
+
 inline dim3 interior_grid(const Grid &g, int zmult = 1)
 {
     return dim3((unsigned)((g.nx + BX - 1) / BX), (unsigned)((g.ny + BY - 1) / BY), (unsigned)(g.nz * zmult));
@@ -660,14 +749,15 @@ inline dim3 march_grid(const Grid &g, int &kc)
     return dim3((unsigned)((g.nx + BX - 1) / BX), (unsigned)((g.ny + BY - 1) / BY), (unsigned)nch);
 }
 
-void launch_tendency(Model &m, int tau)
+template <class T>
+static void tendency_impl(Model &m, const T *u, const T *v, const T *w, const T *b, T *const *tend, int tau)
 {
     ProfScope ps(m, "tendency");
     const Grid &g = m.g;
     const int64_t off = (int64_t)(tau - 1) * g.H;
-    TendArgs a{};
-    a.u = m.u; a.v = m.v; a.w = m.w; a.b = m.b;
-    a.du = m.du + off; a.dv = m.dv + off; a.dw = m.dw + off; a.db = m.db + off;
+    TendArgs<T> a{};
+    a.u = u; a.v = v; a.w = w; a.b = b;
+    a.du = tend[0] + off; a.dv = tend[1] + off; a.dw = tend[2] + off; a.db = tend[3] + off;
     a.tiles_x = (g.nx + TTX - 1) / TTX;
     a.tiles_y = (g.ny + TTY - 1) / TTY;
     const int tiles = a.tiles_x * a.tiles_y;
@@ -678,8 +768,18 @@ void launch_tendency(Model &m, int tau)
     a.kc = (g.nz + nchunk - 1) / nchunk;
     a.nchunk = (g.nz + a.kc - 1) / a.kc;
     a.nblocks = tiles * a.nchunk;
-    hipLaunchKernelGGL(k_tendency_lds, dim3((unsigned)a.nblocks), dim3(TTHR), 0, m.stream, m.c, a);
+    hipLaunchKernelGGL(k_tendency_lds<T>, dim3((unsigned)a.nblocks), dim3(TTHR), 0, m.stream, m.c, a);
     PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void launch_tendency(Model &m, int tau)
+{
+    if (m.mixed) {
+        tendency_impl<float>(m, m.fh[0], m.fh[1], m.fh[2], m.fh[3], m.tf, tau);
+    } else {
+        double *tend[4] = {m.du, m.dv, m.dw, m.db};
+        tendency_impl<double>(m, m.u, m.v, m.w, m.b, tend, tau);
+    }
 }
 
 void launch_buoyancy_advection(Model &m, int tau)
@@ -793,7 +893,7 @@ void launch_ab_exchange(Model &m, int itt, int tau, int taum1, int taum2)
     const Grid &g = m.g;
     const int64_t nhalo = g.H - g.N;
     ProfScope ps(m, "halo");
-    hipLaunchKernelGGL(k_halo, dim3((unsigned)((nhalo + 255) / 256), 4), dim3(256), 0, m.stream, m.c, h, nhalo);
+    hipLaunchKernelGGL(k_halo<double>, dim3((unsigned)((nhalo + 255) / 256), 4), dim3(256), 0, m.stream, m.c, h, nhalo);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
@@ -806,16 +906,19 @@ bool fused_friction_ok(const Model &m)
 // the friction chain and AB of all four fields in one launch, then the halo
 // exchange.  The new fields are written to the second buffers, which then
 // become the canonical ones.
-void launch_friction_ab(Model &m, int itt, int tau, int taum1, int taum2)
+template <class T>
+static void friction_ab_impl(Model &m, const T *const *in, const T *const *in_lo, T *const *out, T *const *out_lo,
+                             T *const *tend, int itt, int tau, int taum1, int taum2)
 {
     const ps3d_params &p = m.prm;
     const Grid &g = m.g;
-    FricArgs a{};
-    double *tend[4] = {m.du, m.dv, m.dw, m.db};
+    FricArgs<T> a{};
     for (int f = 0; f < 4; ++f) {
-        FricField &F = a.f[f];
-        F.in = m.field_buf(f);
-        F.out = m.alt[f];
+        FricField<T> &F = a.f[f];
+        F.in = in[f];
+        F.in_lo = in_lo ? in_lo[f] : nullptr;
+        F.out = out[f];
+        F.out_lo = out_lo ? out_lo[f] : nullptr;
         F.t0 = tend[f] + (int64_t)(tau - 1) * g.H;
         F.t1 = tend[f] + (int64_t)(taum1 - 1) * g.H;
         F.t2 = tend[f] + (int64_t)(taum2 - 1) * g.H;
@@ -855,21 +958,50 @@ void launch_friction_ab(Model &m, int itt, int tau, int taum1, int taum2)
     a.kc = (g.nz + nchunk - 1) / nchunk;
     a.nchunk = (g.nz + a.kc - 1) / a.kc;
     a.nblocks = tiles * a.nchunk * 4;
-    {
-        ProfScope ps(m, "friction_ab");
-        hipLaunchKernelGGL(k_fric_ab, dim3((unsigned)a.nblocks), dim3(FTHREADS), 0, m.stream, m.c, a);
-        PS3D_HIP_CHECK(hipGetLastError());
+    ProfScope ps(m, "friction_ab");
+    hipLaunchKernelGGL(k_fric_ab<T>, dim3((unsigned)a.nblocks), dim3(FTHREADS), 0, m.stream, m.c, a);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+template <class T>
+static void halo_launch(Model &m, const HaloArgsT<T> &h, int n)
+{
+    const int64_t nhalo = m.g.H - m.g.N;
+    ProfScope ps(m, "halo");
+    hipLaunchKernelGGL(k_halo<T>, dim3((unsigned)((nhalo + 255) / 256), (unsigned)n), dim3(256), 0, m.stream, m.c,
+                       h, nhalo);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+// integrate.f90:59-111 for configs without vertical biharmonic friction:
+// the friction chain and AB of all four fields in one launch, then the halo
+// exchange.  The new fields are written to the second buffers, which then
+// become the canonical ones.
+void launch_friction_ab(Model &m, int itt, int tau, int taum1, int taum2)
+{
+    if (m.mixed) {
+        friction_ab_impl<float>(m, m.fh, m.fl, m.ah, m.al, m.tf, itt, tau, taum1, taum2);
+        for (int f = 0; f < 4; ++f) {
+            std::swap(m.fh[f], m.ah[f]);
+            std::swap(m.fl[f], m.al[f]);
+        }
+        // the stencils read the hi parts only; the lo parts' halos are
+        // rebuilt when a field is copied out
+        HaloArgsT<float> h{};
+        for (int f = 0; f < 4; ++f) h.f[f] = m.fh[f];
+        halo_launch(m, h, 4);
+        return;
     }
+    double *in[4] = {m.u, m.v, m.w, m.b};
+    double *tend[4] = {m.du, m.dv, m.dw, m.db};
+    friction_ab_impl<double>(m, in, nullptr, m.alt, nullptr, tend, itt, tau, taum1, taum2);
     std::swap(m.u, m.alt[0]);
     std::swap(m.v, m.alt[1]);
     std::swap(m.w, m.alt[2]);
     std::swap(m.b, m.alt[3]);
     HaloArgs h{};
     for (int f = 0; f < 4; ++f) { h.f[f] = m.field_buf(f); h.neumann[f] = 0; }
-    const int64_t nhalo = g.H - g.N;
-    ProfScope ps(m, "halo");
-    hipLaunchKernelGGL(k_halo, dim3((unsigned)((nhalo + 255) / 256), 4), dim3(256), 0, m.stream, m.c, h, nhalo);
-    PS3D_HIP_CHECK(hipGetLastError());
+    halo_launch(m, h, 4);
 }
 
 void launch_border_exchg(Model &m, double *a, int neumann_z)
@@ -879,7 +1011,7 @@ void launch_border_exchg(Model &m, double *a, int neumann_z)
     h.neumann[0] = neumann_z;
     const int64_t nhalo = m.g.H - m.g.N;
     ProfScope ps(m, "halo");
-    hipLaunchKernelGGL(k_halo, dim3((unsigned)((nhalo + 255) / 256), 1), dim3(256), 0, m.stream, m.c, h, nhalo);
+    hipLaunchKernelGGL(k_halo<double>, dim3((unsigned)((nhalo + 255) / 256), 1), dim3(256), 0, m.stream, m.c, h, nhalo);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
@@ -896,7 +1028,38 @@ void launch_project_exchange(Model &m)
     h.neumann[3] = m.prm.enable_vertical_boundaries;
     const int64_t nhalo = m.g.H - m.g.N;
     ProfScope ps(m, "halo");
-    hipLaunchKernelGGL(k_halo, dim3((unsigned)((nhalo + 255) / 256), 4), dim3(256), 0, m.stream, m.c, h, nhalo);
+    hipLaunchKernelGGL(k_halo<double>, dim3((unsigned)((nhalo + 255) / 256), 4), dim3(256), 0, m.stream, m.c, h, nhalo);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+// pressure_solver.f90:98-118 of the mixed form
+void launch_project_exchange_f(Model &m)
+{
+    {
+        ProfScope ps(m, "project");
+        hipLaunchKernelGGL(k_project_f, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, m.fh[0], m.fl[0],
+                           m.fh[1], m.fl[1], m.fh[2], m.fl[2], m.pf);
+        PS3D_HIP_CHECK(hipGetLastError());
+    }
+    HaloArgsT<float> h{};
+    h.f[0] = m.fh[0]; h.f[1] = m.fh[1]; h.f[2] = m.fh[2]; h.f[3] = m.pf;
+    h.neumann[3] = m.prm.enable_vertical_boundaries;
+    halo_launch(m, h, 4);
+}
+
+void launch_halo_f(Model &m, float *const *arrays, const int *neumann, int n)
+{
+    PS3D_REQUIRE(n >= 1 && n <= 8, "halo exchange of 1..8 arrays");
+    HaloArgsT<float> h{};
+    for (int q = 0; q < n; ++q) { h.f[q] = arrays[q]; h.neumann[q] = neumann ? neumann[q] : 0; }
+    halo_launch(m, h, n);
+}
+
+void launch_convert(Model &m, int dir, int64_t n, const double *src_d, double *dst_d, const float *src_f,
+                    float *dst_f, const float *src_lo, float *dst_lo)
+{
+    hipLaunchKernelGGL(k_convert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, dir, n, src_d, dst_d,
+                       src_f, dst_f, src_lo, dst_lo);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 

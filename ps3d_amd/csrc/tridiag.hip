@@ -137,6 +137,86 @@ __global__ __launch_bounds__(256) void k_divide_ksqr(double2 *__restrict__ fs, c
     fs[n] = x;
 }
 
+// Mixed form: the right-hand side and the solution are FP32, the
+// elimination runs in FP64 with the same factors, and the forward-sweep
+// values d'(k) stay in FP64 -- in LDS when a block's 64 columns fit
+// (nz <= 64: 64 KiB), else in the dp64 scratch array.
+template <bool IN_LDS>
+__global__ __launch_bounds__(64) void k_thomas_f(float2 *__restrict__ fs, double2 *__restrict__ dpg,
+                                                 const double *__restrict__ ksqr, const double *__restrict__ cp,
+                                                 const double *__restrict__ rm, int64_t ncols, int nz, double fxa,
+                                                 double dz2, double dsqr)
+{
+    extern __shared__ double2 dps[];
+    const int64_t col = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (col >= ncols) return;
+    float2 *d = fs + col;
+    double2 *dp_ = IN_LDS ? dps + threadIdx.x : dpg + col;
+    const int64_t dstr = IN_LDS ? 64 : ncols;
+    const double ks = ksqr[col];
+    if (ks > 0) {
+        constexpr int LOOK = 16;
+        const float2 d0 = d[0];
+        const double b1 = rm[col];
+        double2 dp = make_double2((double)d0.x / b1, (double)d0.y / b1);
+        dp_[0] = dp;
+        for (int k0 = 1; k0 < nz; k0 += LOOK) {
+            float2 dd[LOOK];
+            double rr[LOOK];
+#pragma unroll
+            for (int q = 0; q < LOOK; ++q) {
+                if (k0 + q < nz) {
+                    const int64_t o = (int64_t)(k0 + q) * ncols;
+                    dd[q] = d[o];
+                    rr[q] = rm[col + o];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < LOOK; ++q) {
+                if (k0 + q < nz) {
+                    dp = make_double2(((double)dd[q].x - dp.x * fxa) * rr[q], ((double)dd[q].y - dp.y * fxa) * rr[q]);
+                    dp_[(int64_t)(k0 + q) * dstr] = dp;
+                }
+            }
+        }
+        double2 x = dp;
+        d[(int64_t)(nz - 1) * ncols] = make_float2((float)x.x, (float)x.y);
+        for (int k0 = nz - 2; k0 >= 0; k0 -= LOOK) {
+            double cc[LOOK];
+#pragma unroll
+            for (int q = 0; q < LOOK; ++q)
+                if (k0 - q >= 0) cc[q] = cp[col + (int64_t)(k0 - q) * ncols];
+#pragma unroll
+            for (int q = 0; q < LOOK; ++q) {
+                if (k0 - q >= 0) {
+                    const double2 e = dp_[(int64_t)(k0 - q) * dstr];
+                    x = make_double2(e.x - cc[q] * x.x, e.y - cc[q] * x.y);
+                    d[(int64_t)(k0 - q) * ncols] = make_float2((float)x.x, (float)x.y);
+                }
+            }
+        }
+    } else {
+        // zero mode, pressure_solver.f90:44-50, accumulated in FP64
+        float2 bk = d[0];
+        double2 pm1 = make_double2(0.0, 0.0);
+        d[0] = make_float2(0.f, 0.f);
+        float2 nxt = d[ncols];
+        double2 pk = make_double2((double)bk.x * dz2 * dsqr, (double)bk.y * dz2 * dsqr);
+        d[ncols] = make_float2((float)pk.x, (float)pk.y);
+        bk = nxt;
+        for (int k = 2; k < nz; ++k) {
+            const int64_t o = (int64_t)k * ncols;
+            nxt = d[o];
+            const double2 pn = make_double2(2 * pk.x - pm1.x + (double)bk.x * dz2 * dsqr,
+                                            2 * pk.y - pm1.y + (double)bk.y * dz2 * dsqr);
+            d[o] = make_float2((float)pn.x, (float)pn.y);
+            pm1 = pk;
+            pk = pn;
+            bk = nxt;
+        }
+    }
+}
+
 }  // namespace
 
 void tridiag_factor(Model &m)
@@ -163,6 +243,24 @@ void launch_divide_ksqr(Model &m, double2 *data)
 {
     hipLaunchKernelGGL(k_divide_ksqr, dim3((unsigned)((m.g.S + 255) / 256)), dim3(256), 0, m.stream, data,
                        m.ksqr, m.g.S);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void launch_thomas_f(Model &m, float2 *data)
+{
+    const int64_t ncols = (int64_t)m.g.nkx * m.g.ny;
+    const double fxa = 1.0 / (m.prm.dz * m.prm.dz) / m.prm.dsqr;
+    const unsigned nb = (unsigned)((ncols + 63) / 64);
+    ProfScope ps(m, "thomas");
+    if (m.g.nz <= 64) {
+        const size_t lds = sizeof(double2) * 64 * (size_t)m.g.nz;
+        hipLaunchKernelGGL(k_thomas_f<true>, dim3(nb), dim3(64), lds, m.stream, data, nullptr, m.ksqr, m.tri_cp,
+                           m.tri_rm, ncols, m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr);
+    } else {
+        PS3D_REQUIRE(m.dp64, "Thomas scratch not allocated");
+        hipLaunchKernelGGL(k_thomas_f<false>, dim3(nb), dim3(64), 0, m.stream, data, m.dp64, m.ksqr, m.tri_cp,
+                           m.tri_rm, ncols, m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr);
+    }
     PS3D_HIP_CHECK(hipGetLastError());
 }
 

@@ -55,6 +55,10 @@ def test_version_and_kernel_names():
     (dict(nz=1), "nz must be"),
     (dict(precision=7), "precision"),
     (dict(dt=0.0), "dt"),
+    # the mixed form: fused friction only, compiled-in FFT lengths only
+    (dict(precision=1, Avbi=1e-2), "vertical biharmonic"),
+    (dict(precision=1, nx=40), "compiled-in FFT lengths"),
+    (dict(precision=1, nz=30, enable_vertical_boundaries=False), "compiled-in FFT lengths"),
 ])
 def test_create_rejects_bad_parameters(bad, msg):
     from ps3d_amd import _lib
