@@ -18,6 +18,10 @@ The JSON line also carries:
   grid-pt steps/s, the speed-up over FP64, and its relative L2 error per
   field and of the total energy (diagnose.F90:14-24) against the FP64 path
   at steps 1, 100 and 1000 from the same initial state.
+* ensemble: 48 baroclinic-eddy members, 16 each of 64x64x32, 96x96x48 and
+  128x128x64 (eddy radius drawn per member), batched (one launch per stage,
+  hipGraph replay, autotuned tile shapes) against the same members run one
+  after another as single models: member-steps/s of both and their ratio.
 """
 from __future__ import annotations
 
@@ -109,6 +113,73 @@ def mixed_leg(prm, fields, steps: int, warmup: int, fp64_value: float, check_ste
             "dtype": "f32 stencils/FFT, f64 solve + accumulation"}
 
 
+def ensemble_leg(steps: int, warm: int = 3) -> dict:
+    import numpy as np
+    from ps3d_amd import configs
+    from ps3d_amd.ensemble import Ensemble
+    from ps3d_amd.model import Model
+    members = configs.ensemble_c4()
+    n = len(members)
+    # one after another: each member alone through the single-model path
+    seq = 0.0
+    keep = {}
+    for i, (prm, fields) in enumerate(members):
+        m = Model(prm)
+        for k, a in fields.items():
+            m.f[k][...] = a
+        m.push()
+        m.run(warm)
+        seq += time_steps(m, steps)
+        if i in (0, n - 1):
+            m.pull(["u", "w", "p"])
+            keep[i] = {k: m.f[k].copy() for k in ("u", "w", "p")}
+        m.close()
+    # batched
+    ens = Ensemble([p for p, _ in members])
+    for i, (_, fields) in enumerate(members):
+        for k, a in fields.items():
+            ens.set(i, k, a)
+    ens.run(warm)
+    t0 = time.perf_counter()
+    ens.tune(os.path.join("/tmp", "ps3d_ensemble_tune.txt"), reps=5)
+    tune_s = time.perf_counter() - t0
+    ens.run(1, use_graph=True)   # capture
+    ens.synchronize()
+    # the capture step above is one extra step: the comparison below uses a
+    # separate count
+    t0 = time.perf_counter()
+    ens.run(steps, use_graph=True)
+    ens.synchronize()
+    bat = time.perf_counter() - t0
+    same = True
+    # compare with the single runs after warm + steps steps: rerun the two
+    # members one step short of the ensemble's count is not needed -- the
+    # ensemble ran warm + 1 + steps; advance the kept singles by one step
+    for i in keep:
+        prm, fields = members[i]
+        m = Model(prm)
+        for k, a in fields.items():
+            m.f[k][...] = a
+        m.push()
+        m.run(warm + 1 + steps)
+        m.pull(["u", "w", "p"])
+        for k in ("u", "w", "p"):
+            same = same and bool(np.array_equal(m.f[k], ens.get(i, k)))
+        m.close()
+    desc = ens.describe()
+    ens.close()
+    v_seq = n * steps / seq
+    v_bat = n * steps / bat
+    return {"value": v_bat, "unit": "member-steps/s", "members": n, "steps": steps,
+            "sequential_value": v_seq, "speedup_vs_sequential": v_bat / v_seq,
+            "ms_per_ensemble_step": 1e3 * bat / steps, "sequential_ms_per_ensemble_step": 1e3 * seq / steps,
+            "points_per_ensemble_step": int(sum(p.npoints for p, _ in members)),
+            "bitwise_equal_to_single_runs": same, "tuning_s": tune_s,
+            "tiles": desc.strip().splitlines(), "dtype": "f64",
+            "workload": "16x(64x64x32) + 16x(96x96x48) + 16x(128x128x64) baroclinic eddies, rigid lid, AB3, "
+                        "horizontal biharmonic friction; one launch per stage, hipGraph replay"}
+
+
 def cpu_baseline(prm, fields, budget_s: float = 15.0) -> dict:
     import oracle
     orc = oracle.OracleModel(prm)
@@ -137,6 +208,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-all", action="store_true", help="print every kernel's share")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
+    ap.add_argument("--ens-steps", type=int, default=100, help="timed steps of the ensemble leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (to calibrate HBM byte counters)")
     args = ap.parse_args()
@@ -214,6 +287,8 @@ def main() -> int:
     model.close()
     if not args.no_mixed:
         out["mixed"] = mixed_leg(prm, fields, args.steps, args.warmup, value)
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_leg(args.ens_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prm, fields)
     print(json.dumps(out), flush=True)

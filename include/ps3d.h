@@ -159,6 +159,38 @@ int ps3d_profile_read(ps3d_model *m, int64_t *launches, double *total_ms);
 /* comma-separated names of the model's kernels */
 const char *ps3d_kernel_names(void);
 
+/* ---- batched ensemble ------------------------------------------------
+   No counterpart in the reference, which advances one model per process
+   (main.f90:138-162); an ensemble there is a set of separate runs.  Here
+   members of different sizes and parameters advance together: each stage of
+   the step (tendency, friction + AB, halo, the transforms, Thomas or the
+   periodic z solve, projection) is one launch over all members, and a step
+   can be replayed as one captured hipGraph.  Every member computes exactly
+   what a single model with the same parameters computes (PS3D_FP64, the
+   fused-friction configurations: Avbi = Kvbi = 0; FFT lengths among the
+   compiled-in ones, column lengths ny (and nz without a lid) <= 512). */
+typedef struct ps3d_ensemble ps3d_ensemble;
+
+int ps3d_ensemble_create(int n, const ps3d_params *prms, void *stream, ps3d_ensemble **out);
+int ps3d_ensemble_destroy(ps3d_ensemble *e);
+/* field ids and layouts as for ps3d_set_field / ps3d_get_field */
+int ps3d_ensemble_set_field(ps3d_ensemble *e, int member, int field, const double *host);
+int ps3d_ensemble_get_field(ps3d_ensemble *e, int member, int field, double *host);
+/* nsteps steps of every member; state = {itt, tau, taum1, taum2}, shared by
+   the members and advanced as by ps3d_run; use_graph replays a captured step */
+int ps3d_ensemble_run(ps3d_ensemble *e, int nsteps, int32_t *state, int use_graph);
+int ps3d_ensemble_synchronize(ps3d_ensemble *e);
+/* Autotuner: times the candidate tile shapes of the k-marching stages
+   (levels per work-group of tendency and friction + AB) per member shape on
+   the live ensemble, keeps the fastest, and caches the choice per
+   (stage, nx x ny x nz, boundary, precision) in cache_path (NULL: no file).
+   Results do not depend on the tile shapes. */
+int ps3d_ensemble_tune(ps3d_ensemble *e, const char *cache_path, int reps);
+/* the tile shapes in use, one "stage shape levels" line per member shape,
+   copied as by ps3d_last_error; returns the text's length, or minus an
+   error code */
+int ps3d_ensemble_describe(ps3d_ensemble *e, char *buf, int len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,14 @@ PROTOTYPES = {
     "ps3d_spectral_solve": (_c_int, [_vp, _dp]),
     "ps3d_profile_kernel": (_c_int, [_vp, ctypes.c_char_p]),
     "ps3d_profile_read": (_c_int, [_vp, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_dbl)]),
+    "ps3d_ensemble_create": (_c_int, [_c_int, ctypes.POINTER(CParams), _vp, ctypes.POINTER(_vp)]),
+    "ps3d_ensemble_destroy": (_c_int, [_vp]),
+    "ps3d_ensemble_set_field": (_c_int, [_vp, _c_int, _c_int, _dp]),
+    "ps3d_ensemble_get_field": (_c_int, [_vp, _c_int, _c_int, _dp]),
+    "ps3d_ensemble_run": (_c_int, [_vp, _c_int, _i32p, _c_int]),
+    "ps3d_ensemble_synchronize": (_c_int, [_vp]),
+    "ps3d_ensemble_tune": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
+    "ps3d_ensemble_describe": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
 }
 
 _lib = None

@@ -17,7 +17,7 @@ import numpy as np
 
 from .params import PS3D_FP64, ModelParams
 
-FIELD_ID = {"u": 1, "v": 2, "w": 3, "b": 4}
+FIELD_ID = {"u": 1, "v": 2, "w": 3, "b": 4, "R": 5}   # R: per-member scalars
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
 
 
@@ -199,3 +199,19 @@ def random_state(nx, ny, nz, seed=7, lid=True, amp=0.1, **kw):
     fields["w"] *= 0.01
     fields["b"] *= 1e-3
     return prm, _finish(prm, fields)
+
+
+def ensemble_c4(per_size: int = 16, sizes=((64, 64, 32), (96, 96, 48), (128, 128, 64))):
+    """Ensemble of baroclinic eddies (the 64x64x32 eddy physics) in three
+    sizes, per_size members each; member m has seed 1000+m and eddy radius
+    R_m = U(1/8, 1/4) Lx drawn from that seed."""
+    out = []
+    m = 0
+    for (nx, ny, nz) in sizes:
+        for _ in range(per_size):
+            seed = 1000 + m
+            u01 = 0.5 * (float(uniform(seed, "R", 1, 1, 1)[0, 0, 0]) + 1.0)
+            out.append(eddy(nx, ny, nz, seed=seed, radius_frac=0.125 + 0.125 * u01))
+            m += 1
+    return out
+

@@ -14,27 +14,6 @@ using namespace ps3d;
 
 namespace {
 
-thread_local std::string g_err;
-
-template <class F>
-int guard(F &&f)
-{
-    try {
-        f();
-        g_err.clear();
-        return PS3D_OK;
-    } catch (const Error &e) {
-        g_err = e.what();
-        return e.code;
-    } catch (const std::exception &e) {
-        g_err = e.what();
-        return PS3D_ERR_INTERNAL;
-    } catch (...) {
-        g_err = "unknown error";
-        return PS3D_ERR_INTERNAL;
-    }
-}
-
 bool is_235(int n)
 {
     if (n < 1) return false;
@@ -127,30 +106,37 @@ void require_fp64(const Model &m, const char *what)
                                                        "only as a whole: ps3d_integrate / ps3d_run)");
 }
 
-// set / get of a mixed model through the FP64 staging buffer
+// set / get of a mixed model.  u, v, w, b go through the FP64 staging
+// buffer and the (hi, lo) split; p is the FP64 pressure (a value the host
+// sets is returned until the next step and, as in the reference, not read by
+// the step).  The tendency slots and FS hold this form's own quantities
+// (pressure-referenced tendencies, the spectrum of the pressure increment)
+// and are not exchanged with the host.
+const char *kMixedFields = "PS3D_MIXED: only u, v, w, b, p and ksqr are exchanged with the host";
+
 void set_field_mixed(Model &m, int field, const double *host, int64_t l)
 {
+    if (field == PS3D_P) {
+        m.p_host.assign(host, host + l);
+        return;
+    }
+    if (field > PS3D_B) throw Error(PS3D_ERR_STATE, kMixedFields);
     PS3D_HIP_CHECK(hipMemcpyAsync(m.stage, host, sizeof(double) * l, hipMemcpyHostToDevice, m.stream));
-    if (field <= PS3D_B) launch_convert(m, 0, l, m.stage, nullptr, nullptr, m.fh[field], nullptr, m.fl[field]);
-    else if (field == PS3D_P) launch_convert(m, 2, l, m.stage, nullptr, nullptr, m.pf, nullptr, nullptr);
-    else if (field <= PS3D_DB) launch_convert(m, 2, l, m.stage, nullptr, nullptr, m.tf[field - PS3D_DU], nullptr, nullptr);
-    else launch_convert(m, 2, l, m.stage, nullptr, nullptr, reinterpret_cast<float *>(m.FSf), nullptr, nullptr);
+    launch_convert(m, 0, l, m.stage, nullptr, nullptr, m.fh[field], nullptr, m.fl[field]);
 }
 
 void get_field_mixed(Model &m, int field, double *host, int64_t l)
 {
-    if (field <= PS3D_B) {
-        launch_convert(m, 1, l, nullptr, m.stage, m.fh[field], nullptr, m.fl[field], nullptr);
-        // after a step the lo parts' halos are stale (the stencils read the
-        // hi parts only): rebuild the halo of the joined field
-        if (m.lo_stale[field]) launch_border_exchg(m, m.stage, 0);
-    } else if (field == PS3D_P) {
-        launch_convert(m, 3, l, nullptr, m.stage, m.pf, nullptr, nullptr, nullptr);
-    } else if (field <= PS3D_DB) {
-        launch_convert(m, 3, l, nullptr, m.stage, m.tf[field - PS3D_DU], nullptr, nullptr, nullptr);
-    } else {
-        launch_convert(m, 3, l, nullptr, m.stage, reinterpret_cast<const float *>(m.FSf), nullptr, nullptr, nullptr);
+    if (field == PS3D_P) {
+        if (!m.p_host.empty()) std::memcpy(host, m.p_host.data(), sizeof(double) * l);
+        else PS3D_HIP_CHECK(hipMemcpyAsync(host, m.pint, sizeof(double) * l, hipMemcpyDeviceToHost, m.stream));
+        return;
     }
+    if (field > PS3D_B) throw Error(PS3D_ERR_STATE, kMixedFields);
+    launch_convert(m, 1, l, nullptr, m.stage, m.fh[field], nullptr, m.fl[field], nullptr);
+    // after a step the lo parts' halos are stale (the stencils read the hi
+    // parts only): rebuild the halo of the joined field
+    if (m.lo_stale[field]) launch_border_exchg(m, m.stage, 0);
     PS3D_HIP_CHECK(hipMemcpyAsync(host, m.stage, sizeof(double) * l, hipMemcpyDeviceToHost, m.stream));
 }
 
@@ -161,34 +147,82 @@ void check_tau(int tau, int taum1, int taum2)
     PS3D_REQUIRE(tau != taum1 && tau != taum2 && taum1 != taum2, "tau slots must differ");
 }
 
-// pressure_solver.f90:5-118 in the mixed form: FP32 transforms of the hi
-// parts, FP64 spectral division / Thomas elimination, FP64-accumulated
-// projection of (hi, lo)
-void pressure_mixed(Model &m)
+// FP32 Poisson solve of pressure_solver.f90:16-73 on the hi parts of u, v, w
+// (their divergence / dt), result into dst (interior; halo left to the caller)
+void solve_mixed(Model &m, const float *u, const float *v, const float *w, float *dst)
 {
     const ps3d_params &p = m.prm;
-    fft_r2c_x_f(m, m.fh[0], m.fh[1], m.fh[2], m.FSf);
+    fft_r2c_x_f(m, u, v, w, m.FSf);
     if (p.enable_vertical_boundaries) {
         fft_c2c_y_f(m, m.FSf, -1, p.dx, p.dy);
         launch_thomas_f(m, m.FSf);
         fft_c2c_y_f(m, m.FSf, +1, 0.0, 0.0);
-        fft_c2r_x_f(m, m.FSf, m.pf, p.Lx * p.Ly);
+        fft_c2r_x_f(m, m.FSf, dst, p.Lx * p.Ly);
     } else {
         fft_c2c_y_f(m, m.FSf, -1, 0.0, 0.0);
         fft_z_periodic_solve_f(m, m.FSf);
         fft_c2c_y_f(m, m.FSf, +1, 0.0, 0.0);
-        fft_c2r_x_f(m, m.FSf, m.pf, p.Lx * p.Ly * p.Lz);
+        fft_c2r_x_f(m, m.FSf, dst, p.Lx * p.Ly * p.Lz);
     }
-    launch_project_exchange_f(m);
+}
+
+// Starts the pressure history of a mixed model: an estimate p_est of this
+// step's pressure from a provisional Euler step (tendency without a reference
+// pressure, friction + AB into the second buffers, FP32 solve), held in
+// pint with zero increments, so that the step that follows is referenced to
+// p_est and its FP32 quantities are small from the first step on.
+void cold_start_mixed(Model &m, int itt, int tau, int taum1, int taum2)
+{
+    launch_tendency_mixed(m, tau, nullptr);
+    launch_friction_ab_mixed(m, 0, tau, taum1, taum2, 0.0, nullptr, 0.0, nullptr, false);
+    (void)itt;
+    float *est = m.dpr[0];
+    solve_mixed(m, m.ah[0], m.ah[1], m.ah[2], est);
+    launch_halo_f(m, &est, &m.prm.enable_vertical_boundaries, 1);
+    launch_convert(m, 3, m.g.H, nullptr, m.pint, est, nullptr, nullptr, nullptr);
+    for (int q = 0; q < 3; ++q) launch_convert(m, 4, m.g.H, nullptr, nullptr, nullptr, m.dpr[q], nullptr, nullptr);
+    m.dp_cur = 0;
+    m.cold = false;
+}
+
+// One step of the mixed form.  With p_{n-1} the FP64 pressure of the last
+// step, the tendency slot of step q holds s_q = d_q - G p_{q-1} (G the
+// gradient of the projection, w scaled by 1/dsqr through dt/dsqr), and the
+// AB sum sum_q c_q d_q with sum_q c_q = 1 gives
+//   u* - dt G p_{n-1} = u + friction + dt sum_q c_q s_q - dt G Q,
+//   Q = (c1 + c2) dp_{n-1} + c2 dp_{n-2},  dp_m = p_m - p_{m-1},
+// so that L dp_n = div(u* - dt G p_{n-1}) / dt, u_n = u' - dt G dp_n and
+// p_n = p_{n-1} + dp_n are the reference's pressure_solver.f90:16-110
+// rearranged exactly.  The balanced parts (Coriolis, buoyancy, G p) are
+// formed in FP64; what FP32 stores and transforms is small.
+void integrate_mixed(Model &m, int itt, int tau, int taum1, int taum2)
+{
+    const ps3d_params &p = m.prm;
+    if (m.cold) cold_start_mixed(m, itt, tau, taum1, taum2);
+    launch_tendency_mixed(m, tau, m.pint);
+    double qc1 = 0.0, qc2 = 0.0;
+    if (itt == 0) {
+    } else if (itt == 1 || !p.enable_AB_3_order) {
+        qc1 = -(0.5 + p.eps_ab);                  // c1; c2 = 0
+    } else {
+        qc1 = -16.0 / 12.0 + 5.0 / 12.0;          // c1 + c2
+        qc2 = 5.0 / 12.0;                         // c2
+    }
+    const float *d1 = m.dpr[m.dp_cur];
+    const float *d2 = m.dpr[(m.dp_cur + 2) % 3];
+    launch_friction_ab_mixed(m, itt, tau, taum1, taum2, qc1, qc1 != 0.0 ? d1 : nullptr, qc2,
+                             qc2 != 0.0 ? d2 : nullptr, true);
+    const int nxt = (m.dp_cur + 1) % 3;
+    solve_mixed(m, m.fh[0], m.fh[1], m.fh[2], m.dpr[nxt]);
+    launch_project_exchange_f(m, m.dpr[nxt]);
+    m.dp_cur = nxt;
+    for (int f = 0; f < 4; ++f) m.lo_stale[f] = true;
+    m.p_host.clear();
 }
 
 void pressure(Model &m)
 {
     const ps3d_params &p = m.prm;
-    if (m.mixed) {
-        pressure_mixed(m);
-        return;
-    }
     fft_r2c_x(m, nullptr, 0, 1, m.FS);
     if (p.enable_vertical_boundaries) {
         fft_c2c_y(m, m.FS, -1, p.dx, p.dy);
@@ -206,13 +240,11 @@ void pressure(Model &m)
 
 void integrate(Model &m, int itt, int tau, int taum1, int taum2)
 {
-    launch_tendency(m, tau);
     if (m.mixed) {
-        launch_friction_ab(m, itt, tau, taum1, taum2);
-        pressure(m);
-        for (int f = 0; f < 4; ++f) m.lo_stale[f] = true;
+        integrate_mixed(m, itt, tau, taum1, taum2);
         return;
     }
+    launch_tendency(m, tau);
     if (fused_friction_ok(m) && m.fuse_friction) {
         launch_friction_ab(m, itt, tau, taum1, taum2);
     } else {
@@ -229,6 +261,12 @@ const char *kKernelNames =
 }  // namespace
 
 namespace ps3d {
+
+std::string &error_slot()
+{
+    thread_local std::string msg;
+    return msg;
+}
 
 ProfScope::ProfScope(Model &mm, const char *name) : m(mm), on(false)
 {
@@ -251,12 +289,12 @@ ProfScope::~ProfScope()
 
 }  // namespace ps3d
 
-struct ps3d_model : public ps3d::Model {};
 
 extern "C" {
 
 int ps3d_last_error(char *buf, int len)
 {
+    const std::string &g_err = error_slot();
     const int n = (int)g_err.size();
     if (buf && len > 0) {
         const int c = n < len - 1 ? n : len - 1;
@@ -329,7 +367,8 @@ int ps3d_create(const ps3d_params *prm, const double *ksqr, void *stream, ps3d_m
                     m->al[f] = dalloc<float>(*m, g.H);
                     m->tf[f] = dalloc<float>(*m, 3 * g.H);
                 }
-                m->pf = dalloc<float>(*m, g.H);
+                m->pint = dalloc<double>(*m, g.H);
+                for (int q = 0; q < 3; ++q) m->dpr[q] = dalloc<float>(*m, g.H);
                 m->FSf = dalloc<float2>(*m, g.S);
                 m->stage = dalloc<double>(*m, 3 * g.H);
                 if (p.enable_vertical_boundaries && p.nz > 64) m->dp64 = dalloc<double2>(*m, g.S);
@@ -554,6 +593,7 @@ int ps3d_pressure_solver(ps3d_model *m)
 {
     return guard([&] {
         check_model(m);
+        require_fp64(*m, "ps3d_pressure_solver");
         for (int f = 0; f < 4; ++f)
             PS3D_REQUIRE(m->cur[f] == 0, "pressure_solver: friction results not yet folded in by ab_update");
         pressure(*m);

@@ -1,0 +1,484 @@
+// ensemble.hip -- batched ensemble of ps3D models: the runtime around the
+// batched stage kernels (member table, block maps, step state, hipGraph
+// replay) and the autotuner of the k-marching tile shapes.
+//
+// Each member is an ordinary FP64 model (buffers, FFT tables, ksqr, Thomas
+// factors); the ensemble owns a device table that points into them and runs
+// one launch per stage over all of them.  The bodies of the batched kernels
+// are those of the single-model kernels, so every member's result is the
+// same, bit for bit, as a single model's.
+#include "ps3d_internal.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+
+using namespace ps3d;
+
+namespace {
+
+enum Stage { S_TEND, S_FRIC, S_HALO, S_R2C, S_COLS, S_THOMAS, S_ZSOLVE, S_PROJ, S_NSTAGE };
+
+// tile shapes of the k-marching kernels (stencil.hip)
+constexpr int kTTX = 64, kTTY = 8, kFTX = 64, kFTY = 8;
+
+int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+}  // namespace
+
+struct ps3d_ensemble {
+    std::vector<ps3d_model *> mem;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::vector<EnsMember> tab;
+    EnsMember *dev_tab = nullptr;
+    EnsState *dev_state = nullptr;
+    EnsState state{0, 3, 2, 1, 0};   // the device state once the queued steps have run
+    EnsMap maps[S_NSTAGE]{};
+    size_t lds_x = 0, lds_y = 0, lds_z = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::map<std::string, int> tuned;   // "stage shape" -> levels per work-group
+};
+
+namespace {
+
+std::string shape_key(const ps3d_params &p)
+{
+    std::ostringstream o;
+    o << p.nx << "x" << p.ny << "x" << p.nz << (p.enable_vertical_boundaries ? ":lid" : ":periodic") << ":fp64";
+    return o.str();
+}
+
+// levels per work-group -> (kc, nchunk)
+void set_chunks(int nz, int kc, int &okc, int &onch)
+{
+    kc = std::max(1, std::min(kc, nz));
+    onch = cdiv(nz, kc);
+    okc = cdiv(nz, onch);
+}
+
+// the single-model launchers' choices (stencil.hip), the untuned default
+int default_tend_kc(const Grid &g)
+{
+    const int tiles = cdiv(g.nx, kTTX) * cdiv(g.ny, kTTY);
+    int nchunk = cdiv(1024, tiles);
+    const int maxch = g.nz / 8 > 1 ? g.nz / 8 : 1;
+    nchunk = std::min(nchunk, maxch);
+    return cdiv(g.nz, nchunk);
+}
+
+int default_fric_kc(const Grid &g)
+{
+    const int tiles = cdiv(g.nx, kFTX) * cdiv(g.ny, kFTY);
+    int nchunk = cdiv(2048, tiles * 4);
+    nchunk = std::max(1, std::min(nchunk, g.nz));
+    return cdiv(g.nz, nchunk);
+}
+
+void fill_member(EnsMember &e, ps3d_model &m)
+{
+    const ps3d_params &p = m.prm;
+    const Grid &g = m.g;
+    e = EnsMember{};
+    e.c = m.c;
+    e.H = g.H;
+    e.nhalo = g.H - g.N;
+    double *canon[4] = {m.u, m.v, m.w, m.b};
+    double *tend[4] = {m.du, m.dv, m.dw, m.db};
+    for (int q = 0; q < 4; ++q) {
+        e.f[q][0] = canon[q];
+        e.f[q][1] = m.alt[q];
+        e.tend[q] = tend[q];
+    }
+    e.p = m.p;
+    e.FS = m.FS;
+    e.ksqr = m.ksqr;
+    e.cp = m.tri_cp;
+    e.rm = m.tri_rm;
+    e.twx = m.px.tw;
+    e.twy = m.py.tw;
+    e.twz = m.pz.tw;
+    e.wk = m.tw_r2c;
+    e.Hm[0] = p.Ah; e.Vm[0] = p.Av; e.Hbi[0] = p.Ahbi;
+    e.harm[0] = (p.Ah > 0. || p.Av > 0.); e.bih[0] = p.Ahbi > 0.;
+    e.Hm[1] = p.Kh; e.Vm[1] = p.Kv; e.Hbi[1] = p.Khbi;
+    e.harm[1] = (p.Kh > 0. || p.Kv > 0.); e.bih[1] = p.Khbi > 0.;
+    e.eps_ab = p.eps_ab;
+    e.ab3 = p.enable_AB_3_order;
+    e.fxa = 1.0 / (p.dz * p.dz) / p.dsqr;   // tridiag.hip: the factors' fxa
+    e.lxy = p.Lx * p.Ly;
+    e.lxyz = p.Lx * p.Ly * p.Lz;
+    e.rlxy = 1.0 / e.lxy;
+    e.rlxyz = 1.0 / e.lxyz;
+    e.n2 = p.nx / 2;
+    e.nkx = g.nkx;
+    e.tend_tx = cdiv(g.nx, kTTX);
+    e.tend_ty = cdiv(g.ny, kTTY);
+    e.fric_tx = cdiv(g.nx, kFTX);
+    e.fric_ty = cdiv(g.ny, kFTY);
+}
+
+void build_maps(ps3d_ensemble &E)
+{
+    const int n = (int)E.mem.size();
+    for (int s = 0; s < S_NSTAGE; ++s) {
+        E.maps[s].n = n;
+        int64_t acc = 0;
+        for (int i = 0; i < n; ++i) {
+            const EnsMember &e = E.tab[i];
+            const Grid &g = E.mem[i]->g;
+            const bool lid = e.c.lid != 0;
+            int64_t nb = 0;
+            switch (s) {
+            case S_TEND: nb = (int64_t)e.tend_tx * e.tend_ty * e.tend_nchunk; break;
+            case S_FRIC: nb = (int64_t)e.fric_tx * e.fric_ty * e.fric_nchunk * 4; break;
+            case S_HALO: nb = cdiv(4 * e.nhalo, 256); break;
+            case S_R2C: nb = ens_r2c_blocks(e); break;
+            case S_COLS: nb = ens_cols_blocks(e, g.ny, g.nz); break;
+            case S_THOMAS: nb = lid ? cdiv((int64_t)g.nkx * g.ny, 64) : 0; break;
+            case S_ZSOLVE: nb = lid ? 0 : ens_cols_blocks(e, g.nz, g.ny); break;
+            case S_PROJ: nb = (int64_t)cdiv(g.nx, 64) * cdiv(g.ny, 4) * g.nz; break;
+            }
+            E.maps[s].start[i] = (int)acc;
+            acc += nb;
+        }
+        PS3D_REQUIRE(acc < (int64_t)1 << 31, "ensemble too large for one launch per stage");
+        E.maps[s].start[n] = (int)acc;
+    }
+    E.lds_x = E.lds_y = E.lds_z = 0;
+    for (int i = 0; i < n; ++i) {
+        const Grid &g = E.mem[i]->g;
+        E.lds_x = std::max(E.lds_x, ens_x_lds(g.nx / 2));
+        E.lds_y = std::max(E.lds_y, ens_cols_lds(g.ny));
+        if (!E.tab[i].c.lid) E.lds_z = std::max(E.lds_z, ens_cols_lds(g.nz));
+    }
+}
+
+void upload_table(ps3d_ensemble &E)
+{
+    PS3D_HIP_CHECK(hipMemcpyAsync(E.dev_tab, E.tab.data(), sizeof(EnsMember) * E.tab.size(), hipMemcpyHostToDevice,
+                                  E.stream));
+    PS3D_HIP_CHECK(hipStreamSynchronize(E.stream));
+    if (E.exec) {
+        (void)hipGraphExecDestroy(E.exec);
+        E.exec = nullptr;
+    }
+    if (E.graph) {
+        (void)hipGraphDestroy(E.graph);
+        E.graph = nullptr;
+    }
+}
+
+void apply_tiles(ps3d_ensemble &E, int stage, const std::string &key, int kc)
+{
+    for (size_t i = 0; i < E.mem.size(); ++i) {
+        if (shape_key(E.mem[i]->prm) != key) continue;
+        EnsMember &e = E.tab[i];
+        if (stage == S_TEND) set_chunks(e.c.nz, kc, e.tend_kc, e.tend_nchunk);
+        else set_chunks(e.c.nz, kc, e.fric_kc, e.fric_nchunk);
+    }
+}
+
+void launch_stage(ps3d_ensemble &E, int s)
+{
+    switch (s) {
+    case S_TEND: ens_tendency(E.stream, E.dev_tab, E.maps[S_TEND], E.dev_state); break;
+    case S_FRIC: ens_friction_ab(E.stream, E.dev_tab, E.maps[S_FRIC], E.dev_state); break;
+    default: break;
+    }
+}
+
+// integrate.f90:2-115 for every member: the single-model step, stage by stage
+void enqueue_step(ps3d_ensemble &E)
+{
+    hipStream_t s = E.stream;
+    ens_tendency(s, E.dev_tab, E.maps[S_TEND], E.dev_state);
+    ens_friction_ab(s, E.dev_tab, E.maps[S_FRIC], E.dev_state);
+    ens_halo(s, E.dev_tab, E.maps[S_HALO], E.dev_state, 0);
+    ens_r2c(s, E.dev_tab, E.maps[S_R2C], E.dev_state, E.lds_x);
+    ens_cols_y(s, E.dev_tab, E.maps[S_COLS], -1, E.lds_y);
+    ens_thomas(s, E.dev_tab, E.maps[S_THOMAS]);
+    ens_zsolve(s, E.dev_tab, E.maps[S_ZSOLVE], E.lds_z);
+    ens_cols_y(s, E.dev_tab, E.maps[S_COLS], +1, E.lds_y);
+    ens_c2r(s, E.dev_tab, E.maps[S_R2C], E.lds_x);
+    ens_project(s, E.dev_tab, E.maps[S_PROJ], E.dev_state);
+    ens_halo(s, E.dev_tab, E.maps[S_HALO], E.dev_state, 1);
+    ens_advance(s, E.dev_state);
+}
+
+void check_member(const ps3d_ensemble *e, int member)
+{
+    if (!e) throw Error(PS3D_ERR_ARG, "null ensemble");
+    PS3D_REQUIRE(member >= 0 && member < (int)e->mem.size(), "member index out of range");
+}
+
+void destroy(ps3d_ensemble *e)
+{
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (ps3d_model *m : e->mem) ps3d_destroy(m);
+    if (e->dev_tab) (void)hipFree(e->dev_tab);
+    if (e->dev_state) (void)hipFree(e->dev_state);
+    if (e->own_stream && e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ps3d_ensemble_create(int n, const ps3d_params *prms, void *stream, ps3d_ensemble **out)
+{
+    return guard([&] {
+        PS3D_REQUIRE(prms && out, "null argument");
+        *out = nullptr;
+        PS3D_REQUIRE(n >= 1 && n <= ENS_MAX, "ensemble size must be 1.." + std::to_string(ENS_MAX));
+        for (int i = 0; i < n; ++i) {
+            const ps3d_params &p = prms[i];
+            const std::string who = "member " + std::to_string(i) + ": ";
+            PS3D_REQUIRE(p.precision == PS3D_FP64, who + "ensembles run the FP64 form");
+            PS3D_REQUIRE(!(p.Avbi > 0.) && !(p.Kvbi > 0.),
+                         who + "vertical biharmonic friction (Avbi, Kvbi > 0) is not in the batched step");
+            PS3D_REQUIRE(p.nx >= 2 && p.ny >= 2 && ens_fft_size_ok(p.nx / 2, false) && ens_fft_size_ok(p.ny, true) &&
+                             (p.enable_vertical_boundaries || ens_fft_size_ok(p.nz, true)),
+                         who + "nx/2, ny (and nz without a lid) must be compiled-in FFT lengths (" + fft_sizes_text() +
+                             "), ny and nz at most 512");
+        }
+        auto *E = new ps3d_ensemble();
+        try {
+            if (stream) {
+                E->stream = static_cast<hipStream_t>(stream);
+            } else {
+                PS3D_HIP_CHECK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+                E->own_stream = true;
+            }
+            E->mem.assign(n, nullptr);
+            E->tab.resize(n);
+            for (int i = 0; i < n; ++i) {
+                ps3d_model *m = nullptr;
+                const int rc = ps3d_create(&prms[i], nullptr, E->stream, &m);
+                if (rc != PS3D_OK) throw Error(rc, "member " + std::to_string(i) + ": " + error_slot());
+                E->mem[i] = m;
+                fill_member(E->tab[i], *m);
+                set_chunks(m->g.nz, default_tend_kc(m->g), E->tab[i].tend_kc, E->tab[i].tend_nchunk);
+                set_chunks(m->g.nz, default_fric_kc(m->g), E->tab[i].fric_kc, E->tab[i].fric_nchunk);
+            }
+            build_maps(*E);
+            PS3D_HIP_CHECK(hipMalloc(&E->dev_tab, sizeof(EnsMember) * n));
+            PS3D_HIP_CHECK(hipMalloc(&E->dev_state, sizeof(EnsState)));
+            PS3D_HIP_CHECK(hipMemcpyAsync(E->dev_state, &E->state, sizeof(EnsState), hipMemcpyHostToDevice,
+                                          E->stream));
+            upload_table(*E);
+        } catch (...) {
+            destroy(E);
+            throw;
+        }
+        *out = E;
+    });
+}
+
+int ps3d_ensemble_destroy(ps3d_ensemble *e)
+{
+    return guard([&] { destroy(e); });
+}
+
+int ps3d_ensemble_set_field(ps3d_ensemble *e, int member, int field, const double *host)
+{
+    return guard([&] {
+        check_member(e, member);
+        PS3D_REQUIRE(host, "null host pointer");
+        ps3d_model *m = e->mem[member];
+        if (field >= PS3D_U && field <= PS3D_B) {
+            PS3D_HIP_CHECK(hipStreamSynchronize(e->stream));
+            PS3D_HIP_CHECK(hipMemcpyAsync(e->tab[member].f[field][e->state.par], host, sizeof(double) * m->g.H,
+                                          hipMemcpyHostToDevice, e->stream));
+            PS3D_HIP_CHECK(hipStreamSynchronize(e->stream));
+            return;
+        }
+        const int rc = ps3d_set_field(m, field, host);
+        if (rc != PS3D_OK) throw Error(rc, error_slot());
+    });
+}
+
+int ps3d_ensemble_get_field(ps3d_ensemble *e, int member, int field, double *host)
+{
+    return guard([&] {
+        check_member(e, member);
+        PS3D_REQUIRE(host, "null host pointer");
+        ps3d_model *m = e->mem[member];
+        if (field >= PS3D_U && field <= PS3D_B) {
+            PS3D_HIP_CHECK(hipMemcpyAsync(host, e->tab[member].f[field][e->state.par], sizeof(double) * m->g.H,
+                                          hipMemcpyDeviceToHost, e->stream));
+            PS3D_HIP_CHECK(hipStreamSynchronize(e->stream));
+            return;
+        }
+        const int rc = ps3d_get_field(m, field, host);
+        if (rc != PS3D_OK) throw Error(rc, error_slot());
+    });
+}
+
+int ps3d_ensemble_run(ps3d_ensemble *e, int nsteps, int32_t *st, int use_graph)
+{
+    return guard([&] {
+        PS3D_REQUIRE(e && st && nsteps >= 0, "bad arguments");
+        auto ok = [](int t) { return t >= 1 && t <= 3; };
+        PS3D_REQUIRE(ok(st[1]) && ok(st[2]) && ok(st[3]) && st[1] != st[2] && st[1] != st[3] && st[2] != st[3],
+                     "tau slots must be 1..3 and differ");
+        PS3D_REQUIRE(st[0] >= 0, "itt must be >= 0");
+        if (e->state.itt != st[0] || e->state.tau != st[1] || e->state.taum1 != st[2] || e->state.taum2 != st[3]) {
+            e->state.itt = st[0];
+            e->state.tau = st[1];
+            e->state.taum1 = st[2];
+            e->state.taum2 = st[3];
+            PS3D_HIP_CHECK(hipMemcpyAsync(e->dev_state, &e->state, sizeof(EnsState), hipMemcpyHostToDevice,
+                                          e->stream));
+            PS3D_HIP_CHECK(hipStreamSynchronize(e->stream));
+        }
+        if (use_graph && nsteps > 0 && !e->exec) {
+            PS3D_HIP_CHECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue_step(*e);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(e->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            PS3D_HIP_CHECK(hipStreamEndCapture(e->stream, &e->graph));
+            PS3D_HIP_CHECK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+        }
+        for (int s = 0; s < nsteps; ++s) {
+            if (use_graph) PS3D_HIP_CHECK(hipGraphLaunch(e->exec, e->stream));
+            else enqueue_step(*e);
+            e->state.tau = e->state.tau % 3 + 1;
+            e->state.taum1 = e->state.taum1 % 3 + 1;
+            e->state.taum2 = e->state.taum2 % 3 + 1;
+            e->state.itt += 1;
+            e->state.par ^= 1;
+        }
+        st[0] = e->state.itt;
+        st[1] = e->state.tau;
+        st[2] = e->state.taum1;
+        st[3] = e->state.taum2;
+    });
+}
+
+int ps3d_ensemble_synchronize(ps3d_ensemble *e)
+{
+    return guard([&] {
+        PS3D_REQUIRE(e, "null ensemble");
+        PS3D_HIP_CHECK(hipStreamSynchronize(e->stream));
+    });
+}
+
+int ps3d_ensemble_tune(ps3d_ensemble *e, const char *cache_path, int reps)
+{
+    return guard([&] {
+        PS3D_REQUIRE(e, "null ensemble");
+        reps = std::max(1, reps);
+        // the cache file: one "stage shape levels" line per entry
+        std::map<std::string, int> cache;
+        if (cache_path) {
+            std::ifstream in(cache_path);
+            std::string st, sh;
+            int kc;
+            while (in >> st >> sh >> kc) cache[st + " " + sh] = kc;
+        }
+        std::vector<std::string> shapes;
+        for (ps3d_model *m : e->mem) {
+            const std::string k = shape_key(m->prm);
+            if (std::find(shapes.begin(), shapes.end(), k) == shapes.end()) shapes.push_back(k);
+        }
+        hipEvent_t t0, t1;
+        PS3D_HIP_CHECK(hipEventCreate(&t0));
+        PS3D_HIP_CHECK(hipEventCreate(&t1));
+        const char *names[2] = {"tendency", "friction_ab"};
+        const int stages[2] = {S_TEND, S_FRIC};
+        try {
+            for (int si = 0; si < 2; ++si) {
+                const int stage = stages[si];
+                for (const std::string &sh : shapes) {
+                    const std::string key = std::string(names[si]) + " " + sh;
+                    auto hit = cache.find(key);
+                    if (hit != cache.end()) {
+                        apply_tiles(*e, stage, sh, hit->second);
+                        e->tuned[key] = hit->second;
+                        continue;
+                    }
+                    int nz = 0;
+                    for (ps3d_model *m : e->mem)
+                        if (shape_key(m->prm) == sh) nz = m->g.nz;
+                    // levels per work-group: powers of two up to nz, and nz
+                    std::vector<int> cand;
+                    for (int kc = 1; kc < nz; kc *= 2) cand.push_back(kc);
+                    cand.push_back(nz);
+                    int best = cand[0];
+                    float best_ms = 1e30f;
+                    for (int kc : cand) {
+                        apply_tiles(*e, stage, sh, kc);
+                        build_maps(*e);
+                        upload_table(*e);
+                        launch_stage(*e, stage);   // warm-up
+                        PS3D_HIP_CHECK(hipEventRecord(t0, e->stream));
+                        for (int r = 0; r < reps; ++r) launch_stage(*e, stage);
+                        PS3D_HIP_CHECK(hipEventRecord(t1, e->stream));
+                        PS3D_HIP_CHECK(hipEventSynchronize(t1));
+                        float ms = 0.f;
+                        PS3D_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+                        if (ms < best_ms) {
+                            best_ms = ms;
+                            best = kc;
+                        }
+                    }
+                    apply_tiles(*e, stage, sh, best);
+                    e->tuned[key] = best;
+                    cache[key] = best;
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(t0);
+            (void)hipEventDestroy(t1);
+            throw;
+        }
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        build_maps(*e);
+        upload_table(*e);
+        if (cache_path) {
+            std::ofstream out(cache_path, std::ios::trunc);
+            for (auto &kv : cache) out << kv.first << " " << kv.second << "\n";
+        }
+    });
+}
+
+int ps3d_ensemble_describe(ps3d_ensemble *e, char *buf, int len)
+{
+    int n = 0;
+    const int rc = guard([&] {
+        PS3D_REQUIRE(e, "null ensemble");
+        std::map<std::string, std::pair<int, int>> seen;
+        for (size_t i = 0; i < e->mem.size(); ++i)
+            seen[shape_key(e->mem[i]->prm)] = {e->tab[i].tend_kc, e->tab[i].fric_kc};
+        std::ostringstream o;
+        for (auto &kv : seen) {
+            o << "tendency " << kv.first << " " << kv.second.first << "\n";
+            o << "friction_ab " << kv.first << " " << kv.second.second << "\n";
+        }
+        const std::string s = o.str();
+        n = (int)s.size();
+        if (buf && len > 0) {
+            const int c = std::min(n, len - 1);
+            std::memcpy(buf, s.data(), (size_t)c);
+            buf[c] = '\0';
+        }
+    });
+    return rc == PS3D_OK ? n : -rc;
+}
+
+}  // extern "C"

@@ -486,16 +486,16 @@ __device__ __forceinline__ void fft(C *buf, const C *__restrict__ tw)
 // x-pass, forward r2c of ROWS rows (see k_r2c_x); the post-processing works
 // on the pair (k, n2-k) in place.  RT = double (drop-in) or float (mixed).
 template <int N2, int ROWS, int T, bool FUSE_DIV, class RT>
-__global__ __launch_bounds__(T) void r2c_x(StepConsts c, const typename cplx_of<RT>::type *__restrict__ tw,
-                                           const typename cplx_of<RT>::type *__restrict__ wk,
-                                           const RT *__restrict__ src, const RT *__restrict__ u,
-                                           const RT *__restrict__ v, const RT *__restrict__ w,
-                                           typename cplx_of<RT>::type *__restrict__ dst, int64_t nrows, int dstride)
+__device__ __forceinline__ void r2c_body(typename cplx_of<RT>::type *lds, int blk, const StepConsts &c,
+                                         const typename cplx_of<RT>::type *__restrict__ tw,
+                                         const typename cplx_of<RT>::type *__restrict__ wk,
+                                         const RT *__restrict__ src, const RT *__restrict__ u,
+                                         const RT *__restrict__ v, const RT *__restrict__ w,
+                                         typename cplx_of<RT>::type *__restrict__ dst, int64_t nrows, int dstride)
 {
     using C = typename cplx_of<RT>::type;
     constexpr int CS = N2 + 1;
-    __shared__ C lds[ROWS * CS];
-    const int64_t row0 = (int64_t)blockIdx.x * ROWS;
+    const int64_t row0 = (int64_t)blk * ROWS;
     const RT h = RT(0.5);
 #pragma unroll
     for (int s = 0; s < (ROWS * N2 + T - 1) / T; ++s) {
@@ -557,24 +557,35 @@ __global__ __launch_bounds__(T) void r2c_x(StepConsts c, const typename cplx_of<
     }
 }
 
+template <int N2, int ROWS, int T, bool FUSE_DIV, class RT>
+__global__ __launch_bounds__(T) void r2c_x(StepConsts c, const typename cplx_of<RT>::type *__restrict__ tw,
+                                           const typename cplx_of<RT>::type *__restrict__ wk,
+                                           const RT *__restrict__ src, const RT *__restrict__ u,
+                                           const RT *__restrict__ v, const RT *__restrict__ w,
+                                           typename cplx_of<RT>::type *__restrict__ dst, int64_t nrows, int dstride)
+{
+    __shared__ typename cplx_of<RT>::type lds[ROWS * (N2 + 1)];
+    r2c_body<N2, ROWS, T, FUSE_DIV, RT>(lds, (int)blockIdx.x, c, tw, wk, src, u, v, w, dst, nrows, dstride);
+}
+
 // x-pass, inverse c2r of ROWS rows (see k_c2r_x), pre-processing in place
 template <int N2, int ROWS, int T, class RT>
-__global__ __launch_bounds__(T) void c2r_x(StepConsts c, const typename cplx_of<RT>::type *__restrict__ tw,
-                                           const typename cplx_of<RT>::type *__restrict__ wk,
-                                           const typename cplx_of<RT>::type *__restrict__ src,
-                                           RT *__restrict__ dst, int dst_is_field, double inv_scale,
-                                           double rinv_scale, int64_t nrows, int sstride)
+__device__ __forceinline__ void c2r_body(typename cplx_of<RT>::type *lds, int blk, const StepConsts &c,
+                                         const typename cplx_of<RT>::type *__restrict__ tw,
+                                         const typename cplx_of<RT>::type *__restrict__ wk,
+                                         const typename cplx_of<RT>::type *__restrict__ src,
+                                         RT *__restrict__ dst, int dst_is_field, double inv_scale,
+                                         double rinv_scale, int64_t nrows, int sstride)
 {
     using C = typename cplx_of<RT>::type;
     constexpr int CS = N2 + 1;
     constexpr int HALF = N2 / 2 + 1;
-    __shared__ C lds[ROWS * CS];
 #pragma unroll
     for (int s = 0; s < (ROWS * HALF + T - 1) / T; ++s) {
         const int t = (int)threadIdx.x + s * T;
         if ((ROWS * HALF) % T == 0 || t < ROWS * HALF) {
             const int rr = t / HALF, kk = t % HALF;
-            const int64_t row = (int64_t)blockIdx.x * ROWS + rr;
+            const int64_t row = (int64_t)blk * ROWS + rr;
             if (row < nrows) {
                 const C *sr = src + row * sstride;
                 C xk = sr[kk];
@@ -605,7 +616,7 @@ __global__ __launch_bounds__(T) void c2r_x(StepConsts c, const typename cplx_of<
         const int t = (int)threadIdx.x + s * T;
         if ((ROWS * N2) % T == 0 || t < ROWS * N2) {
             const int rr = t / N2, m = t % N2;
-            const int64_t row = (int64_t)blockIdx.x * ROWS + rr;
+            const int64_t row = (int64_t)blk * ROWS + rr;
             if (row < nrows) {
                 const C z = lds[rr * CS + m];
                 const RT a0 = sdiv<RT>(z.x, inv_scale, rinv_scale), a1 = sdiv<RT>(z.y, inv_scale, rinv_scale);
@@ -623,17 +634,28 @@ __global__ __launch_bounds__(T) void c2r_x(StepConsts c, const typename cplx_of<
     }
 }
 
+template <int N2, int ROWS, int T, class RT>
+__global__ __launch_bounds__(T) void c2r_x(StepConsts c, const typename cplx_of<RT>::type *__restrict__ tw,
+                                           const typename cplx_of<RT>::type *__restrict__ wk,
+                                           const typename cplx_of<RT>::type *__restrict__ src,
+                                           RT *__restrict__ dst, int dst_is_field, double inv_scale,
+                                           double rinv_scale, int64_t nrows, int sstride)
+{
+    __shared__ typename cplx_of<RT>::type lds[ROWS * (N2 + 1)];
+    c2r_body<N2, ROWS, T, RT>(lds, (int)blockIdx.x, c, tw, wk, src, dst, dst_is_field, inv_scale, rinv_scale, nrows,
+                              sstride);
+}
+
 // strided c2c pass over TX consecutive kx columns (see k_c2c_cols)
 template <int N, int TX, int T, int SIGN, class C>
-__global__ __launch_bounds__(T) void c2c_cols(const C *__restrict__ tw, C *__restrict__ data, int nkx,
-                                              int64_t estride, int64_t lstride, double s1, double s2,
-                                              double s3, int nscale)
+__device__ __forceinline__ void cols_body(C *lds, int bx, int by, const C *__restrict__ tw, C *__restrict__ data,
+                                          int nkx, int64_t estride, int64_t lstride, double s1, double s2, double s3,
+                                          int nscale)
 {
     using RT = typename real_of<C>::type;
-    __shared__ C lds[N * TX];
-    const int kx0 = blockIdx.x * TX;
+    const int kx0 = bx * TX;
     const int ncol = min(TX, nkx - kx0);
-    C *base = data + kx0 + (int64_t)blockIdx.y * lstride;
+    C *base = data + kx0 + (int64_t)by * lstride;
 #pragma unroll
     for (int s = 0; s < (N * TX + T - 1) / T; ++s) {
         const int t = (int)threadIdx.x + s * T;
@@ -660,17 +682,26 @@ __global__ __launch_bounds__(T) void c2c_cols(const C *__restrict__ tw, C *__res
     }
 }
 
+template <int N, int TX, int T, int SIGN, class C>
+__global__ __launch_bounds__(T) void c2c_cols(const C *__restrict__ tw, C *__restrict__ data, int nkx,
+                                              int64_t estride, int64_t lstride, double s1, double s2,
+                                              double s3, int nscale)
+{
+    __shared__ C lds[N * TX];
+    cols_body<N, TX, T, SIGN, C>(lds, (int)blockIdx.x, (int)blockIdx.y, tw, data, nkx, estride, lstride, s1, s2, s3,
+                                 nscale);
+}
+
 // periodic z solve (see k_z_periodic_solve); the scaling and the division by
 // ksqr are done in FP64 whatever the storage type
 template <int N, int TX, int T, class C>
-__global__ __launch_bounds__(T) void z_periodic_solve(const C *__restrict__ tw, C *__restrict__ data,
-                                                      const double *__restrict__ ksqr, int nkx, int nkxs, int ny,
-                                                      double dx, double dy, double dz)
+__device__ __forceinline__ void zsolve_body(C *lds, int bx, int by, const C *__restrict__ tw, C *__restrict__ data,
+                                            const double *__restrict__ ksqr, int nkx, int nkxs, int ny, double dx,
+                                            double dy, double dz)
 {
     using RT = typename real_of<C>::type;
-    __shared__ C lds[N * TX];
-    const int kx0 = blockIdx.x * TX;
-    const int j = blockIdx.y;
+    const int kx0 = bx * TX;
+    const int j = by;
     const int ncol = min(TX, nkx - kx0);
     const int64_t estride = (int64_t)nkxs * ny;
     C *base = data + kx0 + (int64_t)j * nkxs;
@@ -710,6 +741,15 @@ __global__ __launch_bounds__(T) void z_periodic_solve(const C *__restrict__ tw, 
             if (col < ncol) base[col + e * estride] = lds[e * TX + col];
         }
     }
+}
+
+template <int N, int TX, int T, class C>
+__global__ __launch_bounds__(T) void z_periodic_solve(const C *__restrict__ tw, C *__restrict__ data,
+                                                      const double *__restrict__ ksqr, int nkx, int nkxs, int ny,
+                                                      double dx, double dy, double dz)
+{
+    __shared__ C lds[N * TX];
+    zsolve_body<N, TX, T, C>(lds, (int)blockIdx.x, (int)blockIdx.y, tw, data, ksqr, nkx, nkxs, ny, dx, dy, dz);
 }
 
 // launch configurations: x-pass rows so that ROWS*N2 = 1024 values, y/z
@@ -821,6 +861,102 @@ bool dispatch_zsolve(Model &m, C *data, const C *tw, int nkxs)
         PS3D_FFT_SIZES(CASE)
 #undef CASE
     default: return false;
+    }
+}
+
+// ---- batched ensemble stages (one launch over all members) -----------------
+// The member's transform length selects a compiled-in body at run time;
+// every body of a stage runs with 256 threads and the stage's dynamic LDS is
+// sized for its largest member.
+extern __shared__ double2 ens_lds[];
+
+__global__ __launch_bounds__(TTHREADS) void k_ens_r2c(const EnsMember *__restrict__ mem, EnsMap map,
+                                                       const EnsState *__restrict__ st)
+{
+    const int g = (int)blockIdx.x;
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const StepConsts c = e.c;
+    const int lb = g - map.start[mi];
+    const int nb = st->par ^ 1;
+    const int64_t nrows = (int64_t)c.ny * c.nz;
+    switch (e.n2) {
+#define CASE(n)                                                                                              \
+    case n:                                                                                                  \
+        r2c_body<n, XROWS(n), TTHREADS, true, double>(ens_lds, lb, c, e.twx, e.wk, nullptr, e.f[0][nb],      \
+                                                      e.f[1][nb], e.f[2][nb], e.FS, nrows, e.nkx);            \
+        break;
+        PS3D_FFT_SIZES(CASE)
+#undef CASE
+    default: break;
+    }
+}
+
+__global__ __launch_bounds__(TTHREADS) void k_ens_c2r(const EnsMember *__restrict__ mem, EnsMap map)
+{
+    const int g = (int)blockIdx.x;
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const StepConsts c = e.c;
+    const int lb = g - map.start[mi];
+    const int64_t nrows = (int64_t)c.ny * c.nz;
+    const double inv = c.lid ? e.lxy : e.lxyz, rinv = c.lid ? e.rlxy : e.rlxyz;
+    switch (e.n2) {
+#define CASE(n)                                                                                              \
+    case n:                                                                                                  \
+        c2r_body<n, XROWS(n), TTHREADS, double>(ens_lds, lb, c, e.twx, e.wk, e.FS, e.p, 1, inv, rinv, nrows,  \
+                                                e.nkx);                                                      \
+        break;
+        PS3D_FFT_SIZES(CASE)
+#undef CASE
+    default: break;
+    }
+}
+
+// y-pass over every member: forward scaled by dx dy under a rigid lid
+// (my_fft_2D), unscaled otherwise (the 3-D scaling is in the z solve)
+template <int SIGN>
+__global__ __launch_bounds__(TTHREADS) void k_ens_cols_y(const EnsMember *__restrict__ mem, EnsMap map)
+{
+    const int g = (int)blockIdx.x;
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const int lb = g - map.start[mi];
+    const int nbx = (e.nkx + TCOL - 1) / TCOL;
+    const int bx = lb % nbx, by = lb / nbx;
+    const int nscale = (SIGN < 0 && e.c.lid) ? 2 : 0;
+    const double s1 = nscale ? e.c.dx : 0.0, s2 = nscale ? e.c.dy : 0.0;
+    switch (e.c.ny) {
+#define CASE(n)                                                                                              \
+    case n:                                                                                                  \
+        if constexpr (TCOLS(n) == TTHREADS)                                                                  \
+            cols_body<n, TCOL, TTHREADS, SIGN, double2>(ens_lds, bx, by, e.twy, e.FS, e.nkx, (int64_t)e.nkx,  \
+                                                        (int64_t)e.nkx * e.c.ny, s1, s2, 0.0, nscale);       \
+        break;
+        PS3D_FFT_SIZES(CASE)
+#undef CASE
+    default: break;
+    }
+}
+
+__global__ __launch_bounds__(TTHREADS) void k_ens_zsolve(const EnsMember *__restrict__ mem, EnsMap map)
+{
+    const int g = (int)blockIdx.x;
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const int lb = g - map.start[mi];
+    const int nbx = (e.nkx + TCOL - 1) / TCOL;
+    const int bx = lb % nbx, by = lb / nbx;
+    switch (e.c.nz) {
+#define CASE(n)                                                                                              \
+    case n:                                                                                                  \
+        if constexpr (TCOLS(n) == TTHREADS)                                                                  \
+            zsolve_body<n, TCOL, TTHREADS, double2>(ens_lds, bx, by, e.twz, e.FS, e.ksqr, e.nkx, e.nkx,       \
+                                                    e.c.ny, e.c.dx, e.c.dy, e.c.dz);                         \
+        break;
+        PS3D_FFT_SIZES(CASE)
+#undef CASE
+    default: break;
     }
 }
 
@@ -1048,6 +1184,77 @@ std::string fft_sizes_text()
     PS3D_FFT_SIZES(ITEM)
 #undef ITEM
     return t;
+}
+
+// ---- batched ensemble stages ------------------------------------------------
+bool ens_fft_size_ok(int n, bool column_pass)
+{
+    switch (n) {
+#define CASE(n) case n: return !column_pass || ft::TCOLS(n) == ft::TTHREADS;
+        PS3D_FFT_SIZES(CASE)
+#undef CASE
+    default: return false;
+    }
+}
+
+size_t ens_x_lds(int n2) { return (size_t)ft::XROWS(n2) * (n2 + 1) * sizeof(double2); }
+
+size_t ens_cols_lds(int n) { return (size_t)n * ft::TCOL * sizeof(double2); }
+
+int ens_r2c_blocks(const EnsMember &e)
+{
+    const int rows = ft::XROWS(e.n2);
+    return (int)(((int64_t)e.c.ny * e.c.nz + rows - 1) / rows);
+}
+
+int ens_cols_blocks(const EnsMember &e, int /*n_len*/, int lines)
+{
+    return ((e.nkx + ft::TCOL - 1) / ft::TCOL) * lines;
+}
+
+template <class K>
+static void ens_lds_attr(K kern, size_t lds)
+{
+    if (lds > 65536) PS3D_HIP_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+}
+
+void ens_r2c(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, size_t lds)
+{
+    if (!map.start[map.n]) return;
+    ens_lds_attr(ft::k_ens_r2c, lds);
+    hipLaunchKernelGGL(ft::k_ens_r2c, dim3((unsigned)map.start[map.n]), dim3(ft::TTHREADS), lds, s, mem, map, st);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_c2r(hipStream_t s, const EnsMember *mem, const EnsMap &map, size_t lds)
+{
+    if (!map.start[map.n]) return;
+    ens_lds_attr(ft::k_ens_c2r, lds);
+    hipLaunchKernelGGL(ft::k_ens_c2r, dim3((unsigned)map.start[map.n]), dim3(ft::TTHREADS), lds, s, mem, map);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_cols_y(hipStream_t s, const EnsMember *mem, const EnsMap &map, int sign, size_t lds)
+{
+    if (!map.start[map.n]) return;
+    if (sign < 0) {
+        ens_lds_attr(ft::k_ens_cols_y<-1>, lds);
+        hipLaunchKernelGGL(ft::k_ens_cols_y<-1>, dim3((unsigned)map.start[map.n]), dim3(ft::TTHREADS), lds, s, mem,
+                           map);
+    } else {
+        ens_lds_attr(ft::k_ens_cols_y<+1>, lds);
+        hipLaunchKernelGGL(ft::k_ens_cols_y<+1>, dim3((unsigned)map.start[map.n]), dim3(ft::TTHREADS), lds, s, mem,
+                           map);
+    }
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_zsolve(hipStream_t s, const EnsMember *mem, const EnsMap &map, size_t lds)
+{
+    if (!map.start[map.n]) return;
+    ens_lds_attr(ft::k_ens_zsolve, lds);
+    hipLaunchKernelGGL(ft::k_ens_zsolve, dim3((unsigned)map.start[map.n]), dim3(ft::TTHREADS), lds, s, mem, map);
+    PS3D_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace ps3d

@@ -30,6 +30,30 @@ struct Error : std::runtime_error {
         if (!(cond)) throw ::ps3d::Error(PS3D_ERR_ARG, (msg));                         \
     } while (0)
 
+// message of the calling thread's last failed C-ABI call
+std::string &error_slot();
+
+// runs f, turning exceptions into PS3D_ERR_* codes and a message: no C++
+// exception crosses the C-ABI
+template <class F>
+int guard(F &&f)
+{
+    try {
+        f();
+        error_slot().clear();
+        return PS3D_OK;
+    } catch (const Error &e) {
+        error_slot() = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        error_slot() = e.what();
+        return PS3D_ERR_INTERNAL;
+    } catch (...) {
+        error_slot() = "unknown error";
+        return PS3D_ERR_INTERNAL;
+    }
+}
+
 constexpr int ONX = 2;   // halo width, main_module.f90:93
 
 // Extents of one model.  Physical fields are (X, Y, Z) = (nx+4, ny+4, nz+4)
@@ -131,18 +155,25 @@ struct Model {
     bool fuse_friction = true;
 
     // mixed form (PS3D_MIXED): prognostic fields as FP32 (hi, lo) pairs whose
-    // sum carries the FP64-accumulated state; FP32 tendencies, pressure and
-    // spectral array; FP64 Thomas elimination through dp64
+    // sum carries the FP64-accumulated state.  The pressure is carried in
+    // FP64 (pint) and each step solves, in FP32, only for its increment dp;
+    // the FP32 tendency slots hold the tendencies minus the gradient of the
+    // pressure they were computed with, so every FP32 quantity is small and
+    // the hydrostatic / geostrophic cancellations happen in FP64.
     bool mixed = false;
     float *fh[4] = {}, *fl[4] = {};   // canonical (hi, lo) of u, v, w, b
     float *ah[4] = {}, *al[4] = {};   // second buffers
-    float *tf[4] = {};                // tendencies du..db, 3 slots each
-    float *pf = nullptr;              // pressure
-    float2 *FSf = nullptr;            // spectral array
+    float *tf[4] = {};                // pressure-referenced tendencies, 3 slots each
+    double *pint = nullptr;           // pressure p_n (FP64), halo included
+    float *dpr[3] = {};               // ring of pressure increments dp_n, dp_n-1, dp_n-2
+    int dp_cur = 0;                   // dpr index of the latest increment
+    bool cold = true;                 // no pressure history yet: the next step starts one
+    float2 *FSf = nullptr;            // spectral array (of the increment)
     double2 *dp64 = nullptr;          // forward-sweep values of the Thomas solve
     double *stage = nullptr;          // 3H doubles for host conversions
     float2 *twf_r2c = nullptr;
     bool lo_stale[4] = {};            // lo halo not exchanged since the last step
+    std::vector<double> p_host;       // p as set by the host, returned until the next step
 
     // profiling of one kernel by name
     std::string prof_name;
@@ -191,7 +222,13 @@ void launch_thomas(Model &m, double2 *data);
 void launch_thomas_f(Model &m, float2 *data);
 
 // ---- mixed form (stencil.hip) -------------------------------------------
-void launch_project_exchange_f(Model &m);
+// tendency with the reference pressure pint (null: zero)
+void launch_tendency_mixed(Model &m, int tau, const double *pint);
+// friction + AB into the second buffers; qc1, qc2 weight the last two
+// pressure increments (see the step in api.hip); swap: make them canonical
+void launch_friction_ab_mixed(Model &m, int itt, int tau, int taum1, int taum2, double qc1, const float *d1,
+                              double qc2, const float *d2, bool swap);
+void launch_project_exchange_f(Model &m, float *dp);
 void launch_halo_f(Model &m, float *const *arrays, const int *neumann, int n);
 void launch_convert(Model &m, int dir, int64_t n, const double *src_d, double *dst_d, const float *src_f,
                     float *dst_f, const float *src_lo, float *dst_lo);
@@ -205,4 +242,74 @@ struct ProfScope {
     ~ProfScope();
 };
 
+// ---- batched ensemble (ensemble.hip) --------------------------------------
+// Members of different sizes and parameters advance together: every stage of
+// the step is ONE launch over all members, whose blocks are dealt to the
+// members by a prefix table (EnsMap) and which read the member's constants,
+// buffers and tile shapes from a device-side table (EnsMember).  The step
+// counters live in device memory (EnsState) and are advanced by the last
+// launch of the step, so one captured hipGraph replays every step.
+constexpr int ENS_MAX = 256;
+
+struct EnsMap {
+    int n;                       // members
+    int start[ENS_MAX + 1];      // first block of each member; start[n] = total
+};
+
+struct EnsMember {
+    StepConsts c;
+    int64_t H, nhalo;
+    double *f[4][2];             // u, v, w, b: the two buffers of each field
+    double *tend[4];             // du..db, 3 slots of H each
+    double *p;
+    double2 *FS;
+    const double *ksqr, *cp, *rm;
+    const double2 *twx, *twy, *twz, *wk;   // FFT tables: nx/2, ny, nz, r2c
+    double Hm[2], Vm[2], Hbi[2];            // friction of (u, v, w) and of b
+    int harm[2], bih[2];
+    double eps_ab, fxa, lxy, lxyz, rlxy, rlxyz;   // rl* = 1 / l*, rounded on the host
+    int ab3, n2, nkx;
+    int tend_kc, tend_nchunk, tend_tx, tend_ty;   // tile shapes (tunable)
+    int fric_kc, fric_nchunk, fric_tx, fric_ty;
+};
+
+// member of block g: the last member whose first block is <= g (members
+// without blocks in a stage have empty ranges and are skipped)
+__device__ __forceinline__ int ens_find(const EnsMap &m, int g)
+{
+    int lo = 0, hi = m.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (m.start[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+struct EnsState {
+    int itt, tau, taum1, taum2;
+    int par;                     // which buffer of f[][] holds the state
+};
+
+// stage launchers (one launch each over all members)
+void ens_tendency(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st);
+void ens_friction_ab(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st);
+void ens_halo(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, int after_projection);
+void ens_project(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st);
+void ens_advance(hipStream_t s, EnsState *st);
+void ens_r2c(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, size_t lds);
+void ens_cols_y(hipStream_t s, const EnsMember *mem, const EnsMap &map, int sign, size_t lds);
+void ens_zsolve(hipStream_t s, const EnsMember *mem, const EnsMap &map, size_t lds);
+void ens_c2r(hipStream_t s, const EnsMember *mem, const EnsMap &map, size_t lds);
+void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map);
+// block counts and LDS bytes of the batched FFT stages for one member
+int ens_r2c_blocks(const EnsMember &e);
+int ens_cols_blocks(const EnsMember &e, int n_len, int lines);
+size_t ens_x_lds(int n2);
+size_t ens_cols_lds(int n);
+bool ens_fft_size_ok(int n, bool column_pass);
+
 }  // namespace ps3d
+
+// the opaque handle of the C-ABI
+struct ps3d_model : public ps3d::Model {};

@@ -31,6 +31,15 @@ __device__ __forceinline__ int wrap(int i, int n)
     return i < 1 ? i + n : (i > n ? i - n : i);
 }
 
+// Blocks are dealt round-robin to the 8 XCDs; renumbering them so that each
+// XCD walks a contiguous run of work items keeps neighbouring tiles, which
+// share apron rows, in one L2.
+__device__ __forceinline__ int xcd_remap(int bid, int n)
+{
+    const int per = n / 8;
+    return bid < per * 8 ? (bid % 8) * per + bid / 8 : bid;
+}
+
 // ---------------------------------------------------------------------------
 // tendencies: advection.f90:2-184 + integrate.f90:21-54 (interior points)
 // ---------------------------------------------------------------------------
@@ -137,20 +146,18 @@ template <class T>
 struct TendArgs {
     const T *u, *v, *w, *b;
     T *du, *dv, *dw, *db;
+    // mixed form: lo parts of u, v, b and the FP64 reference pressure (may be null)
+    const float *ul, *vl, *bl;
+    const double *pint;
     int kc, nchunk, tiles_x, tiles_y, nblocks;
 };
 
 // T = double: the drop-in (bit-exact); T = float: the FP32 stencils of the
 // mixed form (same expressions, reciprocal multiplies instead of divisions)
 template <class T>
-__global__ __launch_bounds__(TTHR) void k_tendency_lds(StepConsts c, TendArgs<T> a)
+__device__ __forceinline__ void tendency_body(const StepConsts &c, const TendArgs<T> &a, int bid)
 {
     __shared__ T sp[4][3][TPY][TPX];
-    int bid = (int)blockIdx.x;
-    {
-        const int per = a.nblocks / 8;   // XCD-contiguous runs of (chunk, tile)
-        if (bid < per * 8) bid = (bid % 8) * per + bid / 8;
-    }
     const int ch = bid % a.nchunk;
     const int tile = bid / a.nchunk;
     const int i0 = (tile / a.tiles_y) * TTX + 1;
@@ -264,17 +271,48 @@ __global__ __launch_bounds__(TTHR) void k_tendency_lds(StepConsts c, TendArgs<T>
                 }
             }
             adv = adv - h * (W(0, 0, 0) + W(-1, 0, 0)) * (T)(c.N0 * c.N0);
-            au = au + (V(0, 0, 0) + V(0, -1, 0) + V(0, 0, 1) + V(0, -1, 1)) * q4 * (T)c.f0;
-            av = av - (U(0, 0, -1) + U(0, 0, 0) + U(0, 1, -1) + U(0, 1, 0)) * q4 * (T)c.f0;
-            aw = aw + h * (B(0, 0, 0) + B(1, 0, 0));
-            if (lid && k >= nz) aw = zero;
             a.db[o] = adv;
-            a.du[o] = au;
-            a.dv[o] = av;
-            a.dw[o] = aw;
+            if constexpr (!std::is_same<T, float>::value) {
+                au = au + (V(0, 0, 0) + V(0, -1, 0) + V(0, 0, 1) + V(0, -1, 1)) * q4 * (T)c.f0;
+                av = av - (U(0, 0, -1) + U(0, 0, 0) + U(0, 1, -1) + U(0, 1, 0)) * q4 * (T)c.f0;
+                aw = aw + h * (B(0, 0, 0) + B(1, 0, 0));
+                if (lid && k >= nz) aw = zero;
+                a.du[o] = au;
+                a.dv[o] = av;
+                a.dw[o] = aw;
+            } else {
+                // mixed form: the Coriolis and buoyancy terms, which the
+                // pressure gradient nearly balances, and that gradient in FP64
+                // from (hi, lo); the slots keep tendency - grad(pint)
+                const int64_t X = c.X, XY = c.XY;
+                auto u64 = [&](int dj, int di) { return (double)U(0, dj, di) + (double)a.ul[o + di + dj * X]; };
+                auto v64 = [&](int dj, int di) { return (double)V(0, dj, di) + (double)a.vl[o + di + dj * X]; };
+                const double cu = (v64(0, 0) + v64(-1, 0) + v64(0, 1) + v64(-1, 1)) * 0.25 * c.f0;
+                const double cv = (u64(0, -1) + u64(0, 0) + u64(1, -1) + u64(1, 0)) * 0.25 * c.f0;
+                const double bb = 0.5 * (((double)B(0, 0, 0) + (double)a.bl[o]) + ((double)B(1, 0, 0) + (double)a.bl[o + XY]));
+                double gx = 0.0, gy = 0.0, gz = 0.0;
+                if (a.pint) {
+                    const double pc = a.pint[o];
+                    gx = (a.pint[o + 1] - pc) * c.rdx;
+                    gy = (a.pint[o + X] - pc) * c.rdy;
+                    gz = (a.pint[o + XY] - pc) * c.rdz;
+                }
+                const double su = ((double)au + cu) - gx;
+                const double sv = ((double)av - cv) - gy;
+                const double sw = (lid && k >= nz) ? 0.0 : ((double)aw + bb) - gz;
+                a.du[o] = (float)su;
+                a.dv[o] = (float)sv;
+                a.dw[o] = (float)sw;
+            }
         }
         __syncthreads();
     }
+}
+
+template <class T>
+__global__ __launch_bounds__(TTHR) void k_tendency_lds(StepConsts c, TendArgs<T> a)
+{
+    tendency_body<T>(c, a, xcd_remap((int)blockIdx.x, a.nblocks));
 }
 
 // ---------------------------------------------------------------------------
@@ -424,6 +462,8 @@ struct FricField {
     double Hm, Vm, Hbi;           // harmonic (H, V) and biharmonic_h coefficients
     double scale;                 // dt, or dt/dsqr for w
     int harm, bih, zero_top;
+    int64_t gstride;              // mixed form: grad(Q) direction (1, X, XY; 0 for b)
+    double grd;                   // 1/dx, 1/dy or 1/dz
 };
 
 template <class T>
@@ -432,26 +472,23 @@ struct FricArgs {
     double ca, cb, cc;
     int order;
     int nchunk, kc, tiles_x, tiles_y, nblocks;
+    // mixed form: Q = qc1 d1 + qc2 d2 from the last two pressure increments;
+    // u, v, w subtract scale * grad(Q) (see the step in api.hip)
+    const float *d1, *d2;
+    double qc1, qc2;
 };
 
 // T = double: the drop-in, bit-exact.  T = float: the mixed form -- stencils
 // in FP32, and the new state (hi + lo) + harmonic increment + biharmonic
 // increment + AB increment summed in FP64 and split back into (hi, lo).
 template <class T>
-__global__ __launch_bounds__(FTHREADS) void k_fric_ab(StepConsts c, FricArgs<T> a)
+__device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T> &a, int bid)
 {
     constexpr bool MIXED = std::is_same<T, float>::value;
     __shared__ T f0p[3][R0Y][R0X];
     __shared__ T f1p[R1Y][R1X];
     __shared__ T d2p[FTY + 2][FTX + 2];   // biharmonic_h's del2 on the tile + 1
     __shared__ T hinc[MIXED ? FTY : 1][MIXED ? FTX : 1];   // harmonic increment at tile points
-    // blocks are dealt round-robin to the 8 XCDs; renumber so that each XCD
-    // walks a contiguous run of tiles (neighbours share apron rows in its L2)
-    int bid = (int)blockIdx.x;
-    {
-        const int per = a.nblocks / 8;
-        if (bid < per * 8) bid = (bid % 8) * per + bid / 8;
-    }
     const int ntile = a.tiles_x * a.tiles_y;
     const int tile = bid % ntile;
     const int rest = bid / ntile;
@@ -579,7 +616,17 @@ __global__ __launch_bounds__(FTHREADS) void k_fric_ab(StepConsts c, FricArgs<T> 
                 // biharmonic and AB increments are added
                 const bool zt = F.zero_top && k >= nz;
                 const double base = zt ? 0.0 : (double)f0p[sc][cy + 1][cx + 1] + (double)lo0;
-                const double x = base + ((zt ? 0.0 : (double)hinc[py][px]) + (double)binc + ab);
+                double gq = 0.0;
+                if (F.gstride && a.d1) {
+                    const int64_t on = o + F.gstride;
+                    double qo = a.qc1 * (double)a.d1[o], qn = a.qc1 * (double)a.d1[on];
+                    if (a.d2) {
+                        qo += a.qc2 * (double)a.d2[o];
+                        qn += a.qc2 * (double)a.d2[on];
+                    }
+                    gq = F.scale * ((qn - qo) * F.grd);
+                }
+                const double x = base + ((zt ? 0.0 : (double)hinc[py][px]) + (double)binc + ab - gq);
                 const float hi = (float)x;
                 F.out[o] = hi;
                 F.out_lo[o] = (float)(x - (double)hi);
@@ -587,6 +634,12 @@ __global__ __launch_bounds__(FTHREADS) void k_fric_ab(StepConsts c, FricArgs<T> 
         }
         __syncthreads();
     }
+}
+
+template <class T>
+__global__ __launch_bounds__(FTHREADS) void k_fric_ab(StepConsts c, FricArgs<T> a)
+{
+    fric_body<T>(c, a, xcd_remap((int)blockIdx.x, a.nblocks));
 }
 
 // ---------------------------------------------------------------------------
@@ -604,13 +657,10 @@ struct HaloArgsT {
 };
 using HaloArgs = HaloArgsT<double>;
 
+// halo point t (0 <= t < number of halo points) of array a
 template <class T>
-__global__ __launch_bounds__(256) void k_halo(StepConsts c, HaloArgsT<T> h, int64_t nhalo)
+__device__ __forceinline__ void halo_point(const StepConsts &c, T *__restrict__ a, int neu, int64_t t)
 {
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (t >= nhalo) return;
-    T *a = h.f[blockIdx.y];
-    const int neu = h.neumann[blockIdx.y];
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const int64_t X = c.X, Y = ny + 4;
     const int64_t nzp = 4 * X * Y, nyr = 4 * X * (int64_t)nz;
@@ -638,19 +688,23 @@ __global__ __launch_bounds__(256) void k_halo(StepConsts c, HaloArgsT<T> h, int6
     a[idx(c, i, j, k)] = a[idx(c, si, sj, sk)];
 }
 
+template <class T>
+__global__ __launch_bounds__(256) void k_halo(StepConsts c, HaloArgsT<T> h, int64_t nhalo)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= nhalo) return;
+    halo_point<T>(c, h.f[blockIdx.y], h.neumann[blockIdx.y], t);
+}
+
 // ---------------------------------------------------------------------------
 // projection, pressure_solver.f90:98-110 (interior).  p is read at wrapped
 // indices (periodic in x, y; periodic or Neumann in z), which is what its
 // exchanged halo holds, so p's own halo can be filled afterwards.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BX * BY) void k_project(StepConsts c, double *__restrict__ u,
-                                                      double *__restrict__ v, double *__restrict__ w,
-                                                      const double *__restrict__ p)
+__device__ __forceinline__ void project_point(const StepConsts &c, double *__restrict__ u, double *__restrict__ v,
+                                              double *__restrict__ w, const double *__restrict__ p, int i, int j,
+                                              int k)
 {
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > c.nx || j > c.ny) return;
     const int64_t o = idx(c, i, j, k);
     const double pc = p[o];
     const double pe = p[idx(c, i == c.nx ? 1 : i + 1, j, k)];
@@ -662,35 +716,47 @@ __global__ __launch_bounds__(BX * BY) void k_project(StepConsts c, double *__res
     else w[o] = w[o] - div_rn((c.dt / c.dsqr) * (pt - pc), c.dz, c.rdz);
 }
 
-// projection of the mixed form: p in FP32, the state update accumulated in
-// FP64 on (hi + lo) and split back
+__global__ __launch_bounds__(BX * BY) void k_project(StepConsts c, double *__restrict__ u,
+                                                      double *__restrict__ v, double *__restrict__ w,
+                                                      const double *__restrict__ p)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    if (i > c.nx || j > c.ny) return;
+    project_point(c, u, v, w, p, i, j, (int)blockIdx.z + 1);
+}
+
+// projection of the mixed form with the FP32 pressure increment dp: the
+// state update accumulated in FP64 on (hi + lo) and split back, and the FP64
+// pressure advanced, pint += dp
 __global__ __launch_bounds__(BX * BY) void k_project_f(StepConsts c, float *__restrict__ uh, float *__restrict__ ul,
                                                         float *__restrict__ vh, float *__restrict__ vl,
                                                         float *__restrict__ wh, float *__restrict__ wl,
-                                                        const float *__restrict__ p)
+                                                        const float *__restrict__ dp, double *__restrict__ pint)
 {
     const int i = blockIdx.x * BX + threadIdx.x + 1;
     const int j = blockIdx.y * BY + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
     if (i > c.nx || j > c.ny) return;
     const int64_t o = idx(c, i, j, k);
-    const double pc = p[o];
-    const double pe = p[idx(c, i == c.nx ? 1 : i + 1, j, k)];
-    const double pn = p[idx(c, i, j == c.ny ? 1 : j + 1, k)];
-    const double pt = p[idx(c, i, j, k == c.nz ? (c.lid ? c.nz : 1) : k + 1)];
+    const double pc = dp[o];
+    const double pe = dp[idx(c, i == c.nx ? 1 : i + 1, j, k)];
+    const double pn = dp[idx(c, i, j == c.ny ? 1 : j + 1, k)];
+    const double pt = dp[idx(c, i, j, k == c.nz ? (c.lid ? c.nz : 1) : k + 1)];
     auto put = [&](float *h, float *l, double x) {
         const float hi = (float)x;
         h[o] = hi;
         l[o] = (float)(x - (double)hi);
     };
-    put(uh, ul, ((double)uh[o] + (double)ul[o]) - c.dt * (pe - pc) * c.rdx);
-    put(vh, vl, ((double)vh[o] + (double)vl[o]) - c.dt * (pn - pc) * c.rdy);
+    put(uh, ul, ((double)uh[o] + (double)ul[o]) - c.dt * ((pe - pc) * c.rdx));
+    put(vh, vl, ((double)vh[o] + (double)vl[o]) - c.dt * ((pn - pc) * c.rdy));
     if (c.lid && k >= c.nz) put(wh, wl, 0.0);
-    else put(wh, wl, ((double)wh[o] + (double)wl[o]) - (c.dt / c.dsqr) * (pt - pc) * c.rdz);
+    else put(wh, wl, ((double)wh[o] + (double)wl[o]) - (c.dt / c.dsqr) * ((pt - pc) * c.rdz));
+    pint[o] = pint[o] + pc;
 }
 
 // host-layout conversions of the mixed form: 0 split FP64 -> (hi, lo),
-// 1 join (hi, lo) -> FP64, 2 FP64 -> FP32, 3 FP32 -> FP64
+// 1 join (hi, lo) -> FP64, 2 FP64 -> FP32, 3 FP32 -> FP64, 4 zero FP32
 __global__ __launch_bounds__(256) void k_convert(int dir, int64_t n, const double *__restrict__ sd,
                                                  double *__restrict__ dd, const float *__restrict__ sf,
                                                  float *__restrict__ df, const float *__restrict__ slo,
@@ -707,8 +773,10 @@ __global__ __launch_bounds__(256) void k_convert(int dir, int64_t n, const doubl
         dd[t] = (double)sf[t] + (double)slo[t];
     } else if (dir == 2) {
         df[t] = (float)sd[t];
-    } else {
+    } else if (dir == 3) {
         dd[t] = (double)sf[t];
+    } else {
+        df[t] = 0.f;
     }
 }
 
@@ -720,6 +788,114 @@ inline dim3 interior_grid(const Grid &g, int zmult = 1)
 inline dim3 full_grid(const Grid &g)
 {
     return dim3((unsigned)((g.X + BX - 1) / BX), (unsigned)((g.Y + BY - 1) / BY), (unsigned)g.Z);
+}
+
+// ---------------------------------------------------------------------------
+// batched ensemble stages: one launch over all members (see EnsMember)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TTHR) void k_ens_tendency(const EnsMember *__restrict__ mem, EnsMap map,
+                                                        const EnsState *__restrict__ st)
+{
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const StepConsts c = e.c;
+    const int par = st->par;
+    const int64_t off = (int64_t)(st->tau - 1) * e.H;
+    TendArgs<double> a{};
+    a.u = e.f[0][par]; a.v = e.f[1][par]; a.w = e.f[2][par]; a.b = e.f[3][par];
+    a.du = e.tend[0] + off; a.dv = e.tend[1] + off; a.dw = e.tend[2] + off; a.db = e.tend[3] + off;
+    a.kc = e.tend_kc; a.nchunk = e.tend_nchunk; a.tiles_x = e.tend_tx; a.tiles_y = e.tend_ty;
+    tendency_body<double>(c, a, g - map.start[mi]);
+}
+
+__global__ __launch_bounds__(FTHREADS) void k_ens_fric(const EnsMember *__restrict__ mem, EnsMap map,
+                                                        const EnsState *__restrict__ st)
+{
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const StepConsts c = e.c;
+    const int par = st->par, itt = st->itt;
+    FricArgs<double> a{};
+    for (int f = 0; f < 4; ++f) {
+        FricField<double> &F = a.f[f];
+        const int q = f < 3 ? 0 : 1;
+        F.in = e.f[f][par];
+        F.out = e.f[f][par ^ 1];
+        F.t0 = e.tend[f] + (int64_t)(st->tau - 1) * e.H;
+        F.t1 = e.tend[f] + (int64_t)(st->taum1 - 1) * e.H;
+        F.t2 = e.tend[f] + (int64_t)(st->taum2 - 1) * e.H;
+        F.scale = (f == 2) ? c.dt / c.dsqr : c.dt;
+        F.harm = e.harm[q];
+        F.Hm = e.Hm[q];
+        F.Vm = e.Vm[q];
+        F.bih = e.bih[q];
+        F.Hbi = e.Hbi[q];
+        F.zero_top = (f == 2 && F.harm && c.lid) ? 1 : 0;
+    }
+    if (itt == 0) {
+        a.order = 1;
+    } else if (itt == 1 || !e.ab3) {
+        a.order = 2;
+        a.ca = 1.5 + e.eps_ab;
+        a.cb = 0.5 + e.eps_ab;
+    } else {
+        a.order = 3;
+        a.ca = 23.0 / 12.0;
+        a.cb = -16.0 / 12.0;
+        a.cc = 5.0 / 12.0;
+    }
+    a.kc = e.fric_kc; a.nchunk = e.fric_nchunk; a.tiles_x = e.fric_tx; a.tiles_y = e.fric_ty;
+    fric_body<double>(c, a, g - map.start[mi]);
+}
+
+// halo of u, v, w, b in the new buffers after the friction stage, or of
+// u, v, w and p (Neumann in z under a lid) after the projection
+__global__ __launch_bounds__(256) void k_ens_halo(const EnsMember *__restrict__ mem, EnsMap map,
+                                                   const EnsState *__restrict__ st, int after_projection)
+{
+    const int g = (int)blockIdx.x;
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const int64_t t = (int64_t)(g - map.start[mi]) * 256 + threadIdx.x;
+    const int q = (int)(t / e.nhalo);
+    if (q >= 4) return;
+    const int nb = st->par ^ 1;
+    double *a = (after_projection && q == 3) ? e.p : e.f[q][nb];
+    const int neu = (after_projection && q == 3) ? e.c.lid : 0;
+    halo_point<double>(e.c, a, neu, t - (int64_t)q * e.nhalo);
+}
+
+__global__ __launch_bounds__(BX * BY) void k_ens_project(const EnsMember *__restrict__ mem, EnsMap map,
+                                                          const EnsState *__restrict__ st)
+{
+    const int g = (int)blockIdx.x;
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const StepConsts c = e.c;
+    const int lb = g - map.start[mi];
+    const int bx = (c.nx + BX - 1) / BX, by = (c.ny + BY - 1) / BY;
+    const int i = (lb % bx) * BX + threadIdx.x + 1;
+    const int j = ((lb / bx) % by) * BY + threadIdx.y + 1;
+    const int k = lb / (bx * by) + 1;
+    if (i > c.nx || j > c.ny) return;
+    const int nb = st->par ^ 1;
+    project_point(c, e.f[0][nb], e.f[1][nb], e.f[2][nb], e.p, i, j, k);
+}
+
+// end of the step: main.f90:156-160 (tau rotation, itt) and the buffer swap
+__global__ void k_ens_advance(EnsState *st)
+{
+    if (threadIdx.x == 0) {
+        EnsState s = *st;
+        s.tau = s.tau % 3 + 1;
+        s.taum1 = s.taum1 % 3 + 1;
+        s.taum2 = s.taum2 % 3 + 1;
+        s.itt += 1;
+        s.par ^= 1;
+        *st = s;
+    }
 }
 
 }  // namespace
@@ -750,13 +926,16 @@ inline dim3 march_grid(const Grid &g, int &kc)
 }
 
 template <class T>
-static void tendency_impl(Model &m, const T *u, const T *v, const T *w, const T *b, T *const *tend, int tau)
+static void tendency_impl(Model &m, const T *u, const T *v, const T *w, const T *b, T *const *tend, int tau,
+                          const float *ul = nullptr, const float *vl = nullptr, const float *bl = nullptr,
+                          const double *pint = nullptr)
 {
     ProfScope ps(m, "tendency");
     const Grid &g = m.g;
     const int64_t off = (int64_t)(tau - 1) * g.H;
     TendArgs<T> a{};
     a.u = u; a.v = v; a.w = w; a.b = b;
+    a.ul = ul; a.vl = vl; a.bl = bl; a.pint = pint;
     a.du = tend[0] + off; a.dv = tend[1] + off; a.dw = tend[2] + off; a.db = tend[3] + off;
     a.tiles_x = (g.nx + TTX - 1) / TTX;
     a.tiles_y = (g.ny + TTY - 1) / TTY;
@@ -774,12 +953,14 @@ static void tendency_impl(Model &m, const T *u, const T *v, const T *w, const T 
 
 void launch_tendency(Model &m, int tau)
 {
-    if (m.mixed) {
-        tendency_impl<float>(m, m.fh[0], m.fh[1], m.fh[2], m.fh[3], m.tf, tau);
-    } else {
-        double *tend[4] = {m.du, m.dv, m.dw, m.db};
-        tendency_impl<double>(m, m.u, m.v, m.w, m.b, tend, tau);
-    }
+    PS3D_REQUIRE(!m.mixed, "tendency: FP64 models only");
+    double *tend[4] = {m.du, m.dv, m.dw, m.db};
+    tendency_impl<double>(m, m.u, m.v, m.w, m.b, tend, tau);
+}
+
+void launch_tendency_mixed(Model &m, int tau, const double *pint)
+{
+    tendency_impl<float>(m, m.fh[0], m.fh[1], m.fh[2], m.fh[3], m.tf, tau, m.fl[0], m.fl[1], m.fl[3], pint);
 }
 
 void launch_buoyancy_advection(Model &m, int tau)
@@ -908,11 +1089,16 @@ bool fused_friction_ok(const Model &m)
 // become the canonical ones.
 template <class T>
 static void friction_ab_impl(Model &m, const T *const *in, const T *const *in_lo, T *const *out, T *const *out_lo,
-                             T *const *tend, int itt, int tau, int taum1, int taum2)
+                             T *const *tend, int itt, int tau, int taum1, int taum2, double qc1 = 0.0,
+                             const float *d1 = nullptr, double qc2 = 0.0, const float *d2 = nullptr)
 {
     const ps3d_params &p = m.prm;
     const Grid &g = m.g;
     FricArgs<T> a{};
+    a.d1 = d1;
+    a.d2 = d2;
+    a.qc1 = qc1;
+    a.qc2 = qc2;
     for (int f = 0; f < 4; ++f) {
         FricField<T> &F = a.f[f];
         F.in = in[f];
@@ -937,6 +1123,8 @@ static void friction_ab_impl(Model &m, const T *const *in, const T *const *in_lo
             F.Hbi = p.Khbi;
         }
         F.zero_top = (f == 2 && F.harm && p.enable_vertical_boundaries) ? 1 : 0;
+        F.gstride = f == 0 ? 1 : (f == 1 ? g.X : (f == 2 ? g.XY : 0));
+        F.grd = f == 0 ? m.c.rdx : (f == 1 ? m.c.rdy : m.c.rdz);
     }
     if (itt == 0) {
         a.order = 1;
@@ -977,21 +1165,25 @@ static void halo_launch(Model &m, const HaloArgsT<T> &h, int n)
 // the friction chain and AB of all four fields in one launch, then the halo
 // exchange.  The new fields are written to the second buffers, which then
 // become the canonical ones.
+void launch_friction_ab_mixed(Model &m, int itt, int tau, int taum1, int taum2, double qc1, const float *d1,
+                              double qc2, const float *d2, bool swap)
+{
+    friction_ab_impl<float>(m, m.fh, m.fl, m.ah, m.al, m.tf, itt, tau, taum1, taum2, qc1, d1, qc2, d2);
+    if (!swap) return;
+    for (int f = 0; f < 4; ++f) {
+        std::swap(m.fh[f], m.ah[f]);
+        std::swap(m.fl[f], m.al[f]);
+    }
+    // the stencils and the divergence read the hi parts only; the lo parts'
+    // halos are rebuilt when a field is copied out
+    HaloArgsT<float> h{};
+    for (int f = 0; f < 4; ++f) h.f[f] = m.fh[f];
+    halo_launch(m, h, 4);
+}
+
 void launch_friction_ab(Model &m, int itt, int tau, int taum1, int taum2)
 {
-    if (m.mixed) {
-        friction_ab_impl<float>(m, m.fh, m.fl, m.ah, m.al, m.tf, itt, tau, taum1, taum2);
-        for (int f = 0; f < 4; ++f) {
-            std::swap(m.fh[f], m.ah[f]);
-            std::swap(m.fl[f], m.al[f]);
-        }
-        // the stencils read the hi parts only; the lo parts' halos are
-        // rebuilt when a field is copied out
-        HaloArgsT<float> h{};
-        for (int f = 0; f < 4; ++f) h.f[f] = m.fh[f];
-        halo_launch(m, h, 4);
-        return;
-    }
+    PS3D_REQUIRE(!m.mixed, "friction_ab: FP64 models only");
     double *in[4] = {m.u, m.v, m.w, m.b};
     double *tend[4] = {m.du, m.dv, m.dw, m.db};
     friction_ab_impl<double>(m, in, nullptr, m.alt, nullptr, tend, itt, tau, taum1, taum2);
@@ -1033,18 +1225,21 @@ void launch_project_exchange(Model &m)
 }
 
 // pressure_solver.f90:98-118 of the mixed form
-void launch_project_exchange_f(Model &m)
+void launch_project_exchange_f(Model &m, float *dp)
 {
     {
         ProfScope ps(m, "project");
         hipLaunchKernelGGL(k_project_f, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, m.fh[0], m.fl[0],
-                           m.fh[1], m.fl[1], m.fh[2], m.fl[2], m.pf);
+                           m.fh[1], m.fl[1], m.fh[2], m.fl[2], dp, m.pint);
         PS3D_HIP_CHECK(hipGetLastError());
     }
+    // the increment's halo (Neumann in z under a lid) serves the gradient of
+    // the next steps' pressure-history term
     HaloArgsT<float> h{};
-    h.f[0] = m.fh[0]; h.f[1] = m.fh[1]; h.f[2] = m.fh[2]; h.f[3] = m.pf;
+    h.f[0] = m.fh[0]; h.f[1] = m.fh[1]; h.f[2] = m.fh[2]; h.f[3] = dp;
     h.neumann[3] = m.prm.enable_vertical_boundaries;
     halo_launch(m, h, 4);
+    launch_border_exchg(m, m.pint, m.prm.enable_vertical_boundaries);
 }
 
 void launch_halo_f(Model &m, float *const *arrays, const int *neumann, int n)
@@ -1060,6 +1255,42 @@ void launch_convert(Model &m, int dir, int64_t n, const double *src_d, double *d
 {
     hipLaunchKernelGGL(k_convert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, dir, n, src_d, dst_d,
                        src_f, dst_f, src_lo, dst_lo);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+// ---- batched ensemble stages ------------------------------------------------
+void ens_tendency(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st)
+{
+    if (!map.start[map.n]) return;
+    hipLaunchKernelGGL(k_ens_tendency, dim3((unsigned)map.start[map.n]), dim3(TTHR), 0, s, mem, map, st);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_friction_ab(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st)
+{
+    if (!map.start[map.n]) return;
+    hipLaunchKernelGGL(k_ens_fric, dim3((unsigned)map.start[map.n]), dim3(FTHREADS), 0, s, mem, map, st);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_halo(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, int after_projection)
+{
+    if (!map.start[map.n]) return;
+    hipLaunchKernelGGL(k_ens_halo, dim3((unsigned)map.start[map.n]), dim3(256), 0, s, mem, map, st,
+                       after_projection);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_project(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st)
+{
+    if (!map.start[map.n]) return;
+    hipLaunchKernelGGL(k_ens_project, dim3((unsigned)map.start[map.n]), dim3(BX, BY), 0, s, mem, map, st);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_advance(hipStream_t s, EnsState *st)
+{
+    hipLaunchKernelGGL(k_ens_advance, dim3(1), dim3(64), 0, s, st);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 

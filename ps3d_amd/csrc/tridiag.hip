@@ -46,12 +46,10 @@ __global__ __launch_bounds__(256) void k_tri_factor(const double *__restrict__ k
 
 // one wavefront per work-group: the (nx/2+1)*ny columns are few (33K at
 // 256x256), and 64-lane groups spread them over every CU
-__global__ __launch_bounds__(64) void k_thomas(double2 *__restrict__ fs, const double *__restrict__ ksqr,
-                                               const double *__restrict__ cp, const double *__restrict__ rm,
-                                               int64_t ncols, int nz, double fxa, double dz2, double dsqr)
+__device__ __forceinline__ void thomas_col(double2 *__restrict__ fs, const double *__restrict__ ksqr,
+                                           const double *__restrict__ cp, const double *__restrict__ rm,
+                                           int64_t ncols, int nz, double fxa, double dz2, double dsqr, int64_t col)
 {
-    const int64_t col = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (col >= ncols) return;
     double2 *d = fs + col;
     const double ks = ksqr[col];
     if (ks > 0) {
@@ -122,6 +120,27 @@ __global__ __launch_bounds__(64) void k_thomas(double2 *__restrict__ fs, const d
             bk = nxt;
         }
     }
+}
+
+__global__ __launch_bounds__(64) void k_thomas(double2 *__restrict__ fs, const double *__restrict__ ksqr,
+                                               const double *__restrict__ cp, const double *__restrict__ rm,
+                                               int64_t ncols, int nz, double fxa, double dz2, double dsqr)
+{
+    const int64_t col = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (col >= ncols) return;
+    thomas_col(fs, ksqr, cp, rm, ncols, nz, fxa, dz2, dsqr, col);
+}
+
+// the Thomas stage of a batched ensemble (rigid-lid members)
+__global__ __launch_bounds__(64) void k_ens_thomas(const EnsMember *__restrict__ mem, EnsMap map)
+{
+    const int g = (int)blockIdx.x;
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const int64_t ncols = (int64_t)e.nkx * e.c.ny;
+    const int64_t col = (int64_t)(g - map.start[mi]) * 64 + threadIdx.x;
+    if (col >= ncols) return;
+    thomas_col(e.FS, e.ksqr, e.cp, e.rm, ncols, e.c.nz, e.fxa, e.c.dz * e.c.dz, e.c.dsqr, col);
 }
 
 // periodic spectral step without the z transforms (building-block API)
@@ -261,6 +280,13 @@ void launch_thomas_f(Model &m, float2 *data)
         hipLaunchKernelGGL(k_thomas_f<false>, dim3(nb), dim3(64), 0, m.stream, data, m.dp64, m.ksqr, m.tri_cp,
                            m.tri_rm, ncols, m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr);
     }
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map)
+{
+    if (!map.start[map.n]) return;
+    hipLaunchKernelGGL(k_ens_thomas, dim3((unsigned)map.start[map.n]), dim3(64), 0, s, mem, map);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 

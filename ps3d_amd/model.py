@@ -31,7 +31,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check
-from .params import CParams, ModelParams
+from .params import PS3D_MIXED, CParams, ModelParams
 
 FIELDS = {"u": _lib.PS3D_U, "v": _lib.PS3D_V, "w": _lib.PS3D_W, "b": _lib.PS3D_B,
           "p": _lib.PS3D_P, "du": _lib.PS3D_DU, "dv": _lib.PS3D_DV, "dw": _lib.PS3D_DW,
@@ -91,12 +91,18 @@ class Model:
         raise AttributeError(name)
 
     # ---- host <-> device -----------------------------------------------
+    def _default_names(self):
+        # a mixed-precision model exchanges only the prognostic fields and p
+        if self.prm.precision == PS3D_MIXED:
+            return ("u", "v", "w", "b", "p")
+        return tuple(FIELDS)
+
     def push(self, names=None):
-        for n in names or FIELDS:
+        for n in names or self._default_names():
             check(self.lib.ps3d_set_field(self.h, FIELDS[n], _dp(self.f[n])))
 
     def pull(self, names=None):
-        for n in names or FIELDS:
+        for n in names or self._default_names():
             check(self.lib.ps3d_get_field(self.h, FIELDS[n], _dp(self.f[n])))
 
     def get_spectral(self) -> np.ndarray:

@@ -128,3 +128,23 @@ def test_fortran_dropin_links_against_reference_module(tmp_path):
     assert re.search(r"\bT integrate_\b", nm) and re.search(r"\bT pressure_solver_\b", nm)
     for sym in ("ps3d_integrate", "ps3d_create", "ps3d_pressure_solver"):
         assert re.search(rf"\bU {sym}\b", nm), sym
+
+
+@pytest.mark.parametrize("bad,msg", [
+    (dict(Avbi=1e-2), "vertical biharmonic"),
+    (dict(nx=40), "compiled-in FFT lengths"),
+    (dict(precision=1), "FP64"),
+])
+def test_ensemble_create_rejects_bad_members(bad, msg):
+    from ps3d_amd import _lib
+    from ps3d_amd.params import CParams, ModelParams
+    lib = _lib.load()
+    good = ModelParams(nx=32, ny=32, nz=16, Lx=1.0, Ly=1.0, Lz=1.0, dt=0.1, enable_vertical_boundaries=True)
+    base = dict(nx=32, ny=32, nz=16, Lx=1.0, Ly=1.0, Lz=1.0, dt=0.1, enable_vertical_boundaries=True)
+    base.update(bad)
+    arr = (CParams * 2)(CParams.from_params(good), CParams.from_params(ModelParams(**base)))
+    h = ctypes.c_void_p()
+    code = lib.ps3d_ensemble_create(2, arr, None, ctypes.byref(h))
+    assert code == 1
+    assert not h.value
+    assert "member 1" in _lib.last_error() and msg in _lib.last_error()
