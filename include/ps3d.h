@@ -149,6 +149,21 @@ int ps3d_my_ifft(ps3d_model *m, int three_d, const double *qs, double *qp);
    recurrence with a rigid lid, else the division by -ksqr */
 int ps3d_spectral_solve(ps3d_model *m, double *qs);
 
+/* ---- diagnostics ----------------------------------------------------- */
+
+/* Normal-mode balancing of the model's current u, v, w, b after Warn et al.
+   to third order: diag_balance (diag_balance.f90:89-362) with the discrete
+   normal modes of eigenvectors.f90:6-218 and, under a rigid lid, the
+   vertical-mode transforms of my_fft.f90:112-127, :211-252; FP64 on the
+   device.  out[16] receives u0, v0, w0, b0, u1 .. b1, u2 .. b2, u3 .. b3
+   (full arrays with halo, as module_diag_balance; NULL entries skipped).
+   kx, ky, kz are main_module's wavenumbers (NULL: computed as
+   wavenumber_initialisation does, my_fft.f90:87-110).  filter_width < 0:
+   enable_diag_balance_filter off, else diag_balance_filter_width.  The
+   model's fields, tendency slots and p are left as they were.  FP64 models. */
+int ps3d_diag_balance(ps3d_model *m, int filter_width, const double *kx, const double *ky, const double *kz,
+                      double *const *out);
+
 /* ---- measurement ----------------------------------------------------- */
 
 /* Time every launch of the named kernel with HIP events on the model's

@@ -50,6 +50,7 @@ PROTOTYPES = {
     "ps3d_spectral_solve": (_c_int, [_vp, _dp]),
     "ps3d_profile_kernel": (_c_int, [_vp, ctypes.c_char_p]),
     "ps3d_profile_read": (_c_int, [_vp, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_dbl)]),
+    "ps3d_diag_balance": (_c_int, [_vp, _c_int, _dp, _dp, _dp, ctypes.POINTER(_dp)]),
     "ps3d_ensemble_create": (_c_int, [_c_int, ctypes.POINTER(CParams), _vp, ctypes.POINTER(_vp)]),
     "ps3d_ensemble_destroy": (_c_int, [_vp]),
     "ps3d_ensemble_set_field": (_c_int, [_vp, _c_int, _c_int, _dp]),

@@ -6,10 +6,6 @@
 #include <cstring>
 #include <string>
 
-namespace ps3d {
-void launch_divide_ksqr(Model &m, double2 *data);
-}
-
 using namespace ps3d;
 
 namespace {
@@ -262,6 +258,11 @@ const char *kKernelNames =
 
 namespace ps3d {
 
+void model_step(Model &m, int itt, int tau, int taum1, int taum2)
+{
+    integrate(m, itt, tau, taum1, taum2);
+}
+
 std::string &error_slot()
 {
     thread_local std::string msg;
@@ -430,7 +431,7 @@ int ps3d_destroy(ps3d_model *m)
     return guard([&] {
         if (!m) return;
         if (m->stream) (void)hipStreamSynchronize(m->stream);
-        for (auto &ge : m->graphs) (void)hipGraphExecDestroy(ge.exec);
+        balance_release(*m);
         for (auto ev : m->prof_ev) (void)hipEventDestroy(ev);
         for (void *p : m->allocations) (void)hipFree(p);
         if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);
@@ -651,6 +652,43 @@ int ps3d_spectral_solve(ps3d_model *m, double *qs)
         else launch_divide_ksqr(*m, m->FS);
         PS3D_HIP_CHECK(hipMemcpyAsync(qs, m->FS, sizeof(double2) * m->g.S, hipMemcpyDeviceToHost, m->stream));
         PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
+    });
+}
+
+int ps3d_diag_balance(ps3d_model *m, int filter_width, const double *kx, const double *ky, const double *kz,
+                      double *const *out)
+{
+    return guard([&] {
+        check_model(m);
+        require_fp64(*m, "ps3d_diag_balance");
+        PS3D_REQUIRE(out, "null output table");
+        const ps3d_params &p = m->prm;
+        // my_fft.f90:87-110 where the caller passes no wavenumbers
+        const double pi = std::acos(0.0) * 2.0;
+        std::vector<double> wx, wy, wz;
+        if (!kx) {
+            wx.resize(m->g.nkx);
+            for (int i = 1; i <= m->g.nkx; ++i) wx[i - 1] = (double)(2 * (i - 1)) * pi / p.Lx;
+            kx = wx.data();
+        }
+        if (!ky) {
+            wy.resize(p.ny);
+            for (int j = 1; j <= (p.ny + 1) / 2; ++j) wy[j - 1] = (double)(2 * (j - 1)) * pi / p.Ly;
+            for (int j = (p.ny + 1) / 2 + 1; j <= p.ny; ++j) wy[j - 1] = -((double)(2 * (p.ny - j + 1)) * pi / p.Ly);
+            ky = wy.data();
+        }
+        if (!kz) {
+            wz.resize(p.nz);
+            if (p.enable_vertical_boundaries) {
+                for (int k = 1; k <= p.nz; ++k) wz[k - 1] = (double)(float)(k - 1) * pi / p.Lz;
+            } else {
+                for (int k = 1; k <= (p.nz + 1) / 2; ++k) wz[k - 1] = (double)(2 * (k - 1)) * pi / p.Lz;
+                for (int k = (p.nz + 1) / 2 + 1; k <= p.nz; ++k)
+                    wz[k - 1] = -((double)(2 * (p.nz - k + 1)) * pi / p.Lz);
+            }
+            kz = wz.data();
+        }
+        diag_balance(*m, filter_width, out, kx, ky, kz);
     });
 }
 

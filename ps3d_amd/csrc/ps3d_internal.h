@@ -181,9 +181,8 @@ struct Model {
     int64_t prof_launches = 0;
     double prof_ms = 0.0;
 
-    // graphs of a whole step, keyed by (AB order kind, tau)
-    struct GraphEntry { int key; hipGraphExec_t exec; };
-    std::vector<GraphEntry> graphs;
+    // work arrays of the balancing diagnostic (balance.hip), made on first use
+    struct BalanceWork *bal = nullptr;
 
     double *field_buf(int id) const;
     double *cur_buf(int f) const { return cur[f] ? alt[f] : field_buf(f); }
@@ -220,6 +219,17 @@ std::string fft_sizes_text();
 void tridiag_factor(Model &m);
 void launch_thomas(Model &m, double2 *data);
 void launch_thomas_f(Model &m, float2 *data);
+
+void launch_advection_fields(Model &m, const double *u, const double *v, const double *w, const double *b,
+                             double *du, double *dv, double *dw, double *db);
+// one whole step, integrate.f90:2-115 (api.hip)
+void model_step(Model &m, int itt, int tau, int taum1, int taum2);
+void launch_divide_ksqr(Model &m, double2 *data);
+// frees the host side of the balancing work (device arrays are in allocations)
+void balance_release(Model &m);
+// the balancing diagnostic (balance.hip); out[16] host arrays, nullable
+void diag_balance(Model &m, int filter_width, double *const *out, const double *kx, const double *ky,
+                  const double *kz);
 
 // ---- mixed form (stencil.hip) -------------------------------------------
 // tendency with the reference pressure pint (null: zero)

@@ -973,6 +973,18 @@ void launch_buoyancy_advection(Model &m, int tau)
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
+// advection.f90:2-184 of arbitrary fields (diag_balance.f90:500-535): the
+// advective tendencies only, no Coriolis / buoyancy terms
+void launch_advection_fields(Model &m, const double *u, const double *v, const double *w, const double *b,
+                             double *du, double *dv, double *dw, double *db)
+{
+    int kc = 1;
+    const dim3 grid = march_grid(m.g, kc);
+    hipLaunchKernelGGL((k_tendency<true, true, false>), grid, dim3(BX, BY), 0, m.stream, m.c, u, v, w, b, du, dv,
+                       dw, db, kc);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
 void launch_momentum_advection(Model &m, int tau)
 {
     const int64_t off = (int64_t)(tau - 1) * m.g.H;

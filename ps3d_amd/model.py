@@ -191,6 +191,27 @@ class Model:
         return buf[0] + 1j * buf[1]
 
     # ---- measurement ---------------------------------------------------------
+    def diag_balance(self, filter_width: int | None = None, kx=None, ky=None, kz=None) -> dict:
+        """Normal-mode balancing of the model's current state
+        (diag_balance.f90:89-362): {'u0': .., 'v0': .., ..., 'b3': ..}, full
+        arrays with halo.  The model's own fields are left unchanged."""
+        shp = self.prm.shape
+        names = [f"{c}{o}" for o in range(4) for c in ("u", "v", "w", "b")]
+        out = {n: np.zeros(shp, order="F") for n in names}
+        tab = (ctypes.POINTER(ctypes.c_double) * 16)(*[_dp(out[n]) for n in names])
+        ks = []
+        for k in (kx, ky, kz):
+            if k is None:
+                ks.append(None)
+            else:
+                a = np.ascontiguousarray(k, dtype=np.float64)
+                ks.append(a)
+        self._ks = ks
+        ptr = [None if k is None else _dp(k) for k in ks]
+        check(self.lib.ps3d_diag_balance(self.h, -1 if filter_width is None else int(filter_width),
+                                         ptr[0], ptr[1], ptr[2], tab))
+        return out
+
     def profile_kernel(self, name: str | None):
         check(self.lib.ps3d_profile_kernel(self.h, name.encode() if name else None))
 
