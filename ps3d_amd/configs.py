@@ -132,15 +132,19 @@ def eddy(nx=64, ny=64, nz=32, seed=1, radius_frac=1.0 / 6.0, noise=1e-3, cfl=0.3
     return prm, _finish(prm, {"u": u, "v": v, "w": np.zeros_like(u), "b": b})
 
 
-def jets(nx=256, ny=256, nz=64, seed=2, umax=0.5, noise=1e-3, friction=True, precision=PS3D_FP64):
+def jets(nx=256, ny=256, nz=64, seed=2, umax=0.1, noise=1e-3, friction=True, precision=PS3D_FP64):
     """Two zonal jets of opposite sign, geostrophically balanced, rigid lid.
 
-    Scales follow for_config/jets_unscaled.f90:5-42 (Lx = Ly = 500 km,
-    Lz = 500 m with the lid, f0 = 1e-4, N0 = 50 f0, dt = 0.5/N0).  Jets
-    +-umax at y = Ly/4, 3Ly/4 with Gaussian width Ly/10, times cos(pi z/Lz);
-    b from the thermal-wind cumsum in y of jets_unscaled.f90:80-95; N(0,noise)
-    on u, v, w, b.  Harmonic (Ah, Av, Kh, Kv) and horizontal biharmonic
-    (Ahbi, Khbi) friction sized to the grid.
+    The set-up of for_config/jets_unscaled.f90:5-95 at any resolution:
+    Lx = Ly = 500 km, Lz = 500 m with the lid, f0 = 1e-4, N0 = 50 f0,
+    dt = 0.5/N0; u = umax (jets at y = Ly/4, 3Ly/4 of Gaussian width
+    0.02 Lx + 0.02 sin(10 pi x/Lx) sin(2 pi y/Ly)) cos(pi z/Lz); b from the
+    thermal-wind cumsum in y (:80-95).  Added: N(0, noise) on u and v, and
+    harmonic (Ah, Av, Kh, Kv) and horizontal biharmonic (Ahbi, Khbi)
+    friction sized to the grid.  (Stronger, wider jets -- 0.5 m/s over
+    Ly/10 -- or noise on b of 1e-3 make the thermal-wind stratification
+    statically unstable at N0 = 50 f0, d(b)/dz ~ 1.7e-4 > N0^2 = 2.5e-5,
+    and the run convects and blows up within ~50 steps.)
     """
     f0 = 1e-4
     N0 = 50 * f0
@@ -148,11 +152,13 @@ def jets(nx=256, ny=256, nz=64, seed=2, umax=0.5, noise=1e-3, friction=True, pre
     Lz = 500.0
     dx, dy, dz = Lx / nx, Ly / ny, Lz / nz
     dt = 0.5 / N0
-    width = Ly / 10
+    width = Lx * 0.02
+    xi = (np.arange(1, nx + 1) - 1) * dx
     yj = (np.arange(1, ny + 1) - 1) * dy
     zk = (np.arange(1, nz + 1) - 1) * dz
     prof = (np.exp(-(yj - 3 * Ly / 4) ** 2 / width ** 2) - np.exp(-(yj - Ly / 4) ** 2 / width ** 2))
-    u = umax * prof[None, :, None] * np.cos(math.pi * zk / Lz)[None, None, :] * np.ones((nx, 1, 1))
+    wave = 0.02 * np.sin(xi / Lx * 10 * math.pi)[:, None] * np.sin(yj / Ly * 2 * math.pi)[None, :]
+    u = umax * (prof[None, :] + wave)[:, :, None] * np.cos(zk / Lz * math.pi)[None, None, :]
     # b_y = -f u_z by one-sided/centred differences and a cumulative sum in y
     up = np.concatenate([u[:, :, :1], u, u[:, :, -1:]], axis=2)
     b = np.empty_like(u)
@@ -166,8 +172,7 @@ def jets(nx=256, ny=256, nz=64, seed=2, umax=0.5, noise=1e-3, friction=True, pre
     b = np.cumsum(b, axis=1)
     u = u + noise * normal(seed, "u", nx, ny, nz)
     v = noise * normal(seed, "v", nx, ny, nz)
-    w = noise * normal(seed, "w", nx, ny, nz)
-    b = b + noise * normal(seed, "b", nx, ny, nz)
+    w = np.zeros_like(u)
     kw = {}
     if friction:
         kw = dict(Ah=1e-3 * dx ** 2 / dt, Av=1e-2 * dz ** 2 / dt, Kh=1e-3 * dx ** 2 / dt,

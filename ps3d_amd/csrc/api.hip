@@ -173,12 +173,17 @@ void cold_start_mixed(Model &m, int itt, int tau, int taum1, int taum2)
     launch_friction_ab_mixed(m, 0, tau, taum1, taum2, 0.0, nullptr, 0.0, nullptr, false);
     (void)itt;
     float *est = m.dpr[0];
+    {   // the divergence reads the provisional fields' halos
+        float *hal[3] = {m.ah[0], m.ah[1], m.ah[2]};
+        launch_halo_f(m, hal, nullptr, 3);
+    }
     solve_mixed(m, m.ah[0], m.ah[1], m.ah[2], est);
     launch_halo_f(m, &est, &m.prm.enable_vertical_boundaries, 1);
     launch_convert(m, 3, m.g.H, nullptr, m.pint, est, nullptr, nullptr, nullptr);
     for (int q = 0; q < 3; ++q) launch_convert(m, 4, m.g.H, nullptr, nullptr, nullptr, m.dpr[q], nullptr, nullptr);
     m.dp_cur = 0;
     m.cold = false;
+    m.refine = 2;
 }
 
 // One step of the mixed form.  With p_{n-1} the FP64 pressure of the last
@@ -211,6 +216,18 @@ void integrate_mixed(Model &m, int itt, int tau, int taum1, int taum2)
     const int nxt = (m.dp_cur + 1) % 3;
     solve_mixed(m, m.fh[0], m.fh[1], m.fh[2], m.dpr[nxt]);
     launch_project_exchange_f(m, m.dpr[nxt]);
+    if (m.refine > 0) {
+        // The first steps' increments are large (the first projection
+        // removes the initial state's divergence and the next one takes that
+        // pressure back), and the FP32 solve's error is relative to the
+        // increment: one correction solve on the projected fields, its
+        // result added to the projection, to pint and to the increment.
+        float *corr = m.dpr[(nxt + 1) % 3];   // dp_{n-2}: not needed any more
+        solve_mixed(m, m.fh[0], m.fh[1], m.fh[2], corr);
+        launch_project_exchange_f(m, corr);
+        launch_convert(m, 5, m.g.H, nullptr, nullptr, corr, m.dpr[nxt], nullptr, nullptr);
+        m.refine -= 1;
+    }
     m.dp_cur = nxt;
     for (int f = 0; f < 4; ++f) m.lo_stale[f] = true;
     m.p_host.clear();

@@ -39,6 +39,7 @@ struct ps3d_ensemble {
     EnsState state{0, 3, 2, 1, 0};   // the device state once the queued steps have run
     EnsMap maps[S_NSTAGE]{};
     size_t lds_x = 0, lds_y = 0, lds_z = 0;
+    int ny_max = 0, nz_max = 0;         // longest y and (periodic) z columns
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::map<std::string, int> tuned;   // "stage shape" -> levels per work-group
@@ -150,11 +151,16 @@ void build_maps(ps3d_ensemble &E)
         E.maps[s].start[n] = (int)acc;
     }
     E.lds_x = E.lds_y = E.lds_z = 0;
+    E.ny_max = E.nz_max = 0;
     for (int i = 0; i < n; ++i) {
         const Grid &g = E.mem[i]->g;
         E.lds_x = std::max(E.lds_x, ens_x_lds(g.nx / 2));
         E.lds_y = std::max(E.lds_y, ens_cols_lds(g.ny));
-        if (!E.tab[i].c.lid) E.lds_z = std::max(E.lds_z, ens_cols_lds(g.nz));
+        E.ny_max = std::max(E.ny_max, g.ny);
+        if (!E.tab[i].c.lid) {
+            E.lds_z = std::max(E.lds_z, ens_cols_lds(g.nz));
+            E.nz_max = std::max(E.nz_max, g.nz);
+        }
     }
 }
 
@@ -200,10 +206,10 @@ void enqueue_step(ps3d_ensemble &E)
     ens_friction_ab(s, E.dev_tab, E.maps[S_FRIC], E.dev_state);
     ens_halo(s, E.dev_tab, E.maps[S_HALO], E.dev_state, 0);
     ens_r2c(s, E.dev_tab, E.maps[S_R2C], E.dev_state, E.lds_x);
-    ens_cols_y(s, E.dev_tab, E.maps[S_COLS], -1, E.lds_y);
+    ens_cols_y(s, E.dev_tab, E.maps[S_COLS], -1, E.ny_max, E.lds_y);
     ens_thomas(s, E.dev_tab, E.maps[S_THOMAS]);
-    ens_zsolve(s, E.dev_tab, E.maps[S_ZSOLVE], E.lds_z);
-    ens_cols_y(s, E.dev_tab, E.maps[S_COLS], +1, E.lds_y);
+    ens_zsolve(s, E.dev_tab, E.maps[S_ZSOLVE], E.nz_max, E.lds_z);
+    ens_cols_y(s, E.dev_tab, E.maps[S_COLS], +1, E.ny_max, E.lds_y);
     ens_c2r(s, E.dev_tab, E.maps[S_R2C], E.lds_x);
     ens_project(s, E.dev_tab, E.maps[S_PROJ], E.dev_state);
     ens_halo(s, E.dev_tab, E.maps[S_HALO], E.dev_state, 1);

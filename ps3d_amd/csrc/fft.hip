@@ -915,7 +915,9 @@ __global__ __launch_bounds__(TTHREADS) void k_ens_c2r(const EnsMember *__restric
 
 // y-pass over every member: forward scaled by dx dy under a rigid lid
 // (my_fft_2D), unscaled otherwise (the 3-D scaling is in the z solve)
-template <int SIGN>
+// NMAX bounds the lengths compiled into the kernel: the register budget of
+// the longest body sets the occupancy of every member
+template <int SIGN, int NMAX>
 __global__ __launch_bounds__(TTHREADS) void k_ens_cols_y(const EnsMember *__restrict__ mem, EnsMap map)
 {
     const int g = (int)blockIdx.x;
@@ -929,7 +931,7 @@ __global__ __launch_bounds__(TTHREADS) void k_ens_cols_y(const EnsMember *__rest
     switch (e.c.ny) {
 #define CASE(n)                                                                                              \
     case n:                                                                                                  \
-        if constexpr (TCOLS(n) == TTHREADS)                                                                  \
+        if constexpr (n <= NMAX && TCOLS(n) == TTHREADS)                                                     \
             cols_body<n, TCOL, TTHREADS, SIGN, double2>(ens_lds, bx, by, e.twy, e.FS, e.nkx, (int64_t)e.nkx,  \
                                                         (int64_t)e.nkx * e.c.ny, s1, s2, 0.0, nscale);       \
         break;
@@ -939,6 +941,7 @@ __global__ __launch_bounds__(TTHREADS) void k_ens_cols_y(const EnsMember *__rest
     }
 }
 
+template <int NMAX>
 __global__ __launch_bounds__(TTHREADS) void k_ens_zsolve(const EnsMember *__restrict__ mem, EnsMap map)
 {
     const int g = (int)blockIdx.x;
@@ -950,7 +953,7 @@ __global__ __launch_bounds__(TTHREADS) void k_ens_zsolve(const EnsMember *__rest
     switch (e.c.nz) {
 #define CASE(n)                                                                                              \
     case n:                                                                                                  \
-        if constexpr (TCOLS(n) == TTHREADS)                                                                  \
+        if constexpr (n <= NMAX && TCOLS(n) == TTHREADS)                                                     \
             zsolve_body<n, TCOL, TTHREADS, double2>(ens_lds, bx, by, e.twz, e.FS, e.ksqr, e.nkx, e.nkx,       \
                                                     e.c.ny, e.c.dx, e.c.dy, e.c.dz);                         \
         break;
@@ -1234,26 +1237,46 @@ void ens_c2r(hipStream_t s, const EnsMember *mem, const EnsMap &map, size_t lds)
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
-void ens_cols_y(hipStream_t s, const EnsMember *mem, const EnsMap &map, int sign, size_t lds)
+template <int SIGN, int NMAX>
+static void cols_y_launch(hipStream_t s, const EnsMember *mem, const EnsMap &map, size_t lds)
+{
+    ens_lds_attr(ft::k_ens_cols_y<SIGN, NMAX>, lds);
+    hipLaunchKernelGGL((ft::k_ens_cols_y<SIGN, NMAX>), dim3((unsigned)map.start[map.n]), dim3(ft::TTHREADS), lds, s,
+                       mem, map);
+}
+
+template <int SIGN>
+static void cols_y_nmax(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nmax, size_t lds)
+{
+    if (nmax <= 64) cols_y_launch<SIGN, 64>(s, mem, map, lds);
+    else if (nmax <= 128) cols_y_launch<SIGN, 128>(s, mem, map, lds);
+    else if (nmax <= 256) cols_y_launch<SIGN, 256>(s, mem, map, lds);
+    else cols_y_launch<SIGN, 512>(s, mem, map, lds);
+}
+
+void ens_cols_y(hipStream_t s, const EnsMember *mem, const EnsMap &map, int sign, int nmax, size_t lds)
 {
     if (!map.start[map.n]) return;
-    if (sign < 0) {
-        ens_lds_attr(ft::k_ens_cols_y<-1>, lds);
-        hipLaunchKernelGGL(ft::k_ens_cols_y<-1>, dim3((unsigned)map.start[map.n]), dim3(ft::TTHREADS), lds, s, mem,
-                           map);
-    } else {
-        ens_lds_attr(ft::k_ens_cols_y<+1>, lds);
-        hipLaunchKernelGGL(ft::k_ens_cols_y<+1>, dim3((unsigned)map.start[map.n]), dim3(ft::TTHREADS), lds, s, mem,
-                           map);
-    }
+    if (sign < 0) cols_y_nmax<-1>(s, mem, map, nmax, lds);
+    else cols_y_nmax<+1>(s, mem, map, nmax, lds);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
-void ens_zsolve(hipStream_t s, const EnsMember *mem, const EnsMap &map, size_t lds)
+template <int NMAX>
+static void zsolve_launch(hipStream_t s, const EnsMember *mem, const EnsMap &map, size_t lds)
+{
+    ens_lds_attr(ft::k_ens_zsolve<NMAX>, lds);
+    hipLaunchKernelGGL(ft::k_ens_zsolve<NMAX>, dim3((unsigned)map.start[map.n]), dim3(ft::TTHREADS), lds, s, mem,
+                       map);
+}
+
+void ens_zsolve(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nmax, size_t lds)
 {
     if (!map.start[map.n]) return;
-    ens_lds_attr(ft::k_ens_zsolve, lds);
-    hipLaunchKernelGGL(ft::k_ens_zsolve, dim3((unsigned)map.start[map.n]), dim3(ft::TTHREADS), lds, s, mem, map);
+    if (nmax <= 64) zsolve_launch<64>(s, mem, map, lds);
+    else if (nmax <= 128) zsolve_launch<128>(s, mem, map, lds);
+    else if (nmax <= 256) zsolve_launch<256>(s, mem, map, lds);
+    else zsolve_launch<512>(s, mem, map, lds);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 

@@ -168,6 +168,7 @@ struct Model {
     float *dpr[3] = {};               // ring of pressure increments dp_n, dp_n-1, dp_n-2
     int dp_cur = 0;                   // dpr index of the latest increment
     bool cold = true;                 // no pressure history yet: the next step starts one
+    int refine = 0;                   // steps left that add a second (correction) solve
     float2 *FSf = nullptr;            // spectral array (of the increment)
     double2 *dp64 = nullptr;          // forward-sweep values of the Thomas solve
     double *stage = nullptr;          // 3H doubles for host conversions
@@ -308,8 +309,9 @@ void ens_halo(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsS
 void ens_project(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st);
 void ens_advance(hipStream_t s, EnsState *st);
 void ens_r2c(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, size_t lds);
-void ens_cols_y(hipStream_t s, const EnsMember *mem, const EnsMap &map, int sign, size_t lds);
-void ens_zsolve(hipStream_t s, const EnsMember *mem, const EnsMap &map, size_t lds);
+// nmax: the longest column among the members (selects the compiled variant)
+void ens_cols_y(hipStream_t s, const EnsMember *mem, const EnsMap &map, int sign, int nmax, size_t lds);
+void ens_zsolve(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nmax, size_t lds);
 void ens_c2r(hipStream_t s, const EnsMember *mem, const EnsMap &map, size_t lds);
 void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map);
 // block counts and LDS bytes of the batched FFT stages for one member

@@ -481,8 +481,15 @@ struct FricArgs {
 // T = double: the drop-in, bit-exact.  T = float: the mixed form -- stencils
 // in FP32, and the new state (hi + lo) + harmonic increment + biharmonic
 // increment + AB increment summed in FP64 and split back into (hi, lo).
+// field of work-group bid: blocks run over (tile, chunk, field)
 template <class T>
-__device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T> &a, int bid)
+__device__ __forceinline__ int fric_field(const FricArgs<T> &a, int bid)
+{
+    return bid / (a.tiles_x * a.tiles_y) / a.nchunk;
+}
+
+template <class T>
+__device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T> &a, const FricField<T> &F, int bid)
 {
     constexpr bool MIXED = std::is_same<T, float>::value;
     __shared__ T f0p[3][R0Y][R0X];
@@ -493,8 +500,6 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
     const int tile = bid % ntile;
     const int rest = bid / ntile;
     const int ch = rest % a.nchunk;
-    const int fi = rest / a.nchunk;
-    const FricField<T> &F = a.f[fi];
     const int i0 = (tile / a.tiles_y) * FTX + 1;
     const int j0 = (tile % a.tiles_y) * FTY + 1;
     const int kb = 1 + ch * a.kc;
@@ -639,7 +644,8 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
 template <class T>
 __global__ __launch_bounds__(FTHREADS) void k_fric_ab(StepConsts c, FricArgs<T> a)
 {
-    fric_body<T>(c, a, xcd_remap((int)blockIdx.x, a.nblocks));
+    const int bid = xcd_remap((int)blockIdx.x, a.nblocks);
+    fric_body<T>(c, a, a.f[fric_field(a, bid)], bid);
 }
 
 // ---------------------------------------------------------------------------
@@ -756,7 +762,8 @@ __global__ __launch_bounds__(BX * BY) void k_project_f(StepConsts c, float *__re
 }
 
 // host-layout conversions of the mixed form: 0 split FP64 -> (hi, lo),
-// 1 join (hi, lo) -> FP64, 2 FP64 -> FP32, 3 FP32 -> FP64, 4 zero FP32
+// 1 join (hi, lo) -> FP64, 2 FP64 -> FP32, 3 FP32 -> FP64, 4 zero FP32,
+// 5 FP32 accumulate (df += sf)
 __global__ __launch_bounds__(256) void k_convert(int dir, int64_t n, const double *__restrict__ sd,
                                                  double *__restrict__ dd, const float *__restrict__ sf,
                                                  float *__restrict__ df, const float *__restrict__ slo,
@@ -775,8 +782,10 @@ __global__ __launch_bounds__(256) void k_convert(int dir, int64_t n, const doubl
         df[t] = (float)sd[t];
     } else if (dir == 3) {
         dd[t] = (double)sf[t];
-    } else {
+    } else if (dir == 4) {
         df[t] = 0.f;
+    } else {
+        df[t] = df[t] + sf[t];
     }
 }
 
@@ -817,23 +826,9 @@ __global__ __launch_bounds__(FTHREADS) void k_ens_fric(const EnsMember *__restri
     const EnsMember &e = mem[mi];
     const StepConsts c = e.c;
     const int par = st->par, itt = st->itt;
-    FricArgs<double> a{};
-    for (int f = 0; f < 4; ++f) {
-        FricField<double> &F = a.f[f];
-        const int q = f < 3 ? 0 : 1;
-        F.in = e.f[f][par];
-        F.out = e.f[f][par ^ 1];
-        F.t0 = e.tend[f] + (int64_t)(st->tau - 1) * e.H;
-        F.t1 = e.tend[f] + (int64_t)(st->taum1 - 1) * e.H;
-        F.t2 = e.tend[f] + (int64_t)(st->taum2 - 1) * e.H;
-        F.scale = (f == 2) ? c.dt / c.dsqr : c.dt;
-        F.harm = e.harm[q];
-        F.Hm = e.Hm[q];
-        F.Vm = e.Vm[q];
-        F.bih = e.bih[q];
-        F.Hbi = e.Hbi[q];
-        F.zero_top = (f == 2 && F.harm && c.lid) ? 1 : 0;
-    }
+    const int lb = g - map.start[mi];
+    FricArgs<double> a{};   // tiling and AB coefficients; the field record is built alone
+    a.kc = e.fric_kc; a.nchunk = e.fric_nchunk; a.tiles_x = e.fric_tx; a.tiles_y = e.fric_ty;
     if (itt == 0) {
         a.order = 1;
     } else if (itt == 1 || !e.ab3) {
@@ -846,8 +841,22 @@ __global__ __launch_bounds__(FTHREADS) void k_ens_fric(const EnsMember *__restri
         a.cb = -16.0 / 12.0;
         a.cc = 5.0 / 12.0;
     }
-    a.kc = e.fric_kc; a.nchunk = e.fric_nchunk; a.tiles_x = e.fric_tx; a.tiles_y = e.fric_ty;
-    fric_body<double>(c, a, g - map.start[mi]);
+    const int f = fric_field(a, lb);
+    const int q = f < 3 ? 0 : 1;
+    FricField<double> F{};
+    F.in = e.f[f][par];
+    F.out = e.f[f][par ^ 1];
+    F.t0 = e.tend[f] + (int64_t)(st->tau - 1) * e.H;
+    F.t1 = e.tend[f] + (int64_t)(st->taum1 - 1) * e.H;
+    F.t2 = e.tend[f] + (int64_t)(st->taum2 - 1) * e.H;
+    F.scale = (f == 2) ? c.dt / c.dsqr : c.dt;
+    F.harm = e.harm[q];
+    F.Hm = e.Hm[q];
+    F.Vm = e.Vm[q];
+    F.bih = e.bih[q];
+    F.Hbi = e.Hbi[q];
+    F.zero_top = (f == 2 && F.harm && c.lid) ? 1 : 0;
+    fric_body<double>(c, a, F, lb);
 }
 
 // halo of u, v, w, b in the new buffers after the friction stage, or of

@@ -4,9 +4,14 @@ test_balance_oracle.py).
 
 Advection, transforms and the 10-step model run are bit-identical to the
 oracle; the normal-mode vectors use the device's sin/cos/sqrt and complex
-division, which may differ from the host's in the last bit, so the outputs
-agree to a relative L2 tolerance of 1e-10 per field (measured errors are
-printed).  The model's own state is left untouched.
+division, which may differ from the host's in the last bit.  The zero- and
+first-order fields then agree to 1e-13.  The second and third orders are
+ill-conditioned -- they divide differences of nearly equal nonlinear terms
+by the wave frequency, and the third order also differences f2 over ten
+steps -- so a relative perturbation of 1e-15 of the input state moves them
+by up to ~1e-9 (second) and ~1e-6 (third order).  Their tolerance is that
+conditioning, measured on the oracle itself in the test (two perturbed
+oracle runs), times 20.  The model's own state is left untouched.
 """
 import numpy as np
 import pytest
@@ -14,7 +19,28 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 NAMES = [f"{c}{o}" for o in range(4) for c in ("u", "v", "w", "b")]
-TOL = 1e-10
+TOL_LOW = 1e-13
+
+
+def _sensitivity(prm, fields, filt, ref):
+    """largest relative change of each output under 1e-15 relative noise on
+    the initial state (two seeds), on the oracle"""
+    import oracle
+    from oracle import balance
+    worst = {n: 0.0 for n in NAMES}
+    for seed in (101, 102):
+        rng = np.random.default_rng(seed)
+        orc = oracle.OracleModel(prm)
+        for n, a in fields.items():
+            orc.f[n][...] = a * (1.0 + 1e-15 * rng.standard_normal(a.shape))
+        orc.run(2)
+        r = balance.Balance(orc, filter_width=filt).run()
+        for n in NAMES:
+            nb = float(np.linalg.norm(ref[n]))
+            if nb > 0:
+                worst[n] = max(worst[n], float(np.linalg.norm(r[n] - ref[n])) / nb)
+        orc.close()
+    return worst
 
 
 def _cases():
@@ -47,13 +73,15 @@ def test_diag_balance_matches_oracle(case, filt):
     before = {n: gpu.f[n].copy() for n in ("u", "v", "w", "b", "p", "du", "dv", "dw", "db")}
     print(f"diag_balance case={case} filter={filt}", flush=True)
     out = gpu.diag_balance(filter_width=filt)
-    ref = balance.Balance(orc, filter_width=filt).run()
+    ref = {k: v.copy() for k, v in balance.Balance(orc, filter_width=filt).run().items()}
+    sens = _sensitivity(prm, fields, filt, ref)
     for n in NAMES:
         a, b = out[n], ref[n]
         nb = float(np.linalg.norm(b))
         err = float(np.linalg.norm(a - b)) / nb if nb > 0 else float(np.linalg.norm(a))
-        print(f"  {n}: rel L2 = {err:.2e} (|ref| = {nb:.2e})", flush=True)
-        assert err <= TOL or (nb == 0 and err <= 1e-14), n
+        tol = max(TOL_LOW, 20.0 * sens[n])
+        print(f"  {n}: rel L2 = {err:.2e} (|ref| = {nb:.2e}, oracle conditioning {sens[n]:.1e})", flush=True)
+        assert err <= tol or (nb == 0 and err <= 1e-14), n
     # the model's state is restored
     gpu.pull()
     for n, a in before.items():
