@@ -10,7 +10,9 @@ steps, bracketed by device synchronisation.
 The JSON line also carries:
 * roofline: the kernel with the largest share of the step, timed with HIP
   events around each of its launches on the model's stream during the timed
-  steps; achieved = its algorithmic bytes per launch / mean launch time.
+  steps; achieved = its algorithmic bytes per launch / mean launch time;
+  traffic = its measured HBM bytes per launch (rocprofv3 FETCH_SIZE +
+  WRITE_SIZE, calibrated on a device copy; profiles/).
 * cpu_baseline: the CPU oracle (a C port of the reference step, one core)
   on a bounded sample of the same workload.
 * mixed: the mixed-precision form (FP32 stencils and transforms, FP64
@@ -60,6 +62,18 @@ def algorithmic_bytes(name: str, prm, order: int = 3) -> float:
         "halo": 0.0,
     }
     return float(table[name])
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of the kernel on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/), calibrated on a plain
+    device copy; None where no profile exists."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_fp64_256x256x64_jets.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic_bytes_per_launch"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def build_workload(precision=0):
@@ -278,7 +292,7 @@ def main() -> int:
                                "harmonic + horizontal biharmonic friction",
                    "grid": [prm.nx, prm.ny, prm.nz], "precision": "fp64", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
                      "launches": launches, "mean_launch_us": per_launch_s * 1e6,
                      "algorithmic_bytes_per_launch": traffic_bytes},
     }
