@@ -164,6 +164,29 @@ int ps3d_spectral_solve(ps3d_model *m, double *qs);
 int ps3d_diag_balance(ps3d_model *m, int filter_width, const double *kx, const double *ky, const double *kz,
                       double *const *out);
 
+/* diagnose.F90:11-53 on the device: out[8] = KE, PE (per grid point), max
+   |u_h|, max |w|, horizontal and vertical CFL, horizontal and vertical
+   Peclet numbers; *nan_seen = 1 if |u_h| was NaN (diagnose.F90:42-45, where
+   the reference aborts).  Fields stay on the device. */
+int ps3d_diagnose(ps3d_model *m, double *out, int32_t *nan_seen);
+
+/* ---- restart records (restart.f90:2-82, one rank) ---------------------- */
+
+/* write_restart / read_restart of the model's whole state (u, v, w, b, p and
+   the three tendency slots of du..db, halos included) in the reference's
+   unformatted sequential record layout: (nx, ny, nz, itt, time, taum2, taum1,
+   tau), (is_pe .. ke_pe), (u, v, w, b, p, du, dv, dw, db).  read checks the
+   grid and that the file comes from one rank.  FP64 models. */
+int ps3d_write_restart(ps3d_model *m, const char *path, int itt, double time, int tau, int taum1, int taum2);
+int ps3d_read_restart(ps3d_model *m, const char *path, int32_t *itt, double *time, int32_t *tau, int32_t *taum1,
+                      int32_t *taum2);
+/* the same records from / to host arrays: state = u, v, w, b, p (H doubles
+   each) then du, dv, dw, db (3 H each), H = (nx+4)(ny+4)(nz+4) */
+int ps3d_restart_write_host(const char *path, int nx, int ny, int nz, int itt, double time, int tau, int taum1,
+                            int taum2, const double *state);
+int ps3d_restart_read_host(const char *path, int nx, int ny, int nz, int32_t *itt, double *time, int32_t *tau,
+                           int32_t *taum1, int32_t *taum2, double *state);
+
 /* ---- measurement ----------------------------------------------------- */
 
 /* Time every launch of the named kernel with HIP events on the model's

@@ -40,6 +40,9 @@ pressure_solver         pressure_solver.f90:2-120       discrete divergence afte
                                                         round-off; p solves the discrete Poisson
                                                         problem; projection is idempotent
 integrate / run         integrate.f90, main.f90:139-161 integrate_pre (above) + pressure_solver
+restart records         restart.f90:2-82                the reference's own write_restart /
+                                                        read_restart from the same build: byte-
+                                                        identical files, both directions
 ======================  ==============================  =====================================
 """
 from __future__ import annotations
@@ -157,6 +160,8 @@ def load_reference():
         lib.ref_hat_ksqr.argtypes = [ctypes.c_double, ctypes.c_double]
         lib.ref_ab3.restype = ctypes.c_double
         lib.ref_ab3.argtypes = [ctypes.c_int]
+        if hasattr(lib, "ref_set_clock"):
+            lib.ref_set_clock.argtypes = [ctypes.c_double]
         _ref_lib = lib
     return _ref_lib
 

@@ -81,6 +81,12 @@ contains
   itt = itt_; tau = tau_; taum1 = taum1_; taum2 = taum2_
  end subroutine ref_set_time
 
+ ! main_module's model time (restart.f90 records it)
+ subroutine ref_set_clock(t) bind(C, name="ref_set_clock")
+  real(c_double), value :: t
+  time = t
+ end subroutine ref_set_clock
+
  ! advection.f90 into the current tau slot
  subroutine ref_buoyancy_advection() bind(C, name="ref_buoyancy_advection")
   call buoyancy_advection_2nd(db(:,:,:,tau))

@@ -709,6 +709,58 @@ int ps3d_diag_balance(ps3d_model *m, int filter_width, const double *kx, const d
     });
 }
 
+int ps3d_diagnose(ps3d_model *m, double *out, int32_t *nan_seen)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(out, "null output");
+        const bool nan = monitor(*m, out);
+        if (nan_seen) *nan_seen = nan ? 1 : 0;
+    });
+}
+
+int ps3d_write_restart(ps3d_model *m, const char *path, int itt, double time, int tau, int taum1, int taum2)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(path, "null path");
+        check_tau(tau, taum1, taum2);
+        int32_t it = itt, t0 = tau, t1 = taum1, t2 = taum2;
+        double tm = time;
+        restart_io(*m, path, true, &it, &tm, &t0, &t1, &t2);
+    });
+}
+
+int ps3d_read_restart(ps3d_model *m, const char *path, int32_t *itt, double *time, int32_t *tau, int32_t *taum1,
+                      int32_t *taum2)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(path && itt && time && tau && taum1 && taum2, "null argument");
+        restart_io(*m, path, false, itt, time, tau, taum1, taum2);
+    });
+}
+
+int ps3d_restart_write_host(const char *path, int nx, int ny, int nz, int itt, double time, int tau, int taum1,
+                            int taum2, const double *state)
+{
+    return guard([&] {
+        PS3D_REQUIRE(path && state, "null argument");
+        PS3D_REQUIRE(nx > 0 && ny > 0 && nz > 0, "bad grid size");
+        restart_write_host(path, nx, ny, nz, itt, time, tau, taum1, taum2, state);
+    });
+}
+
+int ps3d_restart_read_host(const char *path, int nx, int ny, int nz, int32_t *itt, double *time, int32_t *tau,
+                           int32_t *taum1, int32_t *taum2, double *state)
+{
+    return guard([&] {
+        PS3D_REQUIRE(path && state && itt && time && tau && taum1 && taum2, "null argument");
+        PS3D_REQUIRE(nx > 0 && ny > 0 && nz > 0, "bad grid size");
+        restart_read_host(path, nx, ny, nz, itt, time, tau, taum1, taum2, state);
+    });
+}
+
 int ps3d_profile_kernel(ps3d_model *m, const char *name)
 {
     return guard([&] {

@@ -184,6 +184,7 @@ struct Model {
 
     // work arrays of the balancing diagnostic (balance.hip), made on first use
     struct BalanceWork *bal = nullptr;
+    void *mon = nullptr;              // partial sums of the monitors (monitor.hip)
 
     double *field_buf(int id) const;
     double *cur_buf(int f) const { return cur[f] ? alt[f] : field_buf(f); }
@@ -228,6 +229,14 @@ void model_step(Model &m, int itt, int tau, int taum1, int taum2);
 void launch_divide_ksqr(Model &m, double2 *data);
 // frees the host side of the balancing work (device arrays are in allocations)
 void balance_release(Model &m);
+// monitors and restart records (monitor.hip)
+bool monitor(Model &m, double *out8);
+void restart_write_host(const char *path, int nx, int ny, int nz, int32_t itt, double time, int32_t tau,
+                        int32_t taum1, int32_t taum2, const double *state);
+void restart_read_host(const char *path, int nx, int ny, int nz, int32_t *itt, double *time, int32_t *tau,
+                       int32_t *taum1, int32_t *taum2, double *state);
+void restart_io(Model &m, const char *path, bool write, int32_t *itt, double *time, int32_t *tau, int32_t *taum1,
+                int32_t *taum2);
 // the balancing diagnostic (balance.hip); out[16] host arrays, nullable
 void diag_balance(Model &m, int filter_width, double *const *out, const double *kx, const double *ky,
                   const double *kz);

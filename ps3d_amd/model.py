@@ -212,6 +212,32 @@ class Model:
                                          ptr[0], ptr[1], ptr[2], tab))
         return out
 
+    def diagnose(self) -> dict:
+        """diagnose.F90:11-53 on the device: KE, PE (per point), max |u_h|,
+        max |w|, CFL and Peclet numbers, and whether a NaN was seen."""
+        out = np.zeros(8)
+        nan = ctypes.c_int32()
+        check(self.lib.ps3d_diagnose(self.h, _dp(out), ctypes.byref(nan)))
+        names = ("ke", "pe", "umax", "wmax", "cfl_h", "cfl_v", "pecl_h", "pecl_v")
+        d = dict(zip(names, out.tolist()))
+        d["values"] = out.tolist()
+        d["nan"] = bool(nan.value)
+        return d
+
+    def write_restart(self, path: str, time: float = 0.0):
+        """restart.f90:2-22 of the model's state and the driver's counters"""
+        check(self.lib.ps3d_write_restart(self.h, path.encode(), self.itt, float(time), self.tau, self.taum1,
+                                          self.taum2))
+
+    def read_restart(self, path: str) -> dict:
+        """restart.f90:27-82; sets the driver's counters from the file"""
+        v = [ctypes.c_int32() for _ in range(4)]
+        tm = ctypes.c_double()
+        check(self.lib.ps3d_read_restart(self.h, path.encode(), ctypes.byref(v[0]), ctypes.byref(tm),
+                                         ctypes.byref(v[1]), ctypes.byref(v[2]), ctypes.byref(v[3])))
+        self.itt, self.tau, self.taum1, self.taum2 = (x.value for x in v)
+        return {"itt": self.itt, "time": tm.value, "tau": self.tau, "taum1": self.taum1, "taum2": self.taum2}
+
     def profile_kernel(self, name: str | None):
         check(self.lib.ps3d_profile_kernel(self.h, name.encode() if name else None))
 
