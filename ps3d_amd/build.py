@@ -22,6 +22,8 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I", os.path.join(ROOT, "include")]
+# extra compile flags, e.g. PS3D_HIPCC_FLAGS="-DPS3D_FRIC_TY=16" (tile variants)
+FLAGS += os.environ.get("PS3D_HIPCC_FLAGS", "").split()
 
 
 def _run(cmd):

@@ -23,7 +23,7 @@ namespace {
 enum Stage { S_TEND, S_FRIC, S_HALO, S_R2C, S_COLS, S_THOMAS, S_ZSOLVE, S_PROJ, S_NSTAGE };
 
 // tile shapes of the k-marching kernels (stencil.hip)
-constexpr int kTTX = 64, kTTY = 8, kFTX = 64, kFTY = 8;
+constexpr int kTTX = TEND_TX, kTTY = TEND_TY, kFTX = FRIC_TX, kFTY = FRIC_TY;
 
 int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
@@ -75,7 +75,7 @@ int default_tend_kc(const Grid &g)
 int default_fric_kc(const Grid &g)
 {
     const int tiles = cdiv(g.nx, kFTX) * cdiv(g.ny, kFTY);
-    int nchunk = cdiv(2048, tiles * 4);
+    int nchunk = cdiv(2048 * 512 / (kFTX * kFTY), tiles * 4);
     nchunk = std::max(1, std::min(nchunk, g.nz));
     return cdiv(g.nz, nchunk);
 }

@@ -56,6 +56,14 @@ int guard(F &&f)
 
 constexpr int ONX = 2;   // halo width, main_module.f90:93
 
+// column tiles of the k-marching kernels (stencil.hip; the ensemble's block
+// counts use the same): tendency TEND_TX x TEND_TY, friction + AB FRIC_TX x FRIC_TY
+#ifndef PS3D_FRIC_TY
+#define PS3D_FRIC_TY 8
+#endif
+constexpr int TEND_TX = 64, TEND_TY = 8;
+constexpr int FRIC_TX = 64, FRIC_TY = PS3D_FRIC_TY;
+
 // Extents of one model.  Physical fields are (X, Y, Z) = (nx+4, ny+4, nz+4)
 // with Fortran index i in -1..nx+2 at offset i+1 (i fastest).
 struct Grid {

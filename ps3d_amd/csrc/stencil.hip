@@ -137,7 +137,7 @@ __global__ __launch_bounds__(BX * BY) void k_tendency(StepConsts c, const double
 // into registers while the current level is computed, so every value is
 // read from memory once per work-group.
 // ---------------------------------------------------------------------------
-constexpr int TTX = 64, TTY = 8;
+constexpr int TTX = TEND_TX, TTY = TEND_TY;
 constexpr int TPX = TTX + 2, TPY = TTY + 2;
 constexpr int TTHR = TTX * TTY;
 constexpr int TPER = (TPX * TPY + TTHR - 1) / TTHR;   // plane values per thread
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(BX * BY) void k_ab(StepConsts c, AbArgs a)
 // biharmonic_h to f1 and the AB update, writing a second buffer (neighbouring
 // tiles still read f0).
 // ---------------------------------------------------------------------------
-constexpr int FTX = 64, FTY = 8;
+constexpr int FTX = FRIC_TX, FTY = FRIC_TY;
 constexpr int R0X = FTX + 6, R0Y = FTY + 6;
 constexpr int R1X = FTX + 4, R1Y = FTY + 4;
 constexpr int FTHREADS = FTX * FTY;
@@ -1162,7 +1162,8 @@ static void friction_ab_impl(Model &m, const T *const *in, const T *const *in_lo
     a.tiles_x = (g.nx + FTX - 1) / FTX;
     a.tiles_y = (g.ny + FTY - 1) / FTY;
     const int tiles = a.tiles_x * a.tiles_y;
-    int nchunk = (2048 + tiles * 4 - 1) / (tiles * 4);
+    const int target = 2048 * 512 / FTHREADS;   // ~8 work-groups of 512 threads per CU
+    int nchunk = (target + tiles * 4 - 1) / (tiles * 4);
     nchunk = nchunk < 1 ? 1 : (nchunk > g.nz ? g.nz : nchunk);
     a.kc = (g.nz + nchunk - 1) / nchunk;
     a.nchunk = (g.nz + a.kc - 1) / a.kc;
