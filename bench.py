@@ -99,6 +99,7 @@ def mixed_leg(prm, fields, steps: int, warmup: int, fp64_value: float, check_ste
         m.f[n][...] = a
     m.push()
     m.run(max(warmup, 3))
+    m.tune(reps=5)
     el = time_steps(m, steps)
     value = prm.npoints * steps / el
     m.close()
@@ -223,6 +224,7 @@ def main() -> int:
     ap.add_argument("--profile-all", action="store_true", help="print every kernel's share")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--ens-steps", type=int, default=100, help="timed steps of the ensemble leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (to calibrate HBM byte counters)")
@@ -251,6 +253,8 @@ def main() -> int:
         model.f[n][...] = a
     model.push()
     model.run(max(args.warmup, 3))   # past Euler and AB2: the timed steps are AB3
+    if not args.no_tune:             # launch-shape autotuner (ps3d_tune), untimed set-up
+        model.tune(reps=5)
     model.synchronize()
 
     # share of each kernel over a few untimed steps (one kernel at a time)
@@ -298,6 +302,7 @@ def main() -> int:
     }
     if args.profile_all:
         out["kernel_shares_ms_per_step"] = {k: v[1] for k, v in sorted(shares.items(), key=lambda x: -x[1][1])}
+    out["tuning"] = model.tuning().strip().splitlines()
     model.close()
     if not args.no_mixed:
         out["mixed"] = mixed_leg(prm, fields, args.steps, args.warmup, value)

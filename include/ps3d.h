@@ -149,6 +149,19 @@ int ps3d_my_ifft(ps3d_model *m, int three_d, const double *qs, double *qp);
    recurrence with a rigid lid, else the division by -ksqr */
 int ps3d_spectral_solve(ps3d_model *m, double *qs);
 
+/* ---- launch-shape autotuner ------------------------------------------ */
+
+/* Times the candidate launch shapes of this model's step on the live model
+   (levels per work-group of the tendency and friction + AB kernels, columns
+   per wavefront of the Thomas solve) and keeps the fastest; the choice is
+   cached per (stage, grid, boundary, precision) in cache_path (NULL: none).
+   next_tau is the slot the next step writes (it is used as scratch).  The
+   model's state is unchanged and results do not depend on the shapes.  No
+   counterpart in the reference (its loop shapes are fixed by the compiler). */
+int ps3d_tune(ps3d_model *m, int next_tau, const char *cache_path, int reps);
+/* the shapes in use, "stage shape value" lines; returns the text length */
+int ps3d_tuning(ps3d_model *m, char *buf, int len);
+
 /* ---- diagnostics ----------------------------------------------------- */
 
 /* Normal-mode balancing of the model's current u, v, w, b after Warn et al.

@@ -50,6 +50,8 @@ PROTOTYPES = {
     "ps3d_spectral_solve": (_c_int, [_vp, _dp]),
     "ps3d_profile_kernel": (_c_int, [_vp, ctypes.c_char_p]),
     "ps3d_profile_read": (_c_int, [_vp, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_dbl)]),
+    "ps3d_tune": (_c_int, [_vp, _c_int, ctypes.c_char_p, _c_int]),
+    "ps3d_tuning": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
     "ps3d_diagnose": (_c_int, [_vp, _dp, _i32p]),
     "ps3d_write_restart": (_c_int, [_vp, ctypes.c_char_p, _c_int, _c_dbl, _c_int, _c_int, _c_int]),
     "ps3d_read_restart": (_c_int, [_vp, ctypes.c_char_p, _i32p, _dp, _i32p, _i32p, _i32p]),

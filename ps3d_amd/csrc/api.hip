@@ -709,6 +709,30 @@ int ps3d_diag_balance(ps3d_model *m, int filter_width, const double *kx, const d
     });
 }
 
+int ps3d_tune(ps3d_model *m, int next_tau, const char *cache_path, int reps)
+{
+    return guard([&] {
+        check_model(m);
+        tune_model(*m, next_tau, cache_path, reps);
+    });
+}
+
+int ps3d_tuning(ps3d_model *m, char *buf, int len)
+{
+    int n = 0;
+    const int rc = guard([&] {
+        check_model(m);
+        const std::string s = describe_tuning(*m);
+        n = (int)s.size();
+        if (buf && len > 0) {
+            const int c = n < len - 1 ? n : len - 1;
+            std::memcpy(buf, s.data(), (size_t)c);
+            buf[c] = '\0';
+        }
+    });
+    return rc == PS3D_OK ? n : -rc;
+}
+
 int ps3d_diagnose(ps3d_model *m, double *out, int32_t *nan_seen)
 {
     return guard([&] {

@@ -20,7 +20,7 @@ using namespace ps3d;
 
 namespace {
 
-enum Stage { S_TEND, S_FRIC, S_HALO, S_R2C, S_COLS, S_THOMAS, S_ZSOLVE, S_PROJ, S_NSTAGE };
+enum Stage { S_TEND, S_FRIC, S_R2C, S_COLS, S_THOMAS, S_ZSOLVE, S_PROJ, S_NSTAGE };
 
 // tile shapes of the k-marching kernels (stencil.hip)
 constexpr int kTTX = TEND_TX, kTTY = TEND_TY, kFTX = FRIC_TX, kFTY = FRIC_TY;
@@ -87,7 +87,6 @@ void fill_member(EnsMember &e, ps3d_model &m)
     e = EnsMember{};
     e.c = m.c;
     e.H = g.H;
-    e.nhalo = g.H - g.N;
     double *canon[4] = {m.u, m.v, m.w, m.b};
     double *tend[4] = {m.du, m.dv, m.dw, m.db};
     for (int q = 0; q < 4; ++q) {
@@ -137,7 +136,6 @@ void build_maps(ps3d_ensemble &E)
             switch (s) {
             case S_TEND: nb = (int64_t)e.tend_tx * e.tend_ty * e.tend_nchunk; break;
             case S_FRIC: nb = (int64_t)e.fric_tx * e.fric_ty * e.fric_nchunk * 4; break;
-            case S_HALO: nb = cdiv(4 * e.nhalo, 256); break;
             case S_R2C: nb = ens_r2c_blocks(e); break;
             case S_COLS: nb = ens_cols_blocks(e, g.ny, g.nz); break;
             case S_THOMAS: nb = lid ? cdiv((int64_t)g.nkx * g.ny, 64) : 0; break;
@@ -203,16 +201,14 @@ void enqueue_step(ps3d_ensemble &E)
 {
     hipStream_t s = E.stream;
     ens_tendency(s, E.dev_tab, E.maps[S_TEND], E.dev_state);
-    ens_friction_ab(s, E.dev_tab, E.maps[S_FRIC], E.dev_state);
-    ens_halo(s, E.dev_tab, E.maps[S_HALO], E.dev_state, 0);
+    ens_friction_ab(s, E.dev_tab, E.maps[S_FRIC], E.dev_state);   // writes the halo images too
     ens_r2c(s, E.dev_tab, E.maps[S_R2C], E.dev_state, E.lds_x);
     ens_cols_y(s, E.dev_tab, E.maps[S_COLS], -1, E.ny_max, E.lds_y);
     ens_thomas(s, E.dev_tab, E.maps[S_THOMAS]);
     ens_zsolve(s, E.dev_tab, E.maps[S_ZSOLVE], E.nz_max, E.lds_z);
     ens_cols_y(s, E.dev_tab, E.maps[S_COLS], +1, E.ny_max, E.lds_y);
     ens_c2r(s, E.dev_tab, E.maps[S_R2C], E.lds_x);
-    ens_project(s, E.dev_tab, E.maps[S_PROJ], E.dev_state);
-    ens_halo(s, E.dev_tab, E.maps[S_HALO], E.dev_state, 1);
+    ens_project(s, E.dev_tab, E.maps[S_PROJ], E.dev_state);       // and of u, v, w, p
     ens_advance(s, E.dev_state);
 }
 

@@ -118,6 +118,40 @@ __device__ __forceinline__ double sdiv<double>(double a, double b, double rb) { 
 template <>
 __device__ __forceinline__ float sdiv<float>(float a, double, double rb) { return a * (float)rb; }
 
+// Writes val, the new value at interior point (i, j, k), to every halo point
+// whose fully wrapped index is (i, j, k): what border_exchg_3D
+// (parallel_mpi.f90:70-211, one rank) copies there.  x and y are periodic;
+// z is periodic, or with neumann_z the halo levels take copies of levels 1
+// and nz (pressure_solver.f90:82-93).  Producers call it for the points they
+// write, so no separate exchange pass is needed.
+template <class T>
+__device__ __forceinline__ void write_images(const StepConsts &c, T *a, int i, int j, int k, T val, int neumann_z)
+{
+    int xs[3], ys[3], zs[3];
+    int nx_ = 0, ny_ = 0, nz_ = 0;
+    xs[nx_++] = i;
+    if (i <= 2) xs[nx_++] = i + c.nx;
+    if (i >= c.nx - 1) xs[nx_++] = i - c.nx;
+    ys[ny_++] = j;
+    if (j <= 2) ys[ny_++] = j + c.ny;
+    if (j >= c.ny - 1) ys[ny_++] = j - c.ny;
+    zs[nz_++] = k;
+    if (neumann_z) {
+        if (k == 1) { zs[nz_++] = 0; zs[nz_++] = -1; }
+        if (k == c.nz) { zs[nz_++] = c.nz + 1; zs[nz_++] = c.nz + 2; }
+    } else {
+        if (k <= 2) zs[nz_++] = k + c.nz;
+        if (k >= c.nz - 1) zs[nz_++] = k - c.nz;
+    }
+    if (nx_ == 1 && ny_ == 1 && nz_ == 1) return;
+    for (int a3 = 0; a3 < nz_; ++a3)
+        for (int a2 = 0; a2 < ny_; ++a2)
+            for (int a1 = 0; a1 < nx_; ++a1) {
+                if (a1 == 0 && a2 == 0 && a3 == 0) continue;
+                a[(int64_t)(xs[a1] + 1) + c.X * (int64_t)(ys[a2] + 1) + c.XY * (int64_t)(zs[a3] + 1)] = val;
+            }
+}
+
 // complex type of a real type
 template <class T> struct cplx_of;
 template <> struct cplx_of<double> { using type = double2; };
@@ -161,6 +195,10 @@ struct Model {
     int cur[4] = {0, 0, 0, 0};
     // use the fused friction + Adams-Bashforth kernel where it applies
     bool fuse_friction = true;
+    // launch shapes (results do not depend on them; set by ps3d_tune):
+    // levels per work-group of the k-marching kernels (0: heuristic) and
+    // columns per 64-lane wave of the Thomas solve (64, 32 or 16)
+    int tend_kc = 0, fric_kc = 0, thomas_cpw = 64;
 
     // mixed form (PS3D_MIXED): prognostic fields as FP32 (hi, lo) pairs whose
     // sum carries the FP64-accumulated state.  The pressure is carried in
@@ -230,6 +268,7 @@ void tridiag_factor(Model &m);
 void launch_thomas(Model &m, double2 *data);
 void launch_thomas_f(Model &m, float2 *data);
 
+void launch_friction_ab_raw(Model &m, int itt, int tau, int taum1, int taum2);
 void launch_advection_fields(Model &m, const double *u, const double *v, const double *w, const double *b,
                              double *du, double *dv, double *dw, double *db);
 // one whole step, integrate.f90:2-115 (api.hip)
@@ -237,6 +276,10 @@ void model_step(Model &m, int itt, int tau, int taum1, int taum2);
 void launch_divide_ksqr(Model &m, double2 *data);
 // frees the host side of the balancing work (device arrays are in allocations)
 void balance_release(Model &m);
+// the single-model autotuner (tune.hip)
+void tune_model(Model &m, int next_tau, const char *cache_path, int reps);
+std::string describe_tuning(const Model &m);
+
 // monitors and restart records (monitor.hip)
 bool monitor(Model &m, double *out8);
 void restart_write_host(const char *path, int nx, int ny, int nz, int32_t itt, double time, int32_t tau,
@@ -286,7 +329,7 @@ struct EnsMap {
 
 struct EnsMember {
     StepConsts c;
-    int64_t H, nhalo;
+    int64_t H;
     double *f[4][2];             // u, v, w, b: the two buffers of each field
     double *tend[4];             // du..db, 3 slots of H each
     double *p;
@@ -322,7 +365,6 @@ struct EnsState {
 // stage launchers (one launch each over all members)
 void ens_tendency(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st);
 void ens_friction_ab(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st);
-void ens_halo(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, int after_projection);
 void ens_project(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st);
 void ens_advance(hipStream_t s, EnsState *st);
 void ens_r2c(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, size_t lds);

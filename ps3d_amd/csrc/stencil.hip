@@ -206,9 +206,26 @@ __device__ __forceinline__ void tendency_body(const StepConsts &c, const TendArg
     const T Ro = (T)c.Ro;
     const double dx = c.dx, dy = c.dy, dz = c.dz;
     const T h = T(0.5), q4 = T(0.25), zero = T(0);
+    constexpr bool MIXED = std::is_same<T, float>::value;
     for (int k = kb; k <= ke; ++k) {
         store(k + 1);
         if (k + 2 <= nz + 2 && k + 1 <= ke) fetch(k + 2);
+        // mixed form: the lo parts and the reference pressure this point
+        // needs, issued before the barrier
+        float lu[4] = {}, lv[4] = {}, lb[2] = {};
+        double pp[4] = {};
+        if constexpr (MIXED) {
+            if (i <= nx && j <= ny) {
+                const int64_t o = idx(c, i, j, k);
+                const int64_t X = c.X, XY = c.XY;
+                lu[0] = a.ul[o - 1]; lu[1] = a.ul[o]; lu[2] = a.ul[o - 1 + X]; lu[3] = a.ul[o + X];
+                lv[0] = a.vl[o]; lv[1] = a.vl[o - X]; lv[2] = a.vl[o + 1]; lv[3] = a.vl[o + 1 - X];
+                lb[0] = a.bl[o]; lb[1] = a.bl[o + XY];
+                if (a.pint) {
+                    pp[0] = a.pint[o]; pp[1] = a.pint[o + 1]; pp[2] = a.pint[o + X]; pp[3] = a.pint[o + XY];
+                }
+            }
+        }
         __syncthreads();
         const int sm = (k - kb) % 3, sc = (k - kb + 1) % 3, sx = (k - kb + 2) % 3;
         if (i <= nx && j <= ny) {
@@ -284,18 +301,16 @@ __device__ __forceinline__ void tendency_body(const StepConsts &c, const TendArg
                 // mixed form: the Coriolis and buoyancy terms, which the
                 // pressure gradient nearly balances, and that gradient in FP64
                 // from (hi, lo); the slots keep tendency - grad(pint)
-                const int64_t X = c.X, XY = c.XY;
-                auto u64 = [&](int dj, int di) { return (double)U(0, dj, di) + (double)a.ul[o + di + dj * X]; };
-                auto v64 = [&](int dj, int di) { return (double)V(0, dj, di) + (double)a.vl[o + di + dj * X]; };
-                const double cu = (v64(0, 0) + v64(-1, 0) + v64(0, 1) + v64(-1, 1)) * 0.25 * c.f0;
-                const double cv = (u64(0, -1) + u64(0, 0) + u64(1, -1) + u64(1, 0)) * 0.25 * c.f0;
-                const double bb = 0.5 * (((double)B(0, 0, 0) + (double)a.bl[o]) + ((double)B(1, 0, 0) + (double)a.bl[o + XY]));
+                auto u64 = [&](int dj, int di, int q) { return (double)U(0, dj, di) + (double)lu[q]; };
+                auto v64 = [&](int dj, int di, int q) { return (double)V(0, dj, di) + (double)lv[q]; };
+                const double cu = (v64(0, 0, 0) + v64(-1, 0, 1) + v64(0, 1, 2) + v64(-1, 1, 3)) * 0.25 * c.f0;
+                const double cv = (u64(0, -1, 0) + u64(0, 0, 1) + u64(1, -1, 2) + u64(1, 0, 3)) * 0.25 * c.f0;
+                const double bb = 0.5 * (((double)B(0, 0, 0) + (double)lb[0]) + ((double)B(1, 0, 0) + (double)lb[1]));
                 double gx = 0.0, gy = 0.0, gz = 0.0;
                 if (a.pint) {
-                    const double pc = a.pint[o];
-                    gx = (a.pint[o + 1] - pc) * c.rdx;
-                    gy = (a.pint[o + X] - pc) * c.rdy;
-                    gz = (a.pint[o + XY] - pc) * c.rdz;
+                    gx = (pp[1] - pp[0]) * c.rdx;
+                    gy = (pp[2] - pp[0]) * c.rdy;
+                    gz = (pp[3] - pp[0]) * c.rdz;
                 }
                 const double su = ((double)au + cu) - gx;
                 const double sv = ((double)av - cv) - gy;
@@ -476,6 +491,7 @@ struct FricArgs {
     // u, v, w subtract scale * grad(Q) (see the step in api.hip)
     const float *d1, *d2;
     double qc1, qc2;
+    int halo;   // also write the new values' halo images (integrate.f90:106-111)
 };
 
 // T = double: the drop-in, bit-exact.  T = float: the mixed form -- stencils
@@ -545,12 +561,23 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
         if (k + 1 <= ke) fetch(k + 2);
         // this level's tendency slots (and lo part), issued before the LDS phases
         T s0 = zero, s1 = zero, s2 = zero, lo0 = zero;
+        float q1o = 0.f, q1n = 0.f, q2o = 0.f, q2n = 0.f;   // mixed form: pressure-history increments
         if (mine) {
             const int64_t o = idx(c, i0 + tpx, j0 + tpy, k);
             s0 = F.t0[o];
             if (a.order >= 2) s1 = F.t1[o];
             if (a.order >= 3) s2 = F.t2[o];
-            if (MIXED) lo0 = F.in_lo[o];
+            if constexpr (MIXED) {
+                lo0 = F.in_lo[o];
+                if (F.gstride && a.d1) {
+                    q1o = a.d1[o];
+                    q1n = a.d1[o + F.gstride];
+                    if (a.d2) {
+                        q2o = a.d2[o];
+                        q2n = a.d2[o + F.gstride];
+                    }
+                }
+            }
         }
         __syncthreads();
         const int sm = slot(k - 1), sc = slot(k), sp = slot(k + 1);
@@ -611,6 +638,7 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
                 else if (a.order == 2) val = f2 + s * (a.ca * s0 - a.cb * s1);
                 else val = f2 + s * (a.ca * s0 + a.cb * s1 + a.cc * s2);
                 F.out[o] = val;
+                if (a.halo) write_images<double>(c, F.out, i, j, k, val, 0);
             } else {
                 const double s = F.scale;
                 double ab;
@@ -623,11 +651,10 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
                 const double base = zt ? 0.0 : (double)f0p[sc][cy + 1][cx + 1] + (double)lo0;
                 double gq = 0.0;
                 if (F.gstride && a.d1) {
-                    const int64_t on = o + F.gstride;
-                    double qo = a.qc1 * (double)a.d1[o], qn = a.qc1 * (double)a.d1[on];
+                    double qo = a.qc1 * (double)q1o, qn = a.qc1 * (double)q1n;
                     if (a.d2) {
-                        qo += a.qc2 * (double)a.d2[o];
-                        qn += a.qc2 * (double)a.d2[on];
+                        qo += a.qc2 * (double)q2o;
+                        qn += a.qc2 * (double)q2n;
                     }
                     gq = F.scale * ((qn - qo) * F.grd);
                 }
@@ -635,6 +662,7 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
                 const float hi = (float)x;
                 F.out[o] = hi;
                 F.out_lo[o] = (float)(x - (double)hi);
+                if (a.halo) write_images<float>(c, F.out, i, j, k, hi, 0);
             }
         }
         __syncthreads();
@@ -707,29 +735,41 @@ __global__ __launch_bounds__(256) void k_halo(StepConsts c, HaloArgsT<T> h, int6
 // indices (periodic in x, y; periodic or Neumann in z), which is what its
 // exchanged halo holds, so p's own halo can be filled afterwards.
 // ---------------------------------------------------------------------------
+// With halo set, the halo images of the new u, v, w and of p (Neumann in z
+// under a lid) are written too: the exchange of pressure_solver.f90:112-118
+// and of p itself (pressure_solver.f90:82-93).  Only interior points of p
+// are read, so its halo can be written in the same pass.
 __device__ __forceinline__ void project_point(const StepConsts &c, double *__restrict__ u, double *__restrict__ v,
-                                              double *__restrict__ w, const double *__restrict__ p, int i, int j,
-                                              int k)
+                                              double *__restrict__ w, double *__restrict__ p, int i, int j, int k,
+                                              int halo)
 {
     const int64_t o = idx(c, i, j, k);
     const double pc = p[o];
     const double pe = p[idx(c, i == c.nx ? 1 : i + 1, j, k)];
     const double pn = p[idx(c, i, j == c.ny ? 1 : j + 1, k)];
     const double pt = p[idx(c, i, j, k == c.nz ? (c.lid ? c.nz : 1) : k + 1)];
-    u[o] = u[o] - div_rn(c.dt * (pe - pc), c.dx, c.rdx);
-    v[o] = v[o] - div_rn(c.dt * (pn - pc), c.dy, c.rdy);
-    if (c.lid && k >= c.nz) w[o] = 0.0;
-    else w[o] = w[o] - div_rn((c.dt / c.dsqr) * (pt - pc), c.dz, c.rdz);
+    const double un = u[o] - div_rn(c.dt * (pe - pc), c.dx, c.rdx);
+    const double vn = v[o] - div_rn(c.dt * (pn - pc), c.dy, c.rdy);
+    const double wn = (c.lid && k >= c.nz) ? 0.0 : w[o] - div_rn((c.dt / c.dsqr) * (pt - pc), c.dz, c.rdz);
+    u[o] = un;
+    v[o] = vn;
+    w[o] = wn;
+    if (halo) {
+        write_images<double>(c, u, i, j, k, un, 0);
+        write_images<double>(c, v, i, j, k, vn, 0);
+        write_images<double>(c, w, i, j, k, wn, 0);
+        write_images<double>(c, p, i, j, k, pc, c.lid);
+    }
 }
 
 __global__ __launch_bounds__(BX * BY) void k_project(StepConsts c, double *__restrict__ u,
                                                       double *__restrict__ v, double *__restrict__ w,
-                                                      const double *__restrict__ p)
+                                                      double *__restrict__ p, int halo)
 {
     const int i = blockIdx.x * BX + threadIdx.x + 1;
     const int j = blockIdx.y * BY + threadIdx.y + 1;
     if (i > c.nx || j > c.ny) return;
-    project_point(c, u, v, w, p, i, j, (int)blockIdx.z + 1);
+    project_point(c, u, v, w, p, i, j, (int)blockIdx.z + 1, halo);
 }
 
 // projection of the mixed form with the FP32 pressure increment dp: the
@@ -738,7 +778,7 @@ __global__ __launch_bounds__(BX * BY) void k_project(StepConsts c, double *__res
 __global__ __launch_bounds__(BX * BY) void k_project_f(StepConsts c, float *__restrict__ uh, float *__restrict__ ul,
                                                         float *__restrict__ vh, float *__restrict__ vl,
                                                         float *__restrict__ wh, float *__restrict__ wl,
-                                                        const float *__restrict__ dp, double *__restrict__ pint)
+                                                        float *__restrict__ dp, double *__restrict__ pint, int halo)
 {
     const int i = blockIdx.x * BX + threadIdx.x + 1;
     const int j = blockIdx.y * BY + threadIdx.y + 1;
@@ -749,16 +789,24 @@ __global__ __launch_bounds__(BX * BY) void k_project_f(StepConsts c, float *__re
     const double pe = dp[idx(c, i == c.nx ? 1 : i + 1, j, k)];
     const double pn = dp[idx(c, i, j == c.ny ? 1 : j + 1, k)];
     const double pt = dp[idx(c, i, j, k == c.nz ? (c.lid ? c.nz : 1) : k + 1)];
+    // with halo set, the halo images of the hi parts, of the increment and of
+    // pint are written too (Neumann in z under a lid for the last two)
     auto put = [&](float *h, float *l, double x) {
         const float hi = (float)x;
         h[o] = hi;
         l[o] = (float)(x - (double)hi);
+        if (halo) write_images<float>(c, h, i, j, k, hi, 0);
     };
     put(uh, ul, ((double)uh[o] + (double)ul[o]) - c.dt * ((pe - pc) * c.rdx));
     put(vh, vl, ((double)vh[o] + (double)vl[o]) - c.dt * ((pn - pc) * c.rdy));
     if (c.lid && k >= c.nz) put(wh, wl, 0.0);
     else put(wh, wl, ((double)wh[o] + (double)wl[o]) - (c.dt / c.dsqr) * ((pt - pc) * c.rdz));
-    pint[o] = pint[o] + pc;
+    const double pn1 = pint[o] + pc;
+    pint[o] = pn1;
+    if (halo) {
+        write_images<float>(c, dp, i, j, k, (float)pc, c.lid);
+        write_images<double>(c, pint, i, j, k, pn1, c.lid);
+    }
 }
 
 // host-layout conversions of the mixed form: 0 split FP64 -> (hi, lo),
@@ -856,24 +904,8 @@ __global__ __launch_bounds__(FTHREADS) void k_ens_fric(const EnsMember *__restri
     F.bih = e.bih[q];
     F.Hbi = e.Hbi[q];
     F.zero_top = (f == 2 && F.harm && c.lid) ? 1 : 0;
+    a.halo = 1;
     fric_body<double>(c, a, F, lb);
-}
-
-// halo of u, v, w, b in the new buffers after the friction stage, or of
-// u, v, w and p (Neumann in z under a lid) after the projection
-__global__ __launch_bounds__(256) void k_ens_halo(const EnsMember *__restrict__ mem, EnsMap map,
-                                                   const EnsState *__restrict__ st, int after_projection)
-{
-    const int g = (int)blockIdx.x;
-    const int mi = ens_find(map, g);
-    const EnsMember &e = mem[mi];
-    const int64_t t = (int64_t)(g - map.start[mi]) * 256 + threadIdx.x;
-    const int q = (int)(t / e.nhalo);
-    if (q >= 4) return;
-    const int nb = st->par ^ 1;
-    double *a = (after_projection && q == 3) ? e.p : e.f[q][nb];
-    const int neu = (after_projection && q == 3) ? e.c.lid : 0;
-    halo_point<double>(e.c, a, neu, t - (int64_t)q * e.nhalo);
 }
 
 __global__ __launch_bounds__(BX * BY) void k_ens_project(const EnsMember *__restrict__ mem, EnsMap map,
@@ -890,7 +922,7 @@ __global__ __launch_bounds__(BX * BY) void k_ens_project(const EnsMember *__rest
     const int k = lb / (bx * by) + 1;
     if (i > c.nx || j > c.ny) return;
     const int nb = st->par ^ 1;
-    project_point(c, e.f[0][nb], e.f[1][nb], e.f[2][nb], e.p, i, j, k);
+    project_point(c, e.f[0][nb], e.f[1][nb], e.f[2][nb], e.p, i, j, k, 1);
 }
 
 // end of the step: main.f90:156-160 (tau rotation, itt) and the buffer swap
@@ -950,9 +982,11 @@ static void tendency_impl(Model &m, const T *u, const T *v, const T *w, const T 
     a.tiles_y = (g.ny + TTY - 1) / TTY;
     const int tiles = a.tiles_x * a.tiles_y;
     // about 4 work-groups per CU over 256 CUs, chunks of at least 8 levels
+    // (or the tuned number of levels)
     int nchunk = (1024 + tiles - 1) / tiles;
     const int maxch = g.nz / 8 > 1 ? g.nz / 8 : 1;
     nchunk = nchunk > maxch ? maxch : nchunk;
+    if (m.tend_kc > 0) nchunk = (g.nz + m.tend_kc - 1) / m.tend_kc;
     a.kc = (g.nz + nchunk - 1) / nchunk;
     a.nchunk = (g.nz + a.kc - 1) / a.kc;
     a.nblocks = tiles * a.nchunk;
@@ -1105,9 +1139,10 @@ bool fused_friction_ok(const Model &m)
 }
 
 // integrate.f90:59-111 for configs without vertical biharmonic friction:
-// the friction chain and AB of all four fields in one launch, then the halo
-// exchange.  The new fields are written to the second buffers, which then
-// become the canonical ones.
+// the friction chain and AB of all four fields in one launch, which also
+// writes the new values' halo images (the exchange of integrate.f90:106-111).
+// The new fields are written to the second buffers, which then become the
+// canonical ones.
 template <class T>
 static void friction_ab_impl(Model &m, const T *const *in, const T *const *in_lo, T *const *out, T *const *out_lo,
                              T *const *tend, int itt, int tau, int taum1, int taum2, double qc1 = 0.0,
@@ -1165,9 +1200,11 @@ static void friction_ab_impl(Model &m, const T *const *in, const T *const *in_lo
     const int target = 2048 * 512 / FTHREADS;   // ~8 work-groups of 512 threads per CU
     int nchunk = (target + tiles * 4 - 1) / (tiles * 4);
     nchunk = nchunk < 1 ? 1 : (nchunk > g.nz ? g.nz : nchunk);
+    if (m.fric_kc > 0) nchunk = (g.nz + m.fric_kc - 1) / m.fric_kc;
     a.kc = (g.nz + nchunk - 1) / nchunk;
     a.nchunk = (g.nz + a.kc - 1) / a.kc;
     a.nblocks = tiles * a.nchunk * 4;
+    a.halo = 1;   // every extent is >= 2, so each halo point is an image of one interior point
     ProfScope ps(m, "friction_ab");
     hipLaunchKernelGGL(k_fric_ab<T>, dim3((unsigned)a.nblocks), dim3(FTHREADS), 0, m.stream, m.c, a);
     PS3D_HIP_CHECK(hipGetLastError());
@@ -1184,9 +1221,10 @@ static void halo_launch(Model &m, const HaloArgsT<T> &h, int n)
 }
 
 // integrate.f90:59-111 for configs without vertical biharmonic friction:
-// the friction chain and AB of all four fields in one launch, then the halo
-// exchange.  The new fields are written to the second buffers, which then
-// become the canonical ones.
+// the friction chain and AB of all four fields in one launch, which also
+// writes the new values' halo images (the exchange of integrate.f90:106-111).
+// The new fields are written to the second buffers, which then become the
+// canonical ones.
 void launch_friction_ab_mixed(Model &m, int itt, int tau, int taum1, int taum2, double qc1, const float *d1,
                               double qc2, const float *d2, bool swap)
 {
@@ -1196,11 +1234,21 @@ void launch_friction_ab_mixed(Model &m, int itt, int tau, int taum1, int taum2, 
         std::swap(m.fh[f], m.ah[f]);
         std::swap(m.fl[f], m.al[f]);
     }
-    // the stencils and the divergence read the hi parts only; the lo parts'
-    // halos are rebuilt when a field is copied out
-    HaloArgsT<float> h{};
-    for (int f = 0; f < 4; ++f) h.f[f] = m.fh[f];
-    halo_launch(m, h, 4);
+    // the kernel wrote the hi parts' halo images, which the stencils and the
+    // divergence read; the lo parts' halos are rebuilt when a field is copied out
+}
+
+// the fused kernel alone, into the second buffers (no swap, no halo): the
+// autotuner's timing launch
+void launch_friction_ab_raw(Model &m, int itt, int tau, int taum1, int taum2)
+{
+    if (m.mixed) {
+        launch_friction_ab_mixed(m, itt, tau, taum1, taum2, 0.0, nullptr, 0.0, nullptr, false);
+        return;
+    }
+    double *in[4] = {m.u, m.v, m.w, m.b};
+    double *tend[4] = {m.du, m.dv, m.dw, m.db};
+    friction_ab_impl<double>(m, in, nullptr, m.alt, nullptr, tend, itt, tau, taum1, taum2);
 }
 
 void launch_friction_ab(Model &m, int itt, int tau, int taum1, int taum2)
@@ -1213,9 +1261,6 @@ void launch_friction_ab(Model &m, int itt, int tau, int taum1, int taum2)
     std::swap(m.v, m.alt[1]);
     std::swap(m.w, m.alt[2]);
     std::swap(m.b, m.alt[3]);
-    HaloArgs h{};
-    for (int f = 0; f < 4; ++f) { h.f[f] = m.field_buf(f); h.neumann[f] = 0; }
-    halo_launch(m, h, 4);
 }
 
 void launch_border_exchg(Model &m, double *a, int neumann_z)
@@ -1229,39 +1274,24 @@ void launch_border_exchg(Model &m, double *a, int neumann_z)
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
-// pressure_solver.f90:75-118 after p's interior is in place
+// pressure_solver.f90:75-118 after p's interior is in place: the projection
+// writes the halo images of u, v, w and p as well
 void launch_project_exchange(Model &m)
 {
-    {
-        ProfScope ps(m, "project");
-        hipLaunchKernelGGL(k_project, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, m.u, m.v, m.w, m.p);
-        PS3D_HIP_CHECK(hipGetLastError());
-    }
-    HaloArgs h{};
-    h.f[0] = m.u; h.f[1] = m.v; h.f[2] = m.w; h.f[3] = m.p;
-    h.neumann[3] = m.prm.enable_vertical_boundaries;
-    const int64_t nhalo = m.g.H - m.g.N;
-    ProfScope ps(m, "halo");
-    hipLaunchKernelGGL(k_halo<double>, dim3((unsigned)((nhalo + 255) / 256), 4), dim3(256), 0, m.stream, m.c, h, nhalo);
+    ProfScope ps(m, "project");
+    hipLaunchKernelGGL(k_project, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, m.u, m.v, m.w, m.p, 1);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
 // pressure_solver.f90:98-118 of the mixed form
 void launch_project_exchange_f(Model &m, float *dp)
 {
-    {
-        ProfScope ps(m, "project");
-        hipLaunchKernelGGL(k_project_f, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, m.fh[0], m.fl[0],
-                           m.fh[1], m.fl[1], m.fh[2], m.fl[2], dp, m.pint);
-        PS3D_HIP_CHECK(hipGetLastError());
-    }
-    // the increment's halo (Neumann in z under a lid) serves the gradient of
-    // the next steps' pressure-history term
-    HaloArgsT<float> h{};
-    h.f[0] = m.fh[0]; h.f[1] = m.fh[1]; h.f[2] = m.fh[2]; h.f[3] = dp;
-    h.neumann[3] = m.prm.enable_vertical_boundaries;
-    halo_launch(m, h, 4);
-    launch_border_exchg(m, m.pint, m.prm.enable_vertical_boundaries);
+    // the increment's halo (Neumann in z under a lid), written by the same
+    // kernel, serves the gradient of the next steps' pressure-history term
+    ProfScope ps(m, "project");
+    hipLaunchKernelGGL(k_project_f, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, m.fh[0], m.fl[0], m.fh[1],
+                       m.fl[1], m.fh[2], m.fl[2], dp, m.pint, 1);
+    PS3D_HIP_CHECK(hipGetLastError());
 }
 
 void launch_halo_f(Model &m, float *const *arrays, const int *neumann, int n)
@@ -1292,14 +1322,6 @@ void ens_friction_ab(hipStream_t s, const EnsMember *mem, const EnsMap &map, con
 {
     if (!map.start[map.n]) return;
     hipLaunchKernelGGL(k_ens_fric, dim3((unsigned)map.start[map.n]), dim3(FTHREADS), 0, s, mem, map, st);
-    PS3D_HIP_CHECK(hipGetLastError());
-}
-
-void ens_halo(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, int after_projection)
-{
-    if (!map.start[map.n]) return;
-    hipLaunchKernelGGL(k_ens_halo, dim3((unsigned)map.start[map.n]), dim3(256), 0, s, mem, map, st,
-                       after_projection);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 

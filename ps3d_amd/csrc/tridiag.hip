@@ -122,11 +122,15 @@ __device__ __forceinline__ void thomas_col(double2 *__restrict__ fs, const doubl
     }
 }
 
+// cpw columns per 64-lane work-group: with few columns (33K at 256x256) a
+// full wavefront per 64 columns leaves SIMDs idle and the sequential sweeps'
+// latency exposed; fewer columns per wave give more waves to interleave
 __global__ __launch_bounds__(64) void k_thomas(double2 *__restrict__ fs, const double *__restrict__ ksqr,
                                                const double *__restrict__ cp, const double *__restrict__ rm,
-                                               int64_t ncols, int nz, double fxa, double dz2, double dsqr)
+                                               int64_t ncols, int nz, double fxa, double dz2, double dsqr, int cpw)
 {
-    const int64_t col = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if ((int)threadIdx.x >= cpw) return;
+    const int64_t col = (int64_t)blockIdx.x * cpw + threadIdx.x;
     if (col >= ncols) return;
     thomas_col(fs, ksqr, cp, rm, ncols, nz, fxa, dz2, dsqr, col);
 }
@@ -252,9 +256,10 @@ void launch_thomas(Model &m, double2 *data)
 {
     const int64_t ncols = (int64_t)m.g.nkx * m.g.ny;
     const double fxa = 1.0 / (m.prm.dz * m.prm.dz) / m.prm.dsqr;
+    const int cpw = (m.thomas_cpw == 16 || m.thomas_cpw == 32) ? m.thomas_cpw : 64;
     ProfScope ps(m, "thomas");
-    hipLaunchKernelGGL(k_thomas, dim3((unsigned)((ncols + 63) / 64)), dim3(64), 0, m.stream, data, m.ksqr,
-                       m.tri_cp, m.tri_rm, ncols, m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr);
+    hipLaunchKernelGGL(k_thomas, dim3((unsigned)((ncols + cpw - 1) / cpw)), dim3(64), 0, m.stream, data, m.ksqr,
+                       m.tri_cp, m.tri_rm, ncols, m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr, cpw);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 

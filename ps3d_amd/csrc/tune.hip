@@ -1,0 +1,132 @@
+// tune.hip -- autotuner of one model's launch shapes: levels per work-group
+// of the two k-marching kernels (tendency; friction + Adams-Bashforth) and
+// columns per wavefront of the Thomas solve.  Each candidate is timed with
+// HIP events on the live model; the launches timed do not change the model's
+// state (the tendency writes the slot the next step overwrites anyway, the
+// friction kernel writes the second buffers, Thomas runs on a scratch copy of
+// the spectral array).  Results do not depend on the shapes.  The choice is
+// cached per (stage, nx x ny x nz, boundary, precision) in a text file of
+// "stage shape value" lines, the same format as the ensemble's.
+#include "ps3d_internal.h"
+
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <vector>
+
+namespace ps3d {
+
+namespace {
+
+std::string shape_of(const Model &m)
+{
+    std::ostringstream o;
+    o << m.g.nx << "x" << m.g.ny << "x" << m.g.nz << (m.prm.enable_vertical_boundaries ? ":lid" : ":periodic")
+      << (m.mixed ? ":mixed" : ":fp64");
+    return o.str();
+}
+
+template <class F>
+float time_it(Model &m, int reps, F &&launch)
+{
+    hipEvent_t a, b;
+    PS3D_HIP_CHECK(hipEventCreate(&a));
+    PS3D_HIP_CHECK(hipEventCreate(&b));
+    float ms = 0.f;
+    try {
+        launch();   // warm-up
+        PS3D_HIP_CHECK(hipEventRecord(a, m.stream));
+        for (int r = 0; r < reps; ++r) launch();
+        PS3D_HIP_CHECK(hipEventRecord(b, m.stream));
+        PS3D_HIP_CHECK(hipEventSynchronize(b));
+        PS3D_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    } catch (...) {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        throw;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms;
+}
+
+}  // namespace
+
+void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
+{
+    PS3D_REQUIRE(next_tau >= 1 && next_tau <= 3, "tune: next_tau must be 1..3");
+    for (int f = 0; f < 4; ++f) PS3D_REQUIRE(m.cur[f] == 0, "tune: friction results not yet folded in by ab_update");
+    reps = reps < 1 ? 1 : reps;
+    std::map<std::string, int> cache;
+    if (cache_path) {
+        std::ifstream in(cache_path);
+        std::string st, sh;
+        int v;
+        while (in >> st >> sh >> v) cache[st + " " + sh] = v;
+    }
+    const std::string sh = shape_of(m);
+    const int nz = m.g.nz;
+    std::vector<int> levels;
+    for (int kc = 2; kc < nz; kc *= 2) levels.push_back(kc);
+    levels.push_back(nz);
+    const int t1 = next_tau % 3 + 1, t2 = t1 % 3 + 1;   // any two other slots
+
+    auto pick = [&](const std::string &stage, const std::vector<int> &cand, int &field, auto &&launch) {
+        const std::string key = stage + " " + sh;
+        auto hit = cache.find(key);
+        if (hit != cache.end()) {
+            field = hit->second;
+            return;
+        }
+        int best = cand[0];
+        float best_ms = 1e30f;
+        for (int v : cand) {
+            field = v;
+            const float ms = time_it(m, reps, launch);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = v;
+            }
+        }
+        field = best;
+        cache[key] = best;
+    };
+
+    pick("tendency", levels, m.tend_kc, [&] {
+        if (m.mixed) launch_tendency_mixed(m, next_tau, m.pint);
+        else launch_tendency(m, next_tau);
+    });
+    if (m.mixed || (fused_friction_ok(m) && m.fuse_friction))
+        pick("friction_ab", levels, m.fric_kc, [&] { launch_friction_ab_raw(m, 3, next_tau, t1, t2); });
+    if (!m.mixed && m.prm.enable_vertical_boundaries) {
+        double2 *scratch = nullptr;
+        PS3D_HIP_CHECK(hipMalloc(&scratch, sizeof(double2) * (size_t)m.g.S));
+        try {
+            PS3D_HIP_CHECK(hipMemcpyAsync(scratch, m.FS, sizeof(double2) * (size_t)m.g.S, hipMemcpyDeviceToDevice,
+                                          m.stream));
+            const std::vector<int> cpw = {64, 32, 16};
+            pick("thomas", cpw, m.thomas_cpw, [&] { launch_thomas(m, scratch); });
+        } catch (...) {
+            (void)hipFree(scratch);
+            throw;
+        }
+        PS3D_HIP_CHECK(hipStreamSynchronize(m.stream));
+        PS3D_HIP_CHECK(hipFree(scratch));
+    }
+    if (cache_path) {
+        std::ofstream out(cache_path, std::ios::trunc);
+        for (auto &kv : cache) out << kv.first << " " << kv.second << "\n";
+    }
+}
+
+std::string describe_tuning(const Model &m)
+{
+    std::ostringstream o;
+    const std::string sh = shape_of(m);
+    o << "tendency " << sh << " " << m.tend_kc << "\n";
+    o << "friction_ab " << sh << " " << m.fric_kc << "\n";
+    o << "thomas " << sh << " " << m.thomas_cpw << "\n";
+    return o.str();
+}
+
+}  // namespace ps3d

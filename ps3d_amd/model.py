@@ -212,6 +212,17 @@ class Model:
                                          ptr[0], ptr[1], ptr[2], tab))
         return out
 
+    def tune(self, cache_path: str | None = None, reps: int = 5):
+        """Autotune the launch shapes (ps3d_tune); the next step's slot is scratch."""
+        check(self.lib.ps3d_tune(self.h, self.tau, cache_path.encode() if cache_path else None, reps))
+
+    def tuning(self) -> str:
+        buf = ctypes.create_string_buffer(4096)
+        n = self.lib.ps3d_tuning(self.h, buf, len(buf))
+        if n < 0:
+            check(-n)
+        return buf.value.decode()
+
     def diagnose(self) -> dict:
         """diagnose.F90:11-53 on the device: KE, PE (per point), max |u_h|,
         max |w|, CFL and Peclet numbers, and whether a NaN was seen."""

@@ -230,3 +230,21 @@ def test_divergence_roundoff_c2_size():
     assert np.abs(div).max() <= 1e-12 * scale
     assert np.isfinite(u).all() and np.isfinite(w).all()
     gpu.close()
+
+
+@pytest.mark.parametrize("case", ["lid_fused", "periodic_pow2"])
+def test_tuned_shapes_give_the_same_bits(case, tmp_path):
+    """ps3d_tune changes launch shapes only: a tuned model steps bit for bit
+    like the oracle, and the cached choice is reused."""
+    prm, gpu, orc = _pair(case)
+    cache = str(tmp_path / "tune.txt")
+    gpu.tune(cache, reps=2)
+    print(gpu.tuning(), flush=True)
+    gpu.run(6)
+    orc.run(6)
+    gpu.pull()
+    for n in ("u", "v", "w", "b", "p"):
+        assert np.array_equal(gpu.f[n], orc.f[n]), n
+    text = open(cache).read()
+    assert "tendency" in text
+    gpu.close()
