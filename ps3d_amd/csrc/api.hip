@@ -370,6 +370,7 @@ int ps3d_create(const ps3d_params *prm, const double *ksqr, void *stream, ps3d_m
             c.dx = p.dx; c.dy = p.dy; c.dz = p.dz; c.dt = p.dt;
             c.Ro = p.Ro; c.dsqr = p.dsqr; c.f0 = p.f0; c.N0 = p.N0;
             c.rdx = 1.0 / p.dx; c.rdy = 1.0 / p.dy; c.rdz = 1.0 / p.dz; c.rdt = 1.0 / p.dt;
+            m->fuse_halo = halo_images_ok(p.nx, p.ny, p.nz);
             if (stream) {
                 m->stream = static_cast<hipStream_t>(stream);
             } else {

@@ -251,6 +251,8 @@ int ps3d_ensemble_create(int n, const ps3d_params *prms, void *stream, ps3d_ense
                              (p.enable_vertical_boundaries || ens_fft_size_ok(p.nz, true)),
                          who + "nx/2, ny (and nz without a lid) must be compiled-in FFT lengths (" + fft_sizes_text() +
                              "), ny and nz at most 512");
+            // the batched friction and projection write the halo images themselves
+            PS3D_REQUIRE(halo_images_ok(p.nx, p.ny, p.nz), who + "nx, ny and nz must be at least 4");
         }
         auto *E = new ps3d_ensemble();
         try {

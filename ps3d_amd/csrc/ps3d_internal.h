@@ -123,33 +123,39 @@ __device__ __forceinline__ float sdiv<float>(float a, double, double rb) { retur
 // (parallel_mpi.f90:70-211, one rank) copies there.  x and y are periodic;
 // z is periodic, or with neumann_z the halo levels take copies of levels 1
 // and nz (pressure_solver.f90:82-93).  Producers call it for the points they
-// write, so no separate exchange pass is needed.
+// write, so no separate exchange pass is needed.  Requires nx, ny, nz >= 4
+// (halo_images_ok): a point then has at most one image per axis (two in z
+// under Neumann), and only lanes next to a face take the stores.
+inline bool halo_images_ok(int nx, int ny, int nz) { return nx >= 4 && ny >= 4 && nz >= 4; }
+
 template <class T>
 __device__ __forceinline__ void write_images(const StepConsts &c, T *a, int i, int j, int k, T val, int neumann_z)
 {
-    int xs[3], ys[3], zs[3];
-    int nx_ = 0, ny_ = 0, nz_ = 0;
-    xs[nx_++] = i;
-    if (i <= 2) xs[nx_++] = i + c.nx;
-    if (i >= c.nx - 1) xs[nx_++] = i - c.nx;
-    ys[ny_++] = j;
-    if (j <= 2) ys[ny_++] = j + c.ny;
-    if (j >= c.ny - 1) ys[ny_++] = j - c.ny;
-    zs[nz_++] = k;
-    if (neumann_z) {
-        if (k == 1) { zs[nz_++] = 0; zs[nz_++] = -1; }
-        if (k == c.nz) { zs[nz_++] = c.nz + 1; zs[nz_++] = c.nz + 2; }
-    } else {
-        if (k <= 2) zs[nz_++] = k + c.nz;
-        if (k >= c.nz - 1) zs[nz_++] = k - c.nz;
+    const int ox = i <= 2 ? c.nx : (i >= c.nx - 1 ? -c.nx : 0);
+    const int oy = j <= 2 ? c.ny : (j >= c.ny - 1 ? -c.ny : 0);
+    int oz;
+    if (neumann_z) oz = k == 1 ? -1 : (k == c.nz ? 1 : 0);
+    else oz = k <= 2 ? c.nz : (k >= c.nz - 1 ? -c.nz : 0);
+    if ((ox | oy | oz) == 0) return;
+    T *p = a + (int64_t)(i + 1) + c.X * (int64_t)(j + 1) + c.XY * (int64_t)(k + 1);
+    const int64_t sx = ox, sy = (int64_t)oy * c.X;
+    if (ox) p[sx] = val;
+    if (oy) {
+        p[sy] = val;
+        if (ox) p[sx + sy] = val;
     }
-    if (nx_ == 1 && ny_ == 1 && nz_ == 1) return;
-    for (int a3 = 0; a3 < nz_; ++a3)
-        for (int a2 = 0; a2 < ny_; ++a2)
-            for (int a1 = 0; a1 < nx_; ++a1) {
-                if (a1 == 0 && a2 == 0 && a3 == 0) continue;
-                a[(int64_t)(xs[a1] + 1) + c.X * (int64_t)(ys[a2] + 1) + c.XY * (int64_t)(zs[a3] + 1)] = val;
+    if (oz) {
+        const int nzi = neumann_z ? 2 : 1;   // Neumann: two copies, at distance 1 and 2
+        for (int r = 1; r <= nzi; ++r) {
+            T *q = p + (int64_t)(oz * r) * c.XY;
+            q[0] = val;
+            if (ox) q[sx] = val;
+            if (oy) {
+                q[sy] = val;
+                if (ox) q[sx + sy] = val;
             }
+        }
+    }
 }
 
 // complex type of a real type
@@ -195,6 +201,8 @@ struct Model {
     int cur[4] = {0, 0, 0, 0};
     // use the fused friction + Adams-Bashforth kernel where it applies
     bool fuse_friction = true;
+    // producers write the halo images of what they update (halo_images_ok)
+    bool fuse_halo = false;
     // launch shapes (results do not depend on them; set by ps3d_tune):
     // levels per work-group of the k-marching kernels (0: heuristic) and
     // columns per 64-lane wave of the Thomas solve (64, 32 or 16)

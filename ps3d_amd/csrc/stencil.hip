@@ -665,7 +665,10 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
                 if (a.halo) write_images<float>(c, F.out, i, j, k, hi, 0);
             }
         }
-        __syncthreads();
+        // no barrier here: the next level's first barrier orders this level's
+        // reads of f1p, d2p and hinc before their next writes, and the f0p
+        // slot refilled next held level k-2, last read before this level's
+        // second barrier
     }
 }
 
@@ -1204,7 +1207,7 @@ static void friction_ab_impl(Model &m, const T *const *in, const T *const *in_lo
     a.kc = (g.nz + nchunk - 1) / nchunk;
     a.nchunk = (g.nz + a.kc - 1) / a.kc;
     a.nblocks = tiles * a.nchunk * 4;
-    a.halo = 1;   // every extent is >= 2, so each halo point is an image of one interior point
+    a.halo = m.fuse_halo ? 1 : 0;
     ProfScope ps(m, "friction_ab");
     hipLaunchKernelGGL(k_fric_ab<T>, dim3((unsigned)a.nblocks), dim3(FTHREADS), 0, m.stream, m.c, a);
     PS3D_HIP_CHECK(hipGetLastError());
@@ -1234,8 +1237,13 @@ void launch_friction_ab_mixed(Model &m, int itt, int tau, int taum1, int taum2, 
         std::swap(m.fh[f], m.ah[f]);
         std::swap(m.fl[f], m.al[f]);
     }
-    // the kernel wrote the hi parts' halo images, which the stencils and the
-    // divergence read; the lo parts' halos are rebuilt when a field is copied out
+    // the stencils and the divergence read the hi parts' halos (written by the
+    // kernel itself when fuse_halo); the lo parts' halos are rebuilt when a
+    // field is copied out
+    if (m.fuse_halo) return;
+    HaloArgsT<float> h{};
+    for (int f = 0; f < 4; ++f) h.f[f] = m.fh[f];
+    halo_launch(m, h, 4);
 }
 
 // the fused kernel alone, into the second buffers (no swap, no halo): the
@@ -1261,6 +1269,10 @@ void launch_friction_ab(Model &m, int itt, int tau, int taum1, int taum2)
     std::swap(m.v, m.alt[1]);
     std::swap(m.w, m.alt[2]);
     std::swap(m.b, m.alt[3]);
+    if (m.fuse_halo) return;
+    HaloArgs h{};
+    for (int f = 0; f < 4; ++f) { h.f[f] = m.field_buf(f); h.neumann[f] = 0; }
+    halo_launch(m, h, 4);
 }
 
 void launch_border_exchg(Model &m, double *a, int neumann_z)
@@ -1275,23 +1287,40 @@ void launch_border_exchg(Model &m, double *a, int neumann_z)
 }
 
 // pressure_solver.f90:75-118 after p's interior is in place: the projection
-// writes the halo images of u, v, w and p as well
+// writes the halo images of u, v, w and p as well (fuse_halo), or an
+// exchange pass follows
 void launch_project_exchange(Model &m)
 {
-    ProfScope ps(m, "project");
-    hipLaunchKernelGGL(k_project, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, m.u, m.v, m.w, m.p, 1);
-    PS3D_HIP_CHECK(hipGetLastError());
+    {
+        ProfScope ps(m, "project");
+        hipLaunchKernelGGL(k_project, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, m.u, m.v, m.w, m.p,
+                           m.fuse_halo ? 1 : 0);
+        PS3D_HIP_CHECK(hipGetLastError());
+    }
+    if (m.fuse_halo) return;
+    HaloArgs h{};
+    h.f[0] = m.u; h.f[1] = m.v; h.f[2] = m.w; h.f[3] = m.p;
+    h.neumann[3] = m.prm.enable_vertical_boundaries;
+    halo_launch(m, h, 4);
 }
 
 // pressure_solver.f90:98-118 of the mixed form
 void launch_project_exchange_f(Model &m, float *dp)
 {
-    // the increment's halo (Neumann in z under a lid), written by the same
-    // kernel, serves the gradient of the next steps' pressure-history term
-    ProfScope ps(m, "project");
-    hipLaunchKernelGGL(k_project_f, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, m.fh[0], m.fl[0], m.fh[1],
-                       m.fl[1], m.fh[2], m.fl[2], dp, m.pint, 1);
-    PS3D_HIP_CHECK(hipGetLastError());
+    // the increment's halo (Neumann in z under a lid) serves the gradient of
+    // the next steps' pressure-history term
+    {
+        ProfScope ps(m, "project");
+        hipLaunchKernelGGL(k_project_f, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, m.fh[0], m.fl[0],
+                           m.fh[1], m.fl[1], m.fh[2], m.fl[2], dp, m.pint, m.fuse_halo ? 1 : 0);
+        PS3D_HIP_CHECK(hipGetLastError());
+    }
+    if (m.fuse_halo) return;
+    HaloArgsT<float> h{};
+    h.f[0] = m.fh[0]; h.f[1] = m.fh[1]; h.f[2] = m.fh[2]; h.f[3] = dp;
+    h.neumann[3] = m.prm.enable_vertical_boundaries;
+    halo_launch(m, h, 4);
+    launch_border_exchg(m, m.pint, m.prm.enable_vertical_boundaries);
 }
 
 void launch_halo_f(Model &m, float *const *arrays, const int *neumann, int n)

@@ -25,6 +25,9 @@ CASES = {
     # every FFT length compiled in (x: 16, y: 32, z: 16); the cases above mix
     # compiled-in and generic lengths
     "periodic_pow2": dict(nx=32, ny=32, nz=16, lid=False, Avbi=0.0, Kvbi=0.0),
+    # three levels: the kernels leave the halo to a separate exchange pass
+    # (they write halo images themselves only when every extent is >= 4)
+    "thin_lid_fused": dict(nx=12, ny=8, nz=3, lid=True, Avbi=0.0, Kvbi=0.0),
 }
 
 
