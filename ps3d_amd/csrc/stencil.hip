@@ -160,8 +160,8 @@ __device__ __forceinline__ void tendency_body(const StepConsts &c, const TendArg
     __shared__ T sp[4][3][TPY][TPX];
     const int ch = bid % a.nchunk;
     const int tile = bid / a.nchunk;
-    const int i0 = (tile / a.tiles_y) * TTX + 1;
-    const int j0 = (tile % a.tiles_y) * TTY + 1;
+    const int i0 = (tile % a.tiles_x) * TTX + 1;
+    const int j0 = (tile / a.tiles_x) * TTY + 1;
     const int kb = 1 + ch * a.kc;
     const int ke = min(c.nz, kb + a.kc - 1);
     const int t = threadIdx.x;
@@ -516,8 +516,8 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
     const int tile = bid % ntile;
     const int rest = bid / ntile;
     const int ch = rest % a.nchunk;
-    const int i0 = (tile / a.tiles_y) * FTX + 1;
-    const int j0 = (tile % a.tiles_y) * FTY + 1;
+    const int i0 = (tile % a.tiles_x) * FTX + 1;   // x-adjacent tiles run together
+    const int j0 = (tile / a.tiles_x) * FTY + 1;
     const int kb = 1 + ch * a.kc;
     const int ke = min(c.nz, kb + a.kc - 1);
     const int t = threadIdx.x;
@@ -556,29 +556,41 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
     fetch(kb);
     store(kb);
     fetch(kb + 1);
-    for (int k = kb; k <= ke; ++k) {
-        store(k + 1);
-        if (k + 1 <= ke) fetch(k + 2);
-        // this level's tendency slots (and lo part), issued before the LDS phases
-        T s0 = zero, s1 = zero, s2 = zero, lo0 = zero;
-        float q1o = 0.f, q1n = 0.f, q2o = 0.f, q2n = 0.f;   // mixed form: pressure-history increments
+    // the tendency slots (and lo part) of a level are loaded one level ahead,
+    // so their latency overlaps a whole level's LDS phases
+    struct Pt {
+        T s0, s1, s2, lo0;
+        float q1o, q1n, q2o, q2n;   // mixed form: pressure-history increments
+    };
+    auto load_pt = [&](int k) {
+        Pt r{zero, zero, zero, zero, 0.f, 0.f, 0.f, 0.f};
         if (mine) {
             const int64_t o = idx(c, i0 + tpx, j0 + tpy, k);
-            s0 = F.t0[o];
-            if (a.order >= 2) s1 = F.t1[o];
-            if (a.order >= 3) s2 = F.t2[o];
+            r.s0 = F.t0[o];
+            if (a.order >= 2) r.s1 = F.t1[o];
+            if (a.order >= 3) r.s2 = F.t2[o];
             if constexpr (MIXED) {
-                lo0 = F.in_lo[o];
+                r.lo0 = F.in_lo[o];
                 if (F.gstride && a.d1) {
-                    q1o = a.d1[o];
-                    q1n = a.d1[o + F.gstride];
+                    r.q1o = a.d1[o];
+                    r.q1n = a.d1[o + F.gstride];
                     if (a.d2) {
-                        q2o = a.d2[o];
-                        q2n = a.d2[o + F.gstride];
+                        r.q2o = a.d2[o];
+                        r.q2n = a.d2[o + F.gstride];
                     }
                 }
             }
         }
+        return r;
+    };
+    Pt nxt = load_pt(kb);
+    for (int k = kb; k <= ke; ++k) {
+        store(k + 1);
+        if (k + 1 <= ke) fetch(k + 2);
+        const Pt cur = nxt;
+        if (k + 1 <= ke) nxt = load_pt(k + 1);
+        const T s0 = cur.s0, s1 = cur.s1, s2 = cur.s2, lo0 = cur.lo0;
+        const float q1o = cur.q1o, q1n = cur.q1n, q2o = cur.q2o, q2n = cur.q2n;
         __syncthreads();
         const int sm = slot(k - 1), sc = slot(k), sp = slot(k + 1);
         for (int q = t; q < R1X * R1Y; q += FTHREADS) {
