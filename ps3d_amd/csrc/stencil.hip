@@ -566,9 +566,11 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
         Pt r{zero, zero, zero, zero, 0.f, 0.f, 0.f, 0.f};
         if (mine) {
             const int64_t o = idx(c, i0 + tpx, j0 + tpy, k);
-            r.s0 = F.t0[o];
-            if (a.order >= 2) r.s1 = F.t1[o];
-            if (a.order >= 3) r.s2 = F.t2[o];
+            // read once: streamed past the L2, which keeps the f0 planes
+            // that neighbouring tiles share
+            r.s0 = __builtin_nontemporal_load(F.t0 + o);
+            if (a.order >= 2) r.s1 = __builtin_nontemporal_load(F.t1 + o);
+            if (a.order >= 3) r.s2 = __builtin_nontemporal_load(F.t2 + o);
             if constexpr (MIXED) {
                 r.lo0 = F.in_lo[o];
                 if (F.gstride && a.d1) {

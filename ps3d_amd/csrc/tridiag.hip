@@ -10,9 +10,9 @@
 // the reference's double-cumulative-sum recurrence (:44-50).
 //
 // One thread per column; a wavefront covers 64 consecutive kx, so every
-// level of the sweep is one coalesced 1 KiB access.  FS is updated in place:
-// the forward sweep leaves d'(k) in FS, the backward sweep overwrites it
-// with the solution.
+// level of the sweep is one coalesced access.  FS is updated in place: the
+// forward sweep's d'(k) are kept in LDS when a work-group's columns fit
+// (else left in FS), and the backward sweep writes the solution.
 #include "ps3d_internal.h"
 
 #pragma clang fp contract(off)
@@ -45,10 +45,14 @@ __global__ __launch_bounds__(256) void k_tri_factor(const double *__restrict__ k
 }
 
 // one wavefront per work-group: the (nx/2+1)*ny columns are few (33K at
-// 256x256), and 64-lane groups spread them over every CU
+// 256x256), and 64-lane groups spread them over every CU.  IN_LDS: the
+// forward sweep's d'(k) go to LDS (dpl[k * ds]) instead of back to FS, so FS
+// is read once and written once.
+template <bool IN_LDS>
 __device__ __forceinline__ void thomas_col(double2 *__restrict__ fs, const double *__restrict__ ksqr,
                                            const double *__restrict__ cp, const double *__restrict__ rm,
-                                           int64_t ncols, int nz, double fxa, double dz2, double dsqr, int64_t col)
+                                           int64_t ncols, int nz, double fxa, double dz2, double dsqr, int64_t col,
+                                           double2 *dpl = nullptr, int ds = 0)
 {
     double2 *d = fs + col;
     const double ks = ksqr[col];
@@ -59,7 +63,8 @@ __device__ __forceinline__ void thomas_col(double2 *__restrict__ fs, const doubl
         double2 dp = d[0];
         const double b1 = rm[col];
         dp = make_double2(dp.x / b1, dp.y / b1);
-        d[0] = dp;
+        if constexpr (IN_LDS) dpl[0] = dp;
+        else d[0] = dp;
         for (int k0 = 1; k0 < nz; k0 += LOOK) {
             double2 dd[LOOK];
             double rr[LOOK];
@@ -75,11 +80,13 @@ __device__ __forceinline__ void thomas_col(double2 *__restrict__ fs, const doubl
             for (int q = 0; q < LOOK; ++q) {
                 if (k0 + q < nz) {
                     dp = make_double2((dd[q].x - dp.x * fxa) * rr[q], (dd[q].y - dp.y * fxa) * rr[q]);
-                    d[(int64_t)(k0 + q) * ncols] = dp;
+                    if constexpr (IN_LDS) dpl[(k0 + q) * ds] = dp;
+                    else d[(int64_t)(k0 + q) * ncols] = dp;
                 }
             }
         }
         double2 x = dp;
+        if constexpr (IN_LDS) d[(int64_t)(nz - 1) * ncols] = x;
         for (int k0 = nz - 2; k0 >= 0; k0 -= LOOK) {
             double2 dd[LOOK];
             double cc[LOOK];
@@ -87,7 +94,8 @@ __device__ __forceinline__ void thomas_col(double2 *__restrict__ fs, const doubl
             for (int q = 0; q < LOOK; ++q) {
                 if (k0 - q >= 0) {
                     const int64_t o = (int64_t)(k0 - q) * ncols;
-                    dd[q] = d[o];
+                    if constexpr (IN_LDS) dd[q] = dpl[(k0 - q) * ds];
+                    else dd[q] = d[o];
                     cc[q] = cp[col + o];
                 }
             }
@@ -125,14 +133,16 @@ __device__ __forceinline__ void thomas_col(double2 *__restrict__ fs, const doubl
 // cpw columns per 64-lane work-group: with few columns (33K at 256x256) a
 // full wavefront per 64 columns leaves SIMDs idle and the sequential sweeps'
 // latency exposed; fewer columns per wave give more waves to interleave
+template <bool IN_LDS>
 __global__ __launch_bounds__(64) void k_thomas(double2 *__restrict__ fs, const double *__restrict__ ksqr,
                                                const double *__restrict__ cp, const double *__restrict__ rm,
                                                int64_t ncols, int nz, double fxa, double dz2, double dsqr, int cpw)
 {
+    extern __shared__ double2 thomas_lds[];   // cpw x nz when IN_LDS
     if ((int)threadIdx.x >= cpw) return;
     const int64_t col = (int64_t)blockIdx.x * cpw + threadIdx.x;
     if (col >= ncols) return;
-    thomas_col(fs, ksqr, cp, rm, ncols, nz, fxa, dz2, dsqr, col);
+    thomas_col<IN_LDS>(fs, ksqr, cp, rm, ncols, nz, fxa, dz2, dsqr, col, thomas_lds + threadIdx.x, cpw);
 }
 
 // the Thomas stage of a batched ensemble (rigid-lid members)
@@ -144,7 +154,7 @@ __global__ __launch_bounds__(64) void k_ens_thomas(const EnsMember *__restrict__
     const int64_t ncols = (int64_t)e.nkx * e.c.ny;
     const int64_t col = (int64_t)(g - map.start[mi]) * 64 + threadIdx.x;
     if (col >= ncols) return;
-    thomas_col(e.FS, e.ksqr, e.cp, e.rm, ncols, e.c.nz, e.fxa, e.c.dz * e.c.dz, e.c.dsqr, col);
+    thomas_col<false>(e.FS, e.ksqr, e.cp, e.rm, ncols, e.c.nz, e.fxa, e.c.dz * e.c.dz, e.c.dsqr, col);
 }
 
 // periodic spectral step without the z transforms (building-block API)
@@ -257,9 +267,15 @@ void launch_thomas(Model &m, double2 *data)
     const int64_t ncols = (int64_t)m.g.nkx * m.g.ny;
     const double fxa = 1.0 / (m.prm.dz * m.prm.dz) / m.prm.dsqr;
     const int cpw = (m.thomas_cpw == 16 || m.thomas_cpw == 32) ? m.thomas_cpw : 64;
+    const size_t lds = sizeof(double2) * (size_t)cpw * m.g.nz;
+    const dim3 grid((unsigned)((ncols + cpw - 1) / cpw));
     ProfScope ps(m, "thomas");
-    hipLaunchKernelGGL(k_thomas, dim3((unsigned)((ncols + cpw - 1) / cpw)), dim3(64), 0, m.stream, data, m.ksqr,
-                       m.tri_cp, m.tri_rm, ncols, m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr, cpw);
+    if (lds <= 64 * 1024)
+        hipLaunchKernelGGL(k_thomas<true>, grid, dim3(64), lds, m.stream, data, m.ksqr, m.tri_cp, m.tri_rm, ncols,
+                           m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr, cpw);
+    else
+        hipLaunchKernelGGL(k_thomas<false>, grid, dim3(64), 0, m.stream, data, m.ksqr, m.tri_cp, m.tri_rm, ncols,
+                           m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr, cpw);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
