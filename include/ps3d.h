@@ -116,6 +116,36 @@ int ps3d_run(ps3d_model *m, int nsteps, int32_t *state, int use_graph);
 /* wait for the model's stream */
 int ps3d_synchronize(ps3d_model *m);
 
+/* ---- forcing: the set_forcing hook of integrate.f90:82 ----------------
+   A forcing hook runs where the reference calls set_forcing: after the
+   friction block (:59-78) and before the Adams-Bashforth update (:87-102),
+   inside ps3d_integrate and ps3d_run.  fn is called on the host with the
+   model's stream, the driver's itt and tau.  It may launch its own kernels
+   on that stream on the device arrays (ps3d_device_ptr): the device forcing
+   path; or synchronise the stream and go through host copies
+   (ps3d_get_field / ps3d_set_field: u, v, w, b and du..db at tau hold the
+   post-friction state there): the host round trip.  A nonzero return stops
+   the step with PS3D_ERR_ARG.  While a hook or the wavemaker is set the
+   step takes the unfused friction kernels.  FP64 form only; fn = NULL
+   removes the hook.  Replaces the config's set_forcing
+   (for_config/*.f90, e.g. lee_wave1.f90:232-331) for GPU runs. */
+typedef int (*ps3d_forcing_fn)(ps3d_model *m, void *user, int itt, int tau, void *stream);
+int ps3d_set_forcing(ps3d_model *m, ps3d_forcing_fn fn, void *user);
+
+/* Device address of a field array (Fortran layout with halo, as the host
+   array): u, v, w, b, p (slot ignored) or du..db at slot 1..3.  u..b move
+   between two buffers from step to step: query inside the hook. */
+int ps3d_device_ptr(ps3d_model *m, int field, int slot, void **ptr);
+
+/* The wavemaker forcing of for_src/config.f90:224-237 (lee_wave1.f90:320-323
+   with omega = -omega_e) on the device, before any hook:
+     q = q + dt*(cos(omega*t)*q_r + sin(omega*t)*q_i),  t = dt*itt,
+   for q = u, v, w, b.  re[f], im[f] (f = 0..3: u, v, w, b) are host arrays
+   of a field's layout, copied to the device; a NULL pair leaves q unforced.
+   cos and sin are evaluated on the host, as the Fortran driver does. */
+int ps3d_set_wavemaker(ps3d_model *m, double omega, const double *const re[4], const double *const im[4]);
+int ps3d_clear_wavemaker(ps3d_model *m);
+
 /* ---- phases, for drivers whose set_forcing does work ------------------
    integrate.f90 in pieces: tendency (:12-54), friction (:59-78), then the
    driver's set_forcing on host mirrors, then ab_update (:87-102) with the

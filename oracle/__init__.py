@@ -125,6 +125,9 @@ def load_oracle():
         for n in ("orc_integrate_pre", "orc_integrate"):
             getattr(lib, n).argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         lib.orc_pressure_solver.argtypes = [vp]
+        lib.orc_wavemaker.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(_dp), ctypes.POINTER(_dp)]
+        lib.orc_set_wavemaker.argtypes = [vp, ctypes.c_double, ctypes.POINTER(_dp), ctypes.POINTER(_dp)]
+        lib.orc_set_forcing.argtypes = [vp, vp, vp]
         lib.orc_run.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
         for n in ("orc_my_fft_2d", "orc_my_ifft_2d", "orc_my_fft_3d", "orc_my_ifft_3d"):
             getattr(lib, n).argtypes = [vp, _dp, _dp]
@@ -248,6 +251,28 @@ class OracleModel:
 
     def pressure_solver(self):
         self.lib.orc_pressure_solver(self.h)
+
+    def _amp_tables(self, re, im):
+        self._wm_keep = []
+        rp, ip = (_dp * 4)(), (_dp * 4)()
+        for q, n in enumerate(("u", "v", "w", "b")):
+            if n in re:
+                a = np.asfortranarray(re[n], dtype=np.float64)
+                b = np.asfortranarray(im[n], dtype=np.float64)
+                self._wm_keep += [a, b]
+                rp[q], ip[q] = _as_dp(a), _as_dp(b)
+        return rp, ip
+
+    def wavemaker(self, omega, re, im, itt=None):
+        """for_src/config.f90:224-237 on the oracle's fields (whole arrays)"""
+        rp, ip = self._amp_tables(re, im)
+        self.lib.orc_wavemaker(self.h, self.itt if itt is None else itt, omega, rp, ip)
+
+    def set_wavemaker(self, omega, re, im):
+        """the wavemaker as the set_forcing hook of integrate"""
+        rp, ip = self._amp_tables(re, im)
+        self._wm_tables = (rp, ip)
+        self.lib.orc_set_wavemaker(self.h, omega, rp, ip)
 
     def integrate(self):
         self.lib.orc_integrate(self.h, self.itt, self.tau, self.taum1, self.taum2)

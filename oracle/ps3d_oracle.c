@@ -48,6 +48,10 @@ struct orc_model {
     plan1d px, py, pz;        /* lengths nx/2, ny, nz */
     cplx *wk;                 /* exp(-2 pi i k/nx), k = 0..nx/2 */
     cplx *row;                /* 2*max(nx,ny,nz) + 2 */
+    orc_forcing_fn forcing;   /* set_forcing (integrate.f90:82), NULL: empty */
+    void *forcing_user;
+    double wm_omega;          /* orc_set_wavemaker: amplitudes owned by the caller */
+    const double *wm_re[4], *wm_im[4];
 };
 
 #define ONX 2
@@ -583,6 +587,45 @@ void orc_border_exchg_3d(orc_model *m, double *a)
 /* integrate.f90                                                             */
 /* ------------------------------------------------------------------------ */
 
+void orc_set_forcing(orc_model *m, orc_forcing_fn fn, void *user)
+{
+    m->forcing = fn;
+    m->forcing_user = user;
+}
+
+/* wavemaker forcing, for_src/config.f90:224-237 (lee_wave1.f90:320-323 with
+   omega = -omega_e): q = q + dt*(cos(omega*t)*q_r + sin(omega*t)*q_i) over
+   the whole array, t = dt*itt.  re[f], im[f] for f = u, v, w, b; NULL skips. */
+void orc_wavemaker(orc_model *m, int itt, double omega, const double *const *re, const double *const *im)
+{
+    const double dt = m->p.dt;
+    const double t = dt * itt;
+    const double cs = cos(omega * t), sn = sin(omega * t);
+    double *q[4] = {m->f[ORC_U], m->f[ORC_V], m->f[ORC_W], m->f[ORC_B]};
+    for (int f = 0; f < 4; ++f) {
+        if (!re[f]) continue;
+        for (int64_t n = 0; n < m->H; ++n) q[f][n] = q[f][n] + dt * (cs * re[f][n] + sn * im[f][n]);
+    }
+}
+
+static void wavemaker_hook(orc_model *m, void *user, int itt, int tau)
+{
+    (void)user;
+    (void)tau;
+    orc_wavemaker(m, itt, m->wm_omega, m->wm_re, m->wm_im);
+}
+
+/* the wavemaker as the model's set_forcing */
+void orc_set_wavemaker(orc_model *m, double omega, const double *const *re, const double *const *im)
+{
+    m->wm_omega = omega;
+    for (int f = 0; f < 4; ++f) {
+        m->wm_re[f] = re[f];
+        m->wm_im[f] = im[f];
+    }
+    orc_set_forcing(m, wavemaker_hook, NULL);
+}
+
 void orc_integrate_pre(orc_model *m, int itt, int tau, int taum1, int taum2)
 {
     const orc_params *p = &m->p;
@@ -656,7 +699,9 @@ void orc_integrate_pre(orc_model *m, int itt, int tau, int taum1, int taum2)
     if (p->Khbi > 0.) orc_biharmonic_h(m, b, p->Khbi);
     if (p->Kvbi > 0.) orc_biharmonic_v(m, b, p->Kvbi);
 
-    /* set_forcing (:82): the oracle runs configs with an empty hook */
+    /* set_forcing (:82): the caller's hook, else empty as in
+       for_config/eddy1.f90:141-142 */
+    if (m->forcing) m->forcing(m, m->forcing_user, itt, tau);
 
     /* Adams-Bashforth (:87-102), whole arrays */
     const double dt = p->dt, dtw = p->dt / p->dsqr;

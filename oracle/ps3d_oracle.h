@@ -60,6 +60,13 @@ void orc_biharmonic_v(orc_model *m, double *p1, double Vmix);
 void orc_border_exchg_3d(orc_model *m, double *a);
 /* integrate.f90:12-111: everything before the call to pressure_solver */
 void orc_integrate_pre(orc_model *m, int itt, int tau, int taum1, int taum2);
+/* set_forcing (integrate.f90:82) hook of orc_integrate_pre; NULL: empty */
+typedef void (*orc_forcing_fn)(orc_model *m, void *user, int itt, int tau);
+void orc_set_forcing(orc_model *m, orc_forcing_fn fn, void *user);
+/* for_src/config.f90:224-237 wavemaker, whole arrays; re/im: 4 entries */
+void orc_wavemaker(orc_model *m, int itt, double omega, const double *const *re, const double *const *im);
+/* orc_wavemaker as the model's hook (amplitude arrays stay the caller's) */
+void orc_set_wavemaker(orc_model *m, double omega, const double *const *re, const double *const *im);
 /* pressure_solver.f90:2-120 */
 void orc_pressure_solver(orc_model *m);
 /* integrate.f90:2-115 */

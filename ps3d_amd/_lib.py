@@ -25,6 +25,9 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 
 # name -> (restype, argtypes); every function declared in include/ps3d.h
+# int (*)(ps3d_model *m, void *user, int itt, int tau, void *stream)
+FORCING_FN = ctypes.CFUNCTYPE(_c_int, _vp, _vp, _c_int, _c_int, _vp)
+
 PROTOTYPES = {
     "ps3d_last_error": (_c_int, [ctypes.c_char_p, _c_int]),
     "ps3d_version": (ctypes.c_char_p, []),
@@ -53,6 +56,10 @@ PROTOTYPES = {
     "ps3d_tune": (_c_int, [_vp, _c_int, ctypes.c_char_p, _c_int]),
     "ps3d_tuning": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
     "ps3d_diagnose": (_c_int, [_vp, _dp, _i32p]),
+    "ps3d_set_forcing": (_c_int, [_vp, _vp, _vp]),
+    "ps3d_device_ptr": (_c_int, [_vp, _c_int, _c_int, ctypes.POINTER(_vp)]),
+    "ps3d_set_wavemaker": (_c_int, [_vp, _c_dbl, ctypes.POINTER(_dp), ctypes.POINTER(_dp)]),
+    "ps3d_clear_wavemaker": (_c_int, [_vp]),
     "ps3d_write_restart": (_c_int, [_vp, ctypes.c_char_p, _c_int, _c_dbl, _c_int, _c_int, _c_int]),
     "ps3d_read_restart": (_c_int, [_vp, ctypes.c_char_p, _i32p, _dp, _i32p, _i32p, _i32p]),
     "ps3d_restart_write_host": (_c_int, [ctypes.c_char_p, _c_int, _c_int, _c_int, _c_int, _c_dbl, _c_int, _c_int,

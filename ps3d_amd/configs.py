@@ -206,6 +206,21 @@ def random_state(nx, ny, nz, seed=7, lid=True, amp=0.1, **kw):
     return prm, _finish(prm, fields)
 
 
+def wavemaker_amplitudes(prm: ModelParams, seed: int = 11, amp: float = 1e-5):
+    """Amplitude fields (q_r, q_i) of the wavemaker forcing
+    (for_src/config.f90:224-237) for u, v, w, b: uniform noise of size amp on
+    the interior, halos exchanged as set_initial_conditions leaves them."""
+    nx, ny, nz = prm.nx, prm.ny, prm.nz
+    re, im = {}, {}
+    for n in ("u", "v", "w", "b"):
+        for d, stream in ((re, 1), (im, 2)):
+            a = np.zeros(prm.shape, order="F")
+            a[2:2 + nx, 2:2 + ny, 2:2 + nz] = amp * uniform(seed, n, nx, ny, nz, stream=stream)
+            border_exchg(a)
+            d[n] = a
+    return re, im
+
+
 def ensemble_c4(per_size: int = 16, sizes=((64, 64, 32), (96, 96, 48), (128, 128, 64))):
     """Ensemble of baroclinic eddies (the 64x64x32 eddy physics) in three
     sizes, per_size members each; member m has seed 1000+m and eddy radius

@@ -251,6 +251,19 @@ void pressure(Model &m)
     launch_project_exchange(m);
 }
 
+// set_forcing (integrate.f90:82): the wavemaker, then the caller's hook
+void apply_forcing(Model &m, int itt, int tau)
+{
+    if (m.wm_on) {
+        const double t = m.prm.dt * itt;   // config.f90:232
+        launch_wavemaker(m, std::cos(m.wm_omega * t), std::sin(m.wm_omega * t));
+    }
+    if (m.forcing) {
+        const int rc = m.forcing(static_cast<ps3d_model *>(&m), m.forcing_user, itt, tau, m.stream);
+        PS3D_REQUIRE(rc == 0, "forcing hook returned " + std::to_string(rc));
+    }
+}
+
 void integrate(Model &m, int itt, int tau, int taum1, int taum2)
 {
     if (m.mixed) {
@@ -258,10 +271,12 @@ void integrate(Model &m, int itt, int tau, int taum1, int taum2)
         return;
     }
     launch_tendency(m, tau);
-    if (fused_friction_ok(m) && m.fuse_friction) {
+    const bool forced = m.forcing || m.wm_on;
+    if (fused_friction_ok(m) && m.fuse_friction && !forced) {
         launch_friction_ab(m, itt, tau, taum1, taum2);
     } else {
         launch_friction_block(m);
+        if (forced) apply_forcing(m, itt, tau);
         launch_ab_exchange(m, itt, tau, taum1, taum2);
     }
     pressure(m);
@@ -732,6 +747,74 @@ int ps3d_tuning(ps3d_model *m, char *buf, int len)
         }
     });
     return rc == PS3D_OK ? n : -rc;
+}
+
+int ps3d_set_forcing(ps3d_model *m, ps3d_forcing_fn fn, void *user)
+{
+    return guard([&] {
+        check_model(m);
+        require_fp64(*m, "ps3d_set_forcing");
+        m->forcing = fn;
+        m->forcing_user = fn ? user : nullptr;
+    });
+}
+
+int ps3d_device_ptr(ps3d_model *m, int field, int slot, void **ptr)
+{
+    return guard([&] {
+        check_model(m);
+        require_fp64(*m, "ps3d_device_ptr");
+        PS3D_REQUIRE(ptr, "null output");
+        if (field >= PS3D_U && field <= PS3D_B) {
+            *ptr = m->cur_buf(field);
+        } else if (field == PS3D_P) {
+            *ptr = m->p;
+        } else if (field >= PS3D_DU && field <= PS3D_DB) {
+            PS3D_REQUIRE(slot >= 1 && slot <= 3, "tendency slot must be 1..3");
+            double *t[4] = {m->du, m->dv, m->dw, m->db};
+            *ptr = t[field - PS3D_DU] + (int64_t)(slot - 1) * m->g.H;
+        } else {
+            throw Error(PS3D_ERR_ARG, "device_ptr: field must be u, v, w, b, p or du..db");
+        }
+    });
+}
+
+int ps3d_set_wavemaker(ps3d_model *m, double omega, const double *const re[4], const double *const im[4])
+{
+    return guard([&] {
+        check_model(m);
+        require_fp64(*m, "ps3d_set_wavemaker");
+        PS3D_REQUIRE(re && im, "null amplitude table");
+        PS3D_REQUIRE(std::isfinite(omega), "omega must be finite");
+        bool any = false;
+        for (int f = 0; f < 4; ++f) {
+            PS3D_REQUIRE((re[f] == nullptr) == (im[f] == nullptr),
+                         "wavemaker: give both or neither of re[f] and im[f]");
+            m->wm_field[f] = re[f] != nullptr;
+            if (!re[f]) continue;
+            any = true;
+            if (!m->wm_re[f]) {
+                m->wm_re[f] = dalloc<double>(*m, m->g.H);
+                m->wm_im[f] = dalloc<double>(*m, m->g.H);
+            }
+            PS3D_HIP_CHECK(hipMemcpyAsync(m->wm_re[f], re[f], sizeof(double) * (size_t)m->g.H,
+                                          hipMemcpyHostToDevice, m->stream));
+            PS3D_HIP_CHECK(hipMemcpyAsync(m->wm_im[f], im[f], sizeof(double) * (size_t)m->g.H,
+                                          hipMemcpyHostToDevice, m->stream));
+        }
+        PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
+        m->wm_omega = omega;
+        m->wm_on = any;
+    });
+}
+
+int ps3d_clear_wavemaker(ps3d_model *m)
+{
+    return guard([&] {
+        check_model(m);
+        m->wm_on = false;
+        for (int f = 0; f < 4; ++f) m->wm_field[f] = false;
+    });
 }
 
 int ps3d_diagnose(ps3d_model *m, double *out, int32_t *nan_seen)

@@ -236,6 +236,14 @@ struct Model {
     int64_t prof_launches = 0;
     double prof_ms = 0.0;
 
+    // forcing (integrate.f90:82): the caller's hook and the built-in wavemaker
+    ps3d_forcing_fn forcing = nullptr;
+    void *forcing_user = nullptr;
+    double *wm_re[4] = {}, *wm_im[4] = {};   // amplitudes, allocated on first use
+    bool wm_field[4] = {};                    // fields the wavemaker forces
+    double wm_omega = 0.0;
+    bool wm_on = false;
+
     // work arrays of the balancing diagnostic (balance.hip), made on first use
     struct BalanceWork *bal = nullptr;
     void *mon = nullptr;              // partial sums of the monitors (monitor.hip)
@@ -251,6 +259,8 @@ void launch_momentum_advection(Model &m, int tau);
 void launch_friction_block(Model &m);
 void launch_mixing(Model &m, int op, int field, double hmix, double vmix, int zero_top);
 void launch_ab_exchange(Model &m, int itt, int tau, int taum1, int taum2);
+// the wavemaker on the current buffers: q += dt*(cs*re + sn*im), interior
+void launch_wavemaker(Model &m, double cs, double sn);
 bool fused_friction_ok(const Model &m);
 void launch_friction_ab(Model &m, int itt, int tau, int taum1, int taum2);
 void launch_border_exchg(Model &m, double *a, int neumann_z);

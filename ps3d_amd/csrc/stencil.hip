@@ -432,6 +432,27 @@ struct AbArgs {
     int order;                              // 1, 2, 3
 };
 
+// wavemaker forcing, for_src/config.f90:224-237 (lee_wave1.f90:320-323):
+// q = q + dt*(cs*q_r + sn*q_i), cs = cos(omega t) and sn = sin(omega t)
+// from the host.  Interior points only: the Adams-Bashforth update that
+// follows reads the interior, and the exchange after it rebuilds the halo,
+// which leaves the state the reference's whole-array update plus exchange
+// leaves.  blockIdx.z runs over levels x forced fields.
+struct WaveArgs {
+    double *f[4];
+    const double *re[4], *im[4];
+};
+
+__global__ __launch_bounds__(BX * BY) void k_wavemaker(StepConsts c, WaveArgs a, double cs, double sn)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    if (i > c.nx || j > c.ny) return;
+    const int k = (int)blockIdx.z % c.nz + 1, f = (int)blockIdx.z / c.nz;
+    const int64_t o = idx(c, i, j, k);
+    a.f[f][o] = a.f[f][o] + c.dt * (cs * a.re[f][o] + sn * a.im[f][o]);
+}
+
 __global__ __launch_bounds__(BX * BY) void k_ab(StepConsts c, AbArgs a)
 {
     const int i = blockIdx.x * BX + threadIdx.x + 1;
@@ -1106,6 +1127,22 @@ void launch_friction_block(Model &m)
     }
     if (p.Khbi > 0.) launch_mixing(m, PS3D_BIHARMONIC_H, PS3D_B, p.Khbi, 0.0, 0);
     if (p.Kvbi > 0.) launch_mixing(m, PS3D_BIHARMONIC_V, PS3D_B, 0.0, p.Kvbi, 0);
+}
+
+void launch_wavemaker(Model &m, double cs, double sn)
+{
+    WaveArgs a{};
+    int n = 0;
+    for (int f = 0; f < 4; ++f) {
+        if (!m.wm_field[f]) continue;
+        a.f[n] = m.cur_buf(f);
+        a.re[n] = m.wm_re[f];
+        a.im[n] = m.wm_im[f];
+        ++n;
+    }
+    if (!n) return;
+    hipLaunchKernelGGL(k_wavemaker, interior_grid(m.g, n), dim3(BX, BY), 0, m.stream, m.c, a, cs, sn);
+    PS3D_HIP_CHECK(hipGetLastError());
 }
 
 // integrate.f90:87-111

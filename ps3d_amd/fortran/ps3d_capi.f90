@@ -111,6 +111,32 @@ module ps3d_capi
    real(c_double), intent(in) :: qs(*)
    real(c_double), intent(out) :: qp(*)
   end function
+  ! set_forcing hook (integrate.f90:82): fn is a bind(C) function
+  ! integer(c_int) fn(model, user, itt, tau, stream), passed with c_funloc
+  integer(c_int) function ps3d_set_forcing(model, fn, user) bind(C, name="ps3d_set_forcing")
+   import :: c_int, c_ptr, c_funptr
+   type(c_ptr), value :: model
+   type(c_funptr), value :: fn
+   type(c_ptr), value :: user
+  end function
+  integer(c_int) function ps3d_device_ptr(model, field, slot, ptr) bind(C, name="ps3d_device_ptr")
+   import :: c_int, c_ptr
+   type(c_ptr), value :: model
+   integer(c_int), value :: field, slot
+   type(c_ptr), intent(out) :: ptr
+  end function
+  ! wavemaker forcing of for_src/config.f90:224-237; re(f), im(f) = c_loc of
+  ! u_r.. b_r and u_i.. b_i (c_null_ptr leaves a field unforced)
+  integer(c_int) function ps3d_set_wavemaker(model, omega, re, im) bind(C, name="ps3d_set_wavemaker")
+   import :: c_int, c_ptr, c_double
+   type(c_ptr), value :: model
+   real(c_double), value :: omega
+   type(c_ptr), intent(in) :: re(4), im(4)
+  end function
+  integer(c_int) function ps3d_clear_wavemaker(model) bind(C, name="ps3d_clear_wavemaker")
+   import :: c_int, c_ptr
+   type(c_ptr), value :: model
+  end function
  end interface
 
 contains

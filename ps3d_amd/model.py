@@ -223,6 +223,54 @@ class Model:
             check(-n)
         return buf.value.decode()
 
+    # --- forcing (integrate.f90:82) -----------------------------------------
+    def set_forcing(self, fn):
+        """Install fn(model, itt, tau, stream) -> None as the set_forcing hook
+        (ps3d_set_forcing); None removes it.  fn runs between the friction
+        block and the Adams-Bashforth update of every step; it may launch work
+        on `stream` on device_ptr() arrays, or synchronize and use pull/push
+        on u, v, w, b and the tau slot of du..db (the host round trip)."""
+        if fn is None:
+            self._forcing_cb = None
+            check(self.lib.ps3d_set_forcing(self.h, None, None))
+            return
+
+        def tramp(_m, _user, itt, tau, stream):
+            try:
+                fn(self, itt, tau, stream)
+                return 0
+            except Exception as e:   # reported as the step's error
+                self._forcing_error = e
+                return 1
+
+        self._forcing_cb = _lib.FORCING_FN(tramp)
+        check(self.lib.ps3d_set_forcing(self.h, ctypes.cast(self._forcing_cb, ctypes.c_void_p), None))
+
+    def device_ptr(self, name: str, slot: int | None = None) -> int:
+        """Device address of a field array (ps3d_device_ptr)."""
+        out = ctypes.c_void_p()
+        check(self.lib.ps3d_device_ptr(self.h, FIELDS[name], 0 if slot is None else slot, ctypes.byref(out)))
+        return out.value
+
+    def set_wavemaker(self, omega: float, re: dict, im: dict):
+        """Device wavemaker forcing (ps3d_set_wavemaker); re/im map u, v, w, b
+        to arrays of the field layout (missing names are not forced)."""
+        keep = []
+        rp = (_lib._dp * 4)()
+        ip = (_lib._dp * 4)()
+        for q, n in enumerate(("u", "v", "w", "b")):
+            if n in re:
+                a = np.asfortranarray(re[n], dtype=np.float64)
+                b = np.asfortranarray(im[n], dtype=np.float64)
+                assert a.shape == self.f[n].shape and b.shape == self.f[n].shape, n
+                keep += [a, b]
+                rp[q] = _dp(a)
+                ip[q] = _dp(b)
+        check(self.lib.ps3d_set_wavemaker(self.h, float(omega), rp, ip))
+
+    def clear_wavemaker(self):
+        check(self.lib.ps3d_clear_wavemaker(self.h))
+
     def diagnose(self) -> dict:
         """diagnose.F90:11-53 on the device: KE, PE (per point), max |u_h|,
         max |w|, CFL and Peclet numbers, and whether a NaN was seen."""
