@@ -207,6 +207,30 @@ int ps3d_tuning(ps3d_model *m, char *buf, int len);
 int ps3d_diag_balance(ps3d_model *m, int filter_width, const double *kx, const double *ky, const double *kz,
                       double *const *out);
 
+/* Optimal balance, diag_opt_balance.f90:55-179 (FP64): from the model's
+   current state, iterate {integrate backward n_end = int(period/dt)+1 Euler
+   steps with Ro ramped to 0 and no biharmonic friction; project on the
+   geostrophic mode (or average the linear model: time_average); integrate
+   forward with Ro ramped back; restore the geostrophic base point (or the
+   time-average boundary condition)} until the norm of the change
+   (opt_balance_norm, :338-363) is below tol after at least two iterations.
+   out[4] (nullable entries): host arrays u_bal, v_bal, w_bal, b_bal with
+   halo; norms (nullable): max_iterations entries, norm_diff per iteration;
+   *iterations: iterations run.  The model's state (fields, tendency slots,
+   p), dt, Ro and friction are as before on return.  kx, ky, kz as for
+   ps3d_diag_balance (NULL: my_fft.f90:87-110). */
+typedef struct ps3d_opt_balance_params {
+    double period;           /* opt_balance_period (s) */
+    double average;          /* opt_balance_average (s) */
+    double tol;              /* opt_balance_tol */
+    int32_t max_iterations;  /* opt_balance_max_Itts */
+    int32_t average_times;   /* opt_balance_average_times */
+    int32_t time_average;    /* enable_diag_opt_time_ave */
+    int32_t reserved;        /* 0 */
+} ps3d_opt_balance_params;
+int ps3d_opt_balance(ps3d_model *m, const ps3d_opt_balance_params *ob, const double *kx, const double *ky,
+                     const double *kz, double *const out[4], double *norms, int32_t *iterations);
+
 /* diagnose.F90:11-53 on the device: out[8] = KE, PE (per grid point), max
    |u_h|, max |w|, horizontal and vertical CFL, horizontal and vertical
    Peclet numbers; *nan_seen = 1 if |u_h| was NaN (diagnose.F90:42-45, where

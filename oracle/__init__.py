@@ -128,6 +128,7 @@ def load_oracle():
         lib.orc_wavemaker.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(_dp), ctypes.POINTER(_dp)]
         lib.orc_set_wavemaker.argtypes = [vp, ctypes.c_double, ctypes.POINTER(_dp), ctypes.POINTER(_dp)]
         lib.orc_set_forcing.argtypes = [vp, vp, vp]
+        lib.orc_set_dynamic.argtypes = [vp] + [ctypes.c_double] * 6
         lib.orc_run.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
         for n in ("orc_my_fft_2d", "orc_my_ifft_2d", "orc_my_fft_3d", "orc_my_ifft_3d"):
             getattr(lib, n).argtypes = [vp, _dp, _dp]
@@ -251,6 +252,10 @@ class OracleModel:
 
     def pressure_solver(self):
         self.lib.orc_pressure_solver(self.h)
+
+    def set_dynamic(self, dt, Ro, Ahbi, Avbi, Khbi, Kvbi):
+        """dt, Ro and the biharmonic coefficients of the next steps"""
+        self.lib.orc_set_dynamic(self.h, dt, Ro, Ahbi, Avbi, Khbi, Kvbi)
 
     def _amp_tables(self, re, im):
         self._wm_keep = []

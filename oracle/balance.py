@@ -416,3 +416,132 @@ class Balance:
         self.second_order()
         self.third_order()
         return self.r
+
+
+# ---- diag_opt_balance.f90 -----------------------------------------------------
+def ramp(theta):
+    """diag_opt_balance.f90:367-378"""
+    import math
+    t1 = 1.0 / max(1e-32, theta)
+    t2 = 1.0 / max(1e-32, 1.0 - theta)
+    return math.exp(-t1) / (math.exp(-t1) + math.exp(-t2))
+
+
+class OptBalance:
+    """diag_opt_balance.f90:55-179 on one oracle model: backward integration
+    with the nonlinear terms ramped off (Ro -> 0, dt < 0, no biharmonic
+    friction), projection on the geostrophic mode, forward integration with
+    them ramped on, the geostrophic base point put back (or the time-average
+    variant, enable_diag_opt_time_ave), until the norm of the change is
+    below tol.  The model's state is restored at the end; run() returns
+    (u_bal, v_bal, w_bal, b_bal) and the norm per iteration."""
+
+    def __init__(self, orc, period, max_iterations, tol, time_average=False, average=1.0, average_times=1):
+        self.orc = orc
+        self.prm = orc.prm
+        self.bal = Balance(orc)
+        self.period, self.max_its, self.tol = period, max_iterations, tol
+        self.time_average, self.average, self.average_times = time_average, average, average_times
+
+    def _set(self, dt, Ro, bih=True):
+        p = self.prm
+        if bih:
+            self.orc.set_dynamic(dt, Ro, p.Ahbi, p.Avbi, p.Khbi, p.Kvbi)
+        else:
+            self.orc.set_dynamic(dt, Ro, 0.0, 0.0, 0.0, 0.0)
+
+    def _steps(self, n, dt, ro_of, bih=True, accumulate=None):
+        o = self.orc
+        o.itt, o.taum2, o.taum1, o.tau = 0, 1, 2, 3
+        for s in range(n):
+            self._set(dt, ro_of(s), bih)
+            o.integrate()
+            o.tau = o.tau % 3 + 1
+            o.taum1 = o.taum1 % 3 + 1
+            o.taum2 = o.taum2 % 3 + 1
+            if accumulate is not None:
+                for a, q in zip(accumulate, ("u", "v", "w", "b")):
+                    a += o.f[q]
+
+    def _time_ave(self):
+        # :212-235
+        acc = [np.zeros(self.orc.shape, order="F") for _ in range(4)]
+        self._steps(self.n_end_ave, abs(self.dt_loc), lambda s: 0.0, accumulate=acc)
+        return [a / self.n_end_ave for a in acc]
+
+    def _set_state(self, fields):
+        for q, a in zip(("u", "v", "w", "b"), fields):
+            self.orc.f[q][...] = a
+
+    def _state(self):
+        return [self.orc.f[q].copy() for q in ("u", "v", "w", "b")]
+
+    def _geo(self, fields, g0_base=None, keep=False):
+        M = self.bal.modes
+        S = self.bal.forward(fields)
+        q, pv = M.vec_q(0.0), M.vec_p(0.0)
+        g0 = self.bal._dot(pv, S)
+        if g0_base is None:
+            out = self.bal.backward([g0 * qq for qq in q])
+        else:
+            out = self.bal.backward([S[c] + (g0_base - g0) * q[c] for c in range(4)])
+        return (out, g0) if keep else out
+
+    def _norm(self, cur, prev):
+        # :342-363
+        p = self.prm
+        I = (slice(2, 2 + p.nx), slice(2, 2 + p.ny), slice(2, 2 + p.nz))
+        u, v, w, b = (a[I] for a in cur)
+        ub, vb, wb, bb = (a[I] for a in prev)
+        N02 = p.N0 * p.N0
+        e1 = np.sum(u ** 2 + v ** 2 + p.dsqr * w ** 2 + b ** 2 / N02)
+        e2 = np.sum(ub ** 2 + vb ** 2 + p.dsqr * wb ** 2 + bb ** 2 / N02)
+        d = np.sum((u - ub) ** 2 + (v - vb) ** 2 + p.dsqr * (w - wb) ** 2 + (b - bb) ** 2 / N02)
+        return 2.0 * np.sqrt(d) / (np.sqrt(e1) + np.sqrt(e2))
+
+    def run(self):
+        o, p = self.orc, self.prm
+        names = ("u", "v", "w", "b", "p", "du", "dv", "dw", "db")
+        save = {n: o.f[n].copy() for n in names}
+        st = (o.itt, o.tau, o.taum1, o.taum2)
+        self.dt_loc, Ro_loc = p.dt, p.Ro
+        u_bal = self._state()
+        self.n_end = int(self.period / self.dt_loc) + 1
+        self.n_end_ave = int(self.average / self.dt_loc) + 1
+        if self.time_average:
+            for _ in range(self.average_times):
+                base = self._time_ave()
+                self._set_state(base)
+            self._set_state(u_bal)
+        else:
+            _, g0_base = self._geo(self._state(), keep=True)
+        norms = []
+        for n in range(1, self.max_its + 1):
+            # :190-198 backward, friction off, Ro ramped to 0
+            self._steps(self.n_end, -abs(self.dt_loc),
+                        lambda s: (1.0 - ramp(s * self.dt_loc / self.period)) * Ro_loc, bih=False)
+            if self.time_average:
+                for _ in range(self.average_times):
+                    self._set_state(self._time_ave())
+            else:
+                self._set_state(self._geo(self._state()))
+            self._steps(self.n_end, abs(self.dt_loc), lambda s: ramp(s * self.dt_loc / self.period) * Ro_loc)
+            if self.time_average:
+                keep = self._state()
+                for _ in range(self.average_times):
+                    ave = self._time_ave()
+                    self._set_state(ave)
+                self._set_state([keep[c] - ave[c] + base[c] for c in range(4)])
+            else:
+                self._set_state(self._geo(self._state(), g0_base=g0_base))
+            cur = self._state()
+            nd = self._norm(cur, u_bal)
+            norms.append(nd)
+            u_bal = cur
+            if n > 1 and nd < self.tol:
+                break
+        for n_ in names:
+            o.f[n_][...] = save[n_]
+        self._set(self.dt_loc, Ro_loc)
+        o.itt, o.tau, o.taum1, o.taum2 = st
+        return u_bal, norms

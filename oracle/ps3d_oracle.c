@@ -587,6 +587,18 @@ void orc_border_exchg_3d(orc_model *m, double *a)
 /* integrate.f90                                                             */
 /* ------------------------------------------------------------------------ */
 
+/* the parameters diag_opt_balance.f90 changes between steps (:90-91,
+   :190-198): dt, Ro and the biharmonic coefficients */
+void orc_set_dynamic(orc_model *m, double dt, double Ro, double Ahbi, double Avbi, double Khbi, double Kvbi)
+{
+    m->p.dt = dt;
+    m->p.Ro = Ro;
+    m->p.Ahbi = Ahbi;
+    m->p.Avbi = Avbi;
+    m->p.Khbi = Khbi;
+    m->p.Kvbi = Kvbi;
+}
+
 void orc_set_forcing(orc_model *m, orc_forcing_fn fn, void *user)
 {
     m->forcing = fn;

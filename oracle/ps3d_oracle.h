@@ -60,6 +60,8 @@ void orc_biharmonic_v(orc_model *m, double *p1, double Vmix);
 void orc_border_exchg_3d(orc_model *m, double *a);
 /* integrate.f90:12-111: everything before the call to pressure_solver */
 void orc_integrate_pre(orc_model *m, int itt, int tau, int taum1, int taum2);
+/* dt, Ro and the biharmonic coefficients for the next steps */
+void orc_set_dynamic(orc_model *m, double dt, double Ro, double Ahbi, double Avbi, double Khbi, double Kvbi);
 /* set_forcing (integrate.f90:82) hook of orc_integrate_pre; NULL: empty */
 typedef void (*orc_forcing_fn)(orc_model *m, void *user, int itt, int tau);
 void orc_set_forcing(orc_model *m, orc_forcing_fn fn, void *user);

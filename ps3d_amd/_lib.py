@@ -25,6 +25,13 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 
 # name -> (restype, argtypes); every function declared in include/ps3d.h
+class OptBalanceParams(ctypes.Structure):
+    """ps3d_opt_balance_params"""
+    _fields_ = [("period", ctypes.c_double), ("average", ctypes.c_double), ("tol", ctypes.c_double),
+                ("max_iterations", ctypes.c_int32), ("average_times", ctypes.c_int32),
+                ("time_average", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 # int (*)(ps3d_model *m, void *user, int itt, int tau, void *stream)
 FORCING_FN = ctypes.CFUNCTYPE(_c_int, _vp, _vp, _c_int, _c_int, _vp)
 
@@ -67,6 +74,8 @@ PROTOTYPES = {
     "ps3d_restart_read_host": (_c_int, [ctypes.c_char_p, _c_int, _c_int, _c_int, _i32p, _dp, _i32p, _i32p, _i32p,
                                         _dp]),
     "ps3d_diag_balance": (_c_int, [_vp, _c_int, _dp, _dp, _dp, ctypes.POINTER(_dp)]),
+    "ps3d_opt_balance": (_c_int, [_vp, ctypes.POINTER(OptBalanceParams), _dp, _dp, _dp, ctypes.POINTER(_dp), _dp,
+                                  _i32p]),
     "ps3d_ensemble_create": (_c_int, [_c_int, ctypes.POINTER(CParams), _vp, ctypes.POINTER(_vp)]),
     "ps3d_ensemble_destroy": (_c_int, [_vp]),
     "ps3d_ensemble_set_field": (_c_int, [_vp, _c_int, _c_int, _dp]),

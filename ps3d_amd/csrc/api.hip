@@ -688,6 +688,35 @@ int ps3d_spectral_solve(ps3d_model *m, double *qs)
     });
 }
 
+// my_fft.f90:87-110 for the wavenumber tables the caller does not pass
+static void default_wavenumbers(const Model &m, const double *&kx, const double *&ky, const double *&kz,
+                                std::vector<double> &wx, std::vector<double> &wy, std::vector<double> &wz)
+{
+    const ps3d_params &p = m.prm;
+    const double pi = std::acos(0.0) * 2.0;
+    if (!kx) {
+        wx.resize(m.g.nkx);
+        for (int i = 1; i <= m.g.nkx; ++i) wx[i - 1] = (double)(2 * (i - 1)) * pi / p.Lx;
+        kx = wx.data();
+    }
+    if (!ky) {
+        wy.resize(p.ny);
+        for (int j = 1; j <= (p.ny + 1) / 2; ++j) wy[j - 1] = (double)(2 * (j - 1)) * pi / p.Ly;
+        for (int j = (p.ny + 1) / 2 + 1; j <= p.ny; ++j) wy[j - 1] = -((double)(2 * (p.ny - j + 1)) * pi / p.Ly);
+        ky = wy.data();
+    }
+    if (!kz) {
+        wz.resize(p.nz);
+        if (p.enable_vertical_boundaries) {
+            for (int k = 1; k <= p.nz; ++k) wz[k - 1] = (double)(float)(k - 1) * pi / p.Lz;
+        } else {
+            for (int k = 1; k <= (p.nz + 1) / 2; ++k) wz[k - 1] = (double)(2 * (k - 1)) * pi / p.Lz;
+            for (int k = (p.nz + 1) / 2 + 1; k <= p.nz; ++k) wz[k - 1] = -((double)(2 * (p.nz - k + 1)) * pi / p.Lz);
+        }
+        kz = wz.data();
+    }
+}
+
 int ps3d_diag_balance(ps3d_model *m, int filter_width, const double *kx, const double *ky, const double *kz,
                       double *const *out)
 {
@@ -695,33 +724,24 @@ int ps3d_diag_balance(ps3d_model *m, int filter_width, const double *kx, const d
         check_model(m);
         require_fp64(*m, "ps3d_diag_balance");
         PS3D_REQUIRE(out, "null output table");
-        const ps3d_params &p = m->prm;
-        // my_fft.f90:87-110 where the caller passes no wavenumbers
-        const double pi = std::acos(0.0) * 2.0;
         std::vector<double> wx, wy, wz;
-        if (!kx) {
-            wx.resize(m->g.nkx);
-            for (int i = 1; i <= m->g.nkx; ++i) wx[i - 1] = (double)(2 * (i - 1)) * pi / p.Lx;
-            kx = wx.data();
-        }
-        if (!ky) {
-            wy.resize(p.ny);
-            for (int j = 1; j <= (p.ny + 1) / 2; ++j) wy[j - 1] = (double)(2 * (j - 1)) * pi / p.Ly;
-            for (int j = (p.ny + 1) / 2 + 1; j <= p.ny; ++j) wy[j - 1] = -((double)(2 * (p.ny - j + 1)) * pi / p.Ly);
-            ky = wy.data();
-        }
-        if (!kz) {
-            wz.resize(p.nz);
-            if (p.enable_vertical_boundaries) {
-                for (int k = 1; k <= p.nz; ++k) wz[k - 1] = (double)(float)(k - 1) * pi / p.Lz;
-            } else {
-                for (int k = 1; k <= (p.nz + 1) / 2; ++k) wz[k - 1] = (double)(2 * (k - 1)) * pi / p.Lz;
-                for (int k = (p.nz + 1) / 2 + 1; k <= p.nz; ++k)
-                    wz[k - 1] = -((double)(2 * (p.nz - k + 1)) * pi / p.Lz);
-            }
-            kz = wz.data();
-        }
+        default_wavenumbers(*m, kx, ky, kz, wx, wy, wz);
         diag_balance(*m, filter_width, out, kx, ky, kz);
+    });
+}
+
+int ps3d_opt_balance(ps3d_model *m, const ps3d_opt_balance_params *ob, const double *kx, const double *ky,
+                     const double *kz, double *const out[4], double *norms, int32_t *iterations)
+{
+    return guard([&] {
+        check_model(m);
+        require_fp64(*m, "ps3d_opt_balance");
+        PS3D_REQUIRE(ob, "null parameters");
+        std::vector<double> wx, wy, wz;
+        default_wavenumbers(*m, kx, ky, kz, wx, wy, wz);
+        int its = 0;
+        opt_balance(*m, *ob, out, norms, &its, kx, ky, kz);
+        if (iterations) *iterations = its;
     });
 }
 

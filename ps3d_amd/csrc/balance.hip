@@ -12,6 +12,8 @@
 // column loads are coalesced and the matrix entry is uniform).
 #include "ps3d_internal.h"
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <vector>
 
@@ -204,7 +206,9 @@ enum Op {
     OP_PTG0,      // out = (-i I_00) q0                                  (:213-217)
     OP_SECOND,    // f2p, f2m; out = Ro^2 (f2p q+ + f2m q-)               (:250-264)
     OP_THIRD,     // f3p, f3m; out = Ro^3 (f3p q+ + f3m q-)               (:345-356)
-    OP_FILTER     // zero where kz^2 >= cut^2                            (:520-533)
+    OP_FILTER,    // zero where kz^2 >= cut^2                            (:520-533)
+    OP_GEO_BASE,  // OP_ZERO, g0 kept in aux0         (diag_opt_balance.f90:262-272)
+    OP_GEO_EXCH   // g0 = p0.S; out = S + (g0b - g0) q0, g0b = in 4     (:303-313)
 };
 
 struct OpArgs {
@@ -239,6 +243,26 @@ __global__ __launch_bounds__(BT) void k_bal_op(BalConsts P, OpArgs A)
         A.out[1][n] = cm(g0, q.y);
         A.out[2][n] = cm(g0, q.z);
         A.out[3][n] = cm(g0, q.b);
+        break;
+    }
+    case OP_GEO_BASE: {
+        const double2 g0 = pdot(E.vec_p(0.0), in(0), in(1), in(2), in(3));
+        const V4 q = E.vec_q(0.0);
+        A.aux[0][n] = g0;
+        A.out[0][n] = cm(g0, q.x);
+        A.out[1][n] = cm(g0, q.y);
+        A.out[2][n] = cm(g0, q.z);
+        A.out[3][n] = cm(g0, q.b);
+        break;
+    }
+    case OP_GEO_EXCH: {
+        const double2 g0 = pdot(E.vec_p(0.0), in(0), in(1), in(2), in(3));
+        const V4 q = E.vec_q(0.0);
+        const double2 dg = cs(in(4), g0);
+        A.out[0][n] = ca(in(0), cm(dg, q.x));
+        A.out[1][n] = ca(in(1), cm(dg, q.y));
+        A.out[2][n] = ca(in(2), cm(dg, q.z));
+        A.out[3][n] = ca(in(3), cm(dg, q.b));
         break;
     }
     case OP_DOT3:
@@ -382,6 +406,70 @@ __global__ __launch_bounds__(BT) void k_comb(int64_t n, double *__restrict__ out
 }
 
 unsigned blocks(int64_t n) { return (unsigned)((n + BT - 1) / BT); }
+
+// ---- optimal balance (diag_opt_balance.f90) ---------------------------------
+// whole-array updates, as the reference's array expressions
+__global__ __launch_bounds__(BT) void k_ob_acc(int64_t n, double *__restrict__ acc, const double *__restrict__ x)
+{
+    const int64_t t = (int64_t)blockIdx.x * BT + threadIdx.x;
+    if (t < n) acc[t] = acc[t] + x[t];   // :227-230
+}
+
+__global__ __launch_bounds__(BT) void k_ob_div(int64_t n, double *__restrict__ a, double d)
+{
+    const int64_t t = (int64_t)blockIdx.x * BT + threadIdx.x;
+    if (t < n) a[t] = a[t] / d;   // :232-233
+}
+
+__global__ __launch_bounds__(BT) void k_ob_bc(int64_t n, double *__restrict__ out, const double *__restrict__ save,
+                                              const double *__restrict__ ave, const double *__restrict__ base)
+{
+    const int64_t t = (int64_t)blockIdx.x * BT + threadIdx.x;
+    if (t < n) out[t] = save[t] - ave[t] + base[t];   // :153-156
+}
+
+// opt_balance_norm (:338-363): per-block partial sums of e1, e2 and d over
+// the interior, tree-reduced in LDS; the host adds the blocks in order
+__global__ __launch_bounds__(BT) void k_ob_norm(StepConsts c, const double *__restrict__ u,
+                                                const double *__restrict__ v, const double *__restrict__ w,
+                                                const double *__restrict__ b, const double *__restrict__ ub,
+                                                const double *__restrict__ vb, const double *__restrict__ wb,
+                                                const double *__restrict__ bb, double dsqr, double N02,
+                                                double *__restrict__ part)
+{
+    __shared__ double sh[3][BT];
+    const int64_t t = (int64_t)blockIdx.x * BT + threadIdx.x;
+    const int64_t N = (int64_t)c.nx * c.ny * c.nz;
+    double e1 = 0.0, e2 = 0.0, d = 0.0;
+    if (t < N) {
+        const int i = (int)(t % c.nx) + 1;
+        const int j = (int)((t / c.nx) % c.ny) + 1;
+        const int k = (int)(t / ((int64_t)c.nx * c.ny)) + 1;
+        const int64_t o = (int64_t)(i + 1) + c.X * (int64_t)(j + 1) + c.XY * (int64_t)(k + 1);
+        e1 = u[o] * u[o] + v[o] * v[o] + dsqr * (w[o] * w[o]) + (b[o] * b[o]) / N02;
+        e2 = ub[o] * ub[o] + vb[o] * vb[o] + dsqr * (wb[o] * wb[o]) + (bb[o] * bb[o]) / N02;
+        const double du = u[o] - ub[o], dv = v[o] - vb[o], dw = w[o] - wb[o], db = b[o] - bb[o];
+        d = du * du + dv * dv + dsqr * (dw * dw) + (db * db) / N02;
+    }
+    sh[0][threadIdx.x] = e1;
+    sh[1][threadIdx.x] = e2;
+    sh[2][threadIdx.x] = d;
+    __syncthreads();
+    for (int h = BT / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int q = 0; q < 3; ++q) sh[q][threadIdx.x] = sh[q][threadIdx.x] + sh[q][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) part[3 * (int64_t)blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// ramp of diag_opt_balance.f90:367-378
+double ob_ramp(double theta)
+{
+    const double t1 = 1.0 / std::max(1e-32, theta);
+    const double t2 = 1.0 / std::max(1e-32, 1.0 - theta);
+    return std::exp(-t1) / (std::exp(-t1) + std::exp(-t2));
+}
 
 }  // namespace
 
@@ -655,14 +743,9 @@ void vertical_modes(int nz, double Lz, std::vector<double> &phi, std::vector<dou
     }
 }
 
-}  // namespace
-
-// diag_balance.f90:89-106 on the model's current state; out[16] = u0, v0,
-// w0, b0, u1 .. b1, u2 .. b2, u3 .. b3 (host arrays with halo, nullable)
-void diag_balance(Model &m, int filter_width, double *const *out, const double *kx, const double *ky,
-                  const double *kz)
+// the work arrays, made on first use
+BalanceWork &bal_work(Model &m)
 {
-    PS3D_REQUIRE(!m.mixed, "diag_balance: FP64 models only");
     const Grid &g = m.g;
     if (!m.bal) {
         auto *W = new BalanceWork();
@@ -684,8 +767,14 @@ void diag_balance(Model &m, int filter_width, double *const *out, const double *
             }
         }
     }
-    BalanceWork &W = *m.bal;
-    Bal b{m, W};
+    return *m.bal;
+}
+
+// the wavenumbers on the device and the constants of the spectral operators
+void bal_consts(Model &m, Bal &b, int filter_width, const double *kx, const double *ky, const double *kz)
+{
+    const Grid &g = m.g;
+    BalanceWork &W = b.W;
     b.filter_width = filter_width;
     b.kzh.assign(kz, kz + g.nz);
     double *dk = W.k;
@@ -699,8 +788,22 @@ void diag_balance(Model &m, int filter_width, double *const *out, const double *
     P.dx = m.prm.dx; P.dy = m.prm.dy; P.dz = m.prm.dz;
     P.f0 = m.prm.f0; P.N0 = m.prm.N0; P.dsqr = m.prm.dsqr; P.Ro = m.prm.Ro;
     P.kx = dk; P.ky = dk + g.nkx; P.kz = dk + g.nkx + g.ny;
+}
+
+}  // namespace
+
+// diag_balance.f90:89-106 on the model's current state; out[16] = u0, v0,
+// w0, b0, u1 .. b1, u2 .. b2, u3 .. b3 (host arrays with halo, nullable)
+void diag_balance(Model &m, int filter_width, double *const *out, const double *kx, const double *ky,
+                  const double *kz)
+{
+    PS3D_REQUIRE(!m.mixed, "diag_balance: FP64 models only");
+    const Grid &g = m.g;
     for (int f = 0; f < 4; ++f)
         PS3D_REQUIRE(m.cur[f] == 0, "diag_balance: friction results not yet folded in by ab_update");
+    BalanceWork &W = bal_work(m);
+    Bal b{m, W};
+    bal_consts(m, b, filter_width, kx, ky, kz);
     b.zero_order();
     b.first_order();
     b.second_order();
@@ -712,6 +815,176 @@ void diag_balance(Model &m, int filter_width, double *const *out, const double *
         if (out && out[q])
             PS3D_HIP_CHECK(hipMemcpyAsync(out[q], W.r[ids[q]], sizeof(double) * g.H, hipMemcpyDeviceToHost, m.stream));
     PS3D_HIP_CHECK(hipStreamSynchronize(m.stream));
+}
+
+namespace {
+
+// dt and Ro of the next steps, biharmonic friction on or off
+void ob_set(Model &m, const ps3d_params &base, double dt, double Ro, bool bih)
+{
+    m.prm.dt = dt;
+    m.prm.Ro = Ro;
+    m.prm.Ahbi = bih ? base.Ahbi : 0.0;
+    m.prm.Avbi = bih ? base.Avbi : 0.0;
+    m.prm.Khbi = bih ? base.Khbi : 0.0;
+    m.prm.Kvbi = bih ? base.Kvbi : 0.0;
+    m.c.dt = dt;
+    m.c.rdt = 1.0 / dt;
+    m.c.Ro = Ro;
+}
+
+}  // namespace
+
+// diag_opt_balance.f90:55-179 on the model's current state: out[4] =
+// u_bal .. b_bal (host arrays with halo, nullable), norms[i] = norm_diff of
+// iteration i+1 (nullable, max_iterations entries), *iterations = iterations
+// run.  The model's state, dt, Ro and friction are restored at the end.
+void opt_balance(Model &m, const ps3d_opt_balance_params &ob, double *const *out, double *norms, int *iterations,
+                 const double *kx, const double *ky, const double *kz)
+{
+    PS3D_REQUIRE(!m.mixed, "opt_balance: FP64 models only");
+    for (int f = 0; f < 4; ++f)
+        PS3D_REQUIRE(m.cur[f] == 0, "opt_balance: friction results not yet folded in by ab_update");
+    PS3D_REQUIRE(ob.period > 0 && ob.max_iterations >= 1, "opt_balance: period > 0 and max_iterations >= 1");
+    PS3D_REQUIRE(!ob.time_average || (ob.average > 0 && ob.average_times >= 1),
+                 "opt_balance: average > 0 and average_times >= 1 for the time-average variant");
+    const Grid &g = m.g;
+    BalanceWork &W = bal_work(m);
+    Bal b{m, W};
+    bal_consts(m, b, -1, kx, ky, kz);
+    using B = BalanceWork;
+    const int64_t H = g.H;
+    hipStream_t st = m.stream;
+    auto copy = [&](double *dst, const double *src, int64_t n) {
+        PS3D_HIP_CHECK(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    };
+    auto fields = [&]() { return std::array<double *, 4>{m.u, m.v, m.w, m.b}; };   // after buffer swaps
+
+    // save the model state (:80-83)
+    const ps3d_params base = m.prm;
+    {
+        double *now[9] = {m.u, m.v, m.w, m.b, m.p, m.du, m.dv, m.dw, m.db};
+        for (int q = 0; q < 9; ++q) copy(W.save[q], now[q], q < 5 ? H : 3 * H);
+    }
+    double *ubal[4] = {b.R(B::u0), b.R(B::v0), b.R(B::w0), b.R(B::b0)};
+    double *ubase[4] = {b.R(B::u1), b.R(B::v1), b.R(B::w1), b.R(B::b1)};
+    double *usave[4] = {b.R(B::u2), b.R(B::v2), b.R(B::w2), b.R(B::b2)};
+    double *uave[4] = {b.R(B::u3), b.R(B::v3), b.R(B::w3), b.R(B::b3)};
+    for (int q = 0; q < 4; ++q) copy(ubal[q], fields()[q], H);
+    const double dt_loc = base.dt, Ro_loc = base.Ro;
+    const int n_end = (int)(ob.period / dt_loc) + 1;      // :87-88
+    const int n_end_ave = (int)(ob.average / dt_loc) + 1;
+
+    // n Euler steps (itt stays 0) from tau = 3; acc: running sums of u..b
+    auto steps = [&](int n, double dt, auto &&ro_of, bool bih, double *const *acc) {
+        int tau = 3, taum1 = 2, taum2 = 1;
+        for (int s = 0; s < n; ++s) {
+            ob_set(m, base, dt, ro_of(s), bih);
+            model_step(m, 0, tau, taum1, taum2);
+            tau = tau % 3 + 1;
+            taum1 = taum1 % 3 + 1;
+            taum2 = taum2 % 3 + 1;
+            if (acc) {
+                const auto f = fields();
+                for (int q = 0; q < 4; ++q) {
+                    hipLaunchKernelGGL(k_ob_acc, dim3(blocks(H)), dim3(BT), 0, st, H, acc[q], f[q]);
+                    PS3D_HIP_CHECK(hipGetLastError());
+                }
+            }
+        }
+    };
+    // opt_time_ave (:212-235) into dst, then u = dst
+    auto time_ave = [&](double *const *dst) {
+        for (int q = 0; q < 4; ++q) PS3D_HIP_CHECK(hipMemsetAsync(dst[q], 0, sizeof(double) * H, st));
+        steps(n_end_ave, std::fabs(dt_loc), [](int) { return 0.0; }, true, dst);
+        for (int q = 0; q < 4; ++q) {
+            hipLaunchKernelGGL(k_ob_div, dim3(blocks(H)), dim3(BT), 0, st, H, dst[q], (double)n_end_ave);
+            PS3D_HIP_CHECK(hipGetLastError());
+            copy(fields()[q], dst[q], H);
+        }
+    };
+    // the geostrophic operators (:238-314) on the current fields
+    auto geo = [&](int code) {
+        const auto f = fields();
+        const double *in[4] = {f[0], f[1], f[2], f[3]};
+        double2 *o[4] = {b.S(B::US), b.S(B::VS), b.S(B::WS), b.S(B::BS)};
+        b.forward(in, o, nullptr, nullptr);
+        if (code == OP_GEO_EXCH)
+            b.op(OP_GEO_EXCH, {o[0], o[1], o[2], o[3], b.S(B::I_00)}, {o[0], o[1], o[2], o[3]});
+        else
+            b.op(code, {o[0], o[1], o[2], o[3]}, {o[0], o[1], o[2], o[3]}, 0.0, b.S(B::I_00));
+        double *r[4] = {f[0], f[1], f[2], f[3]};
+        if (code == OP_GEO_BASE) {
+            for (int q = 0; q < 4; ++q) r[q] = ubase[q];
+        }
+        b.backward(o, r);
+    };
+    const int64_t nblk = blocks(g.N);
+    std::vector<double> part(3 * (size_t)nblk);
+    double *dpart = b.R(B::t0);   // 3 * nblk <= H doubles of scratch
+    auto norm = [&]() {
+        const auto f = fields();
+        hipLaunchKernelGGL(k_ob_norm, dim3((unsigned)nblk), dim3(BT), 0, st, m.c, f[0], f[1], f[2], f[3], ubal[0],
+                           ubal[1], ubal[2], ubal[3], base.dsqr, base.N0 * base.N0, dpart);
+        PS3D_HIP_CHECK(hipGetLastError());
+        PS3D_HIP_CHECK(hipMemcpyAsync(part.data(), dpart, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+        PS3D_HIP_CHECK(hipStreamSynchronize(st));
+        double e1 = 0.0, e2 = 0.0, d = 0.0;
+        for (int64_t q = 0; q < nblk; ++q) {
+            e1 += part[3 * q];
+            e2 += part[3 * q + 1];
+            d += part[3 * q + 2];
+        }
+        return 2.0 * std::sqrt(d) / (std::sqrt(e1) + std::sqrt(e2));   // :363
+    };
+
+    // base point (:96-110)
+    if (ob.time_average) {
+        for (int q = 0; q < ob.average_times; ++q) time_ave(ubase);
+        for (int q = 0; q < 4; ++q) copy(fields()[q], ubal[q], H);
+    } else {
+        geo(OP_GEO_BASE);
+    }
+    int its = 0;
+    for (int n = 1; n <= ob.max_iterations; ++n) {
+        // :117-118, :183-201: backward, biharmonic friction off, Ro ramped down
+        steps(n_end, -std::fabs(dt_loc), [&](int s) { return (1.0 - ob_ramp(s * dt_loc / ob.period)) * Ro_loc; },
+              false, nullptr);
+        if (ob.time_average)
+            for (int q = 0; q < ob.average_times; ++q) time_ave(uave);
+        else
+            geo(OP_ZERO);
+        // :137-139, :205-223: forward, Ro ramped up
+        steps(n_end, std::fabs(dt_loc), [&](int s) { return ob_ramp(s * dt_loc / ob.period) * Ro_loc; }, true,
+              nullptr);
+        if (ob.time_average) {
+            for (int q = 0; q < 4; ++q) copy(usave[q], fields()[q], H);
+            for (int q = 0; q < ob.average_times; ++q) time_ave(uave);
+            const auto f = fields();
+            for (int q = 0; q < 4; ++q) {
+                hipLaunchKernelGGL(k_ob_bc, dim3(blocks(H)), dim3(BT), 0, st, H, f[q], usave[q], uave[q], ubase[q]);
+                PS3D_HIP_CHECK(hipGetLastError());
+            }
+        } else {
+            geo(OP_GEO_EXCH);
+        }
+        const double nd = norm();
+        if (norms) norms[n - 1] = nd;
+        for (int q = 0; q < 4; ++q) copy(ubal[q], fields()[q], H);
+        its = n;
+        if (n > 1 && nd < ob.tol) break;   // :170-175
+    }
+    // results, then the model as it was (:180-184)
+    for (int q = 0; q < 4; ++q)
+        if (out && out[q])
+            PS3D_HIP_CHECK(hipMemcpyAsync(out[q], ubal[q], sizeof(double) * H, hipMemcpyDeviceToHost, st));
+    {
+        double *now[9] = {m.u, m.v, m.w, m.b, m.p, m.du, m.dv, m.dw, m.db};
+        for (int q = 0; q < 9; ++q) copy(now[q], W.save[q], q < 5 ? H : 3 * H);
+    }
+    ob_set(m, base, base.dt, base.Ro, true);
+    PS3D_HIP_CHECK(hipStreamSynchronize(st));
+    if (iterations) *iterations = its;
 }
 
 }  // namespace ps3d

@@ -309,6 +309,8 @@ void restart_io(Model &m, const char *path, bool write, int32_t *itt, double *ti
 // the balancing diagnostic (balance.hip); out[16] host arrays, nullable
 void diag_balance(Model &m, int filter_width, double *const *out, const double *kx, const double *ky,
                   const double *kz);
+void opt_balance(Model &m, const ps3d_opt_balance_params &ob, double *const *out, double *norms, int *iterations,
+                 const double *kx, const double *ky, const double *kz);
 
 // ---- mixed form (stencil.hip) -------------------------------------------
 // tendency with the reference pressure pint (null: zero)

@@ -24,6 +24,12 @@ module ps3d_capi
   integer(c_int32_t) :: reserved(7)
  end type ps3d_params
 
+ ! ps3d_opt_balance_params (include/ps3d.h)
+ type, bind(C) :: ps3d_opt_balance_params
+  real(c_double) :: period, average, tol
+  integer(c_int32_t) :: max_iterations, average_times, time_average, reserved
+ end type
+
  interface
   integer(c_int) function ps3d_last_error(buf, len) bind(C, name="ps3d_last_error")
    import :: c_int, c_char
@@ -110,6 +116,17 @@ module ps3d_capi
    integer(c_int), value :: three_d
    real(c_double), intent(in) :: qs(*)
    real(c_double), intent(out) :: qp(*)
+  end function
+  ! diag_opt_balance.f90:55-179 (ps3d_opt_balance_params as a bind(C) type)
+  integer(c_int) function ps3d_opt_balance(model, ob, kx, ky, kz, out, norms, iterations) &
+    bind(C, name="ps3d_opt_balance")
+   import :: c_int, c_int32_t, c_ptr, c_double, ps3d_opt_balance_params
+   type(c_ptr), value :: model
+   type(ps3d_opt_balance_params), intent(in) :: ob
+   real(c_double), intent(in) :: kx(*), ky(*), kz(*)
+   type(c_ptr), intent(in) :: out(4)
+   real(c_double), intent(out) :: norms(*)
+   integer(c_int32_t), intent(out) :: iterations
   end function
   ! set_forcing hook (integrate.f90:82): fn is a bind(C) function
   ! integer(c_int) fn(model, user, itt, tau, stream), passed with c_funloc

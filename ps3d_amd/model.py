@@ -212,6 +212,27 @@ class Model:
                                          ptr[0], ptr[1], ptr[2], tab))
         return out
 
+    def opt_balance(self, period: float, max_iterations: int, tol: float, time_average: bool = False,
+                    average: float = 1.0, average_times: int = 1, kx=None, ky=None, kz=None) -> dict:
+        """Optimal balance of the model's current state
+        (diag_opt_balance.f90:55-179): {'u': u_bal, 'v': .., 'w': .., 'b': ..,
+        'norms': [norm_diff per iteration], 'iterations': n}.  The model's own
+        state is left unchanged."""
+        shp = self.prm.shape
+        out = {n: np.zeros(shp, order="F") for n in ("u", "v", "w", "b")}
+        tab = (ctypes.POINTER(ctypes.c_double) * 4)(*[_dp(out[n]) for n in ("u", "v", "w", "b")])
+        norms = np.zeros(max(1, int(max_iterations)))
+        its = ctypes.c_int32()
+        ob = _lib.OptBalanceParams(float(period), float(average), float(tol), int(max_iterations),
+                                   int(average_times), int(bool(time_average)), 0)
+        ks = [None if k is None else np.ascontiguousarray(k, dtype=np.float64) for k in (kx, ky, kz)]
+        ptr = [None if k is None else _dp(k) for k in ks]
+        check(self.lib.ps3d_opt_balance(self.h, ctypes.byref(ob), ptr[0], ptr[1], ptr[2], tab, _dp(norms),
+                                        ctypes.byref(its)))
+        out["iterations"] = its.value
+        out["norms"] = list(norms[:its.value])
+        return out
+
     def tune(self, cache_path: str | None = None, reps: int = 5):
         """Autotune the launch shapes (ps3d_tune); the next step's slot is scratch."""
         check(self.lib.ps3d_tune(self.h, self.tau, cache_path.encode() if cache_path else None, reps))
