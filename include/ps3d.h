@@ -109,8 +109,10 @@ int ps3d_integrate(ps3d_model *m, int itt, int tau, int taum1, int taum2);
 
 /* main.f90:139-161 without diagnostics: nsteps calls of integrate with the
    tau rotation.  state = {itt, tau, taum1, taum2}, updated on return.
-   The steps run back to back on the model's stream (graph-captured per
-   tau phase when use_graph != 0); returns when they are queued. */
+   The steps are queued back to back on the model's stream and the call
+   returns when they are queued.  use_graph is accepted for interface
+   stability and has no effect on a single model (its kernels are long
+   enough to hide launch cost; ps3d_ensemble_run does replay a graph). */
 int ps3d_run(ps3d_model *m, int nsteps, int32_t *state, int use_graph);
 
 /* wait for the model's stream */
