@@ -565,7 +565,8 @@ __global__ __launch_bounds__(T) void r2c_x(StepConsts c, const typename cplx_of<
                                            typename cplx_of<RT>::type *__restrict__ dst, int64_t nrows, int dstride)
 {
     __shared__ typename cplx_of<RT>::type lds[ROWS * (N2 + 1)];
-    r2c_body<N2, ROWS, T, FUSE_DIV, RT>(lds, (int)blockIdx.x, c, tw, wk, src, u, v, w, dst, nrows, dstride);
+    r2c_body<N2, ROWS, T, FUSE_DIV, RT>(lds, xcd_remap((int)blockIdx.x, (int)gridDim.x), c, tw, wk, src, u, v, w, dst,
+                                        nrows, dstride);
 }
 
 // x-pass, inverse c2r of ROWS rows (see k_c2r_x), pre-processing in place
@@ -642,8 +643,8 @@ __global__ __launch_bounds__(T) void c2r_x(StepConsts c, const typename cplx_of<
                                            double rinv_scale, int64_t nrows, int sstride)
 {
     __shared__ typename cplx_of<RT>::type lds[ROWS * (N2 + 1)];
-    c2r_body<N2, ROWS, T, RT>(lds, (int)blockIdx.x, c, tw, wk, src, dst, dst_is_field, inv_scale, rinv_scale, nrows,
-                              sstride);
+    c2r_body<N2, ROWS, T, RT>(lds, xcd_remap((int)blockIdx.x, (int)gridDim.x), c, tw, wk, src, dst, dst_is_field,
+                              inv_scale, rinv_scale, nrows, sstride);
 }
 
 // strided c2c pass over TX consecutive kx columns (see k_c2c_cols)
@@ -688,8 +689,11 @@ __global__ __launch_bounds__(T) void c2c_cols(const C *__restrict__ tw, C *__res
                                               double s3, int nscale)
 {
     __shared__ C lds[N * TX];
-    cols_body<N, TX, T, SIGN, C>(lds, (int)blockIdx.x, (int)blockIdx.y, tw, data, nkx, estride, lstride, s1, s2, s3,
-                                 nscale);
+    // the column groups of one row of the grid share cache lines (a group's
+    // rows are not line-aligned): keep them on one XCD's L2
+    const int gx = (int)gridDim.x;
+    const int r = xcd_remap((int)blockIdx.x + gx * (int)blockIdx.y, gx * (int)gridDim.y);
+    cols_body<N, TX, T, SIGN, C>(lds, r % gx, r / gx, tw, data, nkx, estride, lstride, s1, s2, s3, nscale);
 }
 
 // periodic z solve (see k_z_periodic_solve); the scaling and the division by
@@ -749,7 +753,9 @@ __global__ __launch_bounds__(T) void z_periodic_solve(const C *__restrict__ tw, 
                                                       double dx, double dy, double dz)
 {
     __shared__ C lds[N * TX];
-    zsolve_body<N, TX, T, C>(lds, (int)blockIdx.x, (int)blockIdx.y, tw, data, ksqr, nkx, nkxs, ny, dx, dy, dz);
+    const int gx = (int)gridDim.x;
+    const int r = xcd_remap((int)blockIdx.x + gx * (int)blockIdx.y, gx * (int)gridDim.y);
+    zsolve_body<N, TX, T, C>(lds, r % gx, r / gx, tw, data, ksqr, nkx, nkxs, ny, dx, dy, dz);
 }
 
 // launch configurations: x-pass rows so that ROWS*N2 = 1024 values, y/z

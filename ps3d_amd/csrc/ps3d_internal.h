@@ -126,6 +126,15 @@ __device__ __forceinline__ float sdiv<float>(float a, double, double rb) { retur
 // write, so no separate exchange pass is needed.  Requires nx, ny, nz >= 4
 // (halo_images_ok): a point then has at most one image per axis (two in z
 // under Neumann), and only lanes next to a face take the stores.
+// Blocks are dealt round-robin to the 8 XCDs; renumbering them so that each
+// XCD walks a contiguous run of work items keeps neighbouring tiles, which
+// share apron rows, in one L2.
+__device__ __forceinline__ int xcd_remap(int bid, int n)
+{
+    const int per = n / 8;
+    return bid < per * 8 ? (bid % 8) * per + bid / 8 : bid;
+}
+
 inline bool halo_images_ok(int nx, int ny, int nz) { return nx >= 4 && ny >= 4 && nz >= 4; }
 
 template <class T>

@@ -31,14 +31,6 @@ __device__ __forceinline__ int wrap(int i, int n)
     return i < 1 ? i + n : (i > n ? i - n : i);
 }
 
-// Blocks are dealt round-robin to the 8 XCDs; renumbering them so that each
-// XCD walks a contiguous run of work items keeps neighbouring tiles, which
-// share apron rows, in one L2.
-__device__ __forceinline__ int xcd_remap(int bid, int n)
-{
-    const int per = n / 8;
-    return bid < per * 8 ? (bid % 8) * per + bid / 8 : bid;
-}
 
 // ---------------------------------------------------------------------------
 // tendencies: advection.f90:2-184 + integrate.f90:21-54 (interior points)
@@ -804,10 +796,14 @@ __global__ __launch_bounds__(BX * BY) void k_project(StepConsts c, double *__res
                                                       double *__restrict__ v, double *__restrict__ w,
                                                       double *__restrict__ p, int halo)
 {
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    // blocks renumbered per XCD: the rows of p a block reads past its own
+    // tile belong to the next blocks, which then share its L2
+    const int gx = (int)gridDim.x, gy = (int)gridDim.y;
+    const int r = xcd_remap((int)blockIdx.x + gx * ((int)blockIdx.y + gy * (int)blockIdx.z), gx * gy * (int)gridDim.z);
+    const int i = (r % gx) * BX + threadIdx.x + 1;
+    const int j = ((r / gx) % gy) * BY + threadIdx.y + 1;
     if (i > c.nx || j > c.ny) return;
-    project_point(c, u, v, w, p, i, j, (int)blockIdx.z + 1, halo);
+    project_point(c, u, v, w, p, i, j, r / (gx * gy) + 1, halo);
 }
 
 // projection of the mixed form with the FP32 pressure increment dp: the
@@ -818,9 +814,11 @@ __global__ __launch_bounds__(BX * BY) void k_project_f(StepConsts c, float *__re
                                                         float *__restrict__ wh, float *__restrict__ wl,
                                                         float *__restrict__ dp, double *__restrict__ pint, int halo)
 {
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+    const int gx = (int)gridDim.x, gy = (int)gridDim.y;
+    const int r = xcd_remap((int)blockIdx.x + gx * ((int)blockIdx.y + gy * (int)blockIdx.z), gx * gy * (int)gridDim.z);
+    const int i = (r % gx) * BX + threadIdx.x + 1;
+    const int j = ((r / gx) % gy) * BY + threadIdx.y + 1;
+    const int k = r / (gx * gy) + 1;
     if (i > c.nx || j > c.ny) return;
     const int64_t o = idx(c, i, j, k);
     const double pc = dp[o];
