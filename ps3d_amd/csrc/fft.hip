@@ -879,7 +879,7 @@ extern __shared__ double2 ens_lds[];
 __global__ __launch_bounds__(TTHREADS) void k_ens_r2c(const EnsMember *__restrict__ mem, EnsMap map,
                                                        const EnsState *__restrict__ st)
 {
-    const int g = (int)blockIdx.x;
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
     const int mi = ens_find(map, g);
     const EnsMember &e = mem[mi];
     const StepConsts c = e.c;
@@ -900,7 +900,7 @@ __global__ __launch_bounds__(TTHREADS) void k_ens_r2c(const EnsMember *__restric
 
 __global__ __launch_bounds__(TTHREADS) void k_ens_c2r(const EnsMember *__restrict__ mem, EnsMap map)
 {
-    const int g = (int)blockIdx.x;
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
     const int mi = ens_find(map, g);
     const EnsMember &e = mem[mi];
     const StepConsts c = e.c;
@@ -926,7 +926,7 @@ __global__ __launch_bounds__(TTHREADS) void k_ens_c2r(const EnsMember *__restric
 template <int SIGN, int NMAX>
 __global__ __launch_bounds__(TTHREADS) void k_ens_cols_y(const EnsMember *__restrict__ mem, EnsMap map)
 {
-    const int g = (int)blockIdx.x;
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
     const int mi = ens_find(map, g);
     const EnsMember &e = mem[mi];
     const int lb = g - map.start[mi];
@@ -950,7 +950,7 @@ __global__ __launch_bounds__(TTHREADS) void k_ens_cols_y(const EnsMember *__rest
 template <int NMAX>
 __global__ __launch_bounds__(TTHREADS) void k_ens_zsolve(const EnsMember *__restrict__ mem, EnsMap map)
 {
-    const int g = (int)blockIdx.x;
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
     const int mi = ens_find(map, g);
     const EnsMember &e = mem[mi];
     const int lb = g - map.start[mi];

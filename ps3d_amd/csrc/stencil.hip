@@ -947,7 +947,7 @@ __global__ __launch_bounds__(FTHREADS) void k_ens_fric(const EnsMember *__restri
 __global__ __launch_bounds__(BX * BY) void k_ens_project(const EnsMember *__restrict__ mem, EnsMap map,
                                                           const EnsState *__restrict__ st)
 {
-    const int g = (int)blockIdx.x;
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
     const int mi = ens_find(map, g);
     const EnsMember &e = mem[mi];
     const StepConsts c = e.c;

@@ -140,7 +140,7 @@ __global__ __launch_bounds__(64) void k_thomas(double2 *__restrict__ fs, const d
 {
     extern __shared__ double2 thomas_lds[];   // cpw x nz when IN_LDS
     if ((int)threadIdx.x >= cpw) return;
-    const int64_t col = (int64_t)blockIdx.x * cpw + threadIdx.x;
+    const int64_t col = (int64_t)xcd_remap((int)blockIdx.x, (int)gridDim.x) * cpw + threadIdx.x;
     if (col >= ncols) return;
     thomas_col<IN_LDS>(fs, ksqr, cp, rm, ncols, nz, fxa, dz2, dsqr, col, thomas_lds + threadIdx.x, cpw);
 }
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(64) void k_thomas(double2 *__restrict__ fs, const d
 // the Thomas stage of a batched ensemble (rigid-lid members)
 __global__ __launch_bounds__(64) void k_ens_thomas(const EnsMember *__restrict__ mem, EnsMap map)
 {
-    const int g = (int)blockIdx.x;
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
     const int mi = ens_find(map, g);
     const EnsMember &e = mem[mi];
     const int64_t ncols = (int64_t)e.nkx * e.c.ny;
