@@ -145,6 +145,89 @@ __global__ __launch_bounds__(64) void k_thomas(double2 *__restrict__ fs, const d
     thomas_col<IN_LDS>(fs, ksqr, cp, rm, ncols, nz, fxa, dz2, dsqr, col, thomas_lds + threadIdx.x, cpw);
 }
 
+// The sweeps with the elimination factors formed on the fly, by the same
+// operations as k_tri_factor (so the same bits), d'(k) and c'(k) kept in
+// LDS (dpl, cpl: stride ds): FS is read once and written once and no factor
+// table is read.  The factor recurrence (one division per level) is the
+// sweep's critical path; the FS loads are independent of it and go LOOK
+// levels ahead.
+__device__ __forceinline__ void thomas_col_fly(double2 *__restrict__ fs, const double *__restrict__ ksqr,
+                                               int64_t ncols, int nz, double fxa, double dz2, double dsqr,
+                                               int64_t col, double2 *dpl, double *cpl, int ds)
+{
+    double2 *d = fs + col;
+    const double ks = ksqr[col];
+    if (ks > 0) {
+        constexpr int LOOK = 16;
+        const double b1 = -ks - fxa;
+        double cprev = fxa / b1;
+        cpl[0] = cprev;
+        double2 dp = d[0];
+        dp = make_double2(dp.x / b1, dp.y / b1);
+        dpl[0] = dp;
+        for (int k0 = 1; k0 < nz; k0 += LOOK) {
+            double2 dd[LOOK];
+#pragma unroll
+            for (int q = 0; q < LOOK; ++q)
+                if (k0 + q < nz) dd[q] = d[(int64_t)(k0 + q) * ncols];
+#pragma unroll
+            for (int q = 0; q < LOOK; ++q) {
+                if (k0 + q < nz) {
+                    const int k = k0 + q + 1;   // level, 1-based
+                    const double bk = (k == nz) ? -ks - fxa : -ks - 2 * fxa;
+                    const double mm = bk - cprev * fxa;
+                    const double r = 1.0 / mm;
+                    const double ck = (k == nz) ? 0.0 : fxa * r;
+                    cpl[(k0 + q) * ds] = ck;
+                    cprev = ck;
+                    dp = make_double2((dd[q].x - dp.x * fxa) * r, (dd[q].y - dp.y * fxa) * r);
+                    dpl[(k0 + q) * ds] = dp;
+                }
+            }
+        }
+        double2 x = dp;
+        d[(int64_t)(nz - 1) * ncols] = x;
+        for (int k = nz - 2; k >= 0; --k) {
+            const double2 e = dpl[k * ds];
+            const double c = cpl[k * ds];
+            x = make_double2(e.x - c * x.x, e.y - c * x.y);
+            d[(int64_t)k * ncols] = x;
+        }
+    } else {
+        // zero mode, as in thomas_col
+        double2 bk = d[0];
+        double2 pm1 = make_double2(0.0, 0.0);
+        d[0] = pm1;
+        double2 nxt = d[ncols];
+        double2 pk = make_double2(bk.x * dz2 * dsqr, bk.y * dz2 * dsqr);
+        d[ncols] = pk;
+        bk = nxt;
+        for (int k = 2; k < nz; ++k) {
+            const int64_t o = (int64_t)k * ncols;
+            nxt = d[o];
+            const double2 pn = make_double2(2 * pk.x - pm1.x + bk.x * dz2 * dsqr,
+                                            2 * pk.y - pm1.y + bk.y * dz2 * dsqr);
+            d[o] = pn;
+            pm1 = pk;
+            pk = pn;
+            bk = nxt;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_thomas_fly(double2 *__restrict__ fs, const double *__restrict__ ksqr,
+                                                   int64_t ncols, int nz, double fxa, double dz2, double dsqr,
+                                                   int cpw)
+{
+    extern __shared__ double thomas_fly_lds[];   // cpw x nz double2 (d'), then cpw x nz double (c')
+    if ((int)threadIdx.x >= cpw) return;
+    const int64_t col = (int64_t)xcd_remap((int)blockIdx.x, (int)gridDim.x) * cpw + threadIdx.x;
+    if (col >= ncols) return;
+    double2 *dpl = reinterpret_cast<double2 *>(thomas_fly_lds) + threadIdx.x;
+    double *cpl = thomas_fly_lds + 2 * (int64_t)cpw * nz + threadIdx.x;
+    thomas_col_fly(fs, ksqr, ncols, nz, fxa, dz2, dsqr, col, dpl, cpl, cpw);
+}
+
 // the Thomas stage of a batched ensemble (rigid-lid members)
 __global__ __launch_bounds__(64) void k_ens_thomas(const EnsMember *__restrict__ mem, EnsMap map)
 {
@@ -268,9 +351,13 @@ void launch_thomas(Model &m, double2 *data)
     const double fxa = 1.0 / (m.prm.dz * m.prm.dz) / m.prm.dsqr;
     const int cpw = (m.thomas_cpw == 16 || m.thomas_cpw == 32) ? m.thomas_cpw : 64;
     const size_t lds = sizeof(double2) * (size_t)cpw * m.g.nz;
+    const size_t lds_fly = (sizeof(double2) + sizeof(double)) * (size_t)cpw * m.g.nz;
     const dim3 grid((unsigned)((ncols + cpw - 1) / cpw));
     ProfScope ps(m, "thomas");
-    if (lds <= 64 * 1024)
+    if (lds_fly <= 64 * 1024)
+        hipLaunchKernelGGL(k_thomas_fly, grid, dim3(64), lds_fly, m.stream, data, m.ksqr, ncols, m.g.nz, fxa,
+                           m.prm.dz * m.prm.dz, m.prm.dsqr, cpw);
+    else if (lds <= 64 * 1024)
         hipLaunchKernelGGL(k_thomas<true>, grid, dim3(64), lds, m.stream, data, m.ksqr, m.tri_cp, m.tri_rm, ncols,
                            m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr, cpw);
     else
