@@ -40,6 +40,7 @@ struct ps3d_ensemble {
     EnsMap maps[S_NSTAGE]{};
     size_t lds_x = 0, lds_y = 0, lds_z = 0;
     int ny_max = 0, nz_max = 0;         // longest y and (periodic) z columns
+    int nzl_max = 0;                    // most levels of a rigid-lid member (Thomas LDS)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::map<std::string, int> tuned;   // "stage shape" -> levels per work-group
@@ -138,7 +139,7 @@ void build_maps(ps3d_ensemble &E)
             case S_FRIC: nb = (int64_t)e.fric_tx * e.fric_ty * e.fric_nchunk * 4; break;
             case S_R2C: nb = ens_r2c_blocks(e); break;
             case S_COLS: nb = ens_cols_blocks(e, g.ny, g.nz); break;
-            case S_THOMAS: nb = lid ? cdiv((int64_t)g.nkx * g.ny, 64) : 0; break;
+            case S_THOMAS: nb = lid ? cdiv((int64_t)g.nkx * g.ny, ens_thomas_cols()) : 0; break;
             case S_ZSOLVE: nb = lid ? 0 : ens_cols_blocks(e, g.nz, g.ny); break;
             case S_PROJ: nb = (int64_t)cdiv(g.nx, 64) * cdiv(g.ny, 4) * g.nz; break;
             }
@@ -149,9 +150,10 @@ void build_maps(ps3d_ensemble &E)
         E.maps[s].start[n] = (int)acc;
     }
     E.lds_x = E.lds_y = E.lds_z = 0;
-    E.ny_max = E.nz_max = 0;
+    E.ny_max = E.nz_max = E.nzl_max = 0;
     for (int i = 0; i < n; ++i) {
         const Grid &g = E.mem[i]->g;
+        if (E.tab[i].c.lid) E.nzl_max = std::max(E.nzl_max, g.nz);
         E.lds_x = std::max(E.lds_x, ens_x_lds(g.nx / 2));
         E.lds_y = std::max(E.lds_y, ens_cols_lds(g.ny));
         E.ny_max = std::max(E.ny_max, g.ny);
@@ -204,7 +206,7 @@ void enqueue_step(ps3d_ensemble &E)
     ens_friction_ab(s, E.dev_tab, E.maps[S_FRIC], E.dev_state);   // writes the halo images too
     ens_r2c(s, E.dev_tab, E.maps[S_R2C], E.dev_state, E.lds_x);
     ens_cols_y(s, E.dev_tab, E.maps[S_COLS], -1, E.ny_max, E.lds_y);
-    ens_thomas(s, E.dev_tab, E.maps[S_THOMAS]);
+    ens_thomas(s, E.dev_tab, E.maps[S_THOMAS], E.nzl_max);
     ens_zsolve(s, E.dev_tab, E.maps[S_ZSOLVE], E.nz_max, E.lds_z);
     ens_cols_y(s, E.dev_tab, E.maps[S_COLS], +1, E.ny_max, E.lds_y);
     ens_c2r(s, E.dev_tab, E.maps[S_R2C], E.lds_x);

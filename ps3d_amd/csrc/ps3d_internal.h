@@ -401,7 +401,10 @@ void ens_r2c(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsSt
 void ens_cols_y(hipStream_t s, const EnsMember *mem, const EnsMap &map, int sign, int nmax, size_t lds);
 void ens_zsolve(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nmax, size_t lds);
 void ens_c2r(hipStream_t s, const EnsMember *mem, const EnsMap &map, size_t lds);
-void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map);
+// Thomas stage: ens_thomas_cols() columns per work-group; nz_max: most levels
+// of a rigid-lid member (sizes the LDS of the on-the-fly form)
+int ens_thomas_cols();
+void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max);
 // block counts and LDS bytes of the batched FFT stages for one member
 int ens_r2c_blocks(const EnsMember &e);
 int ens_cols_blocks(const EnsMember &e, int n_len, int lines);

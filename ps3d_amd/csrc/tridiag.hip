@@ -228,16 +228,29 @@ __global__ __launch_bounds__(64) void k_thomas_fly(double2 *__restrict__ fs, con
     thomas_col_fly(fs, ksqr, ncols, nz, fxa, dz2, dsqr, col, dpl, cpl, cpw);
 }
 
-// the Thomas stage of a batched ensemble (rigid-lid members)
-__global__ __launch_bounds__(64) void k_ens_thomas(const EnsMember *__restrict__ mem, EnsMap map)
+// the Thomas stage of a batched ensemble (rigid-lid members), ENS_TC columns
+// per 64-lane work-group; FLY: the on-the-fly form with LDS for ENS_TC
+// columns of nz_max levels, else the factor tables
+constexpr int ENS_TC = 32;
+
+template <bool FLY>
+__global__ __launch_bounds__(64) void k_ens_thomas(const EnsMember *__restrict__ mem, EnsMap map, int nz_max)
 {
+    extern __shared__ double ens_thomas_lds[];   // FLY: ENS_TC x nz_max double2, then ENS_TC x nz_max double
     const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
     const int mi = ens_find(map, g);
     const EnsMember &e = mem[mi];
+    if ((int)threadIdx.x >= ENS_TC) return;
     const int64_t ncols = (int64_t)e.nkx * e.c.ny;
-    const int64_t col = (int64_t)(g - map.start[mi]) * 64 + threadIdx.x;
+    const int64_t col = (int64_t)(g - map.start[mi]) * ENS_TC + threadIdx.x;
     if (col >= ncols) return;
-    thomas_col<false>(e.FS, e.ksqr, e.cp, e.rm, ncols, e.c.nz, e.fxa, e.c.dz * e.c.dz, e.c.dsqr, col);
+    if constexpr (FLY) {
+        double2 *dpl = reinterpret_cast<double2 *>(ens_thomas_lds) + threadIdx.x;
+        double *cpl = ens_thomas_lds + 2 * (int64_t)ENS_TC * nz_max + threadIdx.x;
+        thomas_col_fly(e.FS, e.ksqr, ncols, e.c.nz, e.fxa, e.c.dz * e.c.dz, e.c.dsqr, col, dpl, cpl, ENS_TC);
+    } else {
+        thomas_col<false>(e.FS, e.ksqr, e.cp, e.rm, ncols, e.c.nz, e.fxa, e.c.dz * e.c.dz, e.c.dsqr, col);
+    }
 }
 
 // periodic spectral step without the z transforms (building-block API)
@@ -391,10 +404,16 @@ void launch_thomas_f(Model &m, float2 *data)
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
-void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map)
+int ens_thomas_cols() { return ENS_TC; }
+
+void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max)
 {
     if (!map.start[map.n]) return;
-    hipLaunchKernelGGL(k_ens_thomas, dim3((unsigned)map.start[map.n]), dim3(64), 0, s, mem, map);
+    const size_t lds = (sizeof(double2) + sizeof(double)) * ENS_TC * (size_t)nz_max;
+    if (lds <= 64 * 1024)
+        hipLaunchKernelGGL(k_ens_thomas<true>, dim3((unsigned)map.start[map.n]), dim3(64), lds, s, mem, map, nz_max);
+    else
+        hipLaunchKernelGGL(k_ens_thomas<false>, dim3((unsigned)map.start[map.n]), dim3(64), 0, s, mem, map, nz_max);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
