@@ -405,7 +405,10 @@ int ps3d_create(const ps3d_params *prm, const double *ksqr, void *stream, ps3d_m
                 for (int q = 0; q < 3; ++q) m->dpr[q] = dalloc<float>(*m, g.H);
                 m->FSf = dalloc<float2>(*m, g.S);
                 m->stage = dalloc<double>(*m, 3 * g.H);
-                if (p.enable_vertical_boundaries && p.nz > 64) m->dp64 = dalloc<double2>(*m, g.S);
+                // Thomas scratch for columns whose sweep does not fit the LDS
+                // of the on-the-fly form (32 columns x nz x 24 B > 64 KiB)
+                if (p.enable_vertical_boundaries && 32 * 24 * (int64_t)p.nz > 64 * 1024)
+                    m->dp64 = dalloc<double2>(*m, g.S);
             } else {
                 m->u = dalloc<double>(*m, g.H);
                 m->v = dalloc<double>(*m, g.H);

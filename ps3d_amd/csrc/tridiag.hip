@@ -150,23 +150,31 @@ __global__ __launch_bounds__(64) void k_thomas(double2 *__restrict__ fs, const d
 // LDS (dpl, cpl: stride ds): FS is read once and written once and no factor
 // table is read.  The factor recurrence (one division per level) is the
 // sweep's critical path; the FS loads are independent of it and go LOOK
-// levels ahead.
-__device__ __forceinline__ void thomas_col_fly(double2 *__restrict__ fs, const double *__restrict__ ksqr,
-                                               int64_t ncols, int nz, double fxa, double dz2, double dsqr,
-                                               int64_t col, double2 *dpl, double *cpl, int ds)
+// levels ahead.  C = double2: the FP64 path; C = float2: the mixed form (FP32
+// right-hand side and solution, FP64 elimination).
+__device__ __forceinline__ double2 to_d2(double2 a) { return a; }
+__device__ __forceinline__ double2 to_d2(float2 a) { return make_double2((double)a.x, (double)a.y); }
+template <class C> __device__ __forceinline__ C from_d2(double2 a);
+template <> __device__ __forceinline__ double2 from_d2<double2>(double2 a) { return a; }
+template <> __device__ __forceinline__ float2 from_d2<float2>(double2 a) { return make_float2((float)a.x, (float)a.y); }
+
+template <class C>
+__device__ __forceinline__ void thomas_col_fly(C *__restrict__ fs, const double *__restrict__ ksqr, int64_t ncols,
+                                               int nz, double fxa, double dz2, double dsqr, int64_t col,
+                                               double2 *dpl, double *cpl, int ds)
 {
-    double2 *d = fs + col;
+    C *d = fs + col;
     const double ks = ksqr[col];
     if (ks > 0) {
         constexpr int LOOK = 16;
         const double b1 = -ks - fxa;
         double cprev = fxa / b1;
         cpl[0] = cprev;
-        double2 dp = d[0];
+        double2 dp = to_d2(d[0]);
         dp = make_double2(dp.x / b1, dp.y / b1);
         dpl[0] = dp;
         for (int k0 = 1; k0 < nz; k0 += LOOK) {
-            double2 dd[LOOK];
+            C dd[LOOK];
 #pragma unroll
             for (int q = 0; q < LOOK; ++q)
                 if (k0 + q < nz) dd[q] = d[(int64_t)(k0 + q) * ncols];
@@ -180,34 +188,35 @@ __device__ __forceinline__ void thomas_col_fly(double2 *__restrict__ fs, const d
                     const double ck = (k == nz) ? 0.0 : fxa * r;
                     cpl[(k0 + q) * ds] = ck;
                     cprev = ck;
-                    dp = make_double2((dd[q].x - dp.x * fxa) * r, (dd[q].y - dp.y * fxa) * r);
+                    const double2 e = to_d2(dd[q]);
+                    dp = make_double2((e.x - dp.x * fxa) * r, (e.y - dp.y * fxa) * r);
                     dpl[(k0 + q) * ds] = dp;
                 }
             }
         }
         double2 x = dp;
-        d[(int64_t)(nz - 1) * ncols] = x;
+        d[(int64_t)(nz - 1) * ncols] = from_d2<C>(x);
         for (int k = nz - 2; k >= 0; --k) {
             const double2 e = dpl[k * ds];
             const double c = cpl[k * ds];
             x = make_double2(e.x - c * x.x, e.y - c * x.y);
-            d[(int64_t)k * ncols] = x;
+            d[(int64_t)k * ncols] = from_d2<C>(x);
         }
     } else {
-        // zero mode, as in thomas_col
-        double2 bk = d[0];
+        // zero mode, pressure_solver.f90:44-50, as in thomas_col (FP64 sums)
+        double2 bk = to_d2(d[0]);
         double2 pm1 = make_double2(0.0, 0.0);
-        d[0] = pm1;
-        double2 nxt = d[ncols];
+        d[0] = from_d2<C>(pm1);
+        double2 nxt = to_d2(d[ncols]);
         double2 pk = make_double2(bk.x * dz2 * dsqr, bk.y * dz2 * dsqr);
-        d[ncols] = pk;
+        d[ncols] = from_d2<C>(pk);
         bk = nxt;
         for (int k = 2; k < nz; ++k) {
             const int64_t o = (int64_t)k * ncols;
-            nxt = d[o];
+            nxt = to_d2(d[o]);
             const double2 pn = make_double2(2 * pk.x - pm1.x + bk.x * dz2 * dsqr,
                                             2 * pk.y - pm1.y + bk.y * dz2 * dsqr);
-            d[o] = pn;
+            d[o] = from_d2<C>(pn);
             pm1 = pk;
             pk = pn;
             bk = nxt;
@@ -215,7 +224,8 @@ __device__ __forceinline__ void thomas_col_fly(double2 *__restrict__ fs, const d
     }
 }
 
-__global__ __launch_bounds__(64) void k_thomas_fly(double2 *__restrict__ fs, const double *__restrict__ ksqr,
+template <class C>
+__global__ __launch_bounds__(64) void k_thomas_fly(C *__restrict__ fs, const double *__restrict__ ksqr,
                                                    int64_t ncols, int nz, double fxa, double dz2, double dsqr,
                                                    int cpw)
 {
@@ -225,7 +235,7 @@ __global__ __launch_bounds__(64) void k_thomas_fly(double2 *__restrict__ fs, con
     if (col >= ncols) return;
     double2 *dpl = reinterpret_cast<double2 *>(thomas_fly_lds) + threadIdx.x;
     double *cpl = thomas_fly_lds + 2 * (int64_t)cpw * nz + threadIdx.x;
-    thomas_col_fly(fs, ksqr, ncols, nz, fxa, dz2, dsqr, col, dpl, cpl, cpw);
+    thomas_col_fly<C>(fs, ksqr, ncols, nz, fxa, dz2, dsqr, col, dpl, cpl, cpw);
 }
 
 // the Thomas stage of a batched ensemble (rigid-lid members), ENS_TC columns
@@ -247,7 +257,8 @@ __global__ __launch_bounds__(64) void k_ens_thomas(const EnsMember *__restrict__
     if constexpr (FLY) {
         double2 *dpl = reinterpret_cast<double2 *>(ens_thomas_lds) + threadIdx.x;
         double *cpl = ens_thomas_lds + 2 * (int64_t)ENS_TC * nz_max + threadIdx.x;
-        thomas_col_fly(e.FS, e.ksqr, ncols, e.c.nz, e.fxa, e.c.dz * e.c.dz, e.c.dsqr, col, dpl, cpl, ENS_TC);
+        thomas_col_fly<double2>(e.FS, e.ksqr, ncols, e.c.nz, e.fxa, e.c.dz * e.c.dz, e.c.dsqr, col, dpl, cpl,
+                                ENS_TC);
     } else {
         thomas_col<false>(e.FS, e.ksqr, e.cp, e.rm, ncols, e.c.nz, e.fxa, e.c.dz * e.c.dz, e.c.dsqr, col);
     }
@@ -266,22 +277,20 @@ __global__ __launch_bounds__(256) void k_divide_ksqr(double2 *__restrict__ fs, c
     fs[n] = x;
 }
 
-// Mixed form: the right-hand side and the solution are FP32, the
-// elimination runs in FP64 with the same factors, and the forward-sweep
-// values d'(k) stay in FP64 -- in LDS when a block's 64 columns fit
-// (nz <= 64: 64 KiB), else in the dp64 scratch array.
-template <bool IN_LDS>
+// Mixed form for columns too tall for the LDS of k_thomas_fly<float2>: the
+// right-hand side and the solution are FP32, the elimination runs in FP64
+// with the factor tables, and the forward-sweep values d'(k) stay in FP64 in
+// the dp64 scratch array.
 __global__ __launch_bounds__(64) void k_thomas_f(float2 *__restrict__ fs, double2 *__restrict__ dpg,
                                                  const double *__restrict__ ksqr, const double *__restrict__ cp,
                                                  const double *__restrict__ rm, int64_t ncols, int nz, double fxa,
                                                  double dz2, double dsqr)
 {
-    extern __shared__ double2 dps[];
     const int64_t col = (int64_t)blockIdx.x * 64 + threadIdx.x;
     if (col >= ncols) return;
     float2 *d = fs + col;
-    double2 *dp_ = IN_LDS ? dps + threadIdx.x : dpg + col;
-    const int64_t dstr = IN_LDS ? 64 : ncols;
+    double2 *dp_ = dpg + col;
+    const int64_t dstr = ncols;
     const double ks = ksqr[col];
     if (ks > 0) {
         constexpr int LOOK = 16;
@@ -368,8 +377,8 @@ void launch_thomas(Model &m, double2 *data)
     const dim3 grid((unsigned)((ncols + cpw - 1) / cpw));
     ProfScope ps(m, "thomas");
     if (lds_fly <= 64 * 1024)
-        hipLaunchKernelGGL(k_thomas_fly, grid, dim3(64), lds_fly, m.stream, data, m.ksqr, ncols, m.g.nz, fxa,
-                           m.prm.dz * m.prm.dz, m.prm.dsqr, cpw);
+        hipLaunchKernelGGL(k_thomas_fly<double2>, grid, dim3(64), lds_fly, m.stream, data, m.ksqr, ncols, m.g.nz,
+                           fxa, m.prm.dz * m.prm.dz, m.prm.dsqr, cpw);
     else if (lds <= 64 * 1024)
         hipLaunchKernelGGL(k_thomas<true>, grid, dim3(64), lds, m.stream, data, m.ksqr, m.tri_cp, m.tri_rm, ncols,
                            m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr, cpw);
@@ -392,14 +401,15 @@ void launch_thomas_f(Model &m, float2 *data)
     const double fxa = 1.0 / (m.prm.dz * m.prm.dz) / m.prm.dsqr;
     const unsigned nb = (unsigned)((ncols + 63) / 64);
     ProfScope ps(m, "thomas");
-    if (m.g.nz <= 64) {
-        const size_t lds = sizeof(double2) * 64 * (size_t)m.g.nz;
-        hipLaunchKernelGGL(k_thomas_f<true>, dim3(nb), dim3(64), lds, m.stream, data, nullptr, m.ksqr, m.tri_cp,
-                           m.tri_rm, ncols, m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr);
+    constexpr int cpw = 32;
+    const size_t lds_fly = (sizeof(double2) + sizeof(double)) * cpw * (size_t)m.g.nz;
+    if (lds_fly <= 64 * 1024) {
+        hipLaunchKernelGGL(k_thomas_fly<float2>, dim3((unsigned)((ncols + cpw - 1) / cpw)), dim3(64), lds_fly,
+                           m.stream, data, m.ksqr, ncols, m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr, cpw);
     } else {
         PS3D_REQUIRE(m.dp64, "Thomas scratch not allocated");
-        hipLaunchKernelGGL(k_thomas_f<false>, dim3(nb), dim3(64), 0, m.stream, data, m.dp64, m.ksqr, m.tri_cp,
-                           m.tri_rm, ncols, m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr);
+        hipLaunchKernelGGL(k_thomas_f, dim3(nb), dim3(64), 0, m.stream, data, m.dp64, m.ksqr, m.tri_cp, m.tri_rm,
+                           ncols, m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr);
     }
     PS3D_HIP_CHECK(hipGetLastError());
 }

@@ -71,7 +71,7 @@ void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
     levels.push_back(nz);
     const int t1 = next_tau % 3 + 1, t2 = t1 % 3 + 1;   // any two other slots
 
-    auto pick = [&](const std::string &stage, const std::vector<int> &cand, int &field, auto &&launch) {
+    auto pick = [&](const std::string &stage, const std::vector<int> &cand, int &field, int nrep, auto &&launch) {
         const std::string key = stage + " " + sh;
         auto hit = cache.find(key);
         if (hit != cache.end()) {
@@ -82,7 +82,7 @@ void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
         float best_ms = 1e30f;
         for (int v : cand) {
             field = v;
-            const float ms = time_it(m, reps, launch);
+            const float ms = time_it(m, nrep, launch);
             if (ms < best_ms) {
                 best_ms = ms;
                 best = v;
@@ -92,20 +92,21 @@ void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
         cache[key] = best;
     };
 
-    pick("tendency", levels, m.tend_kc, [&] {
+    pick("tendency", levels, m.tend_kc, reps, [&] {
         if (m.mixed) launch_tendency_mixed(m, next_tau, m.pint);
         else launch_tendency(m, next_tau);
     });
     if (m.mixed || (fused_friction_ok(m) && m.fuse_friction))
-        pick("friction_ab", levels, m.fric_kc, [&] { launch_friction_ab_raw(m, 3, next_tau, t1, t2); });
+        pick("friction_ab", levels, m.fric_kc, reps, [&] { launch_friction_ab_raw(m, 3, next_tau, t1, t2); });
     if (!m.mixed && m.prm.enable_vertical_boundaries) {
         double2 *scratch = nullptr;
         PS3D_HIP_CHECK(hipMalloc(&scratch, sizeof(double2) * (size_t)m.g.S));
         try {
             PS3D_HIP_CHECK(hipMemcpyAsync(scratch, m.FS, sizeof(double2) * (size_t)m.g.S, hipMemcpyDeviceToDevice,
                                           m.stream));
-            const std::vector<int> cpw = {64, 32, 16};
-            pick("thomas", cpw, m.thomas_cpw, [&] { launch_thomas(m, scratch); });
+            // a short kernel: more repetitions for a stable choice
+            const std::vector<int> cpw = {32, 16, 64};
+            pick("thomas", cpw, m.thomas_cpw, 4 * reps, [&] { launch_thomas(m, scratch); });
         } catch (...) {
             (void)hipFree(scratch);
             throw;
