@@ -565,8 +565,8 @@ __global__ __launch_bounds__(T) void r2c_x(StepConsts c, const typename cplx_of<
                                            typename cplx_of<RT>::type *__restrict__ dst, int64_t nrows, int dstride)
 {
     __shared__ typename cplx_of<RT>::type lds[ROWS * (N2 + 1)];
-    r2c_body<N2, ROWS, T, FUSE_DIV, RT>(lds, xcd_remap((int)blockIdx.x, (int)gridDim.x), c, tw, wk, src, u, v, w, dst,
-                                        nrows, dstride);
+    // (measured: the XCD renumbering of the other passes makes this one slower)
+    r2c_body<N2, ROWS, T, FUSE_DIV, RT>(lds, (int)blockIdx.x, c, tw, wk, src, u, v, w, dst, nrows, dstride);
 }
 
 // x-pass, inverse c2r of ROWS rows (see k_c2r_x), pre-processing in place
