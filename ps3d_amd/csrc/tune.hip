@@ -18,6 +18,8 @@ namespace ps3d {
 
 namespace {
 
+constexpr size_t kFlushBytes = size_t(512) << 20;   // > 8 x 4 MiB L2 + 256 MiB Infinity Cache
+
 std::string shape_of(const Model &m)
 {
     std::ostringstream o;
@@ -50,6 +52,39 @@ float time_it(Model &m, int reps, F &&launch)
     return ms;
 }
 
+// Time of reps launches, each after the caches were flushed by writing a
+// buffer larger than the L2s and the Infinity Cache together: for kernels
+// that read static tables, which back-to-back launches would find cached
+// while the step's other kernels evict them.
+template <class F>
+float time_it_cold(Model &m, int reps, void *flush, size_t flush_bytes, F &&launch)
+{
+    hipEvent_t a, b;
+    PS3D_HIP_CHECK(hipEventCreate(&a));
+    PS3D_HIP_CHECK(hipEventCreate(&b));
+    float total = 0.f;
+    try {
+        launch();   // warm-up
+        for (int r = 0; r < reps; ++r) {
+            PS3D_HIP_CHECK(hipMemsetAsync(flush, r & 0xff, flush_bytes, m.stream));
+            PS3D_HIP_CHECK(hipEventRecord(a, m.stream));
+            launch();
+            PS3D_HIP_CHECK(hipEventRecord(b, m.stream));
+            PS3D_HIP_CHECK(hipEventSynchronize(b));
+            float ms = 0.f;
+            PS3D_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+            total += ms;
+        }
+    } catch (...) {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        throw;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return total;
+}
+
 }  // namespace
 
 void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
@@ -71,7 +106,9 @@ void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
     levels.push_back(nz);
     const int t1 = next_tau % 3 + 1, t2 = t1 % 3 + 1;   // any two other slots
 
-    auto pick = [&](const std::string &stage, const std::vector<int> &cand, int &field, int nrep, auto &&launch) {
+    // flush: scratch for time_it_cold (null: back-to-back timing)
+    auto pick = [&](const std::string &stage, const std::vector<int> &cand, int &field, int nrep, void *flush,
+                    auto &&launch) {
         const std::string key = stage + " " + sh;
         auto hit = cache.find(key);
         if (hit != cache.end()) {
@@ -82,7 +119,7 @@ void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
         float best_ms = 1e30f;
         for (int v : cand) {
             field = v;
-            const float ms = time_it(m, nrep, launch);
+            const float ms = flush ? time_it_cold(m, nrep, flush, kFlushBytes, launch) : time_it(m, nrep, launch);
             if (ms < best_ms) {
                 best_ms = ms;
                 best = v;
@@ -92,27 +129,35 @@ void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
         cache[key] = best;
     };
 
-    pick("tendency", levels, m.tend_kc, reps, [&] {
+    pick("tendency", levels, m.tend_kc, reps, nullptr, [&] {
         if (m.mixed) launch_tendency_mixed(m, next_tau, m.pint);
         else launch_tendency(m, next_tau);
     });
     if (m.mixed || (fused_friction_ok(m) && m.fuse_friction))
-        pick("friction_ab", levels, m.fric_kc, reps, [&] { launch_friction_ab_raw(m, 3, next_tau, t1, t2); });
-    if (!m.mixed && m.prm.enable_vertical_boundaries) {
+        pick("friction_ab", levels, m.fric_kc, reps, nullptr,
+             [&] { launch_friction_ab_raw(m, 3, next_tau, t1, t2); });
+    if (!m.mixed && m.prm.enable_vertical_boundaries && !cache.count("thomas " + sh)) {
         double2 *scratch = nullptr;
+        void *flush = nullptr;
         PS3D_HIP_CHECK(hipMalloc(&scratch, sizeof(double2) * (size_t)m.g.S));
         try {
+            PS3D_HIP_CHECK(hipMalloc(&flush, kFlushBytes));
             PS3D_HIP_CHECK(hipMemcpyAsync(scratch, m.FS, sizeof(double2) * (size_t)m.g.S, hipMemcpyDeviceToDevice,
                                           m.stream));
-            // a short kernel: more repetitions for a stable choice
+            // the table form reads per-column factors that stay cached when
+            // launched back to back: timed with cold caches, as in the step
             const std::vector<int> cpw = {32, 16, 64};
-            pick("thomas", cpw, m.thomas_cpw, 4 * reps, [&] { launch_thomas(m, scratch); });
+            pick("thomas", cpw, m.thomas_cpw, 2 * reps, flush, [&] { launch_thomas(m, scratch); });
         } catch (...) {
             (void)hipFree(scratch);
+            if (flush) (void)hipFree(flush);
             throw;
         }
         PS3D_HIP_CHECK(hipStreamSynchronize(m.stream));
         PS3D_HIP_CHECK(hipFree(scratch));
+        PS3D_HIP_CHECK(hipFree(flush));
+    } else if (!m.mixed && m.prm.enable_vertical_boundaries) {
+        m.thomas_cpw = cache["thomas " + sh];
     }
     if (cache_path) {
         std::ofstream out(cache_path, std::ios::trunc);
