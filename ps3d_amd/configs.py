@@ -183,6 +183,48 @@ def jets(nx=256, ny=256, nz=64, seed=2, umax=0.1, noise=1e-3, friction=True, pre
     return prm, _finish(prm, {"u": u, "v": v, "w": w, "b": b})
 
 
+def jets_eddies(nx=1024, ny=1024, nz=128, seed=5, n_eddies=8, eddy_amp=0.02, radius_frac=1.0 / 20.0,
+                precision=PS3D_FP64):
+    """Jets and eddy turbulence at scale: the two balanced jets of jets()
+    plus n_eddies Gaussian buoyancy eddies at positions drawn from the seed,
+    each b' = eddy_amp N0^2 Lz exp(-r^2/R^2) exp(-depth/Lz) with R =
+    radius_frac Lx, the velocities in discrete thermal-wind balance (zero at
+    the bottom) as in eddy().  The eddies keep d(b')/dz well below N0^2 and
+    their speed below the jets'."""
+    prm, f = jets(nx, ny, nz, seed=seed, precision=precision)
+    nx_, ny_, nz_ = prm.nx, prm.ny, prm.nz
+    L, Lz, f0, N0 = prm.Lx, prm.Lz, prm.f0, prm.N0
+    dx, dy, dz = prm.dx, prm.dy, prm.dz
+    A = eddy_amp * N0 ** 2 * Lz
+    R = radius_frac * L
+    pos = 0.5 * (uniform(seed, "R", n_eddies, 2, 1, stream=3)[:, :, 0] + 1.0) * L   # (n_eddies, 2) in [0, L)
+    xi = (np.arange(1, nx_ + 1) - 0.5) * dx
+    yj = (np.arange(1, ny_ + 1) - 0.5) * dy
+    zk = (np.arange(1, nz_ + 1) - 0.5) * dz
+    E = np.exp(-(Lz - zk) / Lz)
+    E0 = math.exp(-1.0)
+
+    def per(d, P):   # periodic distance
+        return (d + 0.5 * P) % P - 0.5 * P
+
+    b = np.zeros((nx_, ny_))
+    gy = np.zeros((nx_, ny_))
+    gx = np.zeros((nx_, ny_))
+    for xc, yc in pos:
+        X, Y = np.meshgrid(per(xi - xc, L), per(yj - yc, L), indexing="ij")
+        b += np.exp(-(X ** 2 + Y ** 2) / R ** 2)
+        Xu, Yu = np.meshgrid(per(xi + 0.5 * dx - xc, L), per(yj - yc, L), indexing="ij")
+        gy += -2 * Yu / R ** 2 * np.exp(-(Xu ** 2 + Yu ** 2) / R ** 2)
+        Xv, Yv = np.meshgrid(per(xi - xc, L), per(yj + 0.5 * dy - yc, L), indexing="ij")
+        gx += -2 * Xv / R ** 2 * np.exp(-(Xv ** 2 + Yv ** 2) / R ** 2)
+    I = (slice(2, 2 + nx_), slice(2, 2 + ny_), slice(2, 2 + nz_))
+    inner = {n: np.array(f[n][I]) for n in ("u", "v", "w", "b")}
+    inner["b"] += A * b[:, :, None] * E[None, None, :]
+    inner["u"] += -(A / f0) * gy[:, :, None] * (Lz * (E - E0))[None, None, :]
+    inner["v"] += (A / f0) * gx[:, :, None] * (Lz * (E - E0))[None, None, :]
+    return prm, _finish(prm, inner)
+
+
 def random_state(nx, ny, nz, seed=7, lid=True, amp=0.1, **kw):
     """A generic test state: smooth large-scale fields plus noise, with every
     friction operator switched on (kw override any parameter)."""

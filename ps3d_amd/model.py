@@ -191,14 +191,16 @@ class Model:
         return buf[0] + 1j * buf[1]
 
     # ---- measurement ---------------------------------------------------------
-    def diag_balance(self, filter_width: int | None = None, kx=None, ky=None, kz=None) -> dict:
+    def diag_balance(self, filter_width: int | None = None, kx=None, ky=None, kz=None, outputs=None) -> dict:
         """Normal-mode balancing of the model's current state
         (diag_balance.f90:89-362): {'u0': .., 'v0': .., ..., 'b3': ..}, full
-        arrays with halo.  The model's own fields are left unchanged."""
+        arrays with halo (outputs: the names wanted, default all 16).  The
+        model's own fields are left unchanged."""
         shp = self.prm.shape
         names = [f"{c}{o}" for o in range(4) for c in ("u", "v", "w", "b")]
-        out = {n: np.zeros(shp, order="F") for n in names}
-        tab = (ctypes.POINTER(ctypes.c_double) * 16)(*[_dp(out[n]) for n in names])
+        want = set(names if outputs is None else outputs)
+        out = {n: np.zeros(shp, order="F") for n in names if n in want}
+        tab = (ctypes.POINTER(ctypes.c_double) * 16)(*[_dp(out[n]) if n in out else None for n in names])
         ks = []
         for k in (kx, ky, kz):
             if k is None:
