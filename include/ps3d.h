@@ -148,6 +148,20 @@ int ps3d_device_ptr(ps3d_model *m, int field, int slot, void **ptr);
 int ps3d_set_wavemaker(ps3d_model *m, double omega, const double *const re[4], const double *const im[4]);
 int ps3d_clear_wavemaker(ps3d_model *m);
 
+/* Advection by a background flow (u_back, 0, 0) and of a background
+   buoyancy b_back, lee_wave1.f90:239-314, on the device: each step adds
+   -Ro * (flux divergences) to du..db at the tau slot, before any other
+   forcing.  u_back, b_back: host arrays of a field's layout (with halo),
+   copied; b_back NULL: zero; u_back NULL: off.  In the mixed form the terms
+   are added before its fused friction + AB, i.e. from the pre-friction
+   state (the FP64 form, like the reference, uses the post-friction one). */
+int ps3d_set_background_flow(ps3d_model *m, const double *u_back, const double *b_back);
+
+/* A steady body force: f[q] (q = 0..3: du, dv, dw, db; host arrays of a
+   field's layout, copied) added to the tendency slot tau each step, after
+   the background advection.  NULL entries (or f = NULL): none. */
+int ps3d_set_body_force(ps3d_model *m, const double *const f[4]);
+
 /* ---- phases, for drivers whose set_forcing does work ------------------
    integrate.f90 in pieces: tendency (:12-54), friction (:59-78), then the
    driver's set_forcing on host mirrors, then ab_update (:87-102) with the

@@ -129,6 +129,9 @@ def load_oracle():
         lib.orc_set_wavemaker.argtypes = [vp, ctypes.c_double, ctypes.POINTER(_dp), ctypes.POINTER(_dp)]
         lib.orc_set_forcing.argtypes = [vp, vp, vp]
         lib.orc_set_dynamic.argtypes = [vp] + [ctypes.c_double] * 6
+        lib.orc_background_flow.argtypes = [vp, ctypes.c_int, _dp, _dp]
+        lib.orc_body_force.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_dp)]
+        lib.orc_set_background.argtypes = [vp, _dp, _dp, ctypes.POINTER(_dp)]
         lib.orc_run.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
         for n in ("orc_my_fft_2d", "orc_my_ifft_2d", "orc_my_fft_3d", "orc_my_ifft_3d"):
             getattr(lib, n).argtypes = [vp, _dp, _dp]
@@ -252,6 +255,26 @@ class OracleModel:
 
     def pressure_solver(self):
         self.lib.orc_pressure_solver(self.h)
+
+    def background_flow(self, u_back, b_back=None, tau=None):
+        """lee_wave1.f90:239-314 into the tau slot"""
+        ub = np.asfortranarray(u_back, dtype=np.float64)
+        bb = None if b_back is None else np.asfortranarray(b_back, dtype=np.float64)
+        self.lib.orc_background_flow(self.h, tau or self.tau, _as_dp(ub), None if bb is None else _as_dp(bb))
+
+    def set_background(self, u_back=None, b_back=None, force=None):
+        """background flow and body force ({'du': .., 'dw': ..}) as the set_forcing hook"""
+        keep = []
+        def arr(a):
+            if a is None:
+                return None
+            a = np.asfortranarray(a, dtype=np.float64)
+            keep.append(a)
+            return _as_dp(a)
+        force = force or {}
+        tab = (_dp * 4)(*[arr(force.get(n)) for n in ("du", "dv", "dw", "db")])
+        self._bg_keep = (keep, tab)
+        self.lib.orc_set_background(self.h, arr(u_back), arr(b_back), tab)
 
     def set_dynamic(self, dt, Ro, Ahbi, Avbi, Khbi, Kvbi):
         """dt, Ro and the biharmonic coefficients of the next steps"""

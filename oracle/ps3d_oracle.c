@@ -52,6 +52,8 @@ struct orc_model {
     void *forcing_user;
     double wm_omega;          /* orc_set_wavemaker: amplitudes owned by the caller */
     const double *wm_re[4], *wm_im[4];
+    const double *bg_u, *bg_b; /* orc_set_background: owned by the caller */
+    const double *bf[4];
 };
 
 #define ONX 2
@@ -618,6 +620,118 @@ void orc_wavemaker(orc_model *m, int itt, double omega, const double *const *re,
         if (!re[f]) continue;
         for (int64_t n = 0; n < m->H; ++n) q[f][n] = q[f][n] + dt * (cs * re[f][n] + sn * im[f][n]);
     }
+}
+
+/* Advection by the background flow (u_back, 0, 0) and of the background
+   buoyancy b_back, lee_wave1.f90:239-314, into the tau slot, with the
+   reference's loop ranges and flux scratch arrays (b_back NULL: zero). */
+void orc_background_flow(orc_model *m, int tau, const double *ub, const double *bb)
+{
+    const orc_params *p = &m->p;
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    const int64_t H = m->H, X = m->X, Y = m->Y;
+    const double Ro = p->Ro, dx = p->dx, dy = p->dy, dz = p->dz, dsqr = p->dsqr;
+    const double *u = m->f[ORC_U], *v = m->f[ORC_V], *w = m->f[ORC_W], *b = m->f[ORC_B];
+    double *du = m->f[ORC_DU] + (tau - 1) * H, *dv = m->f[ORC_DV] + (tau - 1) * H;
+    double *dw = m->f[ORC_DW] + (tau - 1) * H, *db = m->f[ORC_DB] + (tau - 1) * H;
+    double *fe = m->f[ORC_FLUX_EAST], *fn = m->f[ORC_FLUX_NORTH], *ft = m->f[ORC_FLUX_TOP];
+    const int lid = p->enable_vertical_boundaries;
+#define BBK(i, j, k) (bb ? A(bb, i, j, k) : 0.0)
+    /* u (:245-268) */
+    for (int i = 0; i <= nx; ++i)
+        for (int k = -1; k <= nz + 2; ++k)
+            for (int j = -1; j <= ny + 2; ++j)
+                A(fe, i, j, k) = 2 * 0.25 * (A(ub, i, j, k) + A(ub, i + 1, j, k)) * (A(u, i + 1, j, k) + A(u, i, j, k));
+    for (int j = 0; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i)
+            for (int k = -1; k <= nz + 2; ++k)
+                A(fn, i, j, k) = 0.25 * (A(ub, i, j, k) + A(ub, i, j + 1, k)) * (A(v, i + 1, j, k) + A(v, i, j, k));
+    for (int k = 0; k <= nz; ++k)
+        for (int i = 1; i <= nx; ++i)
+            for (int j = -1; j <= ny + 2; ++j)
+                A(ft, i, j, k) = 0.25 * (A(ub, i, j, k + 1) + A(ub, i, j, k)) * (A(w, i, j, k) + A(w, i + 1, j, k));
+    if (lid) {
+        for (int k = 1 - ONX; k <= 0; ++k) memset(&A(ft, -1, -1, k), 0, sizeof(double) * (size_t)(X * Y));
+        for (int k = nz; k <= nz + ONX; ++k) memset(&A(ft, -1, -1, k), 0, sizeof(double) * (size_t)(X * Y));
+    }
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(du, i, j, k) = A(du, i, j, k) - Ro * ((A(fe, i, j, k) - A(fe, i - 1, j, k)) / dx +
+                                                        (A(fn, i, j, k) - A(fn, i, j - 1, k)) / dy +
+                                                        (A(ft, i, j, k) - A(ft, i, j, k - 1)) / dz);
+    /* v (:270-277) */
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 0; i <= nx; ++i)
+            for (int k = -1; k <= nz + 2; ++k)
+                A(fe, i, j, k) = 0.25 * (A(v, i, j, k) + A(v, i + 1, j, k)) * (A(ub, i, j + 1, k) + A(ub, i, j, k));
+    for (int i = 1; i <= nx; ++i)
+        for (int k = -1; k <= nz + 2; ++k)
+            for (int j = -1; j <= ny + 2; ++j)
+                A(dv, i, j, k) = A(dv, i, j, k) - Ro * (A(fe, i, j, k) - A(fe, i - 1, j, k)) / dx;
+    /* w (:279-286) */
+    for (int k = 1; k <= nz; ++k)
+        for (int i = 0; i <= nx; ++i)
+            for (int j = -1; j <= ny + 2; ++j)
+                A(fe, i, j, k) = 0.25 * (A(w, i, j, k) + A(w, i + 1, j, k)) * (A(ub, i, j, k) + A(ub, i, j, k + 1));
+    for (int i = 1; i <= nx; ++i)
+        for (int k = -1; k <= nz + 2; ++k)
+            for (int j = -1; j <= ny + 2; ++j)
+                A(dw, i, j, k) = A(dw, i, j, k) - dsqr * Ro * (A(fe, i, j, k) - A(fe, i - 1, j, k)) / dx;
+    /* b (:288-311) */
+    for (int i = 0; i <= nx; ++i)
+        for (int k = -1; k <= nz + 2; ++k)
+            for (int j = -1; j <= ny + 2; ++j)
+                A(fe, i, j, k) = 0.5 * (A(b, i, j, k) + A(b, i + 1, j, k)) * A(ub, i, j, k) +
+                                 0.5 * (BBK(i, j, k) + BBK(i + 1, j, k)) * A(u, i, j, k);
+    for (int j = 0; j <= ny; ++j)
+        for (int k = -1; k <= nz + 2; ++k)
+            for (int i = -1; i <= nx + 2; ++i)
+                A(fn, i, j, k) = 0.5 * (BBK(i, j, k) + BBK(i, j + 1, k)) * A(v, i, j, k);
+    for (int k = 0; k <= nz; ++k)
+        for (int j = -1; j <= ny + 2; ++j)
+            for (int i = -1; i <= nx + 2; ++i)
+                A(ft, i, j, k) = 0.5 * (BBK(i, j, k) + BBK(i, j, k + 1)) * A(w, i, j, k);
+    if (lid) {
+        for (int k = 1 - ONX; k <= 0; ++k) memset(&A(ft, -1, -1, k), 0, sizeof(double) * (size_t)(X * Y));
+        for (int k = nz; k <= nz + ONX; ++k) memset(&A(ft, -1, -1, k), 0, sizeof(double) * (size_t)(X * Y));
+    }
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(db, i, j, k) = A(db, i, j, k) - Ro * ((A(fe, i, j, k) - A(fe, i - 1, j, k)) / dx +
+                                                        (A(fn, i, j, k) - A(fn, i, j - 1, k)) / dy +
+                                                        (A(ft, i, j, k) - A(ft, i, j, k - 1)) / dz);
+#undef BBK
+}
+
+/* a steady body force added to the tau slot (whole arrays); NULL: none */
+void orc_body_force(orc_model *m, int tau, const double *const *f)
+{
+    double *d[4] = {m->f[ORC_DU] + (tau - 1) * m->H, m->f[ORC_DV] + (tau - 1) * m->H,
+                    m->f[ORC_DW] + (tau - 1) * m->H, m->f[ORC_DB] + (tau - 1) * m->H};
+    for (int q = 0; q < 4; ++q) {
+        if (!f[q]) continue;
+        for (int64_t n = 0; n < m->H; ++n) d[q][n] = d[q][n] + f[q][n];
+    }
+}
+
+static void background_hook(orc_model *m, void *user, int itt, int tau)
+{
+    (void)user;
+    (void)itt;
+    if (m->bg_u) orc_background_flow(m, tau, m->bg_u, m->bg_b);
+    orc_body_force(m, tau, m->bf);
+}
+
+/* background flow (u_back NULL: none) and body force (entries NULL: none)
+   as the model's set_forcing */
+void orc_set_background(orc_model *m, const double *u_back, const double *b_back, const double *const *f)
+{
+    m->bg_u = u_back;
+    m->bg_b = b_back;
+    for (int q = 0; q < 4; ++q) m->bf[q] = f ? f[q] : NULL;
+    orc_set_forcing(m, background_hook, NULL);
 }
 
 static void wavemaker_hook(orc_model *m, void *user, int itt, int tau)

@@ -67,6 +67,12 @@ typedef void (*orc_forcing_fn)(orc_model *m, void *user, int itt, int tau);
 void orc_set_forcing(orc_model *m, orc_forcing_fn fn, void *user);
 /* for_src/config.f90:224-237 wavemaker, whole arrays; re/im: 4 entries */
 void orc_wavemaker(orc_model *m, int itt, double omega, const double *const *re, const double *const *im);
+/* lee_wave1.f90:239-314 background-flow advection into slot tau (b_back NULL: 0) */
+void orc_background_flow(orc_model *m, int tau, const double *u_back, const double *b_back);
+/* steady body force f[0..3] added to du..db at slot tau (NULL entries: none) */
+void orc_body_force(orc_model *m, int tau, const double *const *f);
+/* both as the model's hook (arrays stay the caller's) */
+void orc_set_background(orc_model *m, const double *u_back, const double *b_back, const double *const *f);
 /* orc_wavemaker as the model's hook (amplitude arrays stay the caller's) */
 void orc_set_wavemaker(orc_model *m, double omega, const double *const *re, const double *const *im);
 /* pressure_solver.f90:2-120 */

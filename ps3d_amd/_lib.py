@@ -67,6 +67,8 @@ PROTOTYPES = {
     "ps3d_device_ptr": (_c_int, [_vp, _c_int, _c_int, ctypes.POINTER(_vp)]),
     "ps3d_set_wavemaker": (_c_int, [_vp, _c_dbl, ctypes.POINTER(_dp), ctypes.POINTER(_dp)]),
     "ps3d_clear_wavemaker": (_c_int, [_vp]),
+    "ps3d_set_background_flow": (_c_int, [_vp, _dp, _dp]),
+    "ps3d_set_body_force": (_c_int, [_vp, ctypes.POINTER(_dp)]),
     "ps3d_write_restart": (_c_int, [_vp, ctypes.c_char_p, _c_int, _c_dbl, _c_int, _c_int, _c_int]),
     "ps3d_read_restart": (_c_int, [_vp, ctypes.c_char_p, _i32p, _dp, _i32p, _i32p, _i32p]),
     "ps3d_restart_write_host": (_c_int, [ctypes.c_char_p, _c_int, _c_int, _c_int, _c_int, _c_dbl, _c_int, _c_int,

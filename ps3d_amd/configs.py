@@ -225,6 +225,46 @@ def jets_eddies(nx=1024, ny=1024, nz=128, seed=5, n_eddies=8, eddy_amp=0.02, rad
     return prm, _finish(prm, inner)
 
 
+def lee_wave(nx=512, ny=128, nz=128, seed=3, U=0.1, N0=1e-3, noise=1e-4, precision=PS3D_FP64):
+    """Stratified flow U past a Gaussian bottom obstacle, the obstacle as a
+    body force: a new configuration written against the plug-in surface.
+    The background flow is advected as in for_config/lee_wave1.f90:239-314
+    (constant u_back = U, no background buoyancy).  The obstacle
+    h(x) = 0.05 H exp(-((x - Lx/4)/(Lx/20))^2) lifts the stratification by
+    w = U h'(x); its effect on the buoyancy, -N0^2 U h'(x) exp(-z/(0.05 H))
+    (z the height above the bottom), is the body force on db.  Rigid lid,
+    H = 2 km, Lx = 10 km (obstacle width 500 m against the lee wavelength
+    2 pi U/N0 = 630 m), square cells in x-y, f0 = 1e-4, dt at an advective
+    CFL of 0.2, harmonic and horizontal biharmonic friction sized to the
+    grid, noise * U(-1, 1) on u, v, w (seed).
+
+    Returns (prm, fields, forcing) with forcing = {'u_back': .., 'force':
+    {'db': ..}} for Model.set_background_flow / set_body_force."""
+    H = 2000.0
+    Lx = 10e3
+    dx = Lx / nx
+    Ly = dx * ny
+    dz = H / nz
+    f0 = 1e-4
+    dt = 0.2 * dx / U
+    kw = dict(Ah=1e-3 * dx ** 2 / dt, Av=1e-3 * dz ** 2 / dt, Kh=1e-3 * dx ** 2 / dt, Kv=1e-3 * dz ** 2 / dt,
+              Ahbi=dx ** 4 / (30 * 86400.0), Khbi=dx ** 4 / (30 * 86400.0))
+    prm = ModelParams(nx=nx, ny=ny, nz=nz, Lx=Lx, Ly=Ly, Lz=H, dx=dx, dy=dx, dz=dz, dt=dt, f0=f0, N0=N0,
+                      enable_vertical_boundaries=True, enable_AB_3_order=True, eps_ab=0.0, precision=precision,
+                      **kw)
+    inner = {n: noise * uniform(seed, n, nx, ny, nz) for n in ("u", "v", "w")}
+    inner["b"] = np.zeros((nx, ny, nz))
+    fields = _finish(prm, inner)
+    xi = (np.arange(-1, nx + 3) - 0.5) * dx                 # cell centres, halo included
+    zk = (np.arange(-1, nz + 3) - 0.5) * dz
+    h0, x0, sig = 0.05 * H, Lx / 4, Lx / 20
+    dh = -2 * (xi - x0) / sig ** 2 * h0 * np.exp(-((xi - x0) / sig) ** 2)
+    decay = np.exp(-np.clip(zk, 0.0, None) / (0.05 * H))
+    fb = (-N0 ** 2 * U) * dh[:, None, None] * np.ones(ny + 4)[None, :, None] * decay[None, None, :]
+    forcing = {"u_back": np.full(prm.shape, U, order="F"), "force": {"db": np.asfortranarray(fb)}}
+    return prm, fields, forcing
+
+
 def random_state(nx, ny, nz, seed=7, lid=True, amp=0.1, **kw):
     """A generic test state: smooth large-scale fields plus noise, with every
     friction operator switched on (kw override any parameter)."""

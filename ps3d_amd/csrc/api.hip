@@ -167,9 +167,15 @@ void solve_mixed(Model &m, const float *u, const float *v, const float *w, float
 // pressure, friction + AB into the second buffers, FP32 solve), held in
 // pint with zero increments, so that the step that follows is referenced to
 // p_est and its FP32 quantities are small from the first step on.
+bool background_on(const Model &m)
+{
+    return m.bg_on || m.bf_on[0] || m.bf_on[1] || m.bf_on[2] || m.bf_on[3];
+}
+
 void cold_start_mixed(Model &m, int itt, int tau, int taum1, int taum2)
 {
     launch_tendency_mixed(m, tau, nullptr);
+    if (background_on(m)) launch_background(m, tau);   // provisional step: pre-friction state is close enough
     launch_friction_ab_mixed(m, 0, tau, taum1, taum2, 0.0, nullptr, 0.0, nullptr, false);
     (void)itt;
     float *est = m.dpr[0];
@@ -211,8 +217,12 @@ void integrate_mixed(Model &m, int itt, int tau, int taum1, int taum2)
     }
     const float *d1 = m.dpr[m.dp_cur];
     const float *d2 = m.dpr[(m.dp_cur + 2) % 3];
-    launch_friction_ab_mixed(m, itt, tau, taum1, taum2, qc1, qc1 != 0.0 ? d1 : nullptr, qc2,
-                             qc2 != 0.0 ? d2 : nullptr, true);
+    if (background_on(m))   // set_forcing after the friction block, as integrate.f90:82
+        launch_friction_forcing_ab_mixed(m, itt, tau, taum1, taum2, qc1, qc1 != 0.0 ? d1 : nullptr, qc2,
+                                         qc2 != 0.0 ? d2 : nullptr);
+    else
+        launch_friction_ab_mixed(m, itt, tau, taum1, taum2, qc1, qc1 != 0.0 ? d1 : nullptr, qc2,
+                                 qc2 != 0.0 ? d2 : nullptr, true);
     const int nxt = (m.dp_cur + 1) % 3;
     solve_mixed(m, m.fh[0], m.fh[1], m.fh[2], m.dpr[nxt]);
     launch_project_exchange_f(m, m.dpr[nxt]);
@@ -251,9 +261,11 @@ void pressure(Model &m)
     launch_project_exchange(m);
 }
 
-// set_forcing (integrate.f90:82): the wavemaker, then the caller's hook
+// set_forcing (integrate.f90:82): background flow and body force, the
+// wavemaker, then the caller's hook
 void apply_forcing(Model &m, int itt, int tau)
 {
+    if (background_on(m)) launch_background(m, tau);
     if (m.wm_on) {
         const double t = m.prm.dt * itt;   // config.f90:232
         launch_wavemaker(m, std::cos(m.wm_omega * t), std::sin(m.wm_omega * t));
@@ -271,7 +283,7 @@ void integrate(Model &m, int itt, int tau, int taum1, int taum2)
         return;
     }
     launch_tendency(m, tau);
-    const bool forced = m.forcing || m.wm_on;
+    const bool forced = m.forcing || m.wm_on || background_on(m);
     if (fused_friction_ok(m) && m.fuse_friction && !forced) {
         launch_friction_ab(m, itt, tau, taum1, taum2);
     } else {
@@ -828,6 +840,43 @@ int ps3d_set_wavemaker(ps3d_model *m, double omega, const double *const re[4], c
         PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
         m->wm_omega = omega;
         m->wm_on = any;
+    });
+}
+
+int ps3d_set_background_flow(ps3d_model *m, const double *u_back, const double *b_back)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(u_back || !b_back, "background flow: b_back needs u_back");
+        const size_t bytes = sizeof(double) * (size_t)m->g.H;
+        if (u_back) {
+            if (!m->bg_u) m->bg_u = dalloc<double>(*m, m->g.H);
+            PS3D_HIP_CHECK(hipMemcpyAsync(m->bg_u, u_back, bytes, hipMemcpyHostToDevice, m->stream));
+        }
+        if (b_back) {
+            if (!m->bg_b) m->bg_b = dalloc<double>(*m, m->g.H);
+            PS3D_HIP_CHECK(hipMemcpyAsync(m->bg_b, b_back, bytes, hipMemcpyHostToDevice, m->stream));
+        }
+        PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
+        m->bg_on = u_back != nullptr;
+        m->bg_b_on = b_back != nullptr;
+    });
+}
+
+int ps3d_set_body_force(ps3d_model *m, const double *const f[4])
+{
+    return guard([&] {
+        check_model(m);
+        const size_t bytes = sizeof(double) * (size_t)m->g.H;
+        for (int q = 0; q < 4; ++q) {
+            const double *src = f ? f[q] : nullptr;
+            if (src) {
+                if (!m->bf[q]) m->bf[q] = dalloc<double>(*m, m->g.H);
+                PS3D_HIP_CHECK(hipMemcpyAsync(m->bf[q], src, bytes, hipMemcpyHostToDevice, m->stream));
+            }
+            m->bf_on[q] = src != nullptr;
+        }
+        PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
     });
 }
 

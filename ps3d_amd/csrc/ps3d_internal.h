@@ -252,6 +252,11 @@ struct Model {
     bool wm_field[4] = {};                    // fields the wavemaker forces
     double wm_omega = 0.0;
     bool wm_on = false;
+    // background flow (u_back, b_back) and body force (per field)
+    double *bg_u = nullptr, *bg_b = nullptr;
+    bool bg_on = false, bg_b_on = false;
+    double *bf[4] = {};
+    bool bf_on[4] = {};
 
     // work arrays of the balancing diagnostic (balance.hip), made on first use
     struct BalanceWork *bal = nullptr;
@@ -270,6 +275,13 @@ void launch_mixing(Model &m, int op, int field, double hmix, double vmix, int ze
 void launch_ab_exchange(Model &m, int itt, int tau, int taum1, int taum2);
 // the wavemaker on the current buffers: q += dt*(cs*re + sn*im), interior
 void launch_wavemaker(Model &m, double cs, double sn);
+// background-flow advection and body force into the tendency slot tau (both
+// forms; mixed_fields: the FP32 hi fields to advect, default the canonical ones)
+void launch_background(Model &m, int tau, float *const *mixed_fields = nullptr);
+// mixed step with background forcing: friction, forcing on the post-friction
+// state, then Adams-Bashforth (the fused kernel's terms in three passes)
+void launch_friction_forcing_ab_mixed(Model &m, int itt, int tau, int taum1, int taum2, double qc1, const float *d1,
+                                      double qc2, const float *d2);
 bool fused_friction_ok(const Model &m);
 void launch_friction_ab(Model &m, int itt, int tau, int taum1, int taum2);
 void launch_border_exchg(Model &m, double *a, int neumann_z);

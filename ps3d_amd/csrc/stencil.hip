@@ -445,6 +445,128 @@ __global__ __launch_bounds__(BX * BY) void k_wavemaker(StepConsts c, WaveArgs a,
     a.f[f][o] = a.f[f][o] + c.dt * (cs * a.re[f][o] + sn * a.im[f][o]);
 }
 
+// Advection by a background flow (u_back, 0, 0) and of a background
+// buoyancy b_back, lee_wave1.f90:239-314, and a steady body force, added to
+// the tendency slot tau at interior points.  The fluxes are the reference's
+// flux_east/north/top values, formed per face where needed (the lid zeroes
+// flux_top at k <= 0 and k >= nz); u, v, w, b are the post-friction fields,
+// whose halos still hold the pre-friction values, as in the reference.
+// The divisions and their association follow the reference's statements:
+// du and db as -Ro*(dfe/dx + dfn/dy + dft/dz), dv as -(Ro*dfe)/dx, dw as
+// -(dsqr*Ro*dfe)/dx.  T = float: the mixed form's version on the FP32 hi
+// parts (applied before its fused friction + AB, see the step).
+template <class T>
+struct BgArgs {
+    const T *u, *v, *w, *b;         // prognostic fields (post-friction, FP64 form)
+    T *du, *dv, *dw, *db;           // tendency slot tau
+    const double *ub, *bb;          // u_back, b_back (null: no background advection)
+    const double *f[4];             // body force per field (null: none)
+};
+
+template <class T>
+__global__ __launch_bounds__(BX * BY) void k_background(StepConsts c, BgArgs<T> a)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    if (i > c.nx || j > c.ny) return;
+    const int k = (int)blockIdx.z + 1;
+    const int64_t X = c.X, XY = c.XY;
+    const int64_t o = idx(c, i, j, k);
+    const double Ro = c.Ro;
+    double du = a.du[o], dv = a.dv[o], dw = a.dw[o], db = a.db[o];
+    if (a.ub) {
+        const double *UB = a.ub, *BB = a.bb;
+        auto U = [&](int64_t q) { return (double)a.u[q]; };
+        auto V = [&](int64_t q) { return (double)a.v[q]; };
+        auto W = [&](int64_t q) { return (double)a.w[q]; };
+        auto B = [&](int64_t q) { return (double)a.b[q]; };
+        const bool lid = c.lid != 0;
+        // u (:245-268); flux_east at face i (q = point i)
+        auto fe_u = [&](int64_t q) { return 0.5 * (UB[q] + UB[q + 1]) * (U(q + 1) + U(q)); };
+        auto fn_u = [&](int64_t q) { return 0.25 * (UB[q] + UB[q + X]) * (V(q + 1) + V(q)); };
+        auto ft_u = [&](int64_t q, int kk) {
+            return (lid && (kk <= 0 || kk >= c.nz)) ? 0.0 : 0.25 * (UB[q + XY] + UB[q]) * (W(q) + W(q + 1));
+        };
+        du = du - Ro * (div_rn(fe_u(o) - fe_u(o - 1), c.dx, c.rdx) + div_rn(fn_u(o) - fn_u(o - X), c.dy, c.rdy) +
+                        div_rn(ft_u(o, k) - ft_u(o - XY, k - 1), c.dz, c.rdz));
+        // v (:270-277)
+        auto fe_v = [&](int64_t q) { return 0.25 * (V(q) + V(q + 1)) * (UB[q + X] + UB[q]); };
+        dv = dv - div_rn(Ro * (fe_v(o) - fe_v(o - 1)), c.dx, c.rdx);
+        // w (:279-286)
+        auto fe_w = [&](int64_t q) { return 0.25 * (W(q) + W(q + 1)) * (UB[q] + UB[q + XY]); };
+        dw = dw - div_rn(c.dsqr * Ro * (fe_w(o) - fe_w(o - 1)), c.dx, c.rdx);
+        // b (:288-311)
+        // (b_back = 0 when BB is null)
+        auto fe_b = [&](int64_t q) {
+            const double t = 0.5 * (B(q) + B(q + 1)) * UB[q];
+            return BB ? t + 0.5 * (BB[q] + BB[q + 1]) * U(q) : t;
+        };
+        auto fn_b = [&](int64_t q) { return BB ? 0.5 * (BB[q] + BB[q + X]) * V(q) : 0.0; };
+        auto ft_b = [&](int64_t q, int kk) {
+            if (lid && (kk <= 0 || kk >= c.nz)) return 0.0;
+            return BB ? 0.5 * (BB[q] + BB[q + XY]) * W(q) : 0.0;
+        };
+        db = db - Ro * (div_rn(fe_b(o) - fe_b(o - 1), c.dx, c.rdx) + div_rn(fn_b(o) - fn_b(o - X), c.dy, c.rdy) +
+                        div_rn(ft_b(o, k) - ft_b(o - XY, k - 1), c.dz, c.rdz));
+    }
+    if (a.f[0]) du = du + a.f[0][o];
+    if (a.f[1]) dv = dv + a.f[1][o];
+    if (a.f[2]) dw = dw + a.f[2][o];
+    if (a.f[3]) db = db + a.f[3][o];
+    a.du[o] = (T)du;
+    a.dv[o] = (T)dv;
+    a.dw[o] = (T)dw;
+    a.db[o] = (T)db;
+}
+
+// Adams-Bashforth of the mixed form after a separate friction pass (models
+// with background forcing, whose terms need the post-friction state): from
+// the post-friction (hi, lo) pair P, x = P + (AB increment - dt grad(Q)) in
+// FP64, split back into (hi, lo), with the halo images of hi.  The same
+// terms as the fused kernel's final stage (fric_body<float>).
+struct AbMixedArgs {
+    const float *ph[4], *pl[4];          // post-friction state
+    float *oh[4], *ol[4];                // new state
+    const float *t0[4], *t1[4], *t2[4];  // slots tau, taum1, taum2
+    double scale[4];
+    int64_t gstride[4];
+    double grd[4];
+    double ca, cb, cc;
+    int order;
+    const float *d1, *d2;
+    double qc1, qc2;
+    int halo;
+};
+
+__global__ __launch_bounds__(BX * BY) void k_ab_mixed(StepConsts c, AbMixedArgs a)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    if (i > c.nx || j > c.ny) return;
+    const int k = (int)blockIdx.z % c.nz + 1, f = (int)blockIdx.z / c.nz;
+    const int64_t o = idx(c, i, j, k);
+    const double s = a.scale[f];
+    double ab;
+    if (a.order == 1) ab = s * (double)a.t0[f][o];
+    else if (a.order == 2) ab = s * (a.ca * (double)a.t0[f][o] - a.cb * (double)a.t1[f][o]);
+    else ab = s * (a.ca * (double)a.t0[f][o] + a.cb * (double)a.t1[f][o] + a.cc * (double)a.t2[f][o]);
+    double gq = 0.0;
+    const int64_t gs = a.gstride[f];
+    if (gs && a.d1) {
+        double qo = a.qc1 * (double)a.d1[o], qn = a.qc1 * (double)a.d1[o + gs];
+        if (a.d2) {
+            qo += a.qc2 * (double)a.d2[o];
+            qn += a.qc2 * (double)a.d2[o + gs];
+        }
+        gq = s * ((qn - qo) * a.grd[f]);
+    }
+    const double x = ((double)a.ph[f][o] + (double)a.pl[f][o]) + (ab - gq);
+    const float hi = (float)x;
+    a.oh[f][o] = hi;
+    a.ol[f][o] = (float)(x - (double)hi);
+    if (a.halo) write_images<float>(c, a.oh[f], i, j, k, hi, 0);
+}
+
 __global__ __launch_bounds__(BX * BY) void k_ab(StepConsts c, AbArgs a)
 {
     const int i = blockIdx.x * BX + threadIdx.x + 1;
@@ -669,7 +791,8 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
             } else {
                 const double s = F.scale;
                 double ab;
-                if (a.order == 1) ab = s * (double)s0;
+                if (a.order == 0) ab = 0.0;   // friction only (the split step of a forced mixed model)
+                else if (a.order == 1) ab = s * (double)s0;
                 else if (a.order == 2) ab = s * (a.ca * (double)s0 - a.cb * (double)s1);
                 else ab = s * (a.ca * (double)s0 + a.cb * (double)s1 + a.cc * (double)s2);
                 // the harmonic pass zeroes w(nz) under a rigid lid before the
@@ -1127,6 +1250,30 @@ void launch_friction_block(Model &m)
     if (p.Kvbi > 0.) launch_mixing(m, PS3D_BIHARMONIC_V, PS3D_B, 0.0, p.Kvbi, 0);
 }
 
+void launch_background(Model &m, int tau, float *const *mixed_fields)
+{
+    const int64_t off = (int64_t)(tau - 1) * m.g.H;
+    if (!m.mixed) {
+        BgArgs<double> a{};
+        a.u = m.cur_buf(0); a.v = m.cur_buf(1); a.w = m.cur_buf(2); a.b = m.cur_buf(3);
+        a.du = m.du + off; a.dv = m.dv + off; a.dw = m.dw + off; a.db = m.db + off;
+        a.ub = m.bg_on ? m.bg_u : nullptr;
+        a.bb = (m.bg_on && m.bg_b_on) ? m.bg_b : nullptr;
+        for (int f = 0; f < 4; ++f) a.f[f] = m.bf_on[f] ? m.bf[f] : nullptr;
+        hipLaunchKernelGGL(k_background<double>, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, a);
+    } else {
+        BgArgs<float> a{};
+        float *const *fl = mixed_fields ? mixed_fields : m.fh;
+        a.u = fl[0]; a.v = fl[1]; a.w = fl[2]; a.b = fl[3];
+        a.du = m.tf[0] + off; a.dv = m.tf[1] + off; a.dw = m.tf[2] + off; a.db = m.tf[3] + off;
+        a.ub = m.bg_on ? m.bg_u : nullptr;
+        a.bb = (m.bg_on && m.bg_b_on) ? m.bg_b : nullptr;
+        for (int f = 0; f < 4; ++f) a.f[f] = m.bf_on[f] ? m.bf[f] : nullptr;
+        hipLaunchKernelGGL(k_background<float>, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, a);
+    }
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
 void launch_wavemaker(Model &m, double cs, double sn)
 {
     WaveArgs a{};
@@ -1234,7 +1381,9 @@ static void friction_ab_impl(Model &m, const T *const *in, const T *const *in_lo
         F.gstride = f == 0 ? 1 : (f == 1 ? g.X : (f == 2 ? g.XY : 0));
         F.grd = f == 0 ? m.c.rdx : (f == 1 ? m.c.rdy : m.c.rdz);
     }
-    if (itt == 0) {
+    if (itt < 0) {
+        a.order = 0;   // friction only (mixed form with background forcing)
+    } else if (itt == 0) {
         a.order = 1;
     } else if (itt == 1 || !p.enable_AB_3_order) {
         a.order = 2;
@@ -1293,6 +1442,64 @@ void launch_friction_ab_mixed(Model &m, int itt, int tau, int taum1, int taum2, 
     HaloArgsT<float> h{};
     for (int f = 0; f < 4; ++f) h.f[f] = m.fh[f];
     halo_launch(m, h, 4);
+}
+
+// the mixed step with background forcing: friction alone into the second
+// buffers (whose halos keep the pre-friction values, as the reference's
+// in-place friction leaves them), the forcing on that post-friction state
+// into the tau slot, then the Adams-Bashforth pass back into the canonical
+// buffers
+void launch_friction_forcing_ab_mixed(Model &m, int itt, int tau, int taum1, int taum2, double qc1, const float *d1,
+                                      double qc2, const float *d2)
+{
+    const size_t bytes = sizeof(float) * (size_t)m.g.H;
+    for (int f = 0; f < 4; ++f) {
+        PS3D_HIP_CHECK(hipMemcpyAsync(m.ah[f], m.fh[f], bytes, hipMemcpyDeviceToDevice, m.stream));
+        PS3D_HIP_CHECK(hipMemcpyAsync(m.al[f], m.fl[f], bytes, hipMemcpyDeviceToDevice, m.stream));
+    }
+    const int keep = m.fuse_halo;
+    m.fuse_halo = false;   // interior only: the halos stay pre-friction
+    friction_ab_impl<float>(m, m.fh, m.fl, m.ah, m.al, m.tf, -1, tau, taum1, taum2);
+    m.fuse_halo = keep;
+    launch_background(m, tau, m.ah);
+    AbMixedArgs a{};
+    const ps3d_params &p = m.prm;
+    for (int f = 0; f < 4; ++f) {
+        a.ph[f] = m.ah[f];
+        a.pl[f] = m.al[f];
+        a.oh[f] = m.fh[f];
+        a.ol[f] = m.fl[f];
+        a.t0[f] = m.tf[f] + (int64_t)(tau - 1) * m.g.H;
+        a.t1[f] = m.tf[f] + (int64_t)(taum1 - 1) * m.g.H;
+        a.t2[f] = m.tf[f] + (int64_t)(taum2 - 1) * m.g.H;
+        a.scale[f] = (f == 2) ? p.dt / p.dsqr : p.dt;
+        a.gstride[f] = f == 0 ? 1 : (f == 1 ? m.g.X : (f == 2 ? m.g.XY : 0));
+        a.grd[f] = f == 0 ? m.c.rdx : (f == 1 ? m.c.rdy : m.c.rdz);
+    }
+    if (itt == 0) {
+        a.order = 1;
+    } else if (itt == 1 || !p.enable_AB_3_order) {
+        a.order = 2;
+        a.ca = 1.5 + p.eps_ab;
+        a.cb = 0.5 + p.eps_ab;
+    } else {
+        a.order = 3;
+        a.ca = 23.0 / 12.0;
+        a.cb = -16.0 / 12.0;
+        a.cc = 5.0 / 12.0;
+    }
+    a.d1 = d1;
+    a.d2 = d2;
+    a.qc1 = qc1;
+    a.qc2 = qc2;
+    a.halo = m.fuse_halo ? 1 : 0;
+    hipLaunchKernelGGL(k_ab_mixed, interior_grid(m.g, 4), dim3(BX, BY), 0, m.stream, m.c, a);
+    PS3D_HIP_CHECK(hipGetLastError());
+    if (!m.fuse_halo) {
+        HaloArgsT<float> h{};
+        for (int f = 0; f < 4; ++f) h.f[f] = m.fh[f];
+        halo_launch(m, h, 4);
+    }
 }
 
 // the fused kernel alone, into the second buffers (no swap, no halo): the

@@ -294,6 +294,23 @@ class Model:
     def clear_wavemaker(self):
         check(self.lib.ps3d_clear_wavemaker(self.h))
 
+    def set_background_flow(self, u_back=None, b_back=None):
+        """Advection by the background flow (u_back, 0, 0) and of b_back,
+        lee_wave1.f90:239-314 (ps3d_set_background_flow); u_back None: off."""
+        ub = None if u_back is None else np.asfortranarray(u_back, dtype=np.float64)
+        bb = None if b_back is None else np.asfortranarray(b_back, dtype=np.float64)
+        check(self.lib.ps3d_set_background_flow(self.h, None if ub is None else _dp(ub),
+                                                None if bb is None else _dp(bb)))
+
+    def set_body_force(self, force: dict | None):
+        """Steady body force added to the tendencies ({'du': .., 'dw': ..};
+        ps3d_set_body_force); None: off."""
+        force = force or {}
+        arrs = [None if force.get(n) is None else np.asfortranarray(force[n], dtype=np.float64)
+                for n in ("du", "dv", "dw", "db")]
+        tab = (ctypes.POINTER(ctypes.c_double) * 4)(*[None if a is None else _dp(a) for a in arrs])
+        check(self.lib.ps3d_set_body_force(self.h, tab))
+
     def diagnose(self) -> dict:
         """diagnose.F90:11-53 on the device: KE, PE (per point), max |u_h|,
         max |w|, CFL and Peclet numbers, and whether a NaN was seen."""
