@@ -152,9 +152,10 @@ int ps3d_clear_wavemaker(ps3d_model *m);
    buoyancy b_back, lee_wave1.f90:239-314, on the device: each step adds
    -Ro * (flux divergences) to du..db at the tau slot, before any other
    forcing.  u_back, b_back: host arrays of a field's layout (with halo),
-   copied; b_back NULL: zero; u_back NULL: off.  In the mixed form the terms
-   are added before its fused friction + AB, i.e. from the pre-friction
-   state (the FP64 form, like the reference, uses the post-friction one). */
+   copied; b_back NULL: zero; u_back NULL: off.  Both forms: the terms are
+   formed from the post-friction state, as the reference's set_forcing sees
+   it (the mixed form then runs friction, forcing and Adams-Bashforth as
+   three passes instead of its fused kernel). */
 int ps3d_set_background_flow(ps3d_model *m, const double *u_back, const double *b_back);
 
 /* A steady body force: f[q] (q = 0..3: du, dv, dw, db; host arrays of a

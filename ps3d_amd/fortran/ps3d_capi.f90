@@ -154,6 +154,18 @@ module ps3d_capi
    import :: c_int, c_ptr
    type(c_ptr), value :: model
   end function
+  ! lee_wave1.f90:239-314 background advection; u_back, b_back = c_loc of
+  ! field-layout arrays (c_null_ptr: off / zero)
+  integer(c_int) function ps3d_set_background_flow(model, u_back, b_back) bind(C, name="ps3d_set_background_flow")
+   import :: c_int, c_ptr
+   type(c_ptr), value :: model, u_back, b_back
+  end function
+  ! steady body force on du, dv, dw, db (c_loc of field-layout arrays or c_null_ptr)
+  integer(c_int) function ps3d_set_body_force(model, f) bind(C, name="ps3d_set_body_force")
+   import :: c_int, c_ptr
+   type(c_ptr), value :: model
+   type(c_ptr), intent(in) :: f(4)
+  end function
  end interface
 
 contains
