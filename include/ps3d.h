@@ -130,7 +130,7 @@ int ps3d_synchronize(ps3d_model *m);
    the step with PS3D_ERR_ARG.  While a hook or the wavemaker is set the
    step takes the unfused friction kernels.  FP64 form only; fn = NULL
    removes the hook.  Replaces the config's set_forcing
-   (for_config/*.f90, e.g. lee_wave1.f90:232-331) for GPU runs. */
+   (the for_config files, e.g. lee_wave1.f90:232-331) for GPU runs. */
 typedef int (*ps3d_forcing_fn)(ps3d_model *m, void *user, int itt, int tau, void *stream);
 int ps3d_set_forcing(ps3d_model *m, ps3d_forcing_fn fn, void *user);
 

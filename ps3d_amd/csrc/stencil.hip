@@ -1342,10 +1342,12 @@ bool fused_friction_ok(const Model &m)
 // writes the new values' halo images (the exchange of integrate.f90:106-111).
 // The new fields are written to the second buffers, which then become the
 // canonical ones.
+// itt < 0: friction only (no AB).  halo: write the new values' halo images
+// (-1: where the model does, fuse_halo)
 template <class T>
 static void friction_ab_impl(Model &m, const T *const *in, const T *const *in_lo, T *const *out, T *const *out_lo,
                              T *const *tend, int itt, int tau, int taum1, int taum2, double qc1 = 0.0,
-                             const float *d1 = nullptr, double qc2 = 0.0, const float *d2 = nullptr)
+                             const float *d1 = nullptr, double qc2 = 0.0, const float *d2 = nullptr, int halo = -1)
 {
     const ps3d_params &p = m.prm;
     const Grid &g = m.g;
@@ -1405,7 +1407,7 @@ static void friction_ab_impl(Model &m, const T *const *in, const T *const *in_lo
     a.kc = (g.nz + nchunk - 1) / nchunk;
     a.nchunk = (g.nz + a.kc - 1) / a.kc;
     a.nblocks = tiles * a.nchunk * 4;
-    a.halo = m.fuse_halo ? 1 : 0;
+    a.halo = halo < 0 ? (m.fuse_halo ? 1 : 0) : halo;
     ProfScope ps(m, "friction_ab");
     hipLaunchKernelGGL(k_fric_ab<T>, dim3((unsigned)a.nblocks), dim3(FTHREADS), 0, m.stream, m.c, a);
     PS3D_HIP_CHECK(hipGetLastError());
@@ -1457,10 +1459,8 @@ void launch_friction_forcing_ab_mixed(Model &m, int itt, int tau, int taum1, int
         PS3D_HIP_CHECK(hipMemcpyAsync(m.ah[f], m.fh[f], bytes, hipMemcpyDeviceToDevice, m.stream));
         PS3D_HIP_CHECK(hipMemcpyAsync(m.al[f], m.fl[f], bytes, hipMemcpyDeviceToDevice, m.stream));
     }
-    const int keep = m.fuse_halo;
-    m.fuse_halo = false;   // interior only: the halos stay pre-friction
-    friction_ab_impl<float>(m, m.fh, m.fl, m.ah, m.al, m.tf, -1, tau, taum1, taum2);
-    m.fuse_halo = keep;
+    // interior only (halo 0): the halos stay pre-friction
+    friction_ab_impl<float>(m, m.fh, m.fl, m.ah, m.al, m.tf, -1, tau, taum1, taum2, 0.0, nullptr, 0.0, nullptr, 0);
     launch_background(m, tau, m.ah);
     AbMixedArgs a{};
     const ps3d_params &p = m.prm;
