@@ -24,6 +24,8 @@ The JSON line also carries:
   128x128x64 (eddy radius drawn per member), batched (one launch per stage,
   hipGraph replay, autotuned tile shapes) against the same members run one
   after another as single models: member-steps/s of both and their ratio.
+* ensemble_mixed: the same 48 members in the mixed-precision form, batched
+  against single mixed models run one after another.
 """
 from __future__ import annotations
 
@@ -128,12 +130,12 @@ def mixed_leg(prm, fields, steps: int, warmup: int, fp64_value: float, check_ste
             "dtype": "f32 stencils/FFT, f64 solve + accumulation"}
 
 
-def ensemble_leg(steps: int, warm: int = 3) -> dict:
+def ensemble_leg(steps: int, warm: int = 3, precision: int = 0) -> dict:
     import numpy as np
     from ps3d_amd import configs
     from ps3d_amd.ensemble import Ensemble
     from ps3d_amd.model import Model
-    members = configs.ensemble_c4()
+    members = configs.ensemble_c4(precision=precision)
     n = len(members)
     # one after another: each member alone through the single-model path
     seq = 0.0
@@ -156,7 +158,7 @@ def ensemble_leg(steps: int, warm: int = 3) -> dict:
             ens.set(i, k, a)
     ens.run(warm)
     t0 = time.perf_counter()
-    ens.tune(os.path.join("/tmp", "ps3d_ensemble_tune.txt"), reps=5)
+    ens.tune(os.path.join("/tmp", f"ps3d_ensemble_tune_{precision}.txt"), reps=5)
     tune_s = time.perf_counter() - t0
     ens.run(1, use_graph=True)   # capture
     ens.synchronize()
@@ -190,7 +192,8 @@ def ensemble_leg(steps: int, warm: int = 3) -> dict:
             "ms_per_ensemble_step": 1e3 * bat / steps, "sequential_ms_per_ensemble_step": 1e3 * seq / steps,
             "points_per_ensemble_step": int(sum(p.npoints for p, _ in members)),
             "bitwise_equal_to_single_runs": same, "tuning_s": tune_s,
-            "tiles": desc.strip().splitlines(), "dtype": "f64",
+            "tiles": desc.strip().splitlines(),
+            "dtype": "f32 stencils/FFT, f64 solve + accumulation" if precision else "f64",
             "workload": "16x(64x64x32) + 16x(96x96x48) + 16x(128x128x64) baroclinic eddies, rigid lid, AB3, "
                         "horizontal biharmonic friction; one launch per stage, hipGraph replay"}
 
@@ -308,6 +311,8 @@ def main() -> int:
         out["mixed"] = mixed_leg(prm, fields, args.steps, args.warmup, value)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ens_steps)
+        if not args.no_mixed:
+            out["ensemble_mixed"] = ensemble_leg(args.ens_steps, precision=1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prm, fields)
     print(json.dumps(out), flush=True)

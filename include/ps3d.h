@@ -288,9 +288,13 @@ const char *ps3d_kernel_names(void);
    the step (tendency, friction + AB, halo, the transforms, Thomas or the
    periodic z solve, projection) is one launch over all members, and a step
    can be replayed as one captured hipGraph.  Every member computes exactly
-   what a single model with the same parameters computes (PS3D_FP64, the
-   fused-friction configurations: Avbi = Kvbi = 0; FFT lengths among the
-   compiled-in ones, column lengths ny (and nz without a lid) <= 512). */
+   what a single model with the same parameters computes (the fused-friction
+   configurations: Avbi = Kvbi = 0; FFT lengths among the compiled-in ones,
+   column lengths ny (and nz without a lid) <= 512).  The members share one
+   precision form: PS3D_FP64, or PS3D_MIXED (rigid-lid members of at most 85
+   levels), whose first run starts every member's pressure history as a
+   single mixed model's first step does; that step and the next one (which
+   add a correction solve) are queued directly, later ones replay the graph. */
 typedef struct ps3d_ensemble ps3d_ensemble;
 
 int ps3d_ensemble_create(int n, const ps3d_params *prms, void *stream, ps3d_ensemble **out);

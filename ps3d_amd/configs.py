@@ -303,7 +303,7 @@ def wavemaker_amplitudes(prm: ModelParams, seed: int = 11, amp: float = 1e-5):
     return re, im
 
 
-def ensemble_c4(per_size: int = 16, sizes=((64, 64, 32), (96, 96, 48), (128, 128, 64))):
+def ensemble_c4(per_size: int = 16, sizes=((64, 64, 32), (96, 96, 48), (128, 128, 64)), precision=PS3D_FP64):
     """Ensemble of baroclinic eddies (the 64x64x32 eddy physics) in three
     sizes, per_size members each; member m has seed 1000+m and eddy radius
     R_m = U(1/8, 1/4) Lx drawn from that seed."""
@@ -313,7 +313,7 @@ def ensemble_c4(per_size: int = 16, sizes=((64, 64, 32), (96, 96, 48), (128, 128
         for _ in range(per_size):
             seed = 1000 + m
             u01 = 0.5 * (float(uniform(seed, "R", 1, 1, 1)[0, 0, 0]) + 1.0)
-            out.append(eddy(nx, ny, nz, seed=seed, radius_frac=0.125 + 0.125 * u01))
+            out.append(eddy(nx, ny, nz, seed=seed, radius_frac=0.125 + 0.125 * u01, precision=precision))
             m += 1
     return out
 

@@ -2,11 +2,11 @@
 // batched stage kernels (member table, block maps, step state, hipGraph
 // replay) and the autotuner of the k-marching tile shapes.
 //
-// Each member is an ordinary FP64 model (buffers, FFT tables, ksqr, Thomas
-// factors); the ensemble owns a device table that points into them and runs
-// one launch per stage over all of them.  The bodies of the batched kernels
-// are those of the single-model kernels, so every member's result is the
-// same, bit for bit, as a single model's.
+// Each member is an ordinary model (buffers, FFT tables, ksqr, Thomas
+// factors), all of one form, FP64 or mixed; the ensemble owns a device table
+// that points into them and runs one launch per stage over all of them.  The
+// bodies of the batched kernels are those of the single-model kernels, so
+// every member's result is the same, bit for bit, as a single model's.
 #include "ps3d_internal.h"
 
 #include <algorithm>
@@ -20,7 +20,8 @@ using namespace ps3d;
 
 namespace {
 
-enum Stage { S_TEND, S_FRIC, S_R2C, S_COLS, S_THOMAS, S_ZSOLVE, S_PROJ, S_NSTAGE };
+// S_PTS: 256 points per work-group over the H points of a member
+enum Stage { S_TEND, S_FRIC, S_R2C, S_COLS, S_THOMAS, S_ZSOLVE, S_PROJ, S_PTS, S_NSTAGE };
 
 // tile shapes of the k-marching kernels (stencil.hip)
 constexpr int kTTX = TEND_TX, kTTY = TEND_TY, kFTX = FRIC_TX, kFTY = FRIC_TY;
@@ -36,7 +37,7 @@ struct ps3d_ensemble {
     std::vector<EnsMember> tab;
     EnsMember *dev_tab = nullptr;
     EnsState *dev_state = nullptr;
-    EnsState state{0, 3, 2, 1, 0};   // the device state once the queued steps have run
+    EnsState state{0, 3, 2, 1, 0, 0};   // the device state once the queued steps have run
     EnsMap maps[S_NSTAGE]{};
     size_t lds_x = 0, lds_y = 0, lds_z = 0;
     int ny_max = 0, nz_max = 0;         // longest y and (periodic) z columns
@@ -44,6 +45,10 @@ struct ps3d_ensemble {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::map<std::string, int> tuned;   // "stage shape" -> levels per work-group
+    // mixed form: the first step starts the pressure history (cold start),
+    // and it and the next one add a correction solve (as a single model)
+    bool mixed = false, cold = true;
+    int refine = 0;
 };
 
 namespace {
@@ -51,7 +56,8 @@ namespace {
 std::string shape_key(const ps3d_params &p)
 {
     std::ostringstream o;
-    o << p.nx << "x" << p.ny << "x" << p.nz << (p.enable_vertical_boundaries ? ":lid" : ":periodic") << ":fp64";
+    o << p.nx << "x" << p.ny << "x" << p.nz << (p.enable_vertical_boundaries ? ":lid" : ":periodic")
+      << (p.precision == PS3D_MIXED ? ":mixed" : ":fp64");
     return o.str();
 }
 
@@ -121,6 +127,22 @@ void fill_member(EnsMember &e, ps3d_model &m)
     e.tend_ty = cdiv(g.ny, kTTY);
     e.fric_tx = cdiv(g.nx, kFTX);
     e.fric_ty = cdiv(g.ny, kFTY);
+    if (m.mixed) {
+        for (int q = 0; q < 4; ++q) {
+            e.fh[q][0] = m.fh[q];
+            e.fh[q][1] = m.ah[q];
+            e.fl[q][0] = m.fl[q];
+            e.fl[q][1] = m.al[q];
+            e.tf[q] = m.tf[q];
+        }
+        e.pint = m.pint;
+        for (int q = 0; q < 3; ++q) e.dpr[q] = m.dpr[q];
+        e.FSf = m.FSf;
+        e.twxf = m.px.twf;
+        e.twyf = m.py.twf;
+        e.twzf = m.pz.twf;
+        e.wkf = m.twf_r2c;
+    }
 }
 
 void build_maps(ps3d_ensemble &E)
@@ -142,6 +164,7 @@ void build_maps(ps3d_ensemble &E)
             case S_THOMAS: nb = lid ? cdiv((int64_t)g.nkx * g.ny, ens_thomas_cols()) : 0; break;
             case S_ZSOLVE: nb = lid ? 0 : ens_cols_blocks(e, g.nz, g.ny); break;
             case S_PROJ: nb = (int64_t)cdiv(g.nx, 64) * cdiv(g.ny, 4) * g.nz; break;
+            case S_PTS: nb = E.mixed ? cdiv(g.H, 256) : 0; break;
             }
             E.maps[s].start[i] = (int)acc;
             acc += nb;
@@ -192,15 +215,68 @@ void apply_tiles(ps3d_ensemble &E, int stage, const std::string &key, int kc)
 void launch_stage(ps3d_ensemble &E, int s)
 {
     switch (s) {
-    case S_TEND: ens_tendency(E.stream, E.dev_tab, E.maps[S_TEND], E.dev_state); break;
-    case S_FRIC: ens_friction_ab(E.stream, E.dev_tab, E.maps[S_FRIC], E.dev_state); break;
+    case S_TEND:
+        if (E.mixed) ens_tendency_f(E.stream, E.dev_tab, E.maps[S_TEND], E.dev_state, 0);
+        else ens_tendency(E.stream, E.dev_tab, E.maps[S_TEND], E.dev_state);
+        break;
+    case S_FRIC:
+        if (E.mixed) ens_friction_ab_f(E.stream, E.dev_tab, E.maps[S_FRIC], E.dev_state, 0);
+        else ens_friction_ab(E.stream, E.dev_tab, E.maps[S_FRIC], E.dev_state);
+        break;
     default: break;
     }
+}
+
+// mixed form: the FP32 Poisson solve of the new state's divergence into the
+// increment selected by which (see ens_c2r_f)
+void enqueue_solve_f(ps3d_ensemble &E, int which)
+{
+    hipStream_t s = E.stream;
+    ens_r2c_f(s, E.dev_tab, E.maps[S_R2C], E.dev_state, E.lds_x);
+    ens_cols_y_f(s, E.dev_tab, E.maps[S_COLS], -1, E.ny_max, E.lds_y);
+    ens_thomas_f(s, E.dev_tab, E.maps[S_THOMAS], E.nzl_max);
+    ens_zsolve_f(s, E.dev_tab, E.maps[S_ZSOLVE], E.nz_max, E.lds_z);
+    ens_cols_y_f(s, E.dev_tab, E.maps[S_COLS], +1, E.ny_max, E.lds_y);
+    ens_c2r_f(s, E.dev_tab, E.maps[S_R2C], E.dev_state, which, E.lds_x);
+}
+
+// mixed form, start of the pressure history of every member (the single
+// model's cold start): a provisional Euler step into the second buffers, its
+// FP32 pressure as the reference pressure pint, zero increments
+void enqueue_cold_mixed(ps3d_ensemble &E)
+{
+    hipStream_t s = E.stream;
+    ens_tendency_f(s, E.dev_tab, E.maps[S_TEND], E.dev_state, 1);
+    ens_friction_ab_f(s, E.dev_tab, E.maps[S_FRIC], E.dev_state, 1);
+    enqueue_solve_f(E, 0);
+    ens_cold_pint(s, E.dev_tab, E.maps[S_PROJ]);
+    ens_points_f(s, E.dev_tab, E.maps[S_PTS], E.dev_state, 0);
+}
+
+// one step of the mixed form for every member (integrate_mixed of a single
+// model); refine: the correction solve of the first two steps
+void enqueue_step_mixed(ps3d_ensemble &E, bool refine)
+{
+    hipStream_t s = E.stream;
+    ens_tendency_f(s, E.dev_tab, E.maps[S_TEND], E.dev_state, 0);
+    ens_friction_ab_f(s, E.dev_tab, E.maps[S_FRIC], E.dev_state, 0);   // writes the halo images too
+    enqueue_solve_f(E, 1);
+    ens_project_f(s, E.dev_tab, E.maps[S_PROJ], E.dev_state, 1);
+    if (refine) {
+        enqueue_solve_f(E, 2);
+        ens_project_f(s, E.dev_tab, E.maps[S_PROJ], E.dev_state, 2);
+        ens_points_f(s, E.dev_tab, E.maps[S_PTS], E.dev_state, 1);
+    }
+    ens_advance(s, E.dev_state);
 }
 
 // integrate.f90:2-115 for every member: the single-model step, stage by stage
 void enqueue_step(ps3d_ensemble &E)
 {
+    if (E.mixed) {
+        enqueue_step_mixed(E, false);
+        return;
+    }
     hipStream_t s = E.stream;
     ens_tendency(s, E.dev_tab, E.maps[S_TEND], E.dev_state);
     ens_friction_ab(s, E.dev_tab, E.maps[S_FRIC], E.dev_state);   // writes the halo images too
@@ -218,6 +294,27 @@ void check_member(const ps3d_ensemble *e, int member)
 {
     if (!e) throw Error(PS3D_ERR_ARG, "null ensemble");
     PS3D_REQUIRE(member >= 0 && member < (int)e->mem.size(), "member index out of range");
+}
+
+// a mixed member's state is in its second buffers when par is 1: exchange
+// the buffer roles around a single-model call that reads or writes it
+template <class F>
+void with_parity(ps3d_model *m, int par, F &&fn)
+{
+    auto flip = [&] {
+        for (int q = 0; q < 4; ++q) {
+            std::swap(m->fh[q], m->ah[q]);
+            std::swap(m->fl[q], m->al[q]);
+        }
+    };
+    if (par) flip();
+    try {
+        fn();
+    } catch (...) {
+        if (par) flip();
+        throw;
+    }
+    if (par) flip();
 }
 
 void destroy(ps3d_ensemble *e)
@@ -246,7 +343,11 @@ int ps3d_ensemble_create(int n, const ps3d_params *prms, void *stream, ps3d_ense
         for (int i = 0; i < n; ++i) {
             const ps3d_params &p = prms[i];
             const std::string who = "member " + std::to_string(i) + ": ";
-            PS3D_REQUIRE(p.precision == PS3D_FP64, who + "ensembles run the FP64 form");
+            PS3D_REQUIRE(p.precision == PS3D_FP64 || p.precision == PS3D_MIXED, who + "unknown precision form");
+            PS3D_REQUIRE(p.precision == prms[0].precision, who + "the members of an ensemble share one precision form");
+            PS3D_REQUIRE(p.precision != PS3D_MIXED || !p.enable_vertical_boundaries ||
+                             (sizeof(double2) + sizeof(double)) * (size_t)ens_thomas_cols() * (size_t)p.nz <= 64 * 1024,
+                         who + "mixed ensembles: rigid-lid members of at most 85 levels");
             PS3D_REQUIRE(!(p.Avbi > 0.) && !(p.Kvbi > 0.),
                          who + "vertical biharmonic friction (Avbi, Kvbi > 0) is not in the batched step");
             PS3D_REQUIRE(p.nx >= 2 && p.ny >= 2 && ens_fft_size_ok(p.nx / 2, false) && ens_fft_size_ok(p.ny, true) &&
@@ -264,6 +365,7 @@ int ps3d_ensemble_create(int n, const ps3d_params *prms, void *stream, ps3d_ense
                 PS3D_HIP_CHECK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
                 E->own_stream = true;
             }
+            E->mixed = prms[0].precision == PS3D_MIXED;
             E->mem.assign(n, nullptr);
             E->tab.resize(n);
             for (int i = 0; i < n; ++i) {
@@ -300,6 +402,14 @@ int ps3d_ensemble_set_field(ps3d_ensemble *e, int member, int field, const doubl
         check_member(e, member);
         PS3D_REQUIRE(host, "null host pointer");
         ps3d_model *m = e->mem[member];
+        if (e->mixed) {
+            PS3D_HIP_CHECK(hipStreamSynchronize(e->stream));
+            with_parity(m, e->state.par, [&] {
+                const int rc = ps3d_set_field(m, field, host);
+                if (rc != PS3D_OK) throw Error(rc, error_slot());
+            });
+            return;
+        }
         if (field >= PS3D_U && field <= PS3D_B) {
             PS3D_HIP_CHECK(hipStreamSynchronize(e->stream));
             PS3D_HIP_CHECK(hipMemcpyAsync(e->tab[member].f[field][e->state.par], host, sizeof(double) * m->g.H,
@@ -318,6 +428,13 @@ int ps3d_ensemble_get_field(ps3d_ensemble *e, int member, int field, double *hos
         check_member(e, member);
         PS3D_REQUIRE(host, "null host pointer");
         ps3d_model *m = e->mem[member];
+        if (e->mixed) {
+            with_parity(m, e->state.par, [&] {
+                const int rc = ps3d_get_field(m, field, host);
+                if (rc != PS3D_OK) throw Error(rc, error_slot());
+            });
+            return;
+        }
         if (field >= PS3D_U && field <= PS3D_B) {
             PS3D_HIP_CHECK(hipMemcpyAsync(host, e->tab[member].f[field][e->state.par], sizeof(double) * m->g.H,
                                           hipMemcpyDeviceToHost, e->stream));
@@ -359,14 +476,34 @@ int ps3d_ensemble_run(ps3d_ensemble *e, int nsteps, int32_t *st, int use_graph)
             PS3D_HIP_CHECK(hipStreamEndCapture(e->stream, &e->graph));
             PS3D_HIP_CHECK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
         }
+        if (e->mixed && e->cold && nsteps > 0) {
+            enqueue_cold_mixed(*e);
+            e->cold = false;
+            e->refine = 2;
+        }
         for (int s = 0; s < nsteps; ++s) {
-            if (use_graph) PS3D_HIP_CHECK(hipGraphLaunch(e->exec, e->stream));
-            else enqueue_step(*e);
+            if (e->mixed && e->refine > 0) {   // not in the graph: the steady step is
+                enqueue_step_mixed(*e, true);
+                e->refine -= 1;
+            } else if (use_graph) {
+                PS3D_HIP_CHECK(hipGraphLaunch(e->exec, e->stream));
+            } else {
+                enqueue_step(*e);
+            }
             e->state.tau = e->state.tau % 3 + 1;
             e->state.taum1 = e->state.taum1 % 3 + 1;
             e->state.taum2 = e->state.taum2 % 3 + 1;
             e->state.itt += 1;
             e->state.par ^= 1;
+            e->state.dp_cur = (e->state.dp_cur + 1) % 3;
+        }
+        if (e->mixed && nsteps > 0) {
+            // as after a single model's step: the lo parts' halos are stale
+            // and a pressure set by the host is superseded
+            for (ps3d_model *m : e->mem) {
+                for (int f = 0; f < 4; ++f) m->lo_stale[f] = true;
+                m->p_host.clear();
+            }
         }
         st[0] = e->state.itt;
         st[1] = e->state.tau;

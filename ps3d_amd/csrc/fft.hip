@@ -969,6 +969,105 @@ __global__ __launch_bounds__(TTHREADS) void k_ens_zsolve(const EnsMember *__rest
     }
 }
 
+// the same stages for members of the mixed form: FP32 data and tables, the
+// bodies and arguments of fft_r2c_x_f, fft_c2c_y_f, fft_z_periodic_solve_f and
+// fft_c2r_x_f; the x-passes read the state of the new parity and the inverse
+// writes the pressure increment selected by which (see ens_c2r_f)
+__global__ __launch_bounds__(TTHREADS) void k_ens_r2c_f(const EnsMember *__restrict__ mem, EnsMap map,
+                                                         const EnsState *__restrict__ st)
+{
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const StepConsts c = e.c;
+    const int lb = g - map.start[mi];
+    const int nb = st->par ^ 1;
+    const int64_t nrows = (int64_t)c.ny * c.nz;
+    float2 *lds = reinterpret_cast<float2 *>(ens_lds);
+    switch (e.n2) {
+#define CASE(n)                                                                                              \
+    case n:                                                                                                  \
+        r2c_body<n, XROWS(n), TTHREADS, true, float>(lds, lb, c, e.twxf, e.wkf, nullptr, e.fh[0][nb],         \
+                                                     e.fh[1][nb], e.fh[2][nb], e.FSf, nrows, e.nkx);          \
+        break;
+        PS3D_FFT_SIZES(CASE)
+#undef CASE
+    default: break;
+    }
+}
+
+__global__ __launch_bounds__(TTHREADS) void k_ens_c2r_f(const EnsMember *__restrict__ mem, EnsMap map,
+                                                         const EnsState *__restrict__ st, int which)
+{
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const StepConsts c = e.c;
+    const int lb = g - map.start[mi];
+    const int64_t nrows = (int64_t)c.ny * c.nz;
+    const double inv = c.lid ? e.lxy : e.lxyz, rinv = c.lid ? e.rlxy : e.rlxyz;
+    float *dst = which == 0 ? e.dpr[0] : e.dpr[(st->dp_cur + which) % 3];
+    float2 *lds = reinterpret_cast<float2 *>(ens_lds);
+    switch (e.n2) {
+#define CASE(n)                                                                                              \
+    case n:                                                                                                  \
+        c2r_body<n, XROWS(n), TTHREADS, float>(lds, lb, c, e.twxf, e.wkf, e.FSf, dst, 1, inv, rinv, nrows,    \
+                                               e.nkx);                                                       \
+        break;
+        PS3D_FFT_SIZES(CASE)
+#undef CASE
+    default: break;
+    }
+}
+
+template <int SIGN, int NMAX>
+__global__ __launch_bounds__(TTHREADS) void k_ens_cols_y_f(const EnsMember *__restrict__ mem, EnsMap map)
+{
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const int lb = g - map.start[mi];
+    const int nbx = (e.nkx + TCOL - 1) / TCOL;
+    const int bx = lb % nbx, by = lb / nbx;
+    const int nscale = (SIGN < 0 && e.c.lid) ? 2 : 0;
+    const double s1 = nscale ? e.c.dx : 0.0, s2 = nscale ? e.c.dy : 0.0;
+    float2 *lds = reinterpret_cast<float2 *>(ens_lds);
+    switch (e.c.ny) {
+#define CASE(n)                                                                                              \
+    case n:                                                                                                  \
+        if constexpr (n <= NMAX && TCOLS(n) == TTHREADS)                                                     \
+            cols_body<n, TCOL, TTHREADS, SIGN, float2>(lds, bx, by, e.twyf, e.FSf, e.nkx, (int64_t)e.nkx,     \
+                                                       (int64_t)e.nkx * e.c.ny, s1, s2, 0.0, nscale);        \
+        break;
+        PS3D_FFT_SIZES(CASE)
+#undef CASE
+    default: break;
+    }
+}
+
+template <int NMAX>
+__global__ __launch_bounds__(TTHREADS) void k_ens_zsolve_f(const EnsMember *__restrict__ mem, EnsMap map)
+{
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const int lb = g - map.start[mi];
+    const int nbx = (e.nkx + TCOL - 1) / TCOL;
+    const int bx = lb % nbx, by = lb / nbx;
+    float2 *lds = reinterpret_cast<float2 *>(ens_lds);
+    switch (e.c.nz) {
+#define CASE(n)                                                                                              \
+    case n:                                                                                                  \
+        if constexpr (n <= NMAX && TCOLS(n) == TTHREADS)                                                     \
+            zsolve_body<n, TCOL, TTHREADS, float2>(lds, bx, by, e.twzf, e.FSf, e.ksqr, e.nkx, e.nkx, e.c.ny,  \
+                                                   e.c.dx, e.c.dy, e.c.dz);                                  \
+        break;
+        PS3D_FFT_SIZES(CASE)
+#undef CASE
+    default: break;
+    }
+}
+
 }  // namespace ft
 
 // rows per block of an x-pass so that every radix's butterflies fit in
@@ -1283,6 +1382,66 @@ void ens_zsolve(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nmax
     else if (nmax <= 128) zsolve_launch<128>(s, mem, map, lds);
     else if (nmax <= 256) zsolve_launch<256>(s, mem, map, lds);
     else zsolve_launch<512>(s, mem, map, lds);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_r2c_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, size_t lds)
+{
+    if (!map.start[map.n]) return;
+    ens_lds_attr(ft::k_ens_r2c_f, lds);
+    hipLaunchKernelGGL(ft::k_ens_r2c_f, dim3((unsigned)map.start[map.n]), dim3(ft::TTHREADS), lds, s, mem, map, st);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_c2r_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, int which, size_t lds)
+{
+    if (!map.start[map.n]) return;
+    ens_lds_attr(ft::k_ens_c2r_f, lds);
+    hipLaunchKernelGGL(ft::k_ens_c2r_f, dim3((unsigned)map.start[map.n]), dim3(ft::TTHREADS), lds, s, mem, map, st,
+                       which);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+template <int SIGN, int NMAX>
+static void cols_y_launch_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, size_t lds)
+{
+    ens_lds_attr(ft::k_ens_cols_y_f<SIGN, NMAX>, lds);
+    hipLaunchKernelGGL((ft::k_ens_cols_y_f<SIGN, NMAX>), dim3((unsigned)map.start[map.n]), dim3(ft::TTHREADS), lds,
+                       s, mem, map);
+}
+
+template <int SIGN>
+static void cols_y_nmax_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nmax, size_t lds)
+{
+    if (nmax <= 64) cols_y_launch_f<SIGN, 64>(s, mem, map, lds);
+    else if (nmax <= 128) cols_y_launch_f<SIGN, 128>(s, mem, map, lds);
+    else if (nmax <= 256) cols_y_launch_f<SIGN, 256>(s, mem, map, lds);
+    else cols_y_launch_f<SIGN, 512>(s, mem, map, lds);
+}
+
+void ens_cols_y_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int sign, int nmax, size_t lds)
+{
+    if (!map.start[map.n]) return;
+    if (sign < 0) cols_y_nmax_f<-1>(s, mem, map, nmax, lds);
+    else cols_y_nmax_f<+1>(s, mem, map, nmax, lds);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+template <int NMAX>
+static void zsolve_launch_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, size_t lds)
+{
+    ens_lds_attr(ft::k_ens_zsolve_f<NMAX>, lds);
+    hipLaunchKernelGGL(ft::k_ens_zsolve_f<NMAX>, dim3((unsigned)map.start[map.n]), dim3(ft::TTHREADS), lds, s, mem,
+                       map);
+}
+
+void ens_zsolve_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nmax, size_t lds)
+{
+    if (!map.start[map.n]) return;
+    if (nmax <= 64) zsolve_launch_f<64>(s, mem, map, lds);
+    else if (nmax <= 128) zsolve_launch_f<128>(s, mem, map, lds);
+    else if (nmax <= 256) zsolve_launch_f<256>(s, mem, map, lds);
+    else zsolve_launch_f<512>(s, mem, map, lds);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 

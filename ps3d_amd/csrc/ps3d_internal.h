@@ -383,6 +383,15 @@ struct EnsMember {
     int ab3, n2, nkx;
     int tend_kc, tend_nchunk, tend_tx, tend_ty;   // tile shapes (tunable)
     int fric_kc, fric_nchunk, fric_tx, fric_ty;
+    // mixed form (PS3D_MIXED members): (hi, lo) of u, v, w, b in two
+    // buffers, the FP32 tendency slots, the FP64 pressure, the ring of
+    // pressure increments, the FP32 spectral array and FFT tables
+    float *fh[4][2], *fl[4][2];
+    float *tf[4];
+    double *pint;
+    float *dpr[3];
+    float2 *FSf;
+    const float2 *twxf, *twyf, *twzf, *wkf;
 };
 
 // member of block g: the last member whose first block is <= g (members
@@ -401,6 +410,7 @@ __device__ __forceinline__ int ens_find(const EnsMap &m, int g)
 struct EnsState {
     int itt, tau, taum1, taum2;
     int par;                     // which buffer of f[][] holds the state
+    int dp_cur;                  // mixed form: dpr index of the latest increment
 };
 
 // stage launchers (one launch each over all members)
@@ -417,6 +427,24 @@ void ens_c2r(hipStream_t s, const EnsMember *mem, const EnsMap &map, size_t lds)
 // of a rigid-lid member (sizes the LDS of the on-the-fly form)
 int ens_thomas_cols();
 void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max);
+// the same stages for members of the mixed form.  cold: the provisional
+// Euler step of the cold start (tendency without the reference pressure,
+// friction + Euler); which: the increment a solve writes and a projection
+// applies (0: dpr[0], the cold start's estimate; 1: the new increment
+// dpr[dp_cur + 1]; 2: the correction dpr[dp_cur + 2])
+void ens_tendency_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, int cold);
+void ens_friction_ab_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, int cold);
+void ens_project_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, int which);
+// pint = the estimate dpr[0] (images included); then the three increments zeroed
+void ens_cold_pint(hipStream_t s, const EnsMember *mem, const EnsMap &map);
+// per-point maps over the H points of each member: zero the ring (what = 0)
+// or add the correction to the new increment (what = 1)
+void ens_points_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, int what);
+void ens_r2c_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, size_t lds);
+void ens_cols_y_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int sign, int nmax, size_t lds);
+void ens_zsolve_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nmax, size_t lds);
+void ens_c2r_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, int which, size_t lds);
+void ens_thomas_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max);
 // block counts and LDS bytes of the batched FFT stages for one member
 int ens_r2c_blocks(const EnsMember &e);
 int ens_cols_blocks(const EnsMember &e, int n_len, int lines);

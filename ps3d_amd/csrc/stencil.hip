@@ -932,17 +932,12 @@ __global__ __launch_bounds__(BX * BY) void k_project(StepConsts c, double *__res
 // projection of the mixed form with the FP32 pressure increment dp: the
 // state update accumulated in FP64 on (hi + lo) and split back, and the FP64
 // pressure advanced, pint += dp
-__global__ __launch_bounds__(BX * BY) void k_project_f(StepConsts c, float *__restrict__ uh, float *__restrict__ ul,
-                                                        float *__restrict__ vh, float *__restrict__ vl,
-                                                        float *__restrict__ wh, float *__restrict__ wl,
-                                                        float *__restrict__ dp, double *__restrict__ pint, int halo)
+__device__ __forceinline__ void project_f_point(const StepConsts &c, float *__restrict__ uh, float *__restrict__ ul,
+                                                float *__restrict__ vh, float *__restrict__ vl,
+                                                float *__restrict__ wh, float *__restrict__ wl,
+                                                float *__restrict__ dp, double *__restrict__ pint, int i, int j,
+                                                int k, int halo)
 {
-    const int gx = (int)gridDim.x, gy = (int)gridDim.y;
-    const int r = xcd_remap((int)blockIdx.x + gx * ((int)blockIdx.y + gy * (int)blockIdx.z), gx * gy * (int)gridDim.z);
-    const int i = (r % gx) * BX + threadIdx.x + 1;
-    const int j = ((r / gx) % gy) * BY + threadIdx.y + 1;
-    const int k = r / (gx * gy) + 1;
-    if (i > c.nx || j > c.ny) return;
     const int64_t o = idx(c, i, j, k);
     const double pc = dp[o];
     const double pe = dp[idx(c, i == c.nx ? 1 : i + 1, j, k)];
@@ -966,6 +961,20 @@ __global__ __launch_bounds__(BX * BY) void k_project_f(StepConsts c, float *__re
         write_images<float>(c, dp, i, j, k, (float)pc, c.lid);
         write_images<double>(c, pint, i, j, k, pn1, c.lid);
     }
+}
+
+__global__ __launch_bounds__(BX * BY) void k_project_f(StepConsts c, float *__restrict__ uh, float *__restrict__ ul,
+                                                        float *__restrict__ vh, float *__restrict__ vl,
+                                                        float *__restrict__ wh, float *__restrict__ wl,
+                                                        float *__restrict__ dp, double *__restrict__ pint, int halo)
+{
+    const int gx = (int)gridDim.x, gy = (int)gridDim.y;
+    const int r = xcd_remap((int)blockIdx.x + gx * ((int)blockIdx.y + gy * (int)blockIdx.z), gx * gy * (int)gridDim.z);
+    const int i = (r % gx) * BX + threadIdx.x + 1;
+    const int j = ((r / gx) % gy) * BY + threadIdx.y + 1;
+    const int k = r / (gx * gy) + 1;
+    if (i > c.nx || j > c.ny) return;
+    project_f_point(c, uh, ul, vh, vl, wh, wl, dp, pint, i, j, k, halo);
 }
 
 // host-layout conversions of the mixed form: 0 split FP64 -> (hi, lo),
@@ -1094,7 +1103,149 @@ __global__ void k_ens_advance(EnsState *st)
         s.taum2 = s.taum2 % 3 + 1;
         s.itt += 1;
         s.par ^= 1;
+        s.dp_cur = (s.dp_cur + 1) % 3;
         *st = s;
+    }
+}
+
+// ---- the same stages for members of the mixed form -------------------------
+// (the bodies and arguments of tendency_impl<float>, friction_ab_impl<float>
+// and k_project_f, with the member's buffers of the current parity)
+__global__ __launch_bounds__(TTHR) void k_ens_tendency_f(const EnsMember *__restrict__ mem, EnsMap map,
+                                                          const EnsState *__restrict__ st, int cold)
+{
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const StepConsts c = e.c;
+    const int par = st->par;
+    const int64_t off = (int64_t)(st->tau - 1) * e.H;
+    TendArgs<float> a{};
+    a.u = e.fh[0][par]; a.v = e.fh[1][par]; a.w = e.fh[2][par]; a.b = e.fh[3][par];
+    a.ul = e.fl[0][par]; a.vl = e.fl[1][par]; a.bl = e.fl[3][par];
+    a.pint = cold ? nullptr : e.pint;
+    a.du = e.tf[0] + off; a.dv = e.tf[1] + off; a.dw = e.tf[2] + off; a.db = e.tf[3] + off;
+    a.kc = e.tend_kc; a.nchunk = e.tend_nchunk; a.tiles_x = e.tend_tx; a.tiles_y = e.tend_ty;
+    tendency_body<float>(c, a, g - map.start[mi]);
+}
+
+__global__ __launch_bounds__(FTHREADS) void k_ens_fric_f(const EnsMember *__restrict__ mem, EnsMap map,
+                                                          const EnsState *__restrict__ st, int cold)
+{
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const StepConsts c = e.c;
+    const int par = st->par, itt = cold ? 0 : st->itt, cur = st->dp_cur;
+    const int lb = g - map.start[mi];
+    FricArgs<float> a{};
+    a.kc = e.fric_kc; a.nchunk = e.fric_nchunk; a.tiles_x = e.fric_tx; a.tiles_y = e.fric_ty;
+    double qc1 = 0.0, qc2 = 0.0;
+    if (itt == 0) {
+        a.order = 1;
+    } else if (itt == 1 || !e.ab3) {
+        a.order = 2;
+        a.ca = 1.5 + e.eps_ab;
+        a.cb = 0.5 + e.eps_ab;
+        qc1 = -(0.5 + e.eps_ab);
+    } else {
+        a.order = 3;
+        a.ca = 23.0 / 12.0;
+        a.cb = -16.0 / 12.0;
+        a.cc = 5.0 / 12.0;
+        qc1 = -16.0 / 12.0 + 5.0 / 12.0;
+        qc2 = 5.0 / 12.0;
+    }
+    // Q = qc1 dp_{n-1} + qc2 dp_{n-2}: the pressure-history term of the step
+    a.qc1 = qc1;
+    a.qc2 = qc2;
+    a.d1 = qc1 != 0.0 ? e.dpr[cur] : nullptr;
+    a.d2 = qc2 != 0.0 ? e.dpr[(cur + 2) % 3] : nullptr;
+    const int f = fric_field(a, lb);
+    const int q = f < 3 ? 0 : 1;
+    FricField<float> F{};
+    F.in = e.fh[f][par];
+    F.in_lo = e.fl[f][par];
+    F.out = e.fh[f][par ^ 1];
+    F.out_lo = e.fl[f][par ^ 1];
+    F.t0 = e.tf[f] + (int64_t)(st->tau - 1) * e.H;
+    F.t1 = e.tf[f] + (int64_t)(st->taum1 - 1) * e.H;
+    F.t2 = e.tf[f] + (int64_t)(st->taum2 - 1) * e.H;
+    F.scale = (f == 2) ? c.dt / c.dsqr : c.dt;
+    F.harm = e.harm[q];
+    F.Hm = e.Hm[q];
+    F.Vm = e.Vm[q];
+    F.bih = e.bih[q];
+    F.Hbi = e.Hbi[q];
+    F.zero_top = (f == 2 && F.harm && c.lid) ? 1 : 0;
+    F.gstride = f == 0 ? 1 : (f == 1 ? c.X : (f == 2 ? c.XY : 0));
+    F.grd = f == 0 ? c.rdx : (f == 1 ? c.rdy : c.rdz);
+    a.halo = 1;
+    fric_body<float>(c, a, F, lb);
+}
+
+__device__ __forceinline__ float *ens_increment(const EnsMember &e, int cur, int which)
+{
+    return which == 0 ? e.dpr[0] : e.dpr[(cur + which) % 3];
+}
+
+__global__ __launch_bounds__(BX * BY) void k_ens_project_f(const EnsMember *__restrict__ mem, EnsMap map,
+                                                            const EnsState *__restrict__ st, int which)
+{
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const StepConsts c = e.c;
+    const int lb = g - map.start[mi];
+    const int bx = (c.nx + BX - 1) / BX, by = (c.ny + BY - 1) / BY;
+    const int i = (lb % bx) * BX + threadIdx.x + 1;
+    const int j = ((lb / bx) % by) * BY + threadIdx.y + 1;
+    const int k = lb / (bx * by) + 1;
+    if (i > c.nx || j > c.ny) return;
+    const int nb = st->par ^ 1;
+    project_f_point(c, e.fh[0][nb], e.fl[0][nb], e.fh[1][nb], e.fl[1][nb], e.fh[2][nb], e.fl[2][nb],
+                    ens_increment(e, st->dp_cur, which), e.pint, i, j, k, 1);
+}
+
+// cold start: pint = the FP32 estimate in dpr[0], with its halo images
+// (Neumann in z under a lid), as the halo exchange of the estimate followed
+// by the conversion to FP64
+__global__ __launch_bounds__(BX * BY) void k_ens_cold_pint(const EnsMember *__restrict__ mem, EnsMap map)
+{
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const StepConsts c = e.c;
+    const int lb = g - map.start[mi];
+    const int bx = (c.nx + BX - 1) / BX, by = (c.ny + BY - 1) / BY;
+    const int i = (lb % bx) * BX + threadIdx.x + 1;
+    const int j = ((lb / bx) % by) * BY + threadIdx.y + 1;
+    const int k = lb / (bx * by) + 1;
+    if (i > c.nx || j > c.ny) return;
+    const int64_t o = idx(c, i, j, k);
+    const double x = (double)e.dpr[0][o];
+    e.pint[o] = x;
+    write_images<double>(c, e.pint, i, j, k, x, c.lid);
+}
+
+// over the H points of each member: what 0 zeroes the three increments,
+// what 1 adds the correction to the new increment
+__global__ __launch_bounds__(256) void k_ens_points_f(const EnsMember *__restrict__ mem, EnsMap map,
+                                                       const EnsState *__restrict__ st, int what)
+{
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const int64_t t = (int64_t)(g - map.start[mi]) * 256 + threadIdx.x;
+    if (t >= e.H) return;
+    if (what == 0) {
+        e.dpr[0][t] = 0.f;
+        e.dpr[1][t] = 0.f;
+        e.dpr[2][t] = 0.f;
+    } else {
+        const int cur = st->dp_cur;
+        float *dn = e.dpr[(cur + 1) % 3];
+        dn[t] = dn[t] + e.dpr[(cur + 2) % 3][t];
     }
 }
 
@@ -1620,6 +1771,41 @@ void ens_project(hipStream_t s, const EnsMember *mem, const EnsMap &map, const E
 void ens_advance(hipStream_t s, EnsState *st)
 {
     hipLaunchKernelGGL(k_ens_advance, dim3(1), dim3(64), 0, s, st);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_tendency_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, int cold)
+{
+    if (!map.start[map.n]) return;
+    hipLaunchKernelGGL(k_ens_tendency_f, dim3((unsigned)map.start[map.n]), dim3(TTHR), 0, s, mem, map, st, cold);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_friction_ab_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, int cold)
+{
+    if (!map.start[map.n]) return;
+    hipLaunchKernelGGL(k_ens_fric_f, dim3((unsigned)map.start[map.n]), dim3(FTHREADS), 0, s, mem, map, st, cold);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_project_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, int which)
+{
+    if (!map.start[map.n]) return;
+    hipLaunchKernelGGL(k_ens_project_f, dim3((unsigned)map.start[map.n]), dim3(BX, BY), 0, s, mem, map, st, which);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_cold_pint(hipStream_t s, const EnsMember *mem, const EnsMap &map)
+{
+    if (!map.start[map.n]) return;
+    hipLaunchKernelGGL(k_ens_cold_pint, dim3((unsigned)map.start[map.n]), dim3(BX, BY), 0, s, mem, map);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_points_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, int what)
+{
+    if (!map.start[map.n]) return;
+    hipLaunchKernelGGL(k_ens_points_f, dim3((unsigned)map.start[map.n]), dim3(256), 0, s, mem, map, st, what);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 

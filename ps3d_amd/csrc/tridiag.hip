@@ -264,6 +264,24 @@ __global__ __launch_bounds__(64) void k_ens_thomas(const EnsMember *__restrict__
     }
 }
 
+// the Thomas stage of a batched ensemble of mixed-form members: FP32
+// right-hand side and solution, FP64 elimination on the fly (the body of
+// k_thomas_fly<float2>)
+__global__ __launch_bounds__(64) void k_ens_thomas_f(const EnsMember *__restrict__ mem, EnsMap map, int nz_max)
+{
+    extern __shared__ double ens_thomas_lds_f[];   // ENS_TC x nz_max double2, then ENS_TC x nz_max double
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    if ((int)threadIdx.x >= ENS_TC) return;
+    const int64_t ncols = (int64_t)e.nkx * e.c.ny;
+    const int64_t col = (int64_t)(g - map.start[mi]) * ENS_TC + threadIdx.x;
+    if (col >= ncols) return;
+    double2 *dpl = reinterpret_cast<double2 *>(ens_thomas_lds_f) + threadIdx.x;
+    double *cpl = ens_thomas_lds_f + 2 * (int64_t)ENS_TC * nz_max + threadIdx.x;
+    thomas_col_fly<float2>(e.FSf, e.ksqr, ncols, e.c.nz, e.fxa, e.c.dz * e.c.dz, e.c.dsqr, col, dpl, cpl, ENS_TC);
+}
+
 // periodic spectral step without the z transforms (building-block API)
 __global__ __launch_bounds__(256) void k_divide_ksqr(double2 *__restrict__ fs, const double *__restrict__ ksqr,
                                                      int64_t S)
@@ -415,6 +433,15 @@ void launch_thomas_f(Model &m, float2 *data)
 }
 
 int ens_thomas_cols() { return ENS_TC; }
+
+void ens_thomas_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max)
+{
+    if (!map.start[map.n]) return;
+    const size_t lds = (sizeof(double2) + sizeof(double)) * ENS_TC * (size_t)nz_max;
+    PS3D_REQUIRE(lds <= 64 * 1024, "mixed ensemble: rigid-lid members of at most 85 levels");
+    hipLaunchKernelGGL(k_ens_thomas_f, dim3((unsigned)map.start[map.n]), dim3(64), lds, s, mem, map, nz_max);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
 
 void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max)
 {

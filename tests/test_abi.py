@@ -133,7 +133,8 @@ def test_fortran_dropin_links_against_reference_module(tmp_path):
 @pytest.mark.parametrize("bad,msg", [
     (dict(Avbi=1e-2), "vertical biharmonic"),
     (dict(nx=40), "compiled-in FFT lengths"),
-    (dict(precision=1), "FP64"),
+    (dict(precision=1), "share one precision form"),
+    (dict(precision=7), "unknown precision form"),
 ])
 def test_ensemble_create_rejects_bad_members(bad, msg):
     from ps3d_amd import _lib
