@@ -730,7 +730,10 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
         const float q1o = cur.q1o, q1n = cur.q1n, q2o = cur.q2o, q2n = cur.q2n;
         __syncthreads();
         const int sm = slot(k - 1), sc = slot(k), sp = slot(k + 1);
-        for (int q = t; q < R1X * R1Y; q += FTHREADS) {
+#pragma unroll
+        for (int sq = 0; sq < (R1X * R1Y + FTHREADS - 1) / FTHREADS; ++sq) {
+            const int q = t + sq * FTHREADS;
+            if (q >= R1X * R1Y) break;
             const int rx = q % R1X, ry = q / R1X;
             const int ii = i0 - 2 + rx, jj = j0 - 2 + ry;
             const int ox = rx + 1, oy = ry + 1;
@@ -755,7 +758,10 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
         if (F.bih) {
             // del2 = (f1e(i) - f1e(i-1))/dx + (f1n(j) - f1n(j-1))/dy, each point once
             const T Hb = (T)F.Hbi;
-            for (int q = t; q < (FTX + 2) * (FTY + 2); q += FTHREADS) {
+#pragma unroll
+            for (int sq = 0; sq < ((FTX + 2) * (FTY + 2) + FTHREADS - 1) / FTHREADS; ++sq) {
+                const int q = t + sq * FTHREADS;
+                if (q >= (FTX + 2) * (FTY + 2)) break;
                 const int xx = q % (FTX + 2) + 1, yy = q / (FTX + 2) + 1;   // f1p coordinates
                 const T f1e1 = -sdiv<T>(Hb * (f1p[yy][xx + 1] - f1p[yy][xx]), dx, rdx);
                 const T f1e0 = -sdiv<T>(Hb * (f1p[yy][xx] - f1p[yy][xx - 1]), dx, rdx);
