@@ -758,12 +758,15 @@ __global__ __launch_bounds__(T) void z_periodic_solve(const C *__restrict__ tw, 
     zsolve_body<N, TX, T, C>(lds, r % gx, r / gx, tw, data, ksqr, nkx, nkxs, ny, dx, dy, dz);
 }
 
-// launch configurations: x-pass rows so that ROWS*N2 = 1024 values, y/z
-// passes 8 columns; 256 threads (more for the longest columns)
+// launch configurations: x-pass rows so that ROWS*N2 = 1024 values; 256
+// threads (more for the longest columns of the y/z passes)
 constexpr int XROWS(int n2) { return n2 >= 1024 ? 1 : 1024 / n2; }
-constexpr int TCOL = 8;
+// y/z passes: the columns of one work-group span one 64-byte segment of each
+// row (4 FP64 or 8 FP32 complex values), which gives the most work-groups
+// per pass at full-segment accesses
+template <class C> constexpr int TCOL = 64 / (int)sizeof(C);
 constexpr int TTHREADS = 256;
-constexpr int TCOLS(int n) { return n * TCOL / 16 > 256 ? n * TCOL / 16 : 256; }
+template <class C> constexpr int TCOLS(int n) { return n * TCOL<C> / 16 > 256 ? n * TCOL<C> / 16 : 256; }
 
 template <int N2, class RT>
 bool launch_r2c(Model &m, const RT *src, const RT *u, const RT *v, const RT *w, int fuse_div,
@@ -797,13 +800,13 @@ template <int N, class C>
 bool launch_cols(const C *tw, hipStream_t st, C *data, int nkx, int64_t estride, int64_t lstride,
                  unsigned ny_or_nz, int sign, double s1, double s2, double s3, int nscale)
 {
-    const dim3 grid((unsigned)((nkx + TCOL - 1) / TCOL), ny_or_nz);
-    constexpr int T = TCOLS(N);
+    const dim3 grid((unsigned)((nkx + TCOL<C> - 1) / TCOL<C>), ny_or_nz);
+    constexpr int T = TCOLS<C>(N);
     if (sign < 0)
-        hipLaunchKernelGGL((c2c_cols<N, TCOL, T, -1, C>), grid, dim3(T), 0, st, tw, data, nkx, estride, lstride,
+        hipLaunchKernelGGL((c2c_cols<N, TCOL<C>, T, -1, C>), grid, dim3(T), 0, st, tw, data, nkx, estride, lstride,
                            s1, s2, s3, nscale);
     else
-        hipLaunchKernelGGL((c2c_cols<N, TCOL, T, +1, C>), grid, dim3(T), 0, st, tw, data, nkx, estride, lstride,
+        hipLaunchKernelGGL((c2c_cols<N, TCOL<C>, T, +1, C>), grid, dim3(T), 0, st, tw, data, nkx, estride, lstride,
                            s1, s2, s3, nscale);
     return true;
 }
@@ -811,9 +814,9 @@ bool launch_cols(const C *tw, hipStream_t st, C *data, int nkx, int64_t estride,
 template <int N, class C>
 bool launch_zsolve(Model &m, C *data, const C *tw, int nkxs)
 {
-    const dim3 grid((unsigned)((m.g.nkx + TCOL - 1) / TCOL), (unsigned)m.g.ny);
-    constexpr int T = TCOLS(N);
-    hipLaunchKernelGGL((z_periodic_solve<N, TCOL, T, C>), grid, dim3(T), 0, m.stream, tw, data, m.ksqr, m.g.nkx,
+    const dim3 grid((unsigned)((m.g.nkx + TCOL<C> - 1) / TCOL<C>), (unsigned)m.g.ny);
+    constexpr int T = TCOLS<C>(N);
+    hipLaunchKernelGGL((z_periodic_solve<N, TCOL<C>, T, C>), grid, dim3(T), 0, m.stream, tw, data, m.ksqr, m.g.nkx,
                        nkxs, m.g.ny, m.prm.dx, m.prm.dy, m.prm.dz);
     return true;
 }
@@ -930,15 +933,15 @@ __global__ __launch_bounds__(TTHREADS) void k_ens_cols_y(const EnsMember *__rest
     const int mi = ens_find(map, g);
     const EnsMember &e = mem[mi];
     const int lb = g - map.start[mi];
-    const int nbx = (e.nkx + TCOL - 1) / TCOL;
+    const int nbx = (e.nkx + TCOL<double2> - 1) / TCOL<double2>;
     const int bx = lb % nbx, by = lb / nbx;
     const int nscale = (SIGN < 0 && e.c.lid) ? 2 : 0;
     const double s1 = nscale ? e.c.dx : 0.0, s2 = nscale ? e.c.dy : 0.0;
     switch (e.c.ny) {
 #define CASE(n)                                                                                              \
     case n:                                                                                                  \
-        if constexpr (n <= NMAX && TCOLS(n) == TTHREADS)                                                     \
-            cols_body<n, TCOL, TTHREADS, SIGN, double2>(ens_lds, bx, by, e.twy, e.FS, e.nkx, (int64_t)e.nkx,  \
+        if constexpr (n <= NMAX && TCOLS<double2>(n) == TTHREADS)                                            \
+            cols_body<n, TCOL<double2>, TTHREADS, SIGN, double2>(ens_lds, bx, by, e.twy, e.FS, e.nkx, (int64_t)e.nkx,  \
                                                         (int64_t)e.nkx * e.c.ny, s1, s2, 0.0, nscale);       \
         break;
         PS3D_FFT_SIZES(CASE)
@@ -954,13 +957,13 @@ __global__ __launch_bounds__(TTHREADS) void k_ens_zsolve(const EnsMember *__rest
     const int mi = ens_find(map, g);
     const EnsMember &e = mem[mi];
     const int lb = g - map.start[mi];
-    const int nbx = (e.nkx + TCOL - 1) / TCOL;
+    const int nbx = (e.nkx + TCOL<double2> - 1) / TCOL<double2>;
     const int bx = lb % nbx, by = lb / nbx;
     switch (e.c.nz) {
 #define CASE(n)                                                                                              \
     case n:                                                                                                  \
-        if constexpr (n <= NMAX && TCOLS(n) == TTHREADS)                                                     \
-            zsolve_body<n, TCOL, TTHREADS, double2>(ens_lds, bx, by, e.twz, e.FS, e.ksqr, e.nkx, e.nkx,       \
+        if constexpr (n <= NMAX && TCOLS<double2>(n) == TTHREADS)                                            \
+            zsolve_body<n, TCOL<double2>, TTHREADS, double2>(ens_lds, bx, by, e.twz, e.FS, e.ksqr, e.nkx, e.nkx,       \
                                                     e.c.ny, e.c.dx, e.c.dy, e.c.dz);                         \
         break;
         PS3D_FFT_SIZES(CASE)
@@ -1027,7 +1030,7 @@ __global__ __launch_bounds__(TTHREADS) void k_ens_cols_y_f(const EnsMember *__re
     const int mi = ens_find(map, g);
     const EnsMember &e = mem[mi];
     const int lb = g - map.start[mi];
-    const int nbx = (e.nkx + TCOL - 1) / TCOL;
+    const int nbx = (e.nkx + TCOL<float2> - 1) / TCOL<float2>;
     const int bx = lb % nbx, by = lb / nbx;
     const int nscale = (SIGN < 0 && e.c.lid) ? 2 : 0;
     const double s1 = nscale ? e.c.dx : 0.0, s2 = nscale ? e.c.dy : 0.0;
@@ -1035,8 +1038,8 @@ __global__ __launch_bounds__(TTHREADS) void k_ens_cols_y_f(const EnsMember *__re
     switch (e.c.ny) {
 #define CASE(n)                                                                                              \
     case n:                                                                                                  \
-        if constexpr (n <= NMAX && TCOLS(n) == TTHREADS)                                                     \
-            cols_body<n, TCOL, TTHREADS, SIGN, float2>(lds, bx, by, e.twyf, e.FSf, e.nkx, (int64_t)e.nkx,     \
+        if constexpr (n <= NMAX && TCOLS<float2>(n) == TTHREADS)                                             \
+            cols_body<n, TCOL<float2>, TTHREADS, SIGN, float2>(lds, bx, by, e.twyf, e.FSf, e.nkx, (int64_t)e.nkx,     \
                                                        (int64_t)e.nkx * e.c.ny, s1, s2, 0.0, nscale);        \
         break;
         PS3D_FFT_SIZES(CASE)
@@ -1052,14 +1055,14 @@ __global__ __launch_bounds__(TTHREADS) void k_ens_zsolve_f(const EnsMember *__re
     const int mi = ens_find(map, g);
     const EnsMember &e = mem[mi];
     const int lb = g - map.start[mi];
-    const int nbx = (e.nkx + TCOL - 1) / TCOL;
+    const int nbx = (e.nkx + TCOL<float2> - 1) / TCOL<float2>;
     const int bx = lb % nbx, by = lb / nbx;
     float2 *lds = reinterpret_cast<float2 *>(ens_lds);
     switch (e.c.nz) {
 #define CASE(n)                                                                                              \
     case n:                                                                                                  \
-        if constexpr (n <= NMAX && TCOLS(n) == TTHREADS)                                                     \
-            zsolve_body<n, TCOL, TTHREADS, float2>(lds, bx, by, e.twzf, e.FSf, e.ksqr, e.nkx, e.nkx, e.c.ny,  \
+        if constexpr (n <= NMAX && TCOLS<float2>(n) == TTHREADS)                                             \
+            zsolve_body<n, TCOL<float2>, TTHREADS, float2>(lds, bx, by, e.twzf, e.FSf, e.ksqr, e.nkx, e.nkx, e.c.ny,  \
                                                    e.c.dx, e.c.dy, e.c.dz);                                  \
         break;
         PS3D_FFT_SIZES(CASE)
@@ -1298,7 +1301,7 @@ std::string fft_sizes_text()
 bool ens_fft_size_ok(int n, bool column_pass)
 {
     switch (n) {
-#define CASE(n) case n: return !column_pass || ft::TCOLS(n) == ft::TTHREADS;
+#define CASE(n) case n: return !column_pass || (ft::TCOLS<double2>(n) == ft::TTHREADS && ft::TCOLS<float2>(n) == ft::TTHREADS);
         PS3D_FFT_SIZES(CASE)
 #undef CASE
     default: return false;
@@ -1307,7 +1310,8 @@ bool ens_fft_size_ok(int n, bool column_pass)
 
 size_t ens_x_lds(int n2) { return (size_t)ft::XROWS(n2) * (n2 + 1) * sizeof(double2); }
 
-size_t ens_cols_lds(int n) { return (size_t)n * ft::TCOL * sizeof(double2); }
+// the same for both forms: TCOL<C> * sizeof(C) = 64 bytes per row
+size_t ens_cols_lds(int n) { return (size_t)n * 64; }
 
 int ens_r2c_blocks(const EnsMember &e)
 {
@@ -1317,7 +1321,8 @@ int ens_r2c_blocks(const EnsMember &e)
 
 int ens_cols_blocks(const EnsMember &e, int /*n_len*/, int lines)
 {
-    return ((e.nkx + ft::TCOL - 1) / ft::TCOL) * lines;
+    const int tc = e.FSf ? ft::TCOL<float2> : ft::TCOL<double2>;
+    return ((e.nkx + tc - 1) / tc) * lines;
 }
 
 template <class K>
