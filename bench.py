@@ -229,6 +229,8 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--ens-steps", type=int, default=100, help="timed steps of the ensemble leg")
+    ap.add_argument("--mixed-check", default="1,100,1000",
+                    help="steps at which the mixed leg reports its error against FP64 (comma list)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (to calibrate HBM byte counters)")
     args = ap.parse_args()
@@ -308,7 +310,8 @@ def main() -> int:
     out["tuning"] = model.tuning().strip().splitlines()
     model.close()
     if not args.no_mixed:
-        out["mixed"] = mixed_leg(prm, fields, args.steps, args.warmup, value)
+        checks = tuple(int(x) for x in args.mixed_check.split(",") if x.strip())
+        out["mixed"] = mixed_leg(prm, fields, args.steps, args.warmup, value, check_steps=checks)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ens_steps)
         if not args.no_mixed:
