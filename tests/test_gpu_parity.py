@@ -183,8 +183,10 @@ def test_integrate_100_steps(case):
     gpu.close()
 
 
-def test_eddy_100_steps_c1_shape():
-    """64x64x32 baroclinic eddy (rigid lid, AB3, biharmonic friction)."""
+def test_eddy_200_steps_c1_shape():
+    """64x64x32 baroclinic eddy (rigid lid, AB3, biharmonic friction), the
+    200 steps of that configuration: bit for bit, and within the drop-in
+    bound."""
     import oracle
     from ps3d_amd import configs
     from ps3d_amd.model import Model
@@ -195,8 +197,8 @@ def test_eddy_100_steps_c1_shape():
         gpu.f[n][...] = a
         orc.f[n][...] = a
     gpu.push()
-    gpu.run(100)
-    orc.run(100)
+    gpu.run(200)
+    orc.run(200)
     gpu.pull()
     for n in ("u", "v", "w", "b", "p"):
         e = rel_l2(interior(gpu.f[n], prm), interior(orc.f[n], prm))
