@@ -42,6 +42,9 @@ subroutine ps3d_gpu_sync(to_device)
   call ps3d_gpu_check(ps3d_set_field(ps3d_model_handle, PS3D_W, w), 'ps3d_set_field w')
   call ps3d_gpu_check(ps3d_set_field(ps3d_model_handle, PS3D_B, b), 'ps3d_set_field b')
   call ps3d_gpu_check(ps3d_set_field(ps3d_model_handle, PS3D_P, p), 'ps3d_set_field p')
+  ! a mixed model's tendency slots hold its own (pressure-referenced FP32)
+  ! quantities and are not exchanged with the host
+  if (ps3d_use_mixed) return
   call ps3d_gpu_check(ps3d_set_field(ps3d_model_handle, PS3D_DU, du), 'ps3d_set_field du')
   call ps3d_gpu_check(ps3d_set_field(ps3d_model_handle, PS3D_DV, dv), 'ps3d_set_field dv')
   call ps3d_gpu_check(ps3d_set_field(ps3d_model_handle, PS3D_DW, dw), 'ps3d_set_field dw')
@@ -52,6 +55,7 @@ subroutine ps3d_gpu_sync(to_device)
   call ps3d_gpu_check(ps3d_get_field(ps3d_model_handle, PS3D_W, w), 'ps3d_get_field w')
   call ps3d_gpu_check(ps3d_get_field(ps3d_model_handle, PS3D_B, b), 'ps3d_get_field b')
   call ps3d_gpu_check(ps3d_get_field(ps3d_model_handle, PS3D_P, p), 'ps3d_get_field p')
+  if (ps3d_use_mixed) return
   call ps3d_gpu_check(ps3d_get_field(ps3d_model_handle, PS3D_DU, du), 'ps3d_get_field du')
   call ps3d_gpu_check(ps3d_get_field(ps3d_model_handle, PS3D_DV, dv), 'ps3d_get_field dv')
   call ps3d_gpu_check(ps3d_get_field(ps3d_model_handle, PS3D_DW, dw), 'ps3d_get_field dw')
@@ -78,6 +82,8 @@ subroutine ps3d_gpu_init
  prm%precision = merge(PS3D_MIXED, PS3D_FP64, ps3d_use_mixed)
  prm%reserved = 0
  if (n_pes /= 1) call halt_stop('ps3d: the GPU step runs the whole domain on one rank')
+ if (ps3d_use_mixed .and. ps3d_forcing_on_host) &
+  call halt_stop('ps3d: set_forcing on host mirrors needs the FP64 form (ps3d_use_mixed = .false.)')
  call ps3d_gpu_check(ps3d_create(prm, ksqr, c_null_ptr, ps3d_model_handle), 'ps3d_create')
  call ps3d_gpu_sync(.true.)
 end subroutine ps3d_gpu_init

@@ -1,0 +1,113 @@
+"""The Fortran drop-in, run as the reference's driver runs it.
+
+oracle/_ref/libps3d_dropin.so holds the reference's own main_module,
+timing_module and parallel_mpi (compiled unchanged), the drop-in
+integrate / pressure_solver of ps3d_amd/fortran/integrate_gpu.f90 with its
+ISO_C_BINDING interfaces, and a driver (oracle/dropin_glue.f90) that sets up
+main_module and steps it like the main loop of main.f90:138-162.  The state
+goes in through main_module's arrays and comes back through them, so this
+checks the whole boundary: module layout, parameters, ksqr, tau rotation,
+the device model behind the Fortran calls and the host refresh.  FP64 must
+equal the CPU oracle bit for bit; the mixed form must equal the mixed model
+driven from Python bit for bit.
+"""
+import ctypes
+import dataclasses
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DROPIN = os.path.join(ROOT, "oracle", "_ref", "libps3d_dropin.so")
+NAMES = ("u", "v", "w", "b", "p")
+REAL = ("Lx", "Ly", "Lz", "dx", "dy", "dz", "dt", "f0", "N0", "Ro", "dsqr", "eps_ab",
+        "Ah", "Av", "Kh", "Kv", "Ahbi", "Avbi", "Khbi", "Kvbi")
+
+
+def _load():
+    if not os.path.exists(DROPIN):
+        pytest.skip("the drop-in driver is built with the reference sources present (oracle/Makefile: dropin)")
+    lib = ctypes.CDLL(DROPIN)
+    lib.dropin_setup.argtypes = [ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]
+    lib.dropin_ptr.argtypes = [ctypes.c_int32]
+    lib.dropin_ptr.restype = ctypes.c_void_p
+    lib.dropin_run.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+    return lib
+
+
+def _view(lib, fid, shape):
+    n = int(np.prod(shape))
+    buf = (ctypes.c_double * n).from_address(lib.dropin_ptr(fid))
+    return np.ndarray(shape, dtype=np.float64, buffer=buf, order="F")
+
+
+def _fortran_run(lib, prm, fields, ksqr, steps, mixed):
+    iprm = (ctypes.c_int32 * 7)(prm.nx, prm.ny, prm.nz, int(prm.enable_nonlinear),
+                                int(prm.enable_vertical_boundaries), int(prm.enable_AB_3_order), int(mixed))
+    rprm = (ctypes.c_double * 20)(*[float(getattr(prm, n)) for n in REAL])
+    lib.dropin_setup(iprm, rprm)
+    shape = (prm.nx + 4, prm.ny + 4, prm.nz + 4)
+    for fid, n in enumerate(NAMES[:4]):
+        _view(lib, fid, shape)[...] = fields[n]
+    _view(lib, 5, ksqr.shape)[...] = ksqr
+    st = (ctypes.c_int32 * 4)()
+    lib.dropin_run(steps, st)
+    out = {n: _view(lib, fid, shape).copy() for fid, n in enumerate(NAMES)}
+    lib.dropin_finish()
+    return out, tuple(st)
+
+
+CASES = {
+    # fused friction + AB, rigid lid
+    "lid_fused": dict(nx=32, ny=24, nz=16, lid=True, Avbi=0.0, Kvbi=0.0),
+    # vertical biharmonic friction: the unfused friction chain, periodic z
+    "periodic_unfused": dict(nx=24, ny=20, nz=10, lid=False),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_fortran_dropin_matches_the_oracle(case):
+    import oracle
+    from ps3d_amd import configs
+    lib = _load()
+    c = dict(CASES[case])
+    nx, ny, nz, lid = c.pop("nx"), c.pop("ny"), c.pop("nz"), c.pop("lid")
+    prm, fields = configs.random_state(nx, ny, nz, seed=31, lid=lid, **c)
+    orc = oracle.OracleModel(prm)
+    for n, a in fields.items():
+        orc.f[n][...] = a
+    steps = 12
+    print(f"fortran drop-in case={case} steps={steps}", flush=True)
+    got, st = _fortran_run(lib, prm, fields, orc.f["ksqr"].copy(), steps, mixed=False)
+    orc.run(steps)
+    assert st == (orc.itt, orc.tau, orc.taum1, orc.taum2)
+    for n in NAMES:
+        assert np.array_equal(got[n], orc.f[n]), (n, float(np.abs(got[n] - orc.f[n]).max()))
+    orc.close()
+
+
+def test_fortran_dropin_mixed_matches_the_python_mixed_model():
+    import oracle
+    from ps3d_amd import configs
+    from ps3d_amd.model import Model
+    from ps3d_amd.params import PS3D_MIXED
+    lib = _load()
+    prm, fields = configs.eddy(64, 64, 32, seed=33)
+    orc = oracle.OracleModel(prm)   # ksqr as the reference's driver would set it
+    ksqr = orc.f["ksqr"].copy()
+    orc.close()
+    steps = 8
+    print(f"fortran drop-in mixed steps={steps}", flush=True)
+    got, _ = _fortran_run(lib, prm, fields, ksqr, steps, mixed=True)
+    m = Model(dataclasses.replace(prm, precision=PS3D_MIXED))
+    for n, a in fields.items():
+        m.f[n][...] = a
+    m.push()
+    m.run(steps)
+    m.pull(list(NAMES))
+    for n in NAMES:
+        assert np.array_equal(got[n], m.f[n]), (n, float(np.abs(got[n] - m.f[n]).max()))
+    m.close()
