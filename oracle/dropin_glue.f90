@@ -14,11 +14,15 @@
 !    (my_fft.f90:5-82) would set, the parameters a config's set_parameter
 !    sets, and the reference's allocate_main_module.
 !  * dropin_run: the main loop, followed by a refresh of the host arrays.
-!  * set_forcing: the config hook, empty as in for_config/eddy1.f90:141-142.
+!  * set_forcing: the config hook; empty as in for_config/eddy1.f90:141-142
+!    unless dropin_set_forcing switched on a forcing on the host mirrors
+!    (db(:,:,:,tau) += a fixed array, whole arrays as lee_wave1.f90:321-324
+!    updates them), which takes the drop-in's host phase path.
 module dropin_glue
  use iso_c_binding
  use main_module
  implicit none
+ real(real_type), allocatable :: dropin_force(:,:,:)
 
 contains
 
@@ -41,6 +45,8 @@ contains
   enable_vertical_boundaries = iprm(5) /= 0
   enable_AB_3_order = iprm(6) /= 0
   ps3d_use_mixed = iprm(7) /= 0
+  ps3d_forcing_on_host = .false.
+  if (allocated(dropin_force)) deallocate(dropin_force)
   Lx = rprm(1); Ly = rprm(2); Lz = rprm(3)
   dx = rprm(4); dy = rprm(5); dz = rprm(6); dt = rprm(7)
   f0 = rprm(8); N0 = rprm(9); Ro = rprm(10); dsqr = rprm(11); eps_ab = rprm(12)
@@ -65,7 +71,15 @@ contains
   taum2 = 1; taum1 = 2; tau = 3
  end subroutine dropin_setup
 
- ! 0..4 u v w b p, 5 ksqr
+ ! a forcing of db on the host mirrors (set its array through dropin_ptr(6))
+ subroutine dropin_set_forcing() bind(C, name="dropin_set_forcing")
+  use ps3d_gpu_state
+  allocate(dropin_force(is_pe-onx:ie_pe+onx, js_pe-onx:je_pe+onx, ks_pe-onx:ke_pe+onx))
+  dropin_force = 0d0
+  ps3d_forcing_on_host = .true.
+ end subroutine dropin_set_forcing
+
+ ! 0..4 u v w b p, 5 ksqr, 6 the forcing array
  function dropin_ptr(id) bind(C, name="dropin_ptr") result(ptr)
   integer(c_int), value :: id
   type(c_ptr) :: ptr
@@ -76,6 +90,12 @@ contains
   case (3); call addr_of(b, ptr)
   case (4); call addr_of(p, ptr)
   case (5); call addr_of(ksqr, ptr)
+  case (6)
+   if (allocated(dropin_force)) then
+    call addr_of(dropin_force, ptr)
+   else
+    ptr = c_null_ptr
+   endif
   case default; ptr = c_null_ptr
   end select
  end function dropin_ptr
@@ -112,6 +132,9 @@ contains
 end module dropin_glue
 
 
-! the config hook (integrate.f90:82), empty as in for_config/eddy1.f90:141-142
+! the config hook (integrate.f90:82)
 subroutine set_forcing
+ use main_module
+ use dropin_glue
+ if (allocated(dropin_force)) db(:,:,:,tau) = db(:,:,:,tau) + dropin_force
 end subroutine set_forcing

@@ -44,7 +44,7 @@ def _view(lib, fid, shape):
     return np.ndarray(shape, dtype=np.float64, buffer=buf, order="F")
 
 
-def _fortran_run(lib, prm, fields, ksqr, steps, mixed):
+def _fortran_run(lib, prm, fields, ksqr, steps, mixed, force=None):
     iprm = (ctypes.c_int32 * 7)(prm.nx, prm.ny, prm.nz, int(prm.enable_nonlinear),
                                 int(prm.enable_vertical_boundaries), int(prm.enable_AB_3_order), int(mixed))
     rprm = (ctypes.c_double * 20)(*[float(getattr(prm, n)) for n in REAL])
@@ -53,6 +53,9 @@ def _fortran_run(lib, prm, fields, ksqr, steps, mixed):
     for fid, n in enumerate(NAMES[:4]):
         _view(lib, fid, shape)[...] = fields[n]
     _view(lib, 5, ksqr.shape)[...] = ksqr
+    if force is not None:
+        lib.dropin_set_forcing()
+        _view(lib, 6, shape)[...] = force
     st = (ctypes.c_int32 * 4)()
     lib.dropin_run(steps, st)
     out = {n: _view(lib, fid, shape).copy() for fid, n in enumerate(NAMES)}
@@ -110,4 +113,47 @@ def test_fortran_dropin_mixed_matches_the_python_mixed_model():
     m.pull(list(NAMES))
     for n in NAMES:
         assert np.array_equal(got[n], m.f[n]), (n, float(np.abs(got[n] - m.f[n]).max()))
+    m.close()
+
+
+def test_fortran_dropin_host_forcing_path():
+    """A config whose set_forcing works on the host mirrors (here db += a
+    fixed array, whole arrays): the drop-in runs tendency and friction on
+    the device, brings fields and tendencies to the host, calls set_forcing
+    there (integrate.f90:82) and finishes the step on the device.  The same
+    forcing as a host round-trip hook of the Python model must give the same
+    bits."""
+    from ps3d_amd import configs
+    from ps3d_amd.model import Model
+    import oracle
+    lib = _load()
+    prm, fields = configs.random_state(32, 24, 16, seed=35, lid=True, Avbi=0.0, Kvbi=0.0)
+    rng = np.random.default_rng(35)
+    force = 1e-7 * rng.standard_normal((prm.nx + 4, prm.ny + 4, prm.nz + 4))
+    orc = oracle.OracleModel(prm)
+    ksqr = orc.f["ksqr"].copy()
+    orc.close()
+    steps = 6
+    print(f"fortran drop-in host forcing steps={steps}", flush=True)
+    got, _ = _fortran_run(lib, prm, fields, ksqr, steps, mixed=False, force=force)
+    m = Model(prm)
+    for n, a in fields.items():
+        m.f[n][...] = a
+    m.push()
+
+    def hook(mm, itt, tau, stream):
+        mm.synchronize()
+        mm.pull(["db"])
+        mm.f["db"][..., tau - 1] = mm.f["db"][..., tau - 1] + force
+        mm.push(["db"])
+
+    m.set_forcing(hook)
+    m.run(steps)
+    m.set_forcing(None)
+    m.pull(list(NAMES))
+    for n in NAMES:
+        assert np.array_equal(got[n], m.f[n]), (n, float(np.abs(got[n] - m.f[n]).max()))
+    # and the forcing did act
+    ref = _fortran_run(lib, prm, fields, ksqr, steps, mixed=False)[0]
+    assert not np.array_equal(ref["b"], got["b"])
     m.close()
