@@ -13,7 +13,8 @@
 !  * dropin_setup: the one-rank decomposition that p3dfft_initialization
 !    (my_fft.f90:5-82) would set, the parameters a config's set_parameter
 !    sets, and the reference's allocate_main_module.
-!  * dropin_run: the main loop, followed by a refresh of the host arrays.
+!  * dropin_run: the main loop, followed by a refresh of the host arrays;
+!    dropin_pressure: one call of pressure_solver on the host arrays.
 !  * set_forcing: the config hook; empty as in for_config/eddy1.f90:141-142
 !    unless dropin_set_forcing switched on a forcing on the host mirrors
 !    (db(:,:,:,tau) += a fixed array, whole arrays as lee_wave1.f90:321-324
@@ -117,6 +118,12 @@ contains
   call ps3d_gpu_sync(.false.)
   st = (/itt, tau, taum1, taum2/)
  end subroutine dropin_run
+
+ ! pressure_solver called outside integrate (as a config's initial
+ ! conditions may call it), then the host arrays as it leaves them
+ subroutine dropin_pressure() bind(C, name="dropin_pressure")
+  call pressure_solver
+ end subroutine dropin_pressure
 
  ! the device model of the drop-in is released (a new setup starts another)
  subroutine dropin_finish() bind(C, name="dropin_finish")

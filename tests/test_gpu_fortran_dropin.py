@@ -157,3 +157,33 @@ def test_fortran_dropin_host_forcing_path():
     ref = _fortran_run(lib, prm, fields, ksqr, steps, mixed=False)[0]
     assert not np.array_equal(ref["b"], got["b"])
     m.close()
+
+
+@pytest.mark.parametrize("lid", [True, False])
+def test_fortran_dropin_pressure_solver(lid):
+    """pressure_solver called on its own through the drop-in (host arrays
+    pushed, projected on the device, pulled back) equals the oracle's
+    pressure_solver.f90:2-120 bit for bit."""
+    import oracle
+    from ps3d_amd import configs
+    lib = _load()
+    prm, fields = configs.random_state(32, 16, 16, seed=37, lid=lid, Avbi=0.0, Kvbi=0.0)
+    orc = oracle.OracleModel(prm)
+    for n, a in fields.items():
+        orc.f[n][...] = a
+    iprm = (ctypes.c_int32 * 7)(prm.nx, prm.ny, prm.nz, int(prm.enable_nonlinear),
+                                int(prm.enable_vertical_boundaries), int(prm.enable_AB_3_order), 0)
+    rprm = (ctypes.c_double * 20)(*[float(getattr(prm, n)) for n in REAL])
+    lib.dropin_setup(iprm, rprm)
+    shape = (prm.nx + 4, prm.ny + 4, prm.nz + 4)
+    for fid, n in enumerate(NAMES[:4]):
+        _view(lib, fid, shape)[...] = fields[n]
+    _view(lib, 5, orc.f["ksqr"].shape)[...] = orc.f["ksqr"]
+    print(f"fortran drop-in pressure_solver lid={lid}", flush=True)
+    lib.dropin_pressure()
+    orc.pressure_solver()
+    for fid, n in enumerate(("u", "v", "w", "b", "p")):
+        got = _view(lib, fid, shape)
+        assert np.array_equal(got, orc.f[n]), (n, float(np.abs(got - orc.f[n]).max()))
+    lib.dropin_finish()
+    orc.close()
