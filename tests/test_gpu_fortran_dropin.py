@@ -187,3 +187,55 @@ def test_fortran_dropin_pressure_solver(lid):
         assert np.array_equal(got, orc.f[n]), (n, float(np.abs(got - orc.f[n]).max()))
     lib.dropin_finish()
     orc.close()
+
+
+JETS = os.path.join(ROOT, "oracle", "_ref", "libps3d_jets.so")
+
+
+def test_reference_config_linked_unchanged():
+    """The reference's own for_config/jets_unscaled.f90 (set_parameter,
+    set_initial_conditions, set_forcing), compiled unchanged and linked with
+    the drop-in, run in main.f90's order (oracle/config_glue.f90): its
+    parameters and initial state come from its own code, the steps run on
+    the GPU through the drop-in integrate, and the result equals the oracle
+    stepping the same state, bit for bit.  (100 x 100 x 10, rigid lid: FFT
+    lengths 50 and 100 take the generic transform kernels.)"""
+    import oracle
+    from ps3d_amd.params import ModelParams
+    if not os.path.exists(JETS):
+        pytest.skip("the config driver is built with the reference sources present (oracle/Makefile: config)")
+    lib = ctypes.CDLL(JETS)
+    lib.config_ptr.argtypes = [ctypes.c_int32]
+    lib.config_ptr.restype = ctypes.c_void_p
+    lib.config_run.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+    lib.config_setup()
+    iprm = (ctypes.c_int32 * 6)()
+    rprm = (ctypes.c_double * 20)()
+    lib.config_params(iprm, rprm)
+    nx, ny, nz, nonlin, lid, ab3 = list(iprm)
+    prm = ModelParams(nx=nx, ny=ny, nz=nz, enable_nonlinear=bool(nonlin), enable_vertical_boundaries=bool(lid),
+                      enable_AB_3_order=bool(ab3), **{n: rprm[q] for q, n in enumerate(REAL)})
+    orc = oracle.OracleModel(prm)
+
+    def view(fid, shape):
+        n = int(np.prod(shape))
+        buf = (ctypes.c_double * n).from_address(lib.config_ptr(fid))
+        return np.ndarray(shape, dtype=np.float64, buffer=buf, order="F")
+
+    view(5, orc.f["ksqr"].shape)[...] = orc.f["ksqr"]
+    lib.config_initial_conditions()
+    shape = (nx + 4, ny + 4, nz + 4)
+    for fid, n in enumerate(NAMES[:4]):
+        orc.f[n][...] = view(fid, shape)
+    assert np.abs(orc.f["u"]).max() > 0.05 and np.abs(orc.f["b"]).max() > 0.0
+    steps = 20
+    print(f"jets_unscaled.f90 through the drop-in: {nx}x{ny}x{nz}, lid={bool(lid)}, {steps} steps", flush=True)
+    st = (ctypes.c_int32 * 4)()
+    lib.config_run(steps, st)
+    orc.run(steps)
+    assert tuple(st) == (orc.itt, orc.tau, orc.taum1, orc.taum2)
+    for fid, n in enumerate(NAMES):
+        got = view(fid, shape)
+        assert np.array_equal(got, orc.f[n]), (n, float(np.abs(got - orc.f[n]).max()))
+    lib.config_finish()
+    orc.close()
