@@ -214,8 +214,9 @@ struct Model {
     bool fuse_halo = false;
     // launch shapes (results do not depend on them; set by ps3d_tune):
     // levels per work-group of the k-marching kernels (0: heuristic) and
-    // columns per 64-lane wave of the Thomas solve (64, 32 or 16)
-    int tend_kc = 0, fric_kc = 0, thomas_cpw = 64;
+    // columns per 64-lane wave of the Thomas solve (64, 32 or 16; 0: the
+    // widest whose on-the-fly form fits its LDS)
+    int tend_kc = 0, fric_kc = 0, thomas_cpw = 0;
 
     // mixed form (PS3D_MIXED): prognostic fields as FP32 (hi, lo) pairs whose
     // sum carries the FP64-accumulated state.  The pressure is carried in

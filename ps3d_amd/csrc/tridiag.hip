@@ -389,7 +389,11 @@ void launch_thomas(Model &m, double2 *data)
 {
     const int64_t ncols = (int64_t)m.g.nkx * m.g.ny;
     const double fxa = 1.0 / (m.prm.dz * m.prm.dz) / m.prm.dsqr;
-    const int cpw = (m.thomas_cpw == 16 || m.thomas_cpw == 32) ? m.thomas_cpw : 64;
+    int cpw = m.thomas_cpw;
+    if (cpw != 16 && cpw != 32 && cpw != 64) {
+        cpw = 64;
+        while (cpw > 16 && (sizeof(double2) + sizeof(double)) * (size_t)cpw * m.g.nz > 64 * 1024) cpw /= 2;
+    }
     const size_t lds = sizeof(double2) * (size_t)cpw * m.g.nz;
     const size_t lds_fly = (sizeof(double2) + sizeof(double)) * (size_t)cpw * m.g.nz;
     const dim3 grid((unsigned)((ncols + cpw - 1) / cpw));
