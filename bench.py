@@ -199,6 +199,41 @@ def ensemble_leg(steps: int, warm: int = 3, precision: int = 0) -> dict:
                         "horizontal biharmonic friction; one launch per stage, hipGraph replay"}
 
 
+def configs_leg(steps: int = 50, warm: int = 5) -> dict:
+    """Throughput of the other configurations, each form from the same
+    initial state after warm-up steps and tuning: the 64x64x32 baroclinic
+    eddy, the 512x128x128 lee wave (background flow and body force on the
+    device) and the 1024x1024x128 jets and eddies."""
+    from ps3d_amd import configs
+    from ps3d_amd.model import Model
+    from ps3d_amd.params import PS3D_FP64, PS3D_MIXED
+    cases = [("eddy 64x64x32, rigid lid, AB3, biharmonic friction", lambda: configs.eddy(64, 64, 32, seed=1) + (None,)),
+             ("lee wave 512x128x128, rigid lid, background flow + body force", lambda: configs.lee_wave(512, 128, 128, seed=3)),
+             ("jets and eddies 1024x1024x128, rigid lid", lambda: configs.jets_eddies(1024, 1024, 128, seed=5) + (None,))]
+    out = {}
+    for name, make in cases:
+        prm, fields, forcing = make()
+        row = {"grid": [prm.nx, prm.ny, prm.nz]}
+        for tag, prec in (("fp64", PS3D_FP64), ("mixed", PS3D_MIXED)):
+            m = Model(prm.replace(precision=prec))
+            for n, a in fields.items():
+                m.f[n][...] = a
+            m.push()
+            if forcing is not None:
+                m.set_background_flow(forcing["u_back"])
+                m.set_body_force(forcing["force"])
+            m.run(warm)
+            m.tune(reps=3)
+            el = time_steps(m, steps)
+            m.close()
+            row[tag] = {"grid_pt_steps_per_s": prm.npoints * steps / el, "ms_per_step": 1e3 * el / steps}
+            print(f"configs: {name} {tag}: {1e3 * el / steps:.3f} ms per step", file=sys.stderr, flush=True)
+        row["mixed_speedup"] = row["mixed"]["grid_pt_steps_per_s"] / row["fp64"]["grid_pt_steps_per_s"]
+        out[name] = row
+        del fields
+    return out
+
+
 def cpu_baseline(prm, fields, budget_s: float = 15.0, threads: int = 0) -> dict:
     """The C oracle stepping the same workload on the host: one independent
     instance per thread, all stepped at once (ctypes releases the GIL inside
@@ -265,6 +300,8 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--ens-steps", type=int, default=100, help="timed steps of the ensemble leg")
+    ap.add_argument("--configs", action="store_true",
+                    help="also time the eddy, lee-wave and 1024x1024x128 configurations (both forms)")
     ap.add_argument("--mixed-check", default="1,100,1000",
                     help="steps at which the mixed leg reports its error against FP64 (comma list)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -352,6 +389,8 @@ def main() -> int:
         out["ensemble"] = ensemble_leg(args.ens_steps)
         if not args.no_mixed:
             out["ensemble_mixed"] = ensemble_leg(args.ens_steps, precision=1)
+    if args.configs:
+        out["configs"] = configs_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prm, fields, threads=args.cpu_threads)
     print(json.dumps(out), flush=True)
