@@ -286,6 +286,11 @@ void integrate(Model &m, int itt, int tau, int taum1, int taum2)
     const bool forced = m.forcing || m.wm_on || background_on(m);
     if (fused_friction_ok(m) && m.fuse_friction && !forced) {
         launch_friction_ab(m, itt, tau, taum1, taum2);
+    } else if (fused_friction_ok(m) && m.fuse_friction) {
+        // set_forcing sees the post-friction state (integrate.f90:82)
+        launch_friction_fused_only(m);
+        apply_forcing(m, itt, tau);
+        launch_ab_exchange(m, itt, tau, taum1, taum2);
     } else {
         launch_friction_block(m);
         if (forced) apply_forcing(m, itt, tau);

@@ -789,7 +789,8 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
             if constexpr (!MIXED) {
                 const double s = F.scale;
                 double val;
-                if (a.order == 1) val = f2 + s * s0;
+                if (a.order == 0) val = f2;   // friction only (the split step of a forced model)
+                else if (a.order == 1) val = f2 + s * s0;
                 else if (a.order == 2) val = f2 + s * (a.ca * s0 - a.cb * s1);
                 else val = f2 + s * (a.ca * s0 + a.cb * s1 + a.cc * s2);
                 F.out[o] = val;
@@ -850,14 +851,12 @@ struct HaloArgsT {
 };
 using HaloArgs = HaloArgsT<double>;
 
-// halo point t (0 <= t < number of halo points) of array a
-template <class T>
-__device__ __forceinline__ void halo_point(const StepConsts &c, T *__restrict__ a, int neu, int64_t t)
+// coordinates of halo point t (0 <= t < number of halo points)
+__device__ __forceinline__ void halo_ijk(const StepConsts &c, int64_t t, int &i, int &j, int &k)
 {
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const int64_t X = c.X, Y = ny + 4;
     const int64_t nzp = 4 * X * Y, nyr = 4 * X * (int64_t)nz;
-    int i, j, k;
     if (t < nzp) {
         const int64_t q = t / (X * Y), r = t % (X * Y);
         k = q < 2 ? (int)q - 1 : nz + (int)q - 1;   // -1, 0, nz+1, nz+2
@@ -876,6 +875,15 @@ __device__ __forceinline__ void halo_point(const StepConsts &c, T *__restrict__ 
         j = (int)(r % ny) + 1;
         k = (int)(r / ny) + 1;
     }
+}
+
+// halo point t (0 <= t < number of halo points) of array a
+template <class T>
+__device__ __forceinline__ void halo_point(const StepConsts &c, T *__restrict__ a, int neu, int64_t t)
+{
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    int i, j, k;
+    halo_ijk(c, t, i, j, k);
     const int si = wrap(i, nx), sj = wrap(j, ny);
     const int sk = neu ? (k < 1 ? 1 : (k > nz ? nz : k)) : wrap(k, nz);
     a[idx(c, i, j, k)] = a[idx(c, si, sj, sk)];
@@ -887,6 +895,26 @@ __global__ __launch_bounds__(256) void k_halo(StepConsts c, HaloArgsT<T> h, int6
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= nhalo) return;
     halo_point<T>(c, h.f[blockIdx.y], h.neumann[blockIdx.y], t);
+}
+
+// dst's halo points take src's values (the same points): the halos of an
+// out-of-place friction result keep the pre-friction values, as the
+// reference's in-place friction leaves them
+template <class T>
+struct HaloCopyArgs {
+    const T *src[4];
+    T *dst[4];
+};
+
+template <class T>
+__global__ __launch_bounds__(256) void k_halo_copy(StepConsts c, HaloCopyArgs<T> h, int64_t nhalo)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= nhalo) return;
+    int i, j, k;
+    halo_ijk(c, t, i, j, k);
+    const int64_t o = idx(c, i, j, k);
+    h.dst[blockIdx.y][o] = h.src[blockIdx.y][o];
 }
 
 // ---------------------------------------------------------------------------
@@ -1570,6 +1598,22 @@ static void friction_ab_impl(Model &m, const T *const *in, const T *const *in_lo
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
+// the halo points of four arrays copied into four others
+template <class T>
+static void halo_copy(Model &m, const T *const *src, T *const *dst)
+{
+    HaloCopyArgs<T> h{};
+    for (int f = 0; f < 4; ++f) {
+        h.src[f] = src[f];
+        h.dst[f] = dst[f];
+    }
+    const int64_t nhalo = m.g.H - m.g.N;
+    ProfScope ps(m, "halo");
+    hipLaunchKernelGGL(k_halo_copy<T>, dim3((unsigned)((nhalo + 255) / 256), 4), dim3(256), 0, m.stream, m.c, h,
+                       nhalo);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
 template <class T>
 static void halo_launch(Model &m, const HaloArgsT<T> &h, int n)
 {
@@ -1611,13 +1655,10 @@ void launch_friction_ab_mixed(Model &m, int itt, int tau, int taum1, int taum2, 
 void launch_friction_forcing_ab_mixed(Model &m, int itt, int tau, int taum1, int taum2, double qc1, const float *d1,
                                       double qc2, const float *d2)
 {
-    const size_t bytes = sizeof(float) * (size_t)m.g.H;
-    for (int f = 0; f < 4; ++f) {
-        PS3D_HIP_CHECK(hipMemcpyAsync(m.ah[f], m.fh[f], bytes, hipMemcpyDeviceToDevice, m.stream));
-        PS3D_HIP_CHECK(hipMemcpyAsync(m.al[f], m.fl[f], bytes, hipMemcpyDeviceToDevice, m.stream));
-    }
-    // interior only (halo 0): the halos stay pre-friction
+    // interior only (halo 0); the hi halos take the pre-friction values, which
+    // the forcing reads (the lo parts and the AB pass use interior points only)
     friction_ab_impl<float>(m, m.fh, m.fl, m.ah, m.al, m.tf, -1, tau, taum1, taum2, 0.0, nullptr, 0.0, nullptr, 0);
+    halo_copy<float>(m, m.fh, m.ah);
     launch_background(m, tau, m.ah);
     AbMixedArgs a{};
     const ps3d_params &p = m.prm;
@@ -1661,6 +1702,21 @@ void launch_friction_forcing_ab_mixed(Model &m, int itt, int tau, int taum1, int
 
 // the fused kernel alone, into the second buffers (no swap, no halo): the
 // autotuner's timing launch
+// integrate.f90:59-78 for a forced FP64 step without vertical biharmonic
+// friction: the fused kernel's friction chain alone (no AB) into the second
+// buffers, whose halos then take the pre-friction values; the forcing and
+// the Adams-Bashforth pass (launch_ab_exchange) read the second buffers
+void launch_friction_fused_only(Model &m)
+{
+    PS3D_REQUIRE(!m.mixed && fused_friction_ok(m), "fused friction: FP64 without vertical biharmonic friction");
+    double *in[4] = {m.field_buf(0), m.field_buf(1), m.field_buf(2), m.field_buf(3)};
+    double *out[4] = {m.alt[0], m.alt[1], m.alt[2], m.alt[3]};
+    double *tend[4] = {m.du, m.dv, m.dw, m.db};
+    friction_ab_impl<double>(m, in, nullptr, out, nullptr, tend, -1, 1, 2, 3, 0.0, nullptr, 0.0, nullptr, 0);
+    halo_copy<double>(m, in, out);
+    for (int f = 0; f < 4; ++f) m.cur[f] = 1;
+}
+
 void launch_friction_ab_raw(Model &m, int itt, int tau, int taum1, int taum2)
 {
     if (m.mixed) {

@@ -2,10 +2,10 @@
 on the GPU: the FP64 form against the oracle with the same hook, bit for bit,
 and the lee-wave configuration at 512x128x128 in the mixed form against the
 FP64 form at steps 1, 100 and 1000 (<= 1e-5 relative L2 per field and in
-total energy).  In the mixed form the forcing terms are formed from the
-pre-friction state (before its fused friction + AB), the FP64 form and the
-reference use the post-friction one; the difference is part of what the
-1e-5 bar covers.
+total energy).  Both forms form the forcing terms from the post-friction
+state, as the reference does (integrate.f90:82): the FP64 form runs the
+fused friction chain alone, the forcing, then the Adams-Bashforth pass; the
+mixed form the same three passes on its (hi, lo) state.
 """
 import numpy as np
 import pytest
