@@ -422,6 +422,7 @@ struct AbArgs {
     double scale[4];                        // dt, dt, dt/dsqr, dt
     double ca, cb, cc;                      // coefficients
     int order;                              // 1, 2, 3
+    int halo;                               // also write the halo images (the exchange of integrate.f90:106-111)
 };
 
 // wavemaker forcing, for_src/config.f90:224-237 (lee_wave1.f90:320-323):
@@ -585,6 +586,7 @@ __global__ __launch_bounds__(BX * BY) void k_ab(StepConsts c, AbArgs a)
         val = a.src[f][o] + s * (a.ca * a.t0[f][o] + a.cb * a.t1[f][o] + a.cc * a.t2[f][o]);
     }
     a.dst[f][o] = val;
+    if (a.halo) write_images<double>(c, a.dst[f], i, j, k, val, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1502,12 +1504,14 @@ void launch_ab_exchange(Model &m, int itt, int tau, int taum1, int taum2)
         a.cb = -16.0 / 12.0;
         a.cc = 5.0 / 12.0;
     }
+    a.halo = m.fuse_halo ? 1 : 0;
     {
         ProfScope ps(m, "ab_update");
         hipLaunchKernelGGL(k_ab, interior_grid(m.g, 4), dim3(BX, BY), 0, m.stream, m.c, a);
         PS3D_HIP_CHECK(hipGetLastError());
     }
     for (int f = 0; f < 4; ++f) m.cur[f] = 0;
+    if (m.fuse_halo) return;   // the images are written
     HaloArgs h{};
     for (int f = 0; f < 4; ++f) { h.f[f] = m.field_buf(f); h.neumann[f] = 0; }
     const Grid &g = m.g;
