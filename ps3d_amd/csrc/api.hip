@@ -289,8 +289,12 @@ void integrate(Model &m, int itt, int tau, int taum1, int taum2)
     } else if (fused_friction_ok(m) && m.fuse_friction) {
         // set_forcing sees the post-friction state (integrate.f90:82)
         launch_friction_fused_only(m);
-        apply_forcing(m, itt, tau);
-        launch_ab_exchange(m, itt, tau, taum1, taum2);
+        if (!m.wm_on && !m.forcing) {
+            launch_background_ab(m, itt, tau, taum1, taum2);   // background flow / body force only
+        } else {
+            apply_forcing(m, itt, tau);
+            launch_ab_exchange(m, itt, tau, taum1, taum2);
+        }
     } else {
         launch_friction_block(m);
         if (forced) apply_forcing(m, itt, tau);

@@ -312,6 +312,8 @@ void launch_friction_ab_raw(Model &m, int itt, int tau, int taum1, int taum2);
 // forced FP64 step: the fused friction chain alone into the second buffers
 // (halos pre-friction), marked current for the forcing and launch_ab_exchange
 void launch_friction_fused_only(Model &m);
+// then the background flow / body force and the AB update in one pass
+void launch_background_ab(Model &m, int itt, int tau, int taum1, int taum2);
 void launch_advection_fields(Model &m, const double *u, const double *v, const double *w, const double *b,
                              double *du, double *dv, double *dw, double *db);
 // one whole step, integrate.f90:2-115 (api.hip)

@@ -464,17 +464,18 @@ struct BgArgs {
     const double *f[4];             // body force per field (null: none)
 };
 
+// the forced tendencies of interior point (i, j, k) = o: slot tau plus the
+// background-flow terms and the body force
 template <class T>
-__global__ __launch_bounds__(BX * BY) void k_background(StepConsts c, BgArgs<T> a)
+__device__ __forceinline__ void bg_point(const StepConsts &c, const BgArgs<T> &a, int k, int64_t o, double &du,
+                                         double &dv, double &dw, double &db)
 {
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
-    if (i > c.nx || j > c.ny) return;
-    const int k = (int)blockIdx.z + 1;
     const int64_t X = c.X, XY = c.XY;
-    const int64_t o = idx(c, i, j, k);
     const double Ro = c.Ro;
-    double du = a.du[o], dv = a.dv[o], dw = a.dw[o], db = a.db[o];
+    du = a.du[o];
+    dv = a.dv[o];
+    dw = a.dw[o];
+    db = a.db[o];
     if (a.ub) {
         const double *UB = a.ub, *BB = a.bb;
         auto U = [&](int64_t q) { return (double)a.u[q]; };
@@ -514,6 +515,18 @@ __global__ __launch_bounds__(BX * BY) void k_background(StepConsts c, BgArgs<T> 
     if (a.f[1]) dv = dv + a.f[1][o];
     if (a.f[2]) dw = dw + a.f[2][o];
     if (a.f[3]) db = db + a.f[3][o];
+}
+
+template <class T>
+__global__ __launch_bounds__(BX * BY) void k_background(StepConsts c, BgArgs<T> a)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    if (i > c.nx || j > c.ny) return;
+    const int k = (int)blockIdx.z + 1;
+    const int64_t o = idx(c, i, j, k);
+    double du, dv, dw, db;
+    bg_point<T>(c, a, k, o, du, dv, dw, db);
     a.du[o] = (T)du;
     a.dv[o] = (T)dv;
     a.dw[o] = (T)dw;
@@ -587,6 +600,41 @@ __global__ __launch_bounds__(BX * BY) void k_ab(StepConsts c, AbArgs a)
     }
     a.dst[f][o] = val;
     if (a.halo) write_images<double>(c, a.dst[f], i, j, k, val, 0);
+}
+
+// the background forcing and the Adams-Bashforth update in one pass (a
+// forced step whose only forcing is the background flow / body force, after
+// the friction-only pass: the fields are read from the second buffers and
+// the new values written to the canonical ones, so no point is read after it
+// is written).  The slot tau keeps the forced tendency for later steps; the
+// same expressions as k_background then k_ab.
+__global__ __launch_bounds__(BX * BY) void k_background_ab(StepConsts c, BgArgs<double> bg, AbArgs a)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    if (i > c.nx || j > c.ny) return;
+    const int k = (int)blockIdx.z + 1;
+    const int64_t o = idx(c, i, j, k);
+    double d[4];
+    bg_point<double>(c, bg, k, o, d[0], d[1], d[2], d[3]);
+    bg.du[o] = d[0];
+    bg.dv[o] = d[1];
+    bg.dw[o] = d[2];
+    bg.db[o] = d[3];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        const double s = a.scale[f];
+        double val;
+        if (a.order == 1) {
+            val = a.src[f][o] + s * d[f];
+        } else if (a.order == 2) {
+            val = a.src[f][o] + s * (a.ca * d[f] - a.cb * a.t1[f][o]);
+        } else {
+            val = a.src[f][o] + s * (a.ca * d[f] + a.cb * a.t1[f][o] + a.cc * a.t2[f][o]);
+        }
+        a.dst[f][o] = val;
+        if (a.halo) write_images<double>(c, a.dst[f], i, j, k, val, 0);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1437,26 +1485,38 @@ void launch_friction_block(Model &m)
     if (p.Kvbi > 0.) launch_mixing(m, PS3D_BIHARMONIC_V, PS3D_B, 0.0, p.Kvbi, 0);
 }
 
-void launch_background(Model &m, int tau, float *const *mixed_fields)
+static BgArgs<double> bg_args(Model &m, int tau)
 {
     const int64_t off = (int64_t)(tau - 1) * m.g.H;
+    BgArgs<double> a{};
+    a.u = m.cur_buf(0); a.v = m.cur_buf(1); a.w = m.cur_buf(2); a.b = m.cur_buf(3);
+    a.du = m.du + off; a.dv = m.dv + off; a.dw = m.dw + off; a.db = m.db + off;
+    a.ub = m.bg_on ? m.bg_u : nullptr;
+    a.bb = (m.bg_on && m.bg_b_on) ? m.bg_b : nullptr;
+    for (int f = 0; f < 4; ++f) a.f[f] = m.bf_on[f] ? m.bf[f] : nullptr;
+    return a;
+}
+
+static BgArgs<float> bg_args_mixed(Model &m, int tau, float *const *fields)
+{
+    const int64_t off = (int64_t)(tau - 1) * m.g.H;
+    BgArgs<float> a{};
+    a.u = fields[0]; a.v = fields[1]; a.w = fields[2]; a.b = fields[3];
+    a.du = m.tf[0] + off; a.dv = m.tf[1] + off; a.dw = m.tf[2] + off; a.db = m.tf[3] + off;
+    a.ub = m.bg_on ? m.bg_u : nullptr;
+    a.bb = (m.bg_on && m.bg_b_on) ? m.bg_b : nullptr;
+    for (int f = 0; f < 4; ++f) a.f[f] = m.bf_on[f] ? m.bf[f] : nullptr;
+    return a;
+}
+
+void launch_background(Model &m, int tau, float *const *mixed_fields)
+{
     if (!m.mixed) {
-        BgArgs<double> a{};
-        a.u = m.cur_buf(0); a.v = m.cur_buf(1); a.w = m.cur_buf(2); a.b = m.cur_buf(3);
-        a.du = m.du + off; a.dv = m.dv + off; a.dw = m.dw + off; a.db = m.db + off;
-        a.ub = m.bg_on ? m.bg_u : nullptr;
-        a.bb = (m.bg_on && m.bg_b_on) ? m.bg_b : nullptr;
-        for (int f = 0; f < 4; ++f) a.f[f] = m.bf_on[f] ? m.bf[f] : nullptr;
-        hipLaunchKernelGGL(k_background<double>, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, a);
+        hipLaunchKernelGGL(k_background<double>, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c,
+                           bg_args(m, tau));
     } else {
-        BgArgs<float> a{};
-        float *const *fl = mixed_fields ? mixed_fields : m.fh;
-        a.u = fl[0]; a.v = fl[1]; a.w = fl[2]; a.b = fl[3];
-        a.du = m.tf[0] + off; a.dv = m.tf[1] + off; a.dw = m.tf[2] + off; a.db = m.tf[3] + off;
-        a.ub = m.bg_on ? m.bg_u : nullptr;
-        a.bb = (m.bg_on && m.bg_b_on) ? m.bg_b : nullptr;
-        for (int f = 0; f < 4; ++f) a.f[f] = m.bf_on[f] ? m.bf[f] : nullptr;
-        hipLaunchKernelGGL(k_background<float>, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, a);
+        hipLaunchKernelGGL(k_background<float>, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c,
+                           bg_args_mixed(m, tau, mixed_fields ? mixed_fields : m.fh));
     }
     PS3D_HIP_CHECK(hipGetLastError());
 }
@@ -1477,8 +1537,9 @@ void launch_wavemaker(Model &m, double cs, double sn)
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
-// integrate.f90:87-111
-void launch_ab_exchange(Model &m, int itt, int tau, int taum1, int taum2)
+// integrate.f90:87-101: the AB update from the current buffers into the
+// canonical ones
+static AbArgs ab_args(Model &m, int itt, int tau, int taum1, int taum2)
 {
     AbArgs a{};
     const int64_t H = m.g.H;
@@ -1505,13 +1566,15 @@ void launch_ab_exchange(Model &m, int itt, int tau, int taum1, int taum2)
         a.cc = 5.0 / 12.0;
     }
     a.halo = m.fuse_halo ? 1 : 0;
-    {
-        ProfScope ps(m, "ab_update");
-        hipLaunchKernelGGL(k_ab, interior_grid(m.g, 4), dim3(BX, BY), 0, m.stream, m.c, a);
-        PS3D_HIP_CHECK(hipGetLastError());
-    }
+    return a;
+}
+
+// after the AB update: the canonical buffers are current; their halos
+// (integrate.f90:106-111) were written as images or are exchanged now
+static void ab_finish(Model &m)
+{
     for (int f = 0; f < 4; ++f) m.cur[f] = 0;
-    if (m.fuse_halo) return;   // the images are written
+    if (m.fuse_halo) return;
     HaloArgs h{};
     for (int f = 0; f < 4; ++f) { h.f[f] = m.field_buf(f); h.neumann[f] = 0; }
     const Grid &g = m.g;
@@ -1519,6 +1582,33 @@ void launch_ab_exchange(Model &m, int itt, int tau, int taum1, int taum2)
     ProfScope ps(m, "halo");
     hipLaunchKernelGGL(k_halo<double>, dim3((unsigned)((nhalo + 255) / 256), 4), dim3(256), 0, m.stream, m.c, h, nhalo);
     PS3D_HIP_CHECK(hipGetLastError());
+}
+
+// integrate.f90:87-111
+void launch_ab_exchange(Model &m, int itt, int tau, int taum1, int taum2)
+{
+    const AbArgs a = ab_args(m, itt, tau, taum1, taum2);
+    {
+        ProfScope ps(m, "ab_update");
+        hipLaunchKernelGGL(k_ab, interior_grid(m.g, 4), dim3(BX, BY), 0, m.stream, m.c, a);
+        PS3D_HIP_CHECK(hipGetLastError());
+    }
+    ab_finish(m);
+}
+
+// set_forcing (background flow and body force only) and integrate.f90:87-111
+// in one pass, after launch_friction_fused_only
+void launch_background_ab(Model &m, int itt, int tau, int taum1, int taum2)
+{
+    for (int f = 0; f < 4; ++f)
+        PS3D_REQUIRE(m.cur[f] == 1, "forcing + AB in one pass: the friction results must be in the second buffers");
+    const AbArgs a = ab_args(m, itt, tau, taum1, taum2);
+    {
+        ProfScope ps(m, "ab_update");
+        hipLaunchKernelGGL(k_background_ab, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, bg_args(m, tau), a);
+        PS3D_HIP_CHECK(hipGetLastError());
+    }
+    ab_finish(m);
 }
 
 bool fused_friction_ok(const Model &m)
