@@ -139,3 +139,25 @@ def test_ensemble_rejects_mixed_forms():
     prms = [members[0][0], dataclasses.replace(members[1][0], precision=PS3D_MIXED)]
     with pytest.raises(Exception, match="share one precision form"):
         Ensemble(prms)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_c4_ensemble_500_steps(mixed):
+    """The 48-member ensemble of three sizes over its 500 steps (one launch
+    per stage, graph replay, tuned): one member of each size equals a
+    single model of its form run alone, bit for bit."""
+    from ps3d_amd import configs
+    from ps3d_amd.params import PS3D_FP64, PS3D_MIXED
+    members = configs.ensemble_c4(precision=PS3D_MIXED if mixed else PS3D_FP64)
+    ens = _ensemble(members)
+    print(f"C4 ensemble, {len(members)} members, mixed={mixed}, 500 steps", flush=True)
+    ens.run(3)
+    ens.tune(None, reps=2)
+    ens.run(497, use_graph=True)
+    for i in (0, 16, 47):
+        prm, fields = members[i]
+        ref = _single(prm, fields, 500)
+        for n in NAMES:
+            got = ens.get(i, n)
+            assert np.array_equal(got, ref[n]), (i, n, float(np.abs(got - ref[n]).max()))
+    ens.close()
