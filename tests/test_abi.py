@@ -149,3 +149,21 @@ def test_ensemble_create_rejects_bad_members(bad, msg):
     assert code == 1
     assert not h.value
     assert "member 1" in _lib.last_error() and msg in _lib.last_error()
+
+
+def test_mixed_ensemble_rejects_tall_lid_members():
+    """A mixed ensemble's Thomas stage keeps 32 columns of FP64 factors in
+    LDS: rigid-lid members of more than 85 levels are refused at creation,
+    before any device work."""
+    from ps3d_amd import _lib
+    from ps3d_amd.params import CParams, ModelParams, PS3D_MIXED
+    lib = _lib.load()
+    base = dict(nx=32, ny=32, Lx=1.0, Ly=1.0, Lz=1.0, dt=0.1, enable_vertical_boundaries=True,
+                precision=PS3D_MIXED)
+    arr = (CParams * 2)(CParams.from_params(ModelParams(nz=16, **base)),
+                        CParams.from_params(ModelParams(nz=96, **base)))
+    h = ctypes.c_void_p()
+    code = lib.ps3d_ensemble_create(2, arr, None, ctypes.byref(h))
+    assert code == 1
+    assert not h.value
+    assert "member 1" in _lib.last_error() and "at most 85 levels" in _lib.last_error()
