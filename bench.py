@@ -199,7 +199,7 @@ def ensemble_leg(steps: int, warm: int = 3, precision: int = 0) -> dict:
                         "horizontal biharmonic friction; one launch per stage, hipGraph replay"}
 
 
-def configs_leg(steps: int = 50, warm: int = 5) -> dict:
+def configs_leg(steps: int = 50, warm: int = 5, cases=("eddy", "lee_wave", "jets_eddies")) -> dict:
     """Throughput of the other configurations, each form from the same
     initial state after warm-up steps and tuning: the 64x64x32 baroclinic
     eddy, the 512x128x128 lee wave (background flow and body force on the
@@ -207,11 +207,15 @@ def configs_leg(steps: int = 50, warm: int = 5) -> dict:
     from ps3d_amd import configs
     from ps3d_amd.model import Model
     from ps3d_amd.params import PS3D_FP64, PS3D_MIXED
-    cases = [("eddy 64x64x32, rigid lid, AB3, biharmonic friction", lambda: configs.eddy(64, 64, 32, seed=1) + (None,)),
-             ("lee wave 512x128x128, rigid lid, background flow + body force", lambda: configs.lee_wave(512, 128, 128, seed=3)),
-             ("jets and eddies 1024x1024x128, rigid lid", lambda: configs.jets_eddies(1024, 1024, 128, seed=5) + (None,))]
+    table = {"eddy": ("eddy 64x64x32, rigid lid, AB3, biharmonic friction",
+                      lambda: configs.eddy(64, 64, 32, seed=1) + (None,)),
+             "lee_wave": ("lee wave 512x128x128, rigid lid, background flow + body force",
+                          lambda: configs.lee_wave(512, 128, 128, seed=3)),
+             "jets_eddies": ("jets and eddies 1024x1024x128, rigid lid",
+                             lambda: configs.jets_eddies(1024, 1024, 128, seed=5) + (None,))}
     out = {}
-    for name, make in cases:
+    for key in cases:
+        name, make = table[key]
         prm, fields, forcing = make()
         row = {"grid": [prm.nx, prm.ny, prm.nz]}
         for tag, prec in (("fp64", PS3D_FP64), ("mixed", PS3D_MIXED)):
@@ -300,8 +304,9 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--ens-steps", type=int, default=100, help="timed steps of the ensemble leg")
-    ap.add_argument("--configs", action="store_true",
-                    help="also time the eddy, lee-wave and 1024x1024x128 configurations (both forms)")
+    ap.add_argument("--configs", nargs="?", const="eddy,lee_wave,jets_eddies", default=None,
+                    help="also time the eddy, lee-wave and 1024x1024x128 configurations (both forms); "
+                         "optionally a comma list of them")
     ap.add_argument("--mixed-check", default="1,100,1000",
                     help="steps at which the mixed leg reports its error against FP64 (comma list)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -390,7 +395,7 @@ def main() -> int:
         if not args.no_mixed:
             out["ensemble_mixed"] = ensemble_leg(args.ens_steps, precision=1)
     if args.configs:
-        out["configs"] = configs_leg()
+        out["configs"] = configs_leg(cases=tuple(c.strip() for c in args.configs.split(",") if c.strip()))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prm, fields, threads=args.cpu_threads)
     print(json.dumps(out), flush=True)

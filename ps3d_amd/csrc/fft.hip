@@ -767,6 +767,9 @@ constexpr int XROWS(int n2) { return n2 >= 1024 ? 1 : 1024 / n2; }
 template <class C> constexpr int TCOL = 64 / (int)sizeof(C);
 constexpr int TTHREADS = 256;
 template <class C> constexpr int TCOLS(int n) { return n * TCOL<C> / 16 > 256 ? n * TCOL<C> / 16 : 256; }
+// single-model passes: half the columns for the longest transforms (LDS per work-group)
+template <class C, int N> constexpr int TCOLN = N >= 1024 ? TCOL<C> / 2 : TCOL<C>;
+template <class C, int N> constexpr int TCOLSN = N * TCOLN<C, N> / 16 > 256 ? N * TCOLN<C, N> / 16 : 256;
 
 template <int N2, class RT>
 bool launch_r2c(Model &m, const RT *src, const RT *u, const RT *v, const RT *w, int fuse_div,
@@ -800,13 +803,13 @@ template <int N, class C>
 bool launch_cols(const C *tw, hipStream_t st, C *data, int nkx, int64_t estride, int64_t lstride,
                  unsigned ny_or_nz, int sign, double s1, double s2, double s3, int nscale)
 {
-    const dim3 grid((unsigned)((nkx + TCOL<C> - 1) / TCOL<C>), ny_or_nz);
-    constexpr int T = TCOLS<C>(N);
+    const dim3 grid((unsigned)((nkx + TCOLN<C, N> - 1) / TCOLN<C, N>), ny_or_nz);
+    constexpr int T = TCOLSN<C, N>;
     if (sign < 0)
-        hipLaunchKernelGGL((c2c_cols<N, TCOL<C>, T, -1, C>), grid, dim3(T), 0, st, tw, data, nkx, estride, lstride,
+        hipLaunchKernelGGL((c2c_cols<N, TCOLN<C, N>, T, -1, C>), grid, dim3(T), 0, st, tw, data, nkx, estride, lstride,
                            s1, s2, s3, nscale);
     else
-        hipLaunchKernelGGL((c2c_cols<N, TCOL<C>, T, +1, C>), grid, dim3(T), 0, st, tw, data, nkx, estride, lstride,
+        hipLaunchKernelGGL((c2c_cols<N, TCOLN<C, N>, T, +1, C>), grid, dim3(T), 0, st, tw, data, nkx, estride, lstride,
                            s1, s2, s3, nscale);
     return true;
 }
@@ -814,9 +817,9 @@ bool launch_cols(const C *tw, hipStream_t st, C *data, int nkx, int64_t estride,
 template <int N, class C>
 bool launch_zsolve(Model &m, C *data, const C *tw, int nkxs)
 {
-    const dim3 grid((unsigned)((m.g.nkx + TCOL<C> - 1) / TCOL<C>), (unsigned)m.g.ny);
-    constexpr int T = TCOLS<C>(N);
-    hipLaunchKernelGGL((z_periodic_solve<N, TCOL<C>, T, C>), grid, dim3(T), 0, m.stream, tw, data, m.ksqr, m.g.nkx,
+    const dim3 grid((unsigned)((m.g.nkx + TCOLN<C, N> - 1) / TCOLN<C, N>), (unsigned)m.g.ny);
+    constexpr int T = TCOLSN<C, N>;
+    hipLaunchKernelGGL((z_periodic_solve<N, TCOLN<C, N>, T, C>), grid, dim3(T), 0, m.stream, tw, data, m.ksqr, m.g.nkx,
                        nkxs, m.g.ny, m.prm.dx, m.prm.dy, m.prm.dz);
     return true;
 }
