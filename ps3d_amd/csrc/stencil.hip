@@ -999,16 +999,17 @@ __device__ __forceinline__ void project_point(const StepConsts &c, double *__res
     }
 }
 
-__global__ __launch_bounds__(BX * BY) void k_project(StepConsts c, double *__restrict__ u,
-                                                      double *__restrict__ v, double *__restrict__ w,
-                                                      double *__restrict__ p, int halo)
+__global__ __launch_bounds__(256) void k_project(StepConsts c, double *__restrict__ u,
+                                                  double *__restrict__ v, double *__restrict__ w,
+                                                  double *__restrict__ p, int halo)
 {
     // blocks renumbered per XCD: the rows of p a block reads past its own
-    // tile belong to the next blocks, which then share its L2
+    // tile belong to the next blocks, which then share its L2.  Work-groups
+    // of 256 x 1 points where rows are that long, else BX x BY.
     const int gx = (int)gridDim.x, gy = (int)gridDim.y;
     const int r = xcd_remap((int)blockIdx.x + gx * ((int)blockIdx.y + gy * (int)blockIdx.z), gx * gy * (int)gridDim.z);
-    const int i = (r % gx) * BX + threadIdx.x + 1;
-    const int j = ((r / gx) % gy) * BY + threadIdx.y + 1;
+    const int i = (r % gx) * (int)blockDim.x + threadIdx.x + 1;
+    const int j = ((r / gx) % gy) * (int)blockDim.y + threadIdx.y + 1;
     if (i > c.nx || j > c.ny) return;
     project_point(c, u, v, w, p, i, j, r / (gx * gy) + 1, halo);
 }
@@ -1047,15 +1048,18 @@ __device__ __forceinline__ void project_f_point(const StepConsts &c, float *__re
     }
 }
 
-__global__ __launch_bounds__(BX * BY) void k_project_f(StepConsts c, float *__restrict__ uh, float *__restrict__ ul,
-                                                        float *__restrict__ vh, float *__restrict__ vl,
-                                                        float *__restrict__ wh, float *__restrict__ wl,
-                                                        float *__restrict__ dp, double *__restrict__ pint, int halo)
+// work-groups of 256 x 1 points where rows are that long (each of the
+// fifteen FP32/FP64 streams is then read and written in whole rows), else
+// BX x BY
+__global__ __launch_bounds__(256) void k_project_f(StepConsts c, float *__restrict__ uh, float *__restrict__ ul,
+                                                    float *__restrict__ vh, float *__restrict__ vl,
+                                                    float *__restrict__ wh, float *__restrict__ wl,
+                                                    float *__restrict__ dp, double *__restrict__ pint, int halo)
 {
     const int gx = (int)gridDim.x, gy = (int)gridDim.y;
     const int r = xcd_remap((int)blockIdx.x + gx * ((int)blockIdx.y + gy * (int)blockIdx.z), gx * gy * (int)gridDim.z);
-    const int i = (r % gx) * BX + threadIdx.x + 1;
-    const int j = ((r / gx) % gy) * BY + threadIdx.y + 1;
+    const int i = (r % gx) * (int)blockDim.x + threadIdx.x + 1;
+    const int j = ((r / gx) % gy) * (int)blockDim.y + threadIdx.y + 1;
     const int k = r / (gx * gy) + 1;
     if (i > c.nx || j > c.ny) return;
     project_f_point(c, uh, ul, vh, vl, wh, wl, dp, pint, i, j, k, halo);
@@ -1856,7 +1860,10 @@ void launch_project_exchange(Model &m)
 {
     {
         ProfScope ps(m, "project");
-        hipLaunchKernelGGL(k_project, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, m.u, m.v, m.w, m.p,
+        const bool rows = m.g.nx >= 256;
+        const dim3 grid = rows ? dim3((unsigned)((m.g.nx + 255) / 256), (unsigned)m.g.ny, (unsigned)m.g.nz)
+                               : interior_grid(m.g);
+        hipLaunchKernelGGL(k_project, grid, rows ? dim3(256, 1) : dim3(BX, BY), 0, m.stream, m.c, m.u, m.v, m.w, m.p,
                            m.fuse_halo ? 1 : 0);
         PS3D_HIP_CHECK(hipGetLastError());
     }
@@ -1874,7 +1881,10 @@ void launch_project_exchange_f(Model &m, float *dp)
     // the next steps' pressure-history term
     {
         ProfScope ps(m, "project");
-        hipLaunchKernelGGL(k_project_f, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, m.fh[0], m.fl[0],
+        const bool rows = m.g.nx >= 256;
+        const dim3 grid = rows ? dim3((unsigned)((m.g.nx + 255) / 256), (unsigned)m.g.ny, (unsigned)m.g.nz)
+                               : interior_grid(m.g);
+        hipLaunchKernelGGL(k_project_f, grid, rows ? dim3(256, 1) : dim3(BX, BY), 0, m.stream, m.c, m.fh[0], m.fl[0],
                            m.fh[1], m.fl[1], m.fh[2], m.fl[2], dp, m.pint, m.fuse_halo ? 1 : 0);
         PS3D_HIP_CHECK(hipGetLastError());
     }
