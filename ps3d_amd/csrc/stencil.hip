@@ -438,8 +438,8 @@ struct WaveArgs {
 
 __global__ __launch_bounds__(BX * BY) void k_wavemaker(StepConsts c, WaveArgs a, double cs, double sn)
 {
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int i = blockIdx.x * (int)blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * (int)blockDim.y + threadIdx.y + 1;
     if (i > c.nx || j > c.ny) return;
     const int k = (int)blockIdx.z % c.nz + 1, f = (int)blockIdx.z / c.nz;
     const int64_t o = idx(c, i, j, k);
@@ -520,8 +520,8 @@ __device__ __forceinline__ void bg_point(const StepConsts &c, const BgArgs<T> &a
 template <class T>
 __global__ __launch_bounds__(BX * BY) void k_background(StepConsts c, BgArgs<T> a)
 {
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int i = blockIdx.x * (int)blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * (int)blockDim.y + threadIdx.y + 1;
     if (i > c.nx || j > c.ny) return;
     const int k = (int)blockIdx.z + 1;
     const int64_t o = idx(c, i, j, k);
@@ -554,8 +554,8 @@ struct AbMixedArgs {
 
 __global__ __launch_bounds__(BX * BY) void k_ab_mixed(StepConsts c, AbMixedArgs a)
 {
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int i = blockIdx.x * (int)blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * (int)blockDim.y + threadIdx.y + 1;
     if (i > c.nx || j > c.ny) return;
     const int k = (int)blockIdx.z % c.nz + 1, f = (int)blockIdx.z / c.nz;
     const int64_t o = idx(c, i, j, k);
@@ -583,8 +583,8 @@ __global__ __launch_bounds__(BX * BY) void k_ab_mixed(StepConsts c, AbMixedArgs 
 
 __global__ __launch_bounds__(BX * BY) void k_ab(StepConsts c, AbArgs a)
 {
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int i = blockIdx.x * (int)blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * (int)blockDim.y + threadIdx.y + 1;
     const int k = (int)(blockIdx.z % (unsigned)c.nz) + 1;
     const int f = (int)(blockIdx.z / (unsigned)c.nz);
     if (i > c.nx || j > c.ny) return;
@@ -610,8 +610,8 @@ __global__ __launch_bounds__(BX * BY) void k_ab(StepConsts c, AbArgs a)
 // same expressions as k_background then k_ab.
 __global__ __launch_bounds__(BX * BY) void k_background_ab(StepConsts c, BgArgs<double> bg, AbArgs a)
 {
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int i = blockIdx.x * (int)blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * (int)blockDim.y + threadIdx.y + 1;
     if (i > c.nx || j > c.ny) return;
     const int k = (int)blockIdx.z + 1;
     const int64_t o = idx(c, i, j, k);
@@ -1098,6 +1098,20 @@ inline dim3 interior_grid(const Grid &g, int zmult = 1)
     return dim3((unsigned)((g.nx + BX - 1) / BX), (unsigned)((g.ny + BY - 1) / BY), (unsigned)(g.nz * zmult));
 }
 
+// launch shape of the per-point kernels over the interior (blockIdx.z =
+// level x zmult): rows of 256 points where nx allows it (each array is then
+// read and written in whole rows), else BX x BY tiles
+struct PointShape {
+    dim3 grid, block;
+};
+
+inline PointShape point_shape(const Grid &g, int zmult = 1)
+{
+    if (g.nx >= 256)
+        return {dim3((unsigned)((g.nx + 255) / 256), (unsigned)g.ny, (unsigned)(g.nz * zmult)), dim3(256, 1)};
+    return {interior_grid(g, zmult), dim3(BX, BY)};
+}
+
 inline dim3 full_grid(const Grid &g)
 {
     return dim3((unsigned)((g.X + BX - 1) / BX), (unsigned)((g.Y + BY - 1) / BY), (unsigned)g.Z);
@@ -1516,10 +1530,12 @@ static BgArgs<float> bg_args_mixed(Model &m, int tau, float *const *fields)
 void launch_background(Model &m, int tau, float *const *mixed_fields)
 {
     if (!m.mixed) {
-        hipLaunchKernelGGL(k_background<double>, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c,
+        const PointShape sh = point_shape(m.g);
+        hipLaunchKernelGGL(k_background<double>, sh.grid, sh.block, 0, m.stream, m.c,
                            bg_args(m, tau));
     } else {
-        hipLaunchKernelGGL(k_background<float>, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c,
+        const PointShape sh = point_shape(m.g);
+        hipLaunchKernelGGL(k_background<float>, sh.grid, sh.block, 0, m.stream, m.c,
                            bg_args_mixed(m, tau, mixed_fields ? mixed_fields : m.fh));
     }
     PS3D_HIP_CHECK(hipGetLastError());
@@ -1537,7 +1553,8 @@ void launch_wavemaker(Model &m, double cs, double sn)
         ++n;
     }
     if (!n) return;
-    hipLaunchKernelGGL(k_wavemaker, interior_grid(m.g, n), dim3(BX, BY), 0, m.stream, m.c, a, cs, sn);
+    const PointShape sh = point_shape(m.g, n);
+    hipLaunchKernelGGL(k_wavemaker, sh.grid, sh.block, 0, m.stream, m.c, a, cs, sn);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
@@ -1594,7 +1611,8 @@ void launch_ab_exchange(Model &m, int itt, int tau, int taum1, int taum2)
     const AbArgs a = ab_args(m, itt, tau, taum1, taum2);
     {
         ProfScope ps(m, "ab_update");
-        hipLaunchKernelGGL(k_ab, interior_grid(m.g, 4), dim3(BX, BY), 0, m.stream, m.c, a);
+        const PointShape sh = point_shape(m.g, 4);
+        hipLaunchKernelGGL(k_ab, sh.grid, sh.block, 0, m.stream, m.c, a);
         PS3D_HIP_CHECK(hipGetLastError());
     }
     ab_finish(m);
@@ -1609,7 +1627,8 @@ void launch_background_ab(Model &m, int itt, int tau, int taum1, int taum2)
     const AbArgs a = ab_args(m, itt, tau, taum1, taum2);
     {
         ProfScope ps(m, "ab_update");
-        hipLaunchKernelGGL(k_background_ab, interior_grid(m.g), dim3(BX, BY), 0, m.stream, m.c, bg_args(m, tau), a);
+        const PointShape sh = point_shape(m.g);
+        hipLaunchKernelGGL(k_background_ab, sh.grid, sh.block, 0, m.stream, m.c, bg_args(m, tau), a);
         PS3D_HIP_CHECK(hipGetLastError());
     }
     ab_finish(m);
@@ -1789,7 +1808,8 @@ void launch_friction_forcing_ab_mixed(Model &m, int itt, int tau, int taum1, int
     a.qc1 = qc1;
     a.qc2 = qc2;
     a.halo = m.fuse_halo ? 1 : 0;
-    hipLaunchKernelGGL(k_ab_mixed, interior_grid(m.g, 4), dim3(BX, BY), 0, m.stream, m.c, a);
+    const PointShape sh = point_shape(m.g, 4);
+    hipLaunchKernelGGL(k_ab_mixed, sh.grid, sh.block, 0, m.stream, m.c, a);
     PS3D_HIP_CHECK(hipGetLastError());
     if (!m.fuse_halo) {
         HaloArgsT<float> h{};
