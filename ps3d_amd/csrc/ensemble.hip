@@ -73,7 +73,7 @@ void set_chunks(int nz, int kc, int &okc, int &onch)
 int default_tend_kc(const Grid &g)
 {
     const int tiles = cdiv(g.nx, kTTX) * cdiv(g.ny, kTTY);
-    int nchunk = cdiv(1024, tiles);
+    int nchunk = cdiv(512, tiles);
     const int maxch = g.nz / 8 > 1 ? g.nz / 8 : 1;
     nchunk = std::min(nchunk, maxch);
     return cdiv(g.nz, nchunk);

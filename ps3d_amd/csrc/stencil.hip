@@ -1393,9 +1393,10 @@ static void tendency_impl(Model &m, const T *u, const T *v, const T *w, const T 
     a.tiles_x = (g.nx + TTX - 1) / TTX;
     a.tiles_y = (g.ny + TTY - 1) / TTY;
     const int tiles = a.tiles_x * a.tiles_y;
-    // about 4 work-groups per CU over 256 CUs, chunks of at least 8 levels
-    // (or the tuned number of levels)
-    int nchunk = (1024 + tiles - 1) / tiles;
+    // about 2 work-groups per CU over 256 CUs, chunks of at least 8 levels
+    // (or the tuned number of levels): longer chunks re-read fewer apron
+    // planes, which at 256x256x64 outweighs the extra work-groups
+    int nchunk = (512 + tiles - 1) / tiles;
     const int maxch = g.nz / 8 > 1 ? g.nz / 8 : 1;
     nchunk = nchunk > maxch ? maxch : nchunk;
     if (m.tend_kc > 0) nchunk = (g.nz + m.tend_kc - 1) / m.tend_kc;
