@@ -201,7 +201,8 @@ int ps3d_spectral_solve(ps3d_model *m, double *qs);
 /* Times the candidate launch shapes of this model's step on the live model
    (levels per work-group of the tendency and friction + AB kernels, columns
    per wavefront of the Thomas solve) and keeps the fastest; the choice is
-   cached per (stage, grid, boundary, precision) in cache_path (NULL: none).
+   cached per (stage, grid, boundary, precision) in cache_path (NULL or "":
+   none).
    next_tau is the slot the next step writes (it is used as scratch).  The
    model's state is unchanged and results do not depend on the shapes.  No
    counterpart in the reference (its loop shapes are fixed by the compiler). */

@@ -47,6 +47,7 @@ contains
   enable_AB_3_order = iprm(6) /= 0
   ps3d_use_mixed = iprm(7) /= 0
   ps3d_forcing_on_host = .false.
+  ps3d_autotune = .false.
   if (allocated(dropin_force)) deallocate(dropin_force)
   Lx = rprm(1); Ly = rprm(2); Lz = rprm(3)
   dx = rprm(4); dy = rprm(5); dz = rprm(6); dt = rprm(7)
@@ -71,6 +72,13 @@ contains
   time = 0.; itt = 0
   taum2 = 1; taum1 = 2; tau = 3
  end subroutine dropin_setup
+
+ ! the drop-in's launch-shape autotuner at set-up (no cache file)
+ subroutine dropin_set_autotune() bind(C, name="dropin_set_autotune")
+  use ps3d_gpu_state
+  ps3d_autotune = .true.
+  ps3d_tune_cache = ''
+ end subroutine dropin_set_autotune
 
  ! a forcing of db on the host mirrors (set its array through dropin_ptr(6))
  subroutine dropin_set_forcing() bind(C, name="dropin_set_forcing")

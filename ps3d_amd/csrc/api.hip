@@ -773,7 +773,7 @@ int ps3d_tune(ps3d_model *m, int next_tau, const char *cache_path, int reps)
 {
     return guard([&] {
         check_model(m);
-        tune_model(*m, next_tau, cache_path, reps);
+        tune_model(*m, next_tau, (cache_path && *cache_path) ? cache_path : nullptr, reps);
     });
 }
 

@@ -18,6 +18,10 @@ module ps3d_gpu_state
  type(c_ptr) :: ps3d_model_handle = c_null_ptr
  logical :: ps3d_forcing_on_host = .false.
  logical :: ps3d_use_mixed = .false.
+ ! time the launch shapes once at set-up (results do not depend on them),
+ ! cached in ps3d_tune_cache ('' : not cached)
+ logical :: ps3d_autotune = .false.
+ character(len=256) :: ps3d_tune_cache = ''
 end module ps3d_gpu_state
 
 
@@ -86,6 +90,8 @@ subroutine ps3d_gpu_init
   call halt_stop('ps3d: set_forcing on host mirrors needs the FP64 form (ps3d_use_mixed = .false.)')
  call ps3d_gpu_check(ps3d_create(prm, ksqr, c_null_ptr, ps3d_model_handle), 'ps3d_create')
  call ps3d_gpu_sync(.true.)
+ if (ps3d_autotune) call ps3d_gpu_check(ps3d_tune(ps3d_model_handle, int(tau, c_int), &
+                                        trim(ps3d_tune_cache)//c_null_char, 5_c_int), 'ps3d_tune')
 end subroutine ps3d_gpu_init
 
 

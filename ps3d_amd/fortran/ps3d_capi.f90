@@ -47,6 +47,13 @@ module ps3d_capi
    import :: c_int, c_ptr
    type(c_ptr), value :: model
   end function
+  ! launch-shape autotuner; cache_path NUL-terminated ('' + c_null_char: no file)
+  integer(c_int) function ps3d_tune(model, next_tau, cache_path, reps) bind(C, name="ps3d_tune")
+   import :: c_int, c_ptr, c_char
+   type(c_ptr), value :: model
+   integer(c_int), value :: next_tau, reps
+   character(kind=c_char), intent(in) :: cache_path(*)
+  end function
   integer(c_int) function ps3d_set_field(model, field, host) bind(C, name="ps3d_set_field")
    import :: c_int, c_double, c_ptr
    type(c_ptr), value :: model

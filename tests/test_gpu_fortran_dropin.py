@@ -44,7 +44,7 @@ def _view(lib, fid, shape):
     return np.ndarray(shape, dtype=np.float64, buffer=buf, order="F")
 
 
-def _fortran_run(lib, prm, fields, ksqr, steps, mixed, force=None):
+def _fortran_run(lib, prm, fields, ksqr, steps, mixed, force=None, autotune=False):
     iprm = (ctypes.c_int32 * 7)(prm.nx, prm.ny, prm.nz, int(prm.enable_nonlinear),
                                 int(prm.enable_vertical_boundaries), int(prm.enable_AB_3_order), int(mixed))
     rprm = (ctypes.c_double * 20)(*[float(getattr(prm, n)) for n in REAL])
@@ -53,6 +53,8 @@ def _fortran_run(lib, prm, fields, ksqr, steps, mixed, force=None):
     for fid, n in enumerate(NAMES[:4]):
         _view(lib, fid, shape)[...] = fields[n]
     _view(lib, 5, ksqr.shape)[...] = ksqr
+    if autotune:
+        lib.dropin_set_autotune()
     if force is not None:
         lib.dropin_set_forcing()
         _view(lib, 6, shape)[...] = force
@@ -71,8 +73,9 @@ CASES = {
 }
 
 
+@pytest.mark.parametrize("autotune", [False, True])
 @pytest.mark.parametrize("case", list(CASES))
-def test_fortran_dropin_matches_the_oracle(case):
+def test_fortran_dropin_matches_the_oracle(case, autotune):
     import oracle
     from ps3d_amd import configs
     lib = _load()
@@ -83,8 +86,8 @@ def test_fortran_dropin_matches_the_oracle(case):
     for n, a in fields.items():
         orc.f[n][...] = a
     steps = 12
-    print(f"fortran drop-in case={case} steps={steps}", flush=True)
-    got, st = _fortran_run(lib, prm, fields, orc.f["ksqr"].copy(), steps, mixed=False)
+    print(f"fortran drop-in case={case} steps={steps} autotune={autotune}", flush=True)
+    got, st = _fortran_run(lib, prm, fields, orc.f["ksqr"].copy(), steps, mixed=False, autotune=autotune)
     orc.run(steps)
     assert st == (orc.itt, orc.tau, orc.taum1, orc.taum2)
     for n in NAMES:
