@@ -14,8 +14,16 @@ inner product, so the vertical-mode transform pair is the identity) and the
 discrete relations psi(k)-psi(k-1) = 2 sin() phi(k); the projections by
 their algebra (the zero-order projection is idempotent, a slow-mode
 eigenvector is reproduced, a fast mode is removed).  The orchestration
-(diag_balance.f90:89-362) calls P3DFFT and cannot be built here: it is
-restated line by line and cross-checked by those invariants.
+(diag_balance.f90:89-362) calls P3DFFT and cannot be built here.  Its
+first, second and third orders are pinned by the slaving relations they
+satisfy, with every piece taken from the pinned oracle step run as a black
+box (tests/test_balance_pinning.py): the rate of each fast correction under
+the linear step plus the nonlinear forcing of that order (the step with and
+without its nonlinear terms) equals the rate of change of the lower-order
+slaved amplitude; and a state independent of x with v = w = 0 has no
+corrections.  Where the periodic form's kz = 0 fast modes
+(eigenvectors.f90:87-92) turn with the opposite sign, the reference's
+correction has the opposite sign as well, and so does this one.
 """
 from __future__ import annotations
 
