@@ -92,6 +92,15 @@ int ps3d_create(const ps3d_params *prm, const double *ksqr, void *stream,
                 ps3d_model **out);
 int ps3d_destroy(ps3d_model *m);
 
+/* Change the parameters that callers of integrate change between steps:
+   main_module's dt, Ro and friction coefficients as diag_opt_balance.f90
+   sets them around its backward and forward integrations (:190-205, :259-
+   292: dt = -|dt|, Ro ramped, Ahbi..Kvbi zeroed), and eps_ab, f0, N0,
+   enable_nonlinear, enable_AB_3_order.  The grid, domain, spacing, dsqr,
+   boundary form and precision must equal the model's (PS3D_ERR_ARG
+   otherwise).  Takes effect at the next step. */
+int ps3d_set_params(ps3d_model *m, const ps3d_params *prm);
+
 /* Host <-> device copies of whole main_module arrays (halo included).
    Synchronous with respect to the host.  ps3d_set_field of PS3D_KSQR
    replaces the wavenumber table (and the solver factors derived from it). */
@@ -195,6 +204,25 @@ int ps3d_my_ifft(ps3d_model *m, int three_d, const double *qs, double *qp);
    per-column solve_tridiag (pressure_solver.f90:126-162) and the zero-mode
    recurrence with a rigid lid, else the division by -ksqr */
 int ps3d_spectral_solve(ps3d_model *m, double *qs);
+
+/* ---- the FFT library of my_fft.f90, on one GPU -------------------------
+   my_fft.f90 (:5-82, :159-252) reaches its transforms through P3DFFT's
+   p3dfft_setup / p3dfft_get_dims / p3dfft_ftran_r2c / p3dfft_btran_c2r.
+   One GPU holds the whole domain, so these are plain on-device transforms
+   (the pencil transposes become LDS transposes inside the FFT kernels),
+   with P3DFFT's semantics: unnormalised, e^{-ikx} forward; op is a string
+   of three characters, 'f' = transform, 'n' = none, per dimension in
+   execution order: forward "fft" (x, y, z) or "ffn" (x, y), backward "fft"
+   or "nff" (y, x).  qp: real (nx, ny, nz); qs: complex (nx/2+1, ny, nz) as
+   interleaved doubles, x fastest (P3DFFT's spectral layout on one rank).
+   Host buffers; synchronous.  nx/2 and ny (and nz for a z transform) must
+   be products of 2, 3 and 5.  ps3d_amd/fortran/p3dfft_gpu.f90 is the
+   Fortran module p3dfft over these entry points. */
+typedef struct ps3d_fft ps3d_fft;
+int ps3d_fft_create(int nx, int ny, int nz, void *stream, ps3d_fft **out);
+int ps3d_fft_destroy(ps3d_fft *f);
+int ps3d_fft_forward(ps3d_fft *f, const char *op, const double *qp, double *qs);
+int ps3d_fft_backward(ps3d_fft *f, const char *op, const double *qs, double *qp);
 
 /* ---- launch-shape autotuner ------------------------------------------ */
 

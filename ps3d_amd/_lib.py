@@ -41,6 +41,7 @@ PROTOTYPES = {
     "ps3d_kernel_names": (ctypes.c_char_p, []),
     "ps3d_create": (_c_int, [ctypes.POINTER(CParams), _dp, _vp, ctypes.POINTER(_vp)]),
     "ps3d_destroy": (_c_int, [_vp]),
+    "ps3d_set_params": (_c_int, [_vp, ctypes.POINTER(CParams)]),
     "ps3d_set_field": (_c_int, [_vp, _c_int, _dp]),
     "ps3d_get_field": (_c_int, [_vp, _c_int, _dp]),
     "ps3d_field_size": (_c_int, [_vp, _c_int, ctypes.POINTER(_c_i64)]),
@@ -61,6 +62,10 @@ PROTOTYPES = {
     "ps3d_profile_kernel": (_c_int, [_vp, ctypes.c_char_p]),
     "ps3d_profile_read": (_c_int, [_vp, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_dbl)]),
     "ps3d_tune": (_c_int, [_vp, _c_int, ctypes.c_char_p, _c_int]),
+    "ps3d_fft_create": (_c_int, [_c_int, _c_int, _c_int, _vp, ctypes.POINTER(_vp)]),
+    "ps3d_fft_destroy": (_c_int, [_vp]),
+    "ps3d_fft_forward": (_c_int, [_vp, ctypes.c_char_p, _dp, _dp]),
+    "ps3d_fft_backward": (_c_int, [_vp, ctypes.c_char_p, _dp, _dp]),
     "ps3d_tuning": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
     "ps3d_diagnose": (_c_int, [_vp, _dp, _i32p]),
     "ps3d_set_forcing": (_c_int, [_vp, _vp, _vp]),

@@ -496,6 +496,39 @@ int ps3d_destroy(ps3d_model *m)
     });
 }
 
+int ps3d_set_params(ps3d_model *m, const ps3d_params *prm)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(prm, "null argument");
+        const ps3d_params &o = m->prm, &p = *prm;
+        // what the grid, the wavenumber tables and the solver factors were
+        // built from stays fixed for the model's lifetime
+        PS3D_REQUIRE(p.nx == o.nx && p.ny == o.ny && p.nz == o.nz && p.Lx == o.Lx && p.Ly == o.Ly &&
+                         p.Lz == o.Lz && p.dx == o.dx && p.dy == o.dy && p.dz == o.dz && p.dsqr == o.dsqr &&
+                         p.enable_vertical_boundaries == o.enable_vertical_boundaries &&
+                         p.precision == o.precision,
+                     "ps3d_set_params: only dt, Ro, eps_ab, f0, N0, the friction coefficients, "
+                     "enable_nonlinear and enable_AB_3_order can change after ps3d_create");
+        PS3D_REQUIRE(p.dt != 0.0, "dt must be nonzero");
+        PS3D_REQUIRE(!m->mixed || (!(p.Avbi > 0.) && !(p.Kvbi > 0.)),
+                     "PS3D_MIXED: vertical biharmonic friction (Avbi, Kvbi > 0) is only in the FP64 form");
+        for (int f = 0; f < 4; ++f)
+            PS3D_REQUIRE(m->cur[f] == 0, "ps3d_set_params: friction results not yet folded in by ab_update");
+        PS3D_HIP_CHECK(hipStreamSynchronize(m->stream));
+        ps3d_params &q = m->prm;
+        q.dt = p.dt; q.Ro = p.Ro; q.eps_ab = p.eps_ab; q.f0 = p.f0; q.N0 = p.N0;
+        q.Ah = p.Ah; q.Av = p.Av; q.Kh = p.Kh; q.Kv = p.Kv;
+        q.Ahbi = p.Ahbi; q.Avbi = p.Avbi; q.Khbi = p.Khbi; q.Kvbi = p.Kvbi;
+        q.enable_nonlinear = p.enable_nonlinear;
+        q.enable_AB_3_order = p.enable_AB_3_order;
+        StepConsts &c = m->c;
+        c.dt = p.dt; c.rdt = 1.0 / p.dt;
+        c.Ro = p.Ro; c.f0 = p.f0; c.N0 = p.N0;
+        c.nonlinear = p.enable_nonlinear;
+    });
+}
+
 int ps3d_field_size(const ps3d_model *m, int field, int64_t *n)
 {
     return guard([&] {

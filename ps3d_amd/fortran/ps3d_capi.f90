@@ -173,6 +173,37 @@ module ps3d_capi
    type(c_ptr), value :: model
    type(c_ptr), intent(in) :: f(4)
   end function
+  ! dt, Ro, friction, eps_ab, f0, N0 and the two switches of the next steps
+  integer(c_int) function ps3d_set_params(model, prm) bind(C, name="ps3d_set_params")
+   import :: c_int, c_ptr, ps3d_params
+   type(c_ptr), value :: model
+   type(ps3d_params), intent(in) :: prm
+  end function
+  ! the FFT library of my_fft.f90 on the device (op strings NUL-terminated)
+  integer(c_int) function ps3d_fft_create(nx, ny, nz, stream, plan) bind(C, name="ps3d_fft_create")
+   import :: c_int, c_ptr
+   integer(c_int), value :: nx, ny, nz
+   type(c_ptr), value :: stream
+   type(c_ptr), intent(out) :: plan
+  end function
+  integer(c_int) function ps3d_fft_destroy(plan) bind(C, name="ps3d_fft_destroy")
+   import :: c_int, c_ptr
+   type(c_ptr), value :: plan
+  end function
+  integer(c_int) function ps3d_fft_forward(plan, op, qp, qs) bind(C, name="ps3d_fft_forward")
+   import :: c_int, c_ptr, c_char, c_double
+   type(c_ptr), value :: plan
+   character(kind=c_char), intent(in) :: op(*)
+   real(c_double), intent(in) :: qp(*)
+   real(c_double), intent(out) :: qs(*)
+  end function
+  integer(c_int) function ps3d_fft_backward(plan, op, qs, qp) bind(C, name="ps3d_fft_backward")
+   import :: c_int, c_ptr, c_char, c_double
+   type(c_ptr), value :: plan
+   character(kind=c_char), intent(in) :: op(*)
+   real(c_double), intent(in) :: qs(*)
+   real(c_double), intent(out) :: qp(*)
+  end function
  end interface
 
 contains

@@ -19,7 +19,12 @@ Model method           reference routine it stands for
 ``border_exchg``       parallel_mpi.f90:70-81
 ``my_fft/my_ifft``     my_fft.f90:159-205
 ``spectral_solve``     pressure_solver.f90:30-71
+``set_params``         assignments to main_module's dt, Ro, friction
+                       between steps (diag_opt_balance.f90:190-205)
 =====================  ===============================================
+
+``FftPlan`` stands for P3DFFT's p3dfft_ftran_r2c / p3dfft_btran_c2r as
+my_fft.f90:159-252 calls them (unnormalised, one GPU).
 
 Device state is authoritative between ``push()`` and ``pull()``.
 """
@@ -131,6 +136,14 @@ class Model:
 
     def integrate(self):
         check(self.lib.ps3d_integrate(self.h, self.itt, self.tau, self.taum1, self.taum2))
+
+    def set_params(self, **changes):
+        """dt, Ro, eps_ab, f0, N0, friction coefficients or the nonlinear / AB3
+        switches of the next steps"""
+        prm = self.prm.replace(**changes)
+        c = CParams.from_params(prm)
+        check(self.lib.ps3d_set_params(self.h, ctypes.byref(c)))
+        self.prm, self._cprm = prm, c
 
     def run(self, nsteps: int, use_graph: bool = False):
         st = (ctypes.c_int32 * 4)(self.itt, self.tau, self.taum1, self.taum2)
@@ -349,3 +362,42 @@ class Model:
 
 def kernel_names() -> list[str]:
     return _lib.load().ps3d_kernel_names().decode().split(",")
+
+
+class FftPlan:
+    """The device transforms behind my_fft.f90 (P3DFFT's r2c / c2r on one GPU):
+    forward(qp, op) with op "ffn" (x, y) or "fft" (x, y, z), backward(qs, op)
+    with op "nff" or "fft"; unnormalised."""
+
+    def __init__(self, nx: int, ny: int, nz: int):
+        self.lib = _lib.load()
+        self.shape = (nx, ny, nz)
+        h = ctypes.c_void_p()
+        check(self.lib.ps3d_fft_create(nx, ny, nz, None, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(self.lib.ps3d_fft_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, qp: np.ndarray, op: str = "fft") -> np.ndarray:
+        nx, ny, nz = self.shape
+        qp = np.asfortranarray(qp, dtype=np.float64)
+        out = np.zeros((2, nx // 2 + 1, ny, nz), order="F")
+        check(self.lib.ps3d_fft_forward(self.h, op.encode(), _dp(qp), _dp(out)))
+        return out[0] + 1j * out[1]
+
+    def backward(self, qs: np.ndarray, op: str = "fft") -> np.ndarray:
+        nx, ny, nz = self.shape
+        buf = np.zeros((2, nx // 2 + 1, ny, nz), order="F")
+        buf[0], buf[1] = qs.real, qs.imag
+        out = np.zeros((nx, ny, nz), order="F")
+        check(self.lib.ps3d_fft_backward(self.h, op.encode(), _dp(buf), _dp(out)))
+        return out

@@ -1,0 +1,195 @@
+"""The reference's own program on the GPU: main.f90, my_fft.f90, diag_balance.f90,
+the diagnostics, restart.f90 and for_config/eddy1.f90, all compiled unchanged,
+linked with the drop-in integrate / pressure_solver and the device FFT module
+p3dfft (oracle/Makefile, target driver: oracle/_ref/ps3d_eddy1.x).  netCDF is
+absent from the image; its routines are replaced by recording no-ops
+(oracle/driver/nf_stub.f90), which also end main.f90's loop after K
+snapshots (by setting runlen = time), so the program writes its own restart.
+
+eddy1 exercises both kinds of integrate callers: its set_initial_conditions
+balances the eddy to third order through diag_balance (whose model_run
+sets the state on the host and integrates it ten times), and the main loop
+calls diag_balance again at every snapshot (diag_balance.f90:465-497), after
+which the main loop must continue from its own state.
+
+Run 1 (no restart file): the first step, the snapshot and balancing
+diagnostic at itt = 0, then the restart file.  Checked: the recorded
+u0..b3 against the oracle's balancing diagnostic of the restart state (zero
+and first order to 1e-12; second and third order to 20x the oracle's own
+conditioning, as in test_gpu_balance.py; the reference's eigenvectors.f90
+and vertical modes, evaluated in Fortran, differ from the oracle's numpy
+ones in the last bits).
+Run 2 (from run 1's restart file): 216 steps with snapshots and balancing
+diagnostics at itt = 108 and 216.  Checked bit for bit against the oracle
+stepping from the same restart state: both snapshots and the final restart
+file (u, v, w, b, p with halos, du..db interiors, itt and the tau slots).
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(ROOT, "oracle", "_ref", "ps3d_eddy1.x")
+RESTART = "restart_pe=000000000.dta"
+NAMES = [f"{c}{o}" for o in range(4) for c in ("u", "v", "w", "b")]
+
+
+def _run(workdir, snapshots):
+    cap = os.path.join(workdir, "cap")
+    os.makedirs(cap, exist_ok=True)
+    env = dict(os.environ, PS3D_DRIVER_CAPTURE=cap, PS3D_DRIVER_SNAPSHOTS=str(snapshots))
+    print(f"running {EXE} in {workdir}, {snapshots} snapshot(s)", flush=True)
+    r = subprocess.run([EXE], cwd=workdir, env=env, capture_output=True, text=True, timeout=600)
+    tail = "\n".join((r.stdout + r.stderr).splitlines()[-25:])
+    print(tail, flush=True)
+    assert r.returncode == 0, tail
+    assert os.path.exists(os.path.join(workdir, RESTART)), tail
+    return cap
+
+
+def _params(cap):
+    from ps3d_amd.params import ModelParams
+    vals = {}
+    for line in open(os.path.join(cap, "params.txt")):
+        k, *rest = line.split()
+        vals[k] = rest
+    nx, ny, nz = (int(v) for v in vals["grid"])
+    nonlin, lid, ab3 = (v == "T" for v in vals["switches"])
+    r = [float(v) for v in vals["reals"]]
+    names = ("Lx", "Ly", "Lz", "dx", "dy", "dz", "dt", "f0", "N0", "Ro", "dsqr", "eps_ab",
+             "Ah", "Av", "Kh", "Kv", "Ahbi", "Avbi", "Khbi", "Kvbi")
+    prm = ModelParams(nx=nx, ny=ny, nz=nz, enable_nonlinear=nonlin, enable_vertical_boundaries=lid,
+                      enable_AB_3_order=ab3, **dict(zip(names, r)))
+    tables = {n: np.fromfile(os.path.join(cap, n + ".bin"), dtype=np.float64) for n in ("kx", "ky", "kz", "ksqr")}
+    return prm, tables
+
+
+def _read_restart(path, prm):
+    """restart.f90:2-22 records, Fortran unformatted sequential (4-byte markers)"""
+    raw = open(path, "rb").read()
+    recs, o = [], 0
+    while o < len(raw):
+        n = int(np.frombuffer(raw, np.int32, 1, o)[0])
+        recs.append(raw[o + 4:o + 4 + n])
+        assert int(np.frombuffer(raw, np.int32, 1, o + 4 + n)[0]) == n
+        o += 8 + n
+    h = recs[0]
+    nx, ny, nz, itt = np.frombuffer(h, np.int32, 4, 0)
+    time = float(np.frombuffer(h, np.float64, 1, 16)[0])
+    taum2, taum1, tau = np.frombuffer(h, np.int32, 3, 24)
+    assert (nx, ny, nz) == (prm.nx, prm.ny, prm.nz)
+    H = (prm.nx + 4) * (prm.ny + 4) * (prm.nz + 4)
+    shp = (prm.nx + 4, prm.ny + 4, prm.nz + 4)
+    a = np.frombuffer(recs[2], np.float64)
+    assert a.size == 17 * H
+    st = {}
+    for q, n in enumerate(("u", "v", "w", "b", "p")):
+        st[n] = a[q * H:(q + 1) * H].reshape(shp, order="F").copy()
+    for q, n in enumerate(("du", "dv", "dw", "db")):
+        st[n] = a[(5 + 3 * q) * H:(8 + 3 * q) * H].reshape(shp + (3,), order="F").copy()
+    return st, (int(itt), int(tau), int(taum1), int(taum2)), time
+
+
+def _oracle(prm, tables, state, counters):
+    import oracle
+    orc = oracle.OracleModel(prm)
+    for n in ("kx", "ky", "kz"):
+        orc.f[n][...] = tables[n]
+    orc.f["ksqr"][...] = tables["ksqr"].reshape(orc.f["ksqr"].shape, order="F")
+    for n, a in state.items():
+        orc.f[n][...] = a
+    orc.itt, orc.tau, orc.taum1, orc.taum2 = counters
+    return orc
+
+
+def _captured(cap, fname, var, rec, prm):
+    a = np.fromfile(os.path.join(cap, f"{fname}.{var}.{rec:04d}"), dtype=np.float64)
+    return a.reshape((prm.nx, prm.ny, prm.nz), order="F")
+
+
+def _interior(a, prm):
+    return a[2:2 + prm.nx, 2:2 + prm.ny, 2:2 + prm.nz]
+
+
+def _sensitivity(prm, tables, state, counters, ref):
+    from oracle import balance
+    worst = {n: 0.0 for n in NAMES}
+    for seed in (201, 202):
+        rng = np.random.default_rng(seed)
+        st = {n: (a * (1.0 + 1e-15 * rng.standard_normal(a.shape)) if n in ("u", "v", "w", "b") else a)
+              for n, a in state.items()}
+        orc = _oracle(prm, tables, st, counters)
+        r = balance.Balance(orc).run()
+        for n in NAMES:
+            nb = float(np.linalg.norm(_interior(ref[n], prm)))
+            if nb > 0:
+                worst[n] = max(worst[n], float(np.linalg.norm(_interior(r[n] - ref[n], prm))) / nb)
+        orc.close()
+    return worst
+
+
+@pytest.fixture(scope="module")
+def runs(tmp_path_factory):
+    if not os.path.exists(EXE):
+        pytest.skip("reference driver not built (make -C oracle driver needs the reference sources)")
+    d1 = str(tmp_path_factory.mktemp("eddy1_run1"))
+    cap1 = _run(d1, 1)
+    d2 = str(tmp_path_factory.mktemp("eddy1_run2"))
+    shutil.copy(os.path.join(d1, RESTART), os.path.join(d2, RESTART))
+    cap2 = _run(d2, 2)
+    return d1, cap1, d2, cap2
+
+
+def test_reference_driver_balancing_matches_oracle(runs):
+    from oracle import balance
+    d1, cap1, _, _ = runs
+    prm, tables = _params(cap1)
+    assert (prm.nx, prm.ny, prm.nz) == (120, 120, 10) and prm.enable_vertical_boundaries
+    state, counters, time = _read_restart(os.path.join(d1, RESTART), prm)
+    assert counters[0] == 1 and time == prm.dt
+    # the snapshot at itt = 0 is the state the restart file holds
+    for n in ("u", "v", "w", "b", "p"):
+        assert np.array_equal(_captured(cap1, "snap.cdf", n, 1, prm), _interior(state[n], prm)), n
+    orc = _oracle(prm, tables, state, counters)
+    ref = {k: v.copy() for k, v in balance.Balance(orc).run().items()}
+    orc.close()
+    sens = _sensitivity(prm, tables, state, counters, ref)
+    for n in NAMES:
+        got = _captured(cap1, "diag_balance.cdf", n, 1, prm)
+        want = _interior(ref[n], prm)
+        nb = float(np.linalg.norm(want))
+        err = float(np.linalg.norm(got - want)) / nb
+        tol = 1e-12 if n[1] in "01" else max(1e-12, 20.0 * sens[n])
+        print(f"  {n}: rel L2 = {err:.2e} (tol {tol:.1e}, |ref| {nb:.2e})", flush=True)
+        assert err <= tol, n
+    # the balanced state is not trivial: the corrections are there
+    assert float(np.linalg.norm(_interior(ref["u1"], prm))) > 1e-6 * float(np.linalg.norm(_interior(ref["u0"], prm)))
+
+
+def test_reference_driver_steps_match_oracle(runs):
+    d1, cap1, d2, cap2 = runs
+    prm, tables = _params(cap1)
+    state, counters, _ = _read_restart(os.path.join(d1, RESTART), prm)
+    orc = _oracle(prm, tables, state, counters)
+    assert orc.itt == 1
+    orc.run(107)                      # steps itt = 1 .. 107
+    orc.run(1)                        # itt = 108: first snapshot of run 2
+    for n in ("u", "v", "w", "b", "p"):
+        assert np.array_equal(_captured(cap2, "snap.cdf", n, 1, prm), _interior(orc.f[n], prm)), n
+    orc.run(108)                      # itt = 109 .. 216, after the balancing diagnostic at 108
+    for n in ("u", "v", "w", "b", "p"):
+        assert np.array_equal(_captured(cap2, "snap.cdf", n, 2, prm), _interior(orc.f[n], prm)), n
+    final, fcounters, ftime = _read_restart(os.path.join(d2, RESTART), prm)
+    assert fcounters == (orc.itt, orc.tau, orc.taum1, orc.taum2) == (217,) + fcounters[1:]
+    for n in ("u", "v", "w", "b", "p"):
+        assert np.array_equal(final[n], orc.f[n]), n
+    for n in ("du", "dv", "dw", "db"):
+        for t in range(3):
+            assert np.array_equal(_interior(final[n][..., t], prm), _interior(orc.f[n][..., t], prm)), (n, t)
+    orc.close()
