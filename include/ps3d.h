@@ -283,6 +283,22 @@ int ps3d_opt_balance(ps3d_model *m, const ps3d_opt_balance_params *ob, const dou
    the reference aborts).  Fields stay on the device. */
 int ps3d_diagnose(ps3d_model *m, double *out, int32_t *nan_seen);
 
+/* The same monitors without any allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc), which ps3d_diagnose's first call on a model does.  u, v, w, b
+   are caller-owned FP64 device arrays in the field layout above (halo
+   included, nx x ny x nz from prm; prm's dt, dx, dz, dsqr, N0 and friction
+   coefficients enter as in ps3d_diagnose); stream: a hipStream_t or NULL.
+   The partial sums go to a fixed device scratch of 2049 records per GPU (one
+   per work-group of at most 2048, which stride over any grid, plus the
+   total), held until the process ends; grids with an extent above 2^20 are
+   refused.  Synchronous: returns after the 8 values are on the host.  Calls
+   that overlap on one GPU (from several threads or streams) race on the
+   scratch.  Same bits as ps3d_diagnose on the same fields. */
+int ps3d_diagnose_noalloc(const ps3d_params *prm, const double *u, const double *v, const double *w,
+                          const double *b, void *stream, double *out, int32_t *nan_seen);
+
 /* ---- restart records (restart.f90:2-82, one rank) ---------------------- */
 
 /* write_restart / read_restart of the model's whole state (u, v, w, b, p and

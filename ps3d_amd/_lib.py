@@ -68,6 +68,7 @@ PROTOTYPES = {
     "ps3d_fft_backward": (_c_int, [_vp, ctypes.c_char_p, _dp, _dp]),
     "ps3d_tuning": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
     "ps3d_diagnose": (_c_int, [_vp, _dp, _i32p]),
+    "ps3d_diagnose_noalloc": (_c_int, [ctypes.POINTER(CParams), _vp, _vp, _vp, _vp, _vp, _dp, _i32p]),
     "ps3d_set_forcing": (_c_int, [_vp, _vp, _vp]),
     "ps3d_device_ptr": (_c_int, [_vp, _c_int, _c_int, ctypes.POINTER(_vp)]),
     "ps3d_set_wavemaker": (_c_int, [_vp, _c_dbl, ctypes.POINTER(_dp), ctypes.POINTER(_dp)]),

@@ -941,6 +941,17 @@ int ps3d_diagnose(ps3d_model *m, double *out, int32_t *nan_seen)
     });
 }
 
+int ps3d_diagnose_noalloc(const ps3d_params *prm, const double *u, const double *v, const double *w, const double *b,
+                          void *stream, double *out, int32_t *nan_seen)
+{
+    return guard([&] {
+        PS3D_REQUIRE(prm && out, "null argument");
+        PS3D_REQUIRE(u && v && w && b, "null field");
+        const bool nan = monitor_noalloc(*prm, u, v, w, b, static_cast<hipStream_t>(stream), out);
+        if (nan_seen) *nan_seen = nan ? 1 : 0;
+    });
+}
+
 int ps3d_write_restart(ps3d_model *m, const char *path, int itt, double time, int tau, int taum1, int taum2)
 {
     return guard([&] {

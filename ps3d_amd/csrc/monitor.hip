@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #pragma clang fp contract(off)
@@ -159,6 +160,70 @@ void get_record(FILE *f, const std::vector<std::pair<void *, size_t>> &parts)
     PS3D_REQUIRE(done == total, "restart: record shorter than expected (grid mismatch?)");
 }
 
+// the monitors from the sums and maxima: diagnose.F90:14-41
+void monitor_finish(const MonOut &tot, const ps3d_params &p, double *out)
+{
+    const double n = (double)p.nx * p.ny * p.nz;
+    const double ke = tot.ke / n;
+    const double pe = tot.pe / n;
+    const double umax = tot.umax, wmax = tot.wmax;
+    const double cfl_h = umax * p.dt / p.dx;
+    const double cfl_v = wmax * p.dt / p.dz;
+    const double uw = std::fmax(umax, wmax);
+    const double pecl_h = std::fmin(uw * p.dx / (1e-32 + p.Ah), uw * p.dx * p.dx * p.dx / (1e-32 + p.Ahbi));
+    const double pecl_v = std::fmin(uw * p.dz / (1e-32 + p.Av), uw * p.dz * p.dz * p.dz / (1e-32 + p.Avbi));
+    const double v[8] = {ke, pe, umax, wmax, cfl_h, cfl_v, pecl_h, pecl_v};
+    std::memcpy(out, v, sizeof(v));
+}
+
+constexpr int kMaxBlocks = 2048;   // work-groups of k_monitor (grid-stride beyond)
+
+int monitor_blocks(int64_t n) { return (int)std::min<int64_t>((n + MT - 1) / MT, kMaxBlocks); }
+
+// The partial records of one k_monitor launch summed in block order into
+// part[nb]: the same sequence of additions and maxima as the host loop of
+// monitor(), so both give the same bits.
+__global__ void k_monitor_total(MonOut *part, int nb)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    MonOut t{0.0, 0.0, 0.0, 0.0, 0};
+    for (int q = 0; q < nb; ++q) {
+        const MonOut r = part[q];
+        t.ke = t.ke + r.ke;
+        t.pe = t.pe + r.pe;
+        t.umax = fmax(t.umax, r.umax);
+        t.wmax = fmax(t.wmax, r.wmax);
+        t.nan = t.nan | r.nan;
+    }
+    part[nb] = t;
+}
+
+// Scratch of ps3d_diagnose_noalloc: the partial record of each work-group of
+// k_monitor and, after them, the total.  One copy per GPU, held until the
+// process ends; its device address is looked up once per device (the name is
+// only a host-side stand-in for it).  Calls that overlap on one GPU race on it.
+constexpr int kScratchRecords = kMaxBlocks + 1;
+__device__ __align__(256) MonOut scratch[kScratchRecords];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+MonOut *scratch_ptr[kMaxDevices] = {};
+
+MonOut *scratch_on_current_device()
+{
+    int dev = -1;
+    PS3D_HIP_CHECK(hipGetDevice(&dev));
+    PS3D_REQUIRE(dev >= 0 && dev < kMaxDevices, "diagnose_noalloc: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        PS3D_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        PS3D_REQUIRE(ptr, "diagnose_noalloc: no device address for the scratch records");
+        scratch_ptr[dev] = static_cast<MonOut *>(ptr);
+    }
+    return scratch_ptr[dev];
+}
+
 }  // namespace
 
 // diagnose.F90:11-53 on the device; out = ke, pe, max|u_h|, max|w|, cfl_h,
@@ -166,8 +231,7 @@ void get_record(FILE *f, const std::vector<std::pair<void *, size_t>> &parts)
 bool monitor(Model &m, double *out)
 {
     const Grid &g = m.g;
-    constexpr int kMaxBlocks = 2048;
-    const int nb = (int)std::min<int64_t>((g.N + MT - 1) / MT, kMaxBlocks);
+    const int nb = monitor_blocks(g.N);
     if (!m.mon) {   // partial results, kept for the model's lifetime
         PS3D_HIP_CHECK(hipMalloc(&m.mon, sizeof(MonOut) * kMaxBlocks));
         m.allocations.push_back(m.mon);
@@ -179,27 +243,50 @@ bool monitor(Model &m, double *out)
     PS3D_HIP_CHECK(hipGetLastError());
     PS3D_HIP_CHECK(hipMemcpyAsync(h.data(), part, sizeof(MonOut) * nb, hipMemcpyDeviceToHost, m.stream));
     PS3D_HIP_CHECK(hipStreamSynchronize(m.stream));
-    double ke = 0.0, pe = 0.0, umax = 0.0, wmax = 0.0;
-    bool nan = false;
+    MonOut tot{0.0, 0.0, 0.0, 0.0, 0};
     for (const MonOut &r : h) {
-        ke = ke + r.ke;
-        pe = pe + r.pe;
-        umax = std::fmax(umax, r.umax);
-        wmax = std::fmax(wmax, r.wmax);
-        nan = nan || r.nan;
+        tot.ke = tot.ke + r.ke;
+        tot.pe = tot.pe + r.pe;
+        tot.umax = std::fmax(tot.umax, r.umax);
+        tot.wmax = std::fmax(tot.wmax, r.wmax);
+        tot.nan = tot.nan | r.nan;
     }
-    const ps3d_params &p = m.prm;
-    const double n = (double)p.nx * p.ny * p.nz;
-    ke = ke / n;
-    pe = pe / n;
-    const double cfl_h = umax * p.dt / p.dx;
-    const double cfl_v = wmax * p.dt / p.dz;
-    const double uw = std::fmax(umax, wmax);
-    const double pecl_h = std::fmin(uw * p.dx / (1e-32 + p.Ah), uw * p.dx * p.dx * p.dx / (1e-32 + p.Ahbi));
-    const double pecl_v = std::fmin(uw * p.dz / (1e-32 + p.Av), uw * p.dz * p.dz * p.dz / (1e-32 + p.Avbi));
-    const double v[8] = {ke, pe, umax, wmax, cfl_h, cfl_v, pecl_h, pecl_v};
-    std::memcpy(out, v, sizeof(v));
-    return nan;
+    monitor_finish(tot, m.prm, out);
+    return tot.nan != 0;
+}
+
+// the same monitors of caller-owned FP64 device fields, with the partial
+// records in the static scratch instead of model memory: no allocation
+bool monitor_noalloc(const ps3d_params &p, const double *u, const double *v, const double *w, const double *b,
+                     hipStream_t stream, double *out)
+{
+    PS3D_REQUIRE(p.nx >= 1 && p.ny >= 1 && p.nz >= 1, "diagnose_noalloc: nx, ny, nz must be >= 1");
+    PS3D_REQUIRE(p.nx <= (1 << 20) && p.ny <= (1 << 20) && p.nz <= (1 << 20),
+                 "diagnose_noalloc: nx, ny, nz must be <= 2^20");
+    StepConsts c{};
+    c.nx = p.nx;
+    c.ny = p.ny;
+    c.nz = p.nz;
+    c.X = p.nx + 2 * ONX;
+    c.XY = c.X * (p.ny + 2 * ONX);
+    c.dsqr = p.dsqr;
+    c.dz = p.dz;
+    const int64_t n = (int64_t)p.nx * p.ny * p.nz;
+    const int nb = monitor_blocks(n);
+    // the scratch holds one record per work-group plus the total; k_monitor
+    // strides over any grid with at most kMaxBlocks work-groups
+    PS3D_REQUIRE(nb + 1 <= kScratchRecords, "diagnose_noalloc: more work-groups than scratch records");
+    MonOut *part = scratch_on_current_device();
+    FieldRef fu{u, nullptr, nullptr}, fv{v, nullptr, nullptr}, fw{w, nullptr, nullptr}, fb{b, nullptr, nullptr};
+    hipLaunchKernelGGL(k_monitor, dim3(nb), dim3(MT), 0, stream, c, fu, fv, fw, fb, p.N0, part);
+    PS3D_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_monitor_total, dim3(1), dim3(64), 0, stream, part, nb);
+    PS3D_HIP_CHECK(hipGetLastError());
+    MonOut tot;
+    PS3D_HIP_CHECK(hipMemcpyAsync(&tot, part + nb, sizeof(MonOut), hipMemcpyDeviceToHost, stream));
+    PS3D_HIP_CHECK(hipStreamSynchronize(stream));
+    monitor_finish(tot, p, out);
+    return tot.nan != 0;
 }
 
 // restart.f90:2-22 / :27-82 on host arrays: state = u, v, w, b, p, du, dv,

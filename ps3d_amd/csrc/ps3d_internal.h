@@ -327,6 +327,10 @@ std::string describe_tuning(const Model &m);
 
 // monitors and restart records (monitor.hip)
 bool monitor(Model &m, double *out8);
+// the same on caller-owned FP64 device fields, partial records in a static
+// device scratch (no allocation); stream may be null (the null stream)
+bool monitor_noalloc(const ps3d_params &p, const double *u, const double *v, const double *w, const double *b,
+                     hipStream_t stream, double *out8);
 void restart_write_host(const char *path, int nx, int ny, int nz, int32_t itt, double time, int32_t tau,
                         int32_t taum1, int32_t taum2, const double *state);
 void restart_read_host(const char *path, int nx, int ny, int nz, int32_t *itt, double *time, int32_t *tau,
