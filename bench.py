@@ -55,6 +55,9 @@ def algorithmic_bytes(name: str, prm, order: int = 3) -> float:
         "biharmonic_v": 2 * e * N,
         "ab_update": 4 * (2 + order) * e * N,  # per field R u, q slots  W u
         "friction_ab": 4 * (2 + order) * e * N,  # friction chain fused into the AB pass: same I/O
+        # tendency + friction + AB: per field R f and the q-1 older slots,
+        # W slot tau and the new f -- the same count
+        "tend_friction_ab": 4 * (2 + order) * e * N,
         "fft_x_r2c": 3 * e * N + 2 * e * S,    # R u,v,w (divergence)  W FS
         "fft_y_fwd": 4 * e * S,                # R + W complex FS
         "fft_y_inv": 4 * e * S,
@@ -77,6 +80,26 @@ def measured_traffic(kernel: str):
             return json.load(f)["traffic_bytes_per_launch"].get(kernel)
     except (OSError, ValueError, KeyError):
         return None
+
+
+def hbm_probe(mib: int = 1024, reps: int = 10) -> dict:
+    """Rates of the library's hand-written streaming kernels (ps3d_hbm_probe,
+    probe.hip) on this GPU: copy, read and write of a MIB-MiB buffer, plain
+    and nontemporal, best of reps launches; 'ceiling' is the faster copy,
+    the achievable rate for kernels that read and write."""
+    import ctypes
+    import numpy as np
+    from ps3d_amd import _lib
+    lib = _lib.load()
+    out = np.zeros(8)
+    _lib.check(lib.ps3d_hbm_probe(mib << 20, reps, 0, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    names = ("copy", "copy_nt", "read", "read_nt", "write", "write_nt")
+    r = {n: float(v) for n, v in zip(names, out[:6])}
+    r["ceiling"] = max(r["copy"], r["copy_nt"])
+    r["workgroups"] = int(out[6])
+    r["buffer_mib"] = mib
+    r["unit"] = "GB/s"
+    return r
 
 
 def build_workload(precision=0):
@@ -291,6 +314,80 @@ def cpu_baseline(prm, fields, budget_s: float = 15.0, threads: int = 0) -> dict:
                       f"oracle/ps3d_oracle.c, {wall:.1f} s"}
 
 
+def _reference_worker(args):
+    """One process: the reference's own integrate.f90 (compiled unchanged into
+    oracle/_ref/libps3d_ref.so; module globals, so one model per process)
+    stepped up to its pressure solve, and the port doing the same part, on
+    the same state; returns (steps, seconds) of each."""
+    budget_s, small = args
+    import oracle
+    if small:   # a test of this leg on a small grid
+        from ps3d_amd import configs
+        prm, fields = configs.jets(32, 32, 16, seed=2)
+    else:
+        prm, fields = build_workload()
+    ref = oracle.ReferenceModel(prm)
+    orc = oracle.OracleModel(prm)
+    for n, a in fields.items():
+        ref.f[n][...] = a
+        orc.f[n][...] = a
+    out = {}
+    for tag, mdl in (("reference", ref), ("port", orc)):
+        mdl.integrate_pre()   # untimed first step (Euler)
+        mdl.rotate()
+        n, t0 = 0, time.perf_counter()
+        while True:
+            mdl.integrate_pre()
+            mdl.rotate()
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= budget_s or n >= 100:
+                break
+        out[tag] = (n, el)
+    # the port's pressure solve on its current state
+    n, t0 = 0, time.perf_counter()
+    while True:
+        orc.pressure_solver()
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s / 2 or n >= 100:
+            break
+    out["port_pressure"] = (n, el)
+    orc.close()
+    return out
+
+
+def reference_cpu_leg(prm, procs: int, budget_s: float = 4.0, small: bool = False) -> dict | None:
+    """The reference's own Fortran timed on the host beside the port: its
+    integrate.f90 up to the pressure solve (pressure_solver.f90 and my_fft.f90
+    need P3DFFT, absent from the image, so they are not built) and the port's
+    pressure solve, one process per core, all at once."""
+    import multiprocessing as mp
+    import oracle
+    if not oracle.reference_available():
+        return None
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(procs) as pool:
+        res = pool.map(_reference_worker, [(budget_s, small)] * procs)
+    N = prm.npoints
+
+    def rate(tag):
+        return sum(N * r[tag][0] / r[tag][1] for r in res)
+
+    ref_pre, port_pre, port_p = rate("reference"), rate("port"), rate("port_pressure")
+    # per-point seconds of the parts add up to a step
+    step = 1.0 / (1.0 / ref_pre + 1.0 / port_p)
+    return {"value": step, "unit": "grid-pt steps/s", "cores": procs, "kind": "reference",
+            "integrate_before_pressure": {"reference": ref_pre, "port": port_pre,
+                                          "port_over_reference": port_pre / ref_pre},
+            "pressure_solve_port": port_p,
+            "sample": f"{procs} processes at once, each one {prm.nx}x{prm.ny}x{prm.nz} jets model: the reference's own "
+                      f"integrate.f90 (for_src/, amdflang -O2, unchanged; oracle/_ref) up to its pressure "
+                      f"solve, and the C port's pressure solve (the reference's needs P3DFFT, absent), "
+                      f"~{budget_s:.0f} s each; value = aggregate step rate of reference part + port "
+                      f"pressure solve"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -310,17 +407,16 @@ def main() -> int:
     ap.add_argument("--mixed-check", default="1,100,1000",
                     help="steps at which the mixed leg reports its error against FP64 (comma list)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="only run a device-to-device copy of MIB MiB (to calibrate HBM byte counters)")
+                    help="only run the streaming probe kernels on MIB MiB (to calibrate HBM byte counters)")
+    ap.add_argument("--tendency-form", choices=("default", "separate", "fused"), default="default",
+                    help="tendencies in a kernel of their own (separate, the default) or formed inside the "
+                         "friction + AB kernel (fused)")
     args = ap.parse_args()
     if args.copy_probe:
-        import torch
-        n = args.copy_probe << 20
-        a = torch.ones(n, dtype=torch.uint8, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": n, "dispatches": 3}), flush=True)
+        # the library's streaming kernels only (k_copy moves 2 x MIB MiB per
+        # dispatch): the known-size kernels that calibrate the byte counters
+        print(json.dumps({"hbm_probe": hbm_probe(args.copy_probe, reps=2),
+                          "copy_bytes_per_dispatch": 2 * (args.copy_probe << 20)}), flush=True)
         return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} is not "
@@ -330,8 +426,11 @@ def main() -> int:
 
     from ps3d_amd.model import Model, kernel_names
 
+    hbm = hbm_probe()
     prm, fields = build_workload()
     model = Model(prm)
+    if args.tendency_form != "default":
+        model.set_fused_tendency({"separate": 0, "fused": 1}[args.tendency_form])
     for n, a in fields.items():
         model.f[n][...] = a
     model.push()
@@ -381,7 +480,9 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
                      "launches": launches, "mean_launch_us": per_launch_s * 1e6,
-                     "algorithmic_bytes_per_launch": traffic_bytes},
+                     "algorithmic_bytes_per_launch": traffic_bytes,
+                     "achievable": hbm["ceiling"], "frac_of_achievable": achieved / hbm["ceiling"],
+                     "probe": hbm},
     }
     if args.profile_all:
         out["kernel_shares_ms_per_step"] = {k: v[1] for k, v in sorted(shares.items(), key=lambda x: -x[1][1])}
@@ -397,7 +498,14 @@ def main() -> int:
     if args.configs:
         out["configs"] = configs_leg(cases=tuple(c.strip() for c in args.configs.split(",") if c.strip()))
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(prm, fields, threads=args.cpu_threads)
+        port = cpu_baseline(prm, fields, threads=args.cpu_threads)
+        ref = reference_cpu_leg(prm, port["cores"])
+        if ref is not None:
+            # the reference's own code where it builds; the port's whole step beside it
+            ref["port_whole_step"] = port
+            out["cpu_baseline"] = ref
+        else:
+            out["cpu_baseline"] = port
     print(json.dumps(out), flush=True)
     return 0
 

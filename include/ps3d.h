@@ -127,6 +127,16 @@ int ps3d_run(ps3d_model *m, int nsteps, int32_t *state, int use_graph);
 /* wait for the model's stream */
 int ps3d_synchronize(ps3d_model *m);
 
+/* Whether an unforced FP64 step forms the tendencies (integrate.f90:12-54)
+   inside the fused friction + Adams-Bashforth kernel (on = 1) or in a
+   kernel of their own that writes slot tau for the friction kernel to read
+   (on = 0, the default).  Both give the same bits; the fused form moves a
+   third fewer bytes (it reads the state once and never reads slot tau back)
+   but is limited by its instruction issue and runs slower on MI355X.  No
+   counterpart in the reference, whose loops are fixed: a switch for tests
+   and measurements. */
+int ps3d_set_fused_tendency(ps3d_model *m, int on);
+
 /* ---- forcing: the set_forcing hook of integrate.f90:82 ----------------
    A forcing hook runs where the reference calls set_forcing: after the
    friction block (:59-78) and before the Adams-Bashforth update (:87-102),
@@ -326,6 +336,16 @@ int ps3d_profile_read(ps3d_model *m, int64_t *launches, double *total_ms);
 /* comma-separated names of the model's kernels */
 const char *ps3d_kernel_names(void);
 
+/* HBM rates of hand-written streaming kernels on the current GPU, GB/s:
+   out[0] copy (read + write bytes counted), out[1] copy with nontemporal
+   loads and stores, out[2] read only, out[3] nontemporal read, out[4] write
+   only, out[5] nontemporal write; out[6] the work-group count used.  bytes:
+   size of each of the two buffers allocated (a multiple of 16, >= 1 MiB);
+   the best of reps launches after one warm-up; grid: work-groups (0: eight
+   per CU, grid-strided).  No counterpart in the reference: it measures the
+   achievable bandwidth that the step's kernels are compared with. */
+int ps3d_hbm_probe(int64_t bytes, int reps, int grid, double *out);
+
 /* ---- batched ensemble ------------------------------------------------
    No counterpart in the reference, which advances one model per process
    (main.f90:138-162); an ensemble there is a set of separate runs.  Here
@@ -336,7 +356,7 @@ const char *ps3d_kernel_names(void);
    what a single model with the same parameters computes (the fused-friction
    configurations: Avbi = Kvbi = 0; FFT lengths among the compiled-in ones,
    column lengths ny (and nz without a lid) <= 512).  The members share one
-   precision form: PS3D_FP64, or PS3D_MIXED (rigid-lid members of at most 85
+   precision form: PS3D_FP64, or PS3D_MIXED (rigid-lid members of at most 213
    levels), whose first run starts every member's pressure history as a
    single mixed model's first step does; that step and the next one (which
    add a correction solve) are queued directly, later ones replay the graph. */

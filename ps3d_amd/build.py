@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libps3d.so")
 SOURCES = ["stencil.hip", "fft.hip", "tridiag.hip", "api.hip", "ensemble.hip", "balance.hip", "monitor.hip", "tune.hip",
-           "transforms.hip"]
+           "transforms.hip", "probe.hip"]
 ARCH = os.environ.get("PS3D_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 # packed FP32 instructions are kept out of the generated code

@@ -282,8 +282,13 @@ void integrate(Model &m, int itt, int tau, int taum1, int taum2)
         integrate_mixed(m, itt, tau, taum1, taum2);
         return;
     }
-    launch_tendency(m, tau);
     const bool forced = m.forcing || m.wm_on || background_on(m);
+    if (fused_tendency_active(m)) {
+        launch_friction_ab(m, itt, tau, taum1, taum2, true);   // tendency formed inside
+        pressure(m);
+        return;
+    }
+    launch_tendency(m, tau);
     if (fused_friction_ok(m) && m.fuse_friction && !forced) {
         launch_friction_ab(m, itt, tau, taum1, taum2);
     } else if (fused_friction_ok(m) && m.fuse_friction) {
@@ -304,7 +309,7 @@ void integrate(Model &m, int itt, int tau, int taum1, int taum2)
 }
 
 const char *kKernelNames =
-    "tendency,friction_ab,harmonic,biharmonic_h,biharmonic_v,ab_update,halo,fft_x_r2c,fft_y_fwd,thomas,"
+    "tendency,friction_ab,tend_friction_ab,harmonic,biharmonic_h,biharmonic_v,ab_update,halo,fft_x_r2c,fft_y_fwd,thomas,"
     "fft_y_inv,fft_x_c2r,z_solve,fft_z_fwd,fft_z_inv,project";
 
 }  // namespace
@@ -928,6 +933,15 @@ int ps3d_clear_wavemaker(ps3d_model *m)
         check_model(m);
         m->wm_on = false;
         for (int f = 0; f < 4; ++f) m->wm_field[f] = false;
+    });
+}
+
+int ps3d_set_fused_tendency(ps3d_model *m, int on)
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(on == 0 || on == 1, "on must be 0 or 1");
+        m->fuse_tendency = on != 0;
     });
 }
 

@@ -346,8 +346,9 @@ int ps3d_ensemble_create(int n, const ps3d_params *prms, void *stream, ps3d_ense
             PS3D_REQUIRE(p.precision == PS3D_FP64 || p.precision == PS3D_MIXED, who + "unknown precision form");
             PS3D_REQUIRE(p.precision == prms[0].precision, who + "the members of an ensemble share one precision form");
             PS3D_REQUIRE(p.precision != PS3D_MIXED || !p.enable_vertical_boundaries ||
-                             (sizeof(double2) + sizeof(double)) * (size_t)ens_thomas_cols() * (size_t)p.nz <= 64 * 1024,
-                         who + "mixed ensembles: rigid-lid members of at most 85 levels");
+                             (sizeof(double2) + sizeof(double)) * (size_t)ens_thomas_cols() * (size_t)p.nz <=
+                                 160 * 1024,
+                         who + "mixed ensembles: rigid-lid members of at most 213 levels");
             PS3D_REQUIRE(!(p.Avbi > 0.) && !(p.Kvbi > 0.),
                          who + "vertical biharmonic friction (Avbi, Kvbi > 0) is not in the batched step");
             PS3D_REQUIRE(p.nx >= 2 && p.ny >= 2 && ens_fft_size_ok(p.nx / 2, false) && ens_fft_size_ok(p.ny, true) &&
@@ -525,14 +526,10 @@ int ps3d_ensemble_tune(ps3d_ensemble *e, const char *cache_path, int reps)
     return guard([&] {
         PS3D_REQUIRE(e, "null ensemble");
         reps = std::max(1, reps);
-        // the cache file: one "stage shape levels" line per entry
-        std::map<std::string, int> cache;
-        if (cache_path) {
-            std::ifstream in(cache_path);
-            std::string st, sh;
-            int kc;
-            while (in >> st >> sh >> kc) cache[st + " " + sh] = kc;
-        }
+        // the cache file: one "stage shape levels" line per entry; the
+        // stages are named ens_* (a batched launch, timed over all members,
+        // is a different measurement from a single model's)
+        std::map<std::string, int> cache = tune_cache_read(cache_path);
         std::vector<std::string> shapes;
         for (ps3d_model *m : e->mem) {
             const std::string k = shape_key(m->prm);
@@ -541,7 +538,7 @@ int ps3d_ensemble_tune(ps3d_ensemble *e, const char *cache_path, int reps)
         hipEvent_t t0, t1;
         PS3D_HIP_CHECK(hipEventCreate(&t0));
         PS3D_HIP_CHECK(hipEventCreate(&t1));
-        const char *names[2] = {"tendency", "friction_ab"};
+        const char *names[2] = {"ens_tendency", "ens_friction_ab"};
         const int stages[2] = {S_TEND, S_FRIC};
         try {
             for (int si = 0; si < 2; ++si) {
@@ -593,10 +590,7 @@ int ps3d_ensemble_tune(ps3d_ensemble *e, const char *cache_path, int reps)
         (void)hipEventDestroy(t1);
         build_maps(*e);
         upload_table(*e);
-        if (cache_path) {
-            std::ofstream out(cache_path, std::ios::trunc);
-            for (auto &kv : cache) out << kv.first << " " << kv.second << "\n";
-        }
+        tune_cache_write(cache_path, cache);
     });
 }
 
@@ -610,8 +604,8 @@ int ps3d_ensemble_describe(ps3d_ensemble *e, char *buf, int len)
             seen[shape_key(e->mem[i]->prm)] = {e->tab[i].tend_kc, e->tab[i].fric_kc};
         std::ostringstream o;
         for (auto &kv : seen) {
-            o << "tendency " << kv.first << " " << kv.second.first << "\n";
-            o << "friction_ab " << kv.first << " " << kv.second.second << "\n";
+            o << "ens_tendency " << kv.first << " " << kv.second.first << "\n";
+            o << "ens_friction_ab " << kv.first << " " << kv.second.second << "\n";
         }
         const std::string s = o.str();
         n = (int)s.size();

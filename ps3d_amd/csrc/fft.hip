@@ -825,7 +825,8 @@ bool launch_zsolve(Model &m, C *data, const C *tw, int nkxs)
 }
 
 // the sizes compiled in; others take the generic FP64 kernels
-#define PS3D_FFT_SIZES(X) X(8) X(16) X(32) X(48) X(64) X(96) X(128) X(256) X(512) X(1024)
+#define PS3D_FFT_SIZES(X) X(8) X(10) X(12) X(15) X(16) X(20) X(24) X(30) X(32) X(40) X(48) X(50) X(60) X(64) \
+    X(80) X(96) X(100) X(120) X(128) X(160) X(240) X(256) X(512) X(1024)
 
 template <class RT>
 bool dispatch_r2c(Model &m, const RT *src, const RT *u, const RT *v, const RT *w, int fuse_div,

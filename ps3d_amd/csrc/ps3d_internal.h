@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -210,6 +211,10 @@ struct Model {
     int cur[4] = {0, 0, 0, 0};
     // use the fused friction + Adams-Bashforth kernel where it applies
     bool fuse_friction = true;
+    // form the tendency inside it (unforced FP64 steps): reads the state
+    // once and never reads slot tau back, but measured slower than the
+    // separate tendency kernel (ps3d_set_fused_tendency), so off by default
+    bool fuse_tendency = false;
     // producers write the halo images of what they update (halo_images_ok)
     bool fuse_halo = false;
     // launch shapes (results do not depend on them; set by ps3d_tune):
@@ -217,6 +222,10 @@ struct Model {
     // columns per 64-lane wave of the Thomas solve (64, 32 or 16; 0: the
     // widest whose on-the-fly form fits its LDS)
     int tend_kc = 0, fric_kc = 0, thomas_cpw = 0;
+    int tfric_kc = 0;   // the same for the kernel with the tendency fused in
+    // FP64 Thomas solve with whole columns in registers where nz is one of
+    // the compiled-in column lengths (k_thomas_reg), else k_thomas_fly
+    bool thomas_reg = true;
 
     // mixed form (PS3D_MIXED): prognostic fields as FP32 (hi, lo) pairs whose
     // sum carries the FP64-accumulated state.  The pressure is carried in
@@ -284,7 +293,9 @@ void launch_background(Model &m, int tau, float *const *mixed_fields = nullptr);
 void launch_friction_forcing_ab_mixed(Model &m, int itt, int tau, int taum1, int taum2, double qc1, const float *d1,
                                       double qc2, const float *d2);
 bool fused_friction_ok(const Model &m);
-void launch_friction_ab(Model &m, int itt, int tau, int taum1, int taum2);
+// with_tendency: the tendency of integrate.f90:12-54 formed in the same
+// kernel (k_tend_fric_ab) instead of read from slot tau, which it writes
+void launch_friction_ab(Model &m, int itt, int tau, int taum1, int taum2, bool with_tendency = false);
 void launch_border_exchg(Model &m, double *a, int neumann_z);
 void launch_project_exchange(Model &m);
 
@@ -308,7 +319,9 @@ void tridiag_factor(Model &m);
 void launch_thomas(Model &m, double2 *data);
 void launch_thomas_f(Model &m, float2 *data);
 
-void launch_friction_ab_raw(Model &m, int itt, int tau, int taum1, int taum2);
+void launch_friction_ab_raw(Model &m, int itt, int tau, int taum1, int taum2, bool with_tendency = false);
+// the step takes k_tend_fric_ab (tendency, friction and AB in one kernel)
+bool fused_tendency_active(const Model &m);
 // forced FP64 step: the fused friction chain alone into the second buffers
 // (halos pre-friction), marked current for the forcing and launch_ab_exchange
 void launch_friction_fused_only(Model &m);
@@ -323,6 +336,10 @@ void launch_divide_ksqr(Model &m, double2 *data);
 void balance_release(Model &m);
 // the single-model autotuner (tune.hip)
 void tune_model(Model &m, int next_tau, const char *cache_path, int reps);
+// tuning cache files of "stage shape value" lines: read (missing file: empty;
+// malformed line: error) and written atomically (temporary file, rename)
+std::map<std::string, int> tune_cache_read(const char *path);
+void tune_cache_write(const char *path, const std::map<std::string, int> &cache);
 std::string describe_tuning(const Model &m);
 
 // monitors and restart records (monitor.hip)

@@ -144,6 +144,107 @@ struct TendArgs {
     int kc, nchunk, tiles_x, tiles_y, nblocks;
 };
 
+// The flux-form advection terms at interior point (i, j, k) of level k,
+// advection.f90:2-184: F(f, dk, dj, di) is field f (0..3: u, v, w, b) at
+// (i+di, j+dj, k+dk).  Each returns -Ro * (flux divergence) (times dsqr for
+// w), the expressions and their association those of the reference, so
+// every kernel that evaluates them through these helpers gets the same bits.
+template <class T, class A>
+__device__ __forceinline__ T adv_b(const StepConsts &c, int k, A F)
+{
+    const bool lid = c.lid != 0;
+    const bool top0 = lid && (k >= c.nz);
+    const bool bot0 = lid && (k - 1 <= 0);
+    const T h = T(0.5), zero = T(0), Ro = (T)c.Ro;
+    const T fe1 = h * (F(3, 0, 0, 0) + F(3, 0, 0, 1)) * F(0, 0, 0, 0);
+    const T fe0 = h * (F(3, 0, 0, -1) + F(3, 0, 0, 0)) * F(0, 0, 0, -1);
+    const T fn1 = h * (F(3, 0, 0, 0) + F(3, 0, 1, 0)) * F(1, 0, 0, 0);
+    const T fn0 = h * (F(3, 0, -1, 0) + F(3, 0, 0, 0)) * F(1, 0, -1, 0);
+    const T ft1 = top0 ? zero : h * (F(3, 0, 0, 0) + F(3, 1, 0, 0)) * F(2, 0, 0, 0);
+    const T ft0 = bot0 ? zero : h * (F(3, -1, 0, 0) + F(3, 0, 0, 0)) * F(2, -1, 0, 0);
+    return -(Ro * (sdiv<T>(fe1 - fe0, c.dx, c.rdx) + sdiv<T>(fn1 - fn0, c.dy, c.rdy) +
+                   sdiv<T>(ft1 - ft0, c.dz, c.rdz)));
+}
+
+template <class T, class A>
+__device__ __forceinline__ T adv_u(const StepConsts &c, int k, A F)
+{
+    const bool lid = c.lid != 0;
+    const bool top0 = lid && (k >= c.nz);
+    const bool bot0 = lid && (k - 1 <= 0);
+    const T q4 = T(0.25), zero = T(0), Ro = (T)c.Ro;
+    const T fe1 = q4 * (F(0, 0, 0, 0) + F(0, 0, 0, 1)) * (F(0, 0, 0, 1) + F(0, 0, 0, 0));
+    const T fe0 = q4 * (F(0, 0, 0, -1) + F(0, 0, 0, 0)) * (F(0, 0, 0, 0) + F(0, 0, 0, -1));
+    const T fn1 = q4 * (F(0, 0, 0, 0) + F(0, 0, 1, 0)) * (F(1, 0, 0, 1) + F(1, 0, 0, 0));
+    const T fn0 = q4 * (F(0, 0, -1, 0) + F(0, 0, 0, 0)) * (F(1, 0, -1, 1) + F(1, 0, -1, 0));
+    const T ft1 = top0 ? zero : q4 * (F(0, 1, 0, 0) + F(0, 0, 0, 0)) * (F(2, 0, 0, 0) + F(2, 0, 0, 1));
+    const T ft0 = bot0 ? zero : q4 * (F(0, 0, 0, 0) + F(0, -1, 0, 0)) * (F(2, -1, 0, 0) + F(2, -1, 0, 1));
+    return -(Ro * (sdiv<T>(fe1 - fe0, c.dx, c.rdx) + sdiv<T>(fn1 - fn0, c.dy, c.rdy) +
+                   sdiv<T>(ft1 - ft0, c.dz, c.rdz)));
+}
+
+template <class T, class A>
+__device__ __forceinline__ T adv_v(const StepConsts &c, int k, A F)
+{
+    const bool lid = c.lid != 0;
+    const bool top0 = lid && (k >= c.nz);
+    const bool bot0 = lid && (k - 1 <= 0);
+    const T q4 = T(0.25), zero = T(0), Ro = (T)c.Ro;
+    const T fe1 = q4 * (F(1, 0, 0, 0) + F(1, 0, 0, 1)) * (F(0, 0, 1, 0) + F(0, 0, 0, 0));
+    const T fe0 = q4 * (F(1, 0, 0, -1) + F(1, 0, 0, 0)) * (F(0, 0, 1, -1) + F(0, 0, 0, -1));
+    const T fn1 = q4 * (F(1, 0, 0, 0) + F(1, 0, 1, 0)) * (F(1, 0, 1, 0) + F(1, 0, 0, 0));
+    const T fn0 = q4 * (F(1, 0, -1, 0) + F(1, 0, 0, 0)) * (F(1, 0, 0, 0) + F(1, 0, -1, 0));
+    const T ft1 = top0 ? zero : q4 * (F(1, 1, 0, 0) + F(1, 0, 0, 0)) * (F(2, 0, 0, 0) + F(2, 0, 1, 0));
+    const T ft0 = bot0 ? zero : q4 * (F(1, 0, 0, 0) + F(1, -1, 0, 0)) * (F(2, -1, 0, 0) + F(2, -1, 1, 0));
+    return -(Ro * (sdiv<T>(fe1 - fe0, c.dx, c.rdx) + sdiv<T>(fn1 - fn0, c.dy, c.rdy) +
+                   sdiv<T>(ft1 - ft0, c.dz, c.rdz)));
+}
+
+template <class T, class A>
+__device__ __forceinline__ T adv_w(const StepConsts &c, int k, A F)
+{
+    const bool lid = c.lid != 0;
+    const bool bot0 = lid && (k - 1 <= 0);
+    const bool top0w = lid && (k >= c.nz - 1);
+    const T q4 = T(0.25), zero = T(0), Ro = (T)c.Ro;
+    const T fe1 = q4 * (F(2, 0, 0, 0) + F(2, 0, 0, 1)) * (F(0, 0, 0, 0) + F(0, 1, 0, 0));
+    const T fe0 = q4 * (F(2, 0, 0, -1) + F(2, 0, 0, 0)) * (F(0, 0, 0, -1) + F(0, 1, 0, -1));
+    const T fn1 = q4 * (F(2, 0, 0, 0) + F(2, 0, 1, 0)) * (F(1, 0, 0, 0) + F(1, 1, 0, 0));
+    const T fn0 = q4 * (F(2, 0, -1, 0) + F(2, 0, 0, 0)) * (F(1, 0, -1, 0) + F(1, 1, -1, 0));
+    const T ft1 = top0w ? zero : q4 * (F(2, 1, 0, 0) + F(2, 0, 0, 0)) * (F(2, 0, 0, 0) + F(2, 1, 0, 0));
+    const T ft0 = (bot0 || (lid && k - 1 >= c.nz - 1))
+                      ? zero
+                      : q4 * (F(2, 0, 0, 0) + F(2, -1, 0, 0)) * (F(2, -1, 0, 0) + F(2, 0, 0, 0));
+    return -(((T)c.dsqr * Ro) * (sdiv<T>(fe1 - fe0, c.dx, c.rdx) + sdiv<T>(fn1 - fn0, c.dy, c.rdy) +
+                                 sdiv<T>(ft1 - ft0, c.dz, c.rdz)));
+}
+
+// The FP64 tendencies of integrate.f90:12-54 at interior point (i, j, k):
+// advection (when enable_nonlinear), then the inline terms -- N0^2 w for b,
+// Coriolis for u and v, buoyancy and the lid for w -- as the reference adds
+// them.  F as for adv_b.
+template <int FLD, class A>
+__device__ __forceinline__ double tendency_fp64(const StepConsts &c, int k, A F)
+{
+    const double h = 0.5, q4 = 0.25;
+    double t = 0.0;
+    if constexpr (FLD == 0) {
+        if (c.nonlinear) t = adv_u<double>(c, k, F);
+        t = t + (F(1, 0, 0, 0) + F(1, 0, -1, 0) + F(1, 0, 0, 1) + F(1, 0, -1, 1)) * q4 * c.f0;
+    } else if constexpr (FLD == 1) {
+        if (c.nonlinear) t = adv_v<double>(c, k, F);
+        t = t - (F(0, 0, 0, -1) + F(0, 0, 0, 0) + F(0, 0, 1, -1) + F(0, 0, 1, 0)) * q4 * c.f0;
+    } else if constexpr (FLD == 2) {
+        if (c.nonlinear) t = adv_w<double>(c, k, F);
+        t = t + h * (F(3, 0, 0, 0) + F(3, 1, 0, 0));
+        if (c.lid != 0 && k >= c.nz) t = 0.0;
+    } else {
+        if (c.nonlinear) t = adv_b<double>(c, k, F);
+        t = t - h * (F(2, 0, 0, 0) + F(2, -1, 0, 0)) * (c.N0 * c.N0);
+    }
+    return t;
+}
+
 // T = double: the drop-in (bit-exact); T = float: the FP32 stencils of the
 // mixed form (same expressions, reciprocal multiplies instead of divisions)
 template <class T>
@@ -195,9 +296,7 @@ __device__ __forceinline__ void tendency_body(const StepConsts &c, const TendArg
     const int i = i0 + px, j = j0 + py;
     const int rx = px + 1, ry = py + 1;
     const bool lid = c.lid != 0;
-    const T Ro = (T)c.Ro;
-    const double dx = c.dx, dy = c.dy, dz = c.dz;
-    const T h = T(0.5), q4 = T(0.25), zero = T(0);
+    const T h = T(0.5), zero = T(0);
     constexpr bool MIXED = std::is_same<T, float>::value;
     for (int k = kb; k <= ke; ++k) {
         store(k + 1);
@@ -230,66 +329,22 @@ __device__ __forceinline__ void tendency_body(const StepConsts &c, const TendArg
             auto V = [&](int dk, int dj, int di) { return F(1, dk, dj, di); };
             auto W = [&](int dk, int dj, int di) { return F(2, dk, dj, di); };
             auto B = [&](int dk, int dj, int di) { return F(3, dk, dj, di); };
-            const bool top0 = lid && (k >= nz);
-            const bool bot0 = lid && (k - 1 <= 0);
-            const bool top0w = lid && (k >= nz - 1);
             const int64_t o = idx(c, i, j, k);
-            T adv = zero, au = zero, av = zero, aw = zero;
-            if (c.nonlinear) {
-                {
-                    const T fe1 = h * (B(0, 0, 0) + B(0, 0, 1)) * U(0, 0, 0);
-                    const T fe0 = h * (B(0, 0, -1) + B(0, 0, 0)) * U(0, 0, -1);
-                    const T fn1 = h * (B(0, 0, 0) + B(0, 1, 0)) * V(0, 0, 0);
-                    const T fn0 = h * (B(0, -1, 0) + B(0, 0, 0)) * V(0, -1, 0);
-                    const T ft1 = top0 ? zero : h * (B(0, 0, 0) + B(1, 0, 0)) * W(0, 0, 0);
-                    const T ft0 = bot0 ? zero : h * (B(-1, 0, 0) + B(0, 0, 0)) * W(-1, 0, 0);
-                    adv = -(Ro * (sdiv<T>(fe1 - fe0, dx, c.rdx) + sdiv<T>(fn1 - fn0, dy, c.rdy) +
-                                  sdiv<T>(ft1 - ft0, dz, c.rdz)));
-                }
-                {
-                    const T fe1 = q4 * (U(0, 0, 0) + U(0, 0, 1)) * (U(0, 0, 1) + U(0, 0, 0));
-                    const T fe0 = q4 * (U(0, 0, -1) + U(0, 0, 0)) * (U(0, 0, 0) + U(0, 0, -1));
-                    const T fn1 = q4 * (U(0, 0, 0) + U(0, 1, 0)) * (V(0, 0, 1) + V(0, 0, 0));
-                    const T fn0 = q4 * (U(0, -1, 0) + U(0, 0, 0)) * (V(0, -1, 1) + V(0, -1, 0));
-                    const T ft1 = top0 ? zero : q4 * (U(1, 0, 0) + U(0, 0, 0)) * (W(0, 0, 0) + W(0, 0, 1));
-                    const T ft0 = bot0 ? zero : q4 * (U(0, 0, 0) + U(-1, 0, 0)) * (W(-1, 0, 0) + W(-1, 0, 1));
-                    au = -(Ro * (sdiv<T>(fe1 - fe0, dx, c.rdx) + sdiv<T>(fn1 - fn0, dy, c.rdy) +
-                                 sdiv<T>(ft1 - ft0, dz, c.rdz)));
-                }
-                {
-                    const T fe1 = q4 * (V(0, 0, 0) + V(0, 0, 1)) * (U(0, 1, 0) + U(0, 0, 0));
-                    const T fe0 = q4 * (V(0, 0, -1) + V(0, 0, 0)) * (U(0, 1, -1) + U(0, 0, -1));
-                    const T fn1 = q4 * (V(0, 0, 0) + V(0, 1, 0)) * (V(0, 1, 0) + V(0, 0, 0));
-                    const T fn0 = q4 * (V(0, -1, 0) + V(0, 0, 0)) * (V(0, 0, 0) + V(0, -1, 0));
-                    const T ft1 = top0 ? zero : q4 * (V(1, 0, 0) + V(0, 0, 0)) * (W(0, 0, 0) + W(0, 1, 0));
-                    const T ft0 = bot0 ? zero : q4 * (V(0, 0, 0) + V(-1, 0, 0)) * (W(-1, 0, 0) + W(-1, 1, 0));
-                    av = -(Ro * (sdiv<T>(fe1 - fe0, dx, c.rdx) + sdiv<T>(fn1 - fn0, dy, c.rdy) +
-                                 sdiv<T>(ft1 - ft0, dz, c.rdz)));
-                }
-                {
-                    const T fe1 = q4 * (W(0, 0, 0) + W(0, 0, 1)) * (U(0, 0, 0) + U(1, 0, 0));
-                    const T fe0 = q4 * (W(0, 0, -1) + W(0, 0, 0)) * (U(0, 0, -1) + U(1, 0, -1));
-                    const T fn1 = q4 * (W(0, 0, 0) + W(0, 1, 0)) * (V(0, 0, 0) + V(1, 0, 0));
-                    const T fn0 = q4 * (W(0, -1, 0) + W(0, 0, 0)) * (V(0, -1, 0) + V(1, -1, 0));
-                    const T ft1 = top0w ? zero : q4 * (W(1, 0, 0) + W(0, 0, 0)) * (W(0, 0, 0) + W(1, 0, 0));
-                    const T ft0 = (bot0 || (lid && k - 1 >= nz - 1))
-                                           ? zero
-                                           : q4 * (W(0, 0, 0) + W(-1, 0, 0)) * (W(-1, 0, 0) + W(0, 0, 0));
-                    aw = -(((T)c.dsqr * Ro) * (sdiv<T>(fe1 - fe0, dx, c.rdx) + sdiv<T>(fn1 - fn0, dy, c.rdy) +
-                                            sdiv<T>(ft1 - ft0, dz, c.rdz)));
-                }
-            }
-            adv = adv - h * (W(0, 0, 0) + W(-1, 0, 0)) * (T)(c.N0 * c.N0);
-            a.db[o] = adv;
-            if constexpr (!std::is_same<T, float>::value) {
-                au = au + (V(0, 0, 0) + V(0, -1, 0) + V(0, 0, 1) + V(0, -1, 1)) * q4 * (T)c.f0;
-                av = av - (U(0, 0, -1) + U(0, 0, 0) + U(0, 1, -1) + U(0, 1, 0)) * q4 * (T)c.f0;
-                aw = aw + h * (B(0, 0, 0) + B(1, 0, 0));
-                if (lid && k >= nz) aw = zero;
-                a.du[o] = au;
-                a.dv[o] = av;
-                a.dw[o] = aw;
+            if constexpr (!MIXED) {
+                a.db[o] = tendency_fp64<3>(c, k, F);
+                a.du[o] = tendency_fp64<0>(c, k, F);
+                a.dv[o] = tendency_fp64<1>(c, k, F);
+                a.dw[o] = tendency_fp64<2>(c, k, F);
             } else {
+                T adv = zero, au = zero, av = zero, aw = zero;
+                if (c.nonlinear) {
+                    adv = adv_b<T>(c, k, F);
+                    au = adv_u<T>(c, k, F);
+                    av = adv_v<T>(c, k, F);
+                    aw = adv_w<T>(c, k, F);
+                }
+                adv = adv - h * (W(0, 0, 0) + W(-1, 0, 0)) * (T)(c.N0 * c.N0);
+                a.db[o] = adv;
                 // mixed form: the Coriolis and buoyancy terms, which the
                 // pressure gradient nearly balances, and that gradient in FP64
                 // from (hi, lo); the slots keep tendency - grad(pint)
@@ -689,9 +744,63 @@ __device__ __forceinline__ int fric_field(const FricArgs<T> &a, int bid)
     return bid / (a.tiles_x * a.tiles_y) / a.nchunk;
 }
 
-template <class T>
+// The tendency fused in (TF = 0..3, the field of the work-group, FP64 only):
+// the work-group forms field TF's tendency at its tile points (the
+// expressions of tendency_fp64) from its own ring of f0 and, for the other
+// fields, values loaded from global memory at the few points the stencil
+// reads -- the work-groups of the same tile and chunk that own those fields
+// run next to it on the same XCD and bring them into L2.  The tendency goes
+// straight into the AB sum and into slot tau, which is never read back.
+template <int TF, class A>
+__device__ __forceinline__ double fused_tendency(const StepConsts &c, int k, A ring, const double *e0,
+                                                 const double *em, const double *ep)
+{
+    auto F = [&](int f, int dk, int dj, int di) -> double {
+        if (f == TF) return ring(dk, dj, di);
+        if constexpr (TF == 0) {   // V at k: (j, i+1), (j, i), (j-1, i+1), (j-1, i); W at k, k-1: (j, i), (j, i+1)
+            if (f == 1) return e0[(dj == 0 ? 0 : 2) + (di == 1 ? 0 : 1)];
+            return dk == 0 ? e0[4 + di] : em[di];
+        } else if constexpr (TF == 1) {   // U at k: (j+1, i), (j, i), (j+1, i-1), (j, i-1); W at k, k-1: (j, i), (j+1, i)
+            if (f == 0) return e0[(di == 0 ? 0 : 2) + (dj == 1 ? 0 : 1)];
+            return dk == 0 ? e0[4 + dj] : em[dj];
+        } else if constexpr (TF == 2) {   // at k and k+1: U (j, i), (j, i-1); V (j, i), (j-1, i); B (j, i)
+            const double *e = dk == 0 ? e0 : ep;
+            if (f == 0) return e[di == 0 ? 0 : 1];
+            if (f == 1) return e[2 + (dj == 0 ? 0 : 1)];
+            return e[4];
+        } else {   // U at k: (j, i), (j, i-1); V at k: (j, i), (j-1, i); W at k, k-1: (j, i)
+            if (f == 0) return e0[di == 0 ? 0 : 1];
+            if (f == 1) return e0[2 + (dj == 0 ? 0 : 1)];
+            return dk == 0 ? e0[4] : em[0];
+        }
+    };
+    return tendency_fp64<TF>(c, k, F);
+}
+
+// the other fields' values of fused_tendency<TF> at level kk (see there)
+template <int TF>
+__device__ __forceinline__ void fused_tendency_load(const StepConsts &c, const FricArgs<double> &a, int i, int j,
+                                                    int kk, double *e)
+{
+    const int64_t o = (int64_t)(i + 1) + c.X * (int64_t)(j + 1) + c.XY * (int64_t)(kk + 1);
+    const int64_t X = c.X;
+    const double *U = a.f[0].in, *V = a.f[1].in, *W = a.f[2].in, *B = a.f[3].in;
+    if constexpr (TF == 0) {
+        e[0] = V[o + 1]; e[1] = V[o]; e[2] = V[o + 1 - X]; e[3] = V[o - X]; e[4] = W[o]; e[5] = W[o + 1];
+    } else if constexpr (TF == 1) {
+        e[0] = U[o + X]; e[1] = U[o]; e[2] = U[o + X - 1]; e[3] = U[o - 1]; e[4] = W[o]; e[5] = W[o + X];
+    } else if constexpr (TF == 2) {
+        e[0] = U[o]; e[1] = U[o - 1]; e[2] = V[o]; e[3] = V[o - X]; e[4] = B[o];
+    } else {
+        e[0] = U[o]; e[1] = U[o - 1]; e[2] = V[o]; e[3] = V[o - X]; e[4] = W[o];
+    }
+}
+
+template <class T, int TF = -1>
 __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T> &a, const FricField<T> &F, int bid)
 {
+    static_assert(TF < 0 || std::is_same<T, double>::value, "the fused tendency is FP64 only");
+    constexpr bool TEND = TF >= 0;
     constexpr bool MIXED = std::is_same<T, float>::value;
     __shared__ T f0p[3][R0Y][R0X];
     __shared__ T f1p[R1Y][R1X];
@@ -753,7 +862,7 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
             const int64_t o = idx(c, i0 + tpx, j0 + tpy, k);
             // read once: streamed past the L2, which keeps the f0 planes
             // that neighbouring tiles share
-            r.s0 = __builtin_nontemporal_load(F.t0 + o);
+            if constexpr (!TEND) r.s0 = __builtin_nontemporal_load(F.t0 + o);
             if (a.order >= 2) r.s1 = __builtin_nontemporal_load(F.t1 + o);
             if (a.order >= 3) r.s2 = __builtin_nontemporal_load(F.t2 + o);
             if constexpr (MIXED) {
@@ -771,11 +880,33 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
         return r;
     };
     Pt nxt = load_pt(kb);
+    // fused tendency: the other fields' values at level k (e0), k - 1 (em:
+    // W, carried from the level before) and k + 1 (ep, field w only)
+    double e0[6] = {}, em[2] = {}, ep[6] = {};
+    if constexpr (TEND) {
+        if (mine) {
+            if constexpr (TF == 2) fused_tendency_load<TF>(c, a, i0 + tpx, j0 + tpy, kb, ep);
+            else fused_tendency_load<TF>(c, a, i0 + tpx, j0 + tpy, kb - 1, e0);
+        }
+    }
     for (int k = kb; k <= ke; ++k) {
         store(k + 1);
         if (k + 1 <= ke) fetch(k + 2);
         const Pt cur = nxt;
         if (k + 1 <= ke) nxt = load_pt(k + 1);
+        if constexpr (TEND) {
+            if (mine) {
+                if constexpr (TF == 2) {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) e0[q] = ep[q];
+                    fused_tendency_load<TF>(c, a, i0 + tpx, j0 + tpy, k + 1, ep);
+                } else {
+                    if constexpr (TF == 3) em[0] = e0[4];
+                    else { em[0] = e0[4]; em[1] = e0[5]; }
+                    fused_tendency_load<TF>(c, a, i0 + tpx, j0 + tpy, k, e0);
+                }
+            }
+        }
         const T s0 = cur.s0, s1 = cur.s1, s2 = cur.s2, lo0 = cur.lo0;
         const float q1o = cur.q1o, q1n = cur.q1n, q2o = cur.q2o, q2n = cur.q2n;
         __syncthreads();
@@ -837,12 +968,24 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
             }
             const int64_t o = idx(c, i, j, k);
             if constexpr (!MIXED) {
+                double s0t = s0;
+                if constexpr (TEND) {
+                    const int sm2 = slot(k - 1), sc2 = slot(k), sp2 = slot(k + 1);
+                    const int ry0 = py + 3, rx0 = px + 3;
+                    auto ring = [&](int dk, int dj, int di) -> double {
+                        const int sl = dk < 0 ? sm2 : (dk > 0 ? sp2 : sc2);
+                        return f0p[sl][ry0 + dj][rx0 + di];
+                    };
+                    s0t = fused_tendency<TF>(c, k, ring, e0, em, ep);
+                    // slot tau (read-only elsewhere in this kernel), for the next steps' AB sums
+                    const_cast<double *>(F.t0)[o] = s0t;
+                }
                 const double s = F.scale;
                 double val;
                 if (a.order == 0) val = f2;   // friction only (the split step of a forced model)
-                else if (a.order == 1) val = f2 + s * s0;
-                else if (a.order == 2) val = f2 + s * (a.ca * s0 - a.cb * s1);
-                else val = f2 + s * (a.ca * s0 + a.cb * s1 + a.cc * s2);
+                else if (a.order == 1) val = f2 + s * s0t;
+                else if (a.order == 2) val = f2 + s * (a.ca * s0t - a.cb * s1);
+                else val = f2 + s * (a.ca * s0t + a.cb * s1 + a.cc * s2);
                 F.out[o] = val;
                 if (a.halo) write_images<double>(c, F.out, i, j, k, val, 0);
             } else {
@@ -884,6 +1027,20 @@ __global__ __launch_bounds__(FTHREADS) void k_fric_ab(StepConsts c, FricArgs<T> 
 {
     const int bid = xcd_remap((int)blockIdx.x, a.nblocks);
     fric_body<T>(c, a, a.f[fric_field(a, bid)], bid);
+}
+
+// tendency (integrate.f90:12-54), friction chain and Adams-Bashforth of the
+// four fields in one launch (FP64): blocks run over (field, tile, chunk) with
+// the field fastest, so the four work-groups of a tile and chunk sit together
+// on one XCD and share the state's planes in its L2
+__global__ __launch_bounds__(FTHREADS) void k_tend_fric_ab(StepConsts c, FricArgs<double> a)
+{
+    const int bid = xcd_remap((int)blockIdx.x, a.nblocks);
+    const int f = bid & 3, rest = bid >> 2;
+    if (f == 0) fric_body<double, 0>(c, a, a.f[0], rest);
+    else if (f == 1) fric_body<double, 1>(c, a, a.f[1], rest);
+    else if (f == 2) fric_body<double, 2>(c, a, a.f[2], rest);
+    else fric_body<double, 3>(c, a, a.f[3], rest);
 }
 
 // ---------------------------------------------------------------------------
@@ -1640,6 +1797,13 @@ bool fused_friction_ok(const Model &m)
     return !(m.prm.Avbi > 0.) && !(m.prm.Kvbi > 0.);
 }
 
+bool fused_tendency_active(const Model &m)
+{
+    const bool forced = m.forcing || m.wm_on || m.bg_on ||
+                        (m.bf_on[0] || m.bf_on[1] || m.bf_on[2] || m.bf_on[3]);
+    return !m.mixed && fused_friction_ok(m) && m.fuse_friction && m.fuse_tendency && !forced;
+}
+
 // integrate.f90:59-111 for configs without vertical biharmonic friction:
 // the friction chain and AB of all four fields in one launch, which also
 // writes the new values' halo images (the exchange of integrate.f90:106-111).
@@ -1650,7 +1814,8 @@ bool fused_friction_ok(const Model &m)
 template <class T>
 static void friction_ab_impl(Model &m, const T *const *in, const T *const *in_lo, T *const *out, T *const *out_lo,
                              T *const *tend, int itt, int tau, int taum1, int taum2, double qc1 = 0.0,
-                             const float *d1 = nullptr, double qc2 = 0.0, const float *d2 = nullptr, int halo = -1)
+                             const float *d1 = nullptr, double qc2 = 0.0, const float *d2 = nullptr, int halo = -1,
+                             bool fuse_tend = false)
 {
     const ps3d_params &p = m.prm;
     const Grid &g = m.g;
@@ -1706,11 +1871,22 @@ static void friction_ab_impl(Model &m, const T *const *in, const T *const *in_lo
     const int target = 2048 * 512 / FTHREADS;   // ~8 work-groups of 512 threads per CU
     int nchunk = (target + tiles * 4 - 1) / (tiles * 4);
     nchunk = nchunk < 1 ? 1 : (nchunk > g.nz ? g.nz : nchunk);
-    if (m.fric_kc > 0) nchunk = (g.nz + m.fric_kc - 1) / m.fric_kc;
+    const int kc_set = fuse_tend ? m.tfric_kc : m.fric_kc;
+    if (kc_set > 0) nchunk = (g.nz + kc_set - 1) / kc_set;
     a.kc = (g.nz + nchunk - 1) / nchunk;
     a.nchunk = (g.nz + a.kc - 1) / a.kc;
     a.nblocks = tiles * a.nchunk * 4;
     a.halo = halo < 0 ? (m.fuse_halo ? 1 : 0) : halo;
+    if constexpr (std::is_same<T, double>::value) {
+        if (fuse_tend) {
+            PS3D_REQUIRE(itt >= 0, "tendency + friction + AB: itt >= 0");
+            ProfScope ps(m, "tend_friction_ab");
+            hipLaunchKernelGGL(k_tend_fric_ab, dim3((unsigned)a.nblocks), dim3(FTHREADS), 0, m.stream, m.c, a);
+            PS3D_HIP_CHECK(hipGetLastError());
+            return;
+        }
+    }
+    PS3D_REQUIRE(!fuse_tend, "tendency + friction + AB: FP64 models only");
     ProfScope ps(m, "friction_ab");
     hipLaunchKernelGGL(k_fric_ab<T>, dim3((unsigned)a.nblocks), dim3(FTHREADS), 0, m.stream, m.c, a);
     PS3D_HIP_CHECK(hipGetLastError());
@@ -1836,7 +2012,7 @@ void launch_friction_fused_only(Model &m)
     for (int f = 0; f < 4; ++f) m.cur[f] = 1;
 }
 
-void launch_friction_ab_raw(Model &m, int itt, int tau, int taum1, int taum2)
+void launch_friction_ab_raw(Model &m, int itt, int tau, int taum1, int taum2, bool with_tendency)
 {
     if (m.mixed) {
         launch_friction_ab_mixed(m, itt, tau, taum1, taum2, 0.0, nullptr, 0.0, nullptr, false);
@@ -1844,15 +2020,17 @@ void launch_friction_ab_raw(Model &m, int itt, int tau, int taum1, int taum2)
     }
     double *in[4] = {m.u, m.v, m.w, m.b};
     double *tend[4] = {m.du, m.dv, m.dw, m.db};
-    friction_ab_impl<double>(m, in, nullptr, m.alt, nullptr, tend, itt, tau, taum1, taum2);
+    friction_ab_impl<double>(m, in, nullptr, m.alt, nullptr, tend, itt, tau, taum1, taum2, 0.0, nullptr, 0.0,
+                             nullptr, -1, with_tendency);
 }
 
-void launch_friction_ab(Model &m, int itt, int tau, int taum1, int taum2)
+void launch_friction_ab(Model &m, int itt, int tau, int taum1, int taum2, bool with_tendency)
 {
     PS3D_REQUIRE(!m.mixed, "friction_ab: FP64 models only");
     double *in[4] = {m.u, m.v, m.w, m.b};
     double *tend[4] = {m.du, m.dv, m.dw, m.db};
-    friction_ab_impl<double>(m, in, nullptr, m.alt, nullptr, tend, itt, tau, taum1, taum2);
+    friction_ab_impl<double>(m, in, nullptr, m.alt, nullptr, tend, itt, tau, taum1, taum2, 0.0, nullptr, 0.0,
+                             nullptr, -1, with_tendency);
     std::swap(m.u, m.alt[0]);
     std::swap(m.v, m.alt[1]);
     std::swap(m.w, m.alt[2]);

@@ -238,6 +238,91 @@ __global__ __launch_bounds__(64) void k_thomas_fly(C *__restrict__ fs, const dou
     thomas_col_fly<C>(fs, ksqr, ncols, nz, fxa, dz2, dsqr, col, dpl, cpl, cpw);
 }
 
+// The sweeps with whole columns in registers, NZ levels known at compile
+// time: one lane per column, 64 columns per wave.  Each lane loads its
+// column's NZ values at once (all loads in flight before the first is used),
+// keeps d'(k) in registers and c'(k) in LDS (NZ x 64 doubles), and stores
+// the solution.  With one wave per work-group and little LDS, every wave of
+// the solve is resident at once, so the sequential sweeps of all columns
+// run concurrently.  The arithmetic is that of thomas_col_fly, operation for
+// operation: the same bits.
+template <int NZ>
+__global__ __launch_bounds__(64, 1) void k_thomas_reg(double2 *__restrict__ fs, const double *__restrict__ ksqr,
+                                                      int64_t ncols, double fxa, double dz2, double dsqr)
+{
+    const int64_t col = (int64_t)xcd_remap((int)blockIdx.x, (int)gridDim.x) * 64 + threadIdx.x;
+    __shared__ double cpl[NZ][64];
+    if (col >= ncols) return;
+    double *cpk = &cpl[0][threadIdx.x];   // c'(level q) at cpk[64 q]
+    // 32-bit byte offsets from the array base (the launcher checks that FS
+    // is below 4 GiB), so each access is one base register pair plus one
+    // offset register
+    char *base = reinterpret_cast<char *>(fs);
+    const uint32_t o0 = (uint32_t)col * 16u, st = (uint32_t)ncols * 16u;
+    auto at = [&](int k) -> double2 & { return *reinterpret_cast<double2 *>(base + (o0 + (uint32_t)k * st)); };
+    double2 d[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) d[k] = at(k);
+    const double ks = ksqr[col];
+    if (ks > 0) {
+        const double b1 = -ks - fxa;
+        double cprev = fxa / b1;
+        cpk[0] = cprev;
+        double2 dp = make_double2(d[0].x / b1, d[0].y / b1);
+        d[0] = dp;
+#pragma unroll
+        for (int q = 1; q < NZ; ++q) {
+            const int k = q + 1;   // level, 1-based
+            const double bk = (k == NZ) ? -ks - fxa : -ks - 2 * fxa;
+            const double mm = bk - cprev * fxa;
+            const double r = 1.0 / mm;
+            const double ck = (k == NZ) ? 0.0 : fxa * r;
+            cpk[64 * q] = ck;
+            cprev = ck;
+            const double2 e = d[q];
+            dp = make_double2((e.x - dp.x * fxa) * r, (e.y - dp.y * fxa) * r);
+            d[q] = dp;
+        }
+        double2 x = dp;
+        // c'(k) is read back from LDS (not kept in registers across the sweep)
+        __syncthreads();
+#pragma unroll
+        for (int q = NZ - 2; q >= 0; --q) {
+            const double2 e = d[q];
+            const double c = cpk[64 * q];
+            x = make_double2(e.x - c * x.x, e.y - c * x.y);
+            d[q] = x;
+        }
+    } else {
+        // zero mode, pressure_solver.f90:44-50, as in thomas_col_fly
+        double2 pm1 = make_double2(0.0, 0.0);
+        double2 pk = make_double2(d[0].x * dz2 * dsqr, d[0].y * dz2 * dsqr);
+        double2 bk = d[1];
+        d[0] = pm1;
+        d[1] = pk;
+#pragma unroll
+        for (int k = 2; k < NZ; ++k) {
+            const double2 nxt = d[k];
+            const double2 pn = make_double2(2 * pk.x - pm1.x + bk.x * dz2 * dsqr,
+                                            2 * pk.y - pm1.y + bk.y * dz2 * dsqr);
+            d[k] = pn;
+            pm1 = pk;
+            pk = pn;
+            bk = nxt;
+        }
+    }
+    // the store offsets are formed here, one after the other (the empty asm
+    // makes each one opaque), rather than computed with the loads' and held
+    // in registers through both sweeps
+    uint32_t off = o0;
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        asm volatile("" : "+v"(off));
+        *reinterpret_cast<double2 *>(base + off) = d[k];
+        off += st;
+    }
+}
+
 // the Thomas stage of a batched ensemble (rigid-lid members), ENS_TC columns
 // per 64-lane work-group; FLY: the on-the-fly form with LDS for ENS_TC
 // columns of nz_max levels, else the factor tables
@@ -375,6 +460,20 @@ __global__ __launch_bounds__(64) void k_thomas_f(float2 *__restrict__ fs, double
 
 }  // namespace
 
+// LDS of the on-the-fly form for cpw columns of nz levels, and the most
+// one work-group may take (gfx950: 160 KiB per CU; above 64 KiB the kernel
+// must ask for it)
+static size_t thomas_fly_lds(int cpw, int nz) { return (sizeof(double2) + sizeof(double)) * (size_t)cpw * nz; }
+constexpr size_t kLdsMax = 160 * 1024;
+
+template <class K>
+static void allow_lds(K kernel, size_t bytes)
+{
+    if (bytes > 64 * 1024)
+        PS3D_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
 void tridiag_factor(Model &m)
 {
     if (!m.prm.enable_vertical_boundaries) return;
@@ -389,19 +488,41 @@ void launch_thomas(Model &m, double2 *data)
 {
     const int64_t ncols = (int64_t)m.g.nkx * m.g.ny;
     const double fxa = 1.0 / (m.prm.dz * m.prm.dz) / m.prm.dsqr;
+    if (m.thomas_reg && (uint64_t)m.g.S * sizeof(double2) < (uint64_t(1) << 32)) {
+        const dim3 grid((unsigned)((ncols + 63) / 64));
+        const double dz2 = m.prm.dz * m.prm.dz, dsqr = m.prm.dsqr;
+        bool done = true;
+        ProfScope ps(m, "thomas");
+        switch (m.g.nz) {
+#define CASE(n)                                                                                             \
+    case n:                                                                                                 \
+        hipLaunchKernelGGL(k_thomas_reg<n>, grid, dim3(64), 0, m.stream, data, m.ksqr, ncols, fxa, dz2, dsqr); \
+        break;
+            CASE(16) CASE(32) CASE(48) CASE(64)
+#undef CASE
+        default: done = false;
+        }
+        if (done) {
+            PS3D_HIP_CHECK(hipGetLastError());
+            return;
+        }
+    }
     int cpw = m.thomas_cpw;
     if (cpw != 16 && cpw != 32 && cpw != 64) {
+        // the widest group whose on-the-fly form fits 64 KiB (several
+        // work-groups per CU), else 16 columns in up to 160 KiB
         cpw = 64;
-        while (cpw > 16 && (sizeof(double2) + sizeof(double)) * (size_t)cpw * m.g.nz > 64 * 1024) cpw /= 2;
+        while (cpw > 16 && thomas_fly_lds(cpw, m.g.nz) > 64 * 1024) cpw /= 2;
     }
     const size_t lds = sizeof(double2) * (size_t)cpw * m.g.nz;
-    const size_t lds_fly = (sizeof(double2) + sizeof(double)) * (size_t)cpw * m.g.nz;
+    const size_t lds_fly = thomas_fly_lds(cpw, m.g.nz);
     const dim3 grid((unsigned)((ncols + cpw - 1) / cpw));
     ProfScope ps(m, "thomas");
-    if (lds_fly <= 64 * 1024)
+    if (lds_fly <= kLdsMax) {
+        allow_lds(k_thomas_fly<double2>, lds_fly);
         hipLaunchKernelGGL(k_thomas_fly<double2>, grid, dim3(64), lds_fly, m.stream, data, m.ksqr, ncols, m.g.nz,
                            fxa, m.prm.dz * m.prm.dz, m.prm.dsqr, cpw);
-    else if (lds <= 64 * 1024)
+    } else if (lds <= 64 * 1024)
         hipLaunchKernelGGL(k_thomas<true>, grid, dim3(64), lds, m.stream, data, m.ksqr, m.tri_cp, m.tri_rm, ncols,
                            m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr, cpw);
     else
@@ -424,8 +545,9 @@ void launch_thomas_f(Model &m, float2 *data)
     const unsigned nb = (unsigned)((ncols + 63) / 64);
     ProfScope ps(m, "thomas");
     constexpr int cpw = 32;
-    const size_t lds_fly = (sizeof(double2) + sizeof(double)) * cpw * (size_t)m.g.nz;
-    if (lds_fly <= 64 * 1024) {
+    const size_t lds_fly = thomas_fly_lds(cpw, m.g.nz);
+    if (lds_fly <= kLdsMax) {
+        allow_lds(k_thomas_fly<float2>, lds_fly);
         hipLaunchKernelGGL(k_thomas_fly<float2>, dim3((unsigned)((ncols + cpw - 1) / cpw)), dim3(64), lds_fly,
                            m.stream, data, m.ksqr, ncols, m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr, cpw);
     } else {
@@ -441,8 +563,9 @@ int ens_thomas_cols() { return ENS_TC; }
 void ens_thomas_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max)
 {
     if (!map.start[map.n]) return;
-    const size_t lds = (sizeof(double2) + sizeof(double)) * ENS_TC * (size_t)nz_max;
-    PS3D_REQUIRE(lds <= 64 * 1024, "mixed ensemble: rigid-lid members of at most 85 levels");
+    const size_t lds = thomas_fly_lds(ENS_TC, nz_max);
+    PS3D_REQUIRE(lds <= kLdsMax, "mixed ensemble: rigid-lid members of at most 213 levels");
+    allow_lds(k_ens_thomas_f, lds);
     hipLaunchKernelGGL(k_ens_thomas_f, dim3((unsigned)map.start[map.n]), dim3(64), lds, s, mem, map, nz_max);
     PS3D_HIP_CHECK(hipGetLastError());
 }
@@ -450,11 +573,13 @@ void ens_thomas_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz
 void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max)
 {
     if (!map.start[map.n]) return;
-    const size_t lds = (sizeof(double2) + sizeof(double)) * ENS_TC * (size_t)nz_max;
-    if (lds <= 64 * 1024)
+    const size_t lds = thomas_fly_lds(ENS_TC, nz_max);
+    if (lds <= kLdsMax) {
+        allow_lds(k_ens_thomas<true>, lds);
         hipLaunchKernelGGL(k_ens_thomas<true>, dim3((unsigned)map.start[map.n]), dim3(64), lds, s, mem, map, nz_max);
-    else
+    } else {
         hipLaunchKernelGGL(k_ens_thomas<false>, dim3((unsigned)map.start[map.n]), dim3(64), 0, s, mem, map, nz_max);
+    }
     PS3D_HIP_CHECK(hipGetLastError());
 }
 

@@ -6,9 +6,13 @@
 // friction kernel writes the second buffers, Thomas runs on a scratch copy of
 // the spectral array).  Results do not depend on the shapes.  The choice is
 // cached per (stage, nx x ny x nz, boundary, precision) in a text file of
-// "stage shape value" lines, the same format as the ensemble's.
+// "stage shape value" lines, the same format as the ensemble's (whose
+// stages carry an "ens_" prefix: a batched launch is timed differently).
 #include "ps3d_internal.h"
 
+#include <unistd.h>
+
+#include <cstdio>
 #include <fstream>
 #include <map>
 #include <sstream>
@@ -87,18 +91,55 @@ float time_it_cold(Model &m, int reps, void *flush, size_t flush_bytes, F &&laun
 
 }  // namespace
 
+// "stage shape value" lines of a tuning cache; a missing file is an empty
+// cache, a malformed line an error (so no entry is silently dropped)
+std::map<std::string, int> tune_cache_read(const char *path)
+{
+    std::map<std::string, int> cache;
+    if (!path || !*path) return cache;
+    std::ifstream in(path);
+    if (!in.is_open()) return cache;
+    std::string line;
+    int lineno = 0;
+    while (std::getline(in, line)) {
+        ++lineno;
+        if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+        std::istringstream ls(line);
+        std::string st, sh, extra;
+        int v = 0;
+        PS3D_REQUIRE(bool(ls >> st >> sh >> v) && !(ls >> extra),
+                     std::string("tune cache ") + path + ": malformed line " + std::to_string(lineno));
+        cache[st + " " + sh] = v;
+    }
+    PS3D_REQUIRE(!in.bad(), std::string("tune cache ") + path + ": read failed");
+    return cache;
+}
+
+// written to a temporary file beside it and renamed over it, so a reader
+// never sees a partly written cache
+void tune_cache_write(const char *path, const std::map<std::string, int> &cache)
+{
+    if (!path || !*path) return;
+    const std::string tmp = std::string(path) + ".tmp" + std::to_string((long long)getpid());
+    {
+        std::ofstream out(tmp, std::ios::trunc);
+        PS3D_REQUIRE(out.is_open(), "tune cache: cannot write " + tmp);
+        for (auto &kv : cache) out << kv.first << " " << kv.second << "\n";
+        out.flush();
+        PS3D_REQUIRE(bool(out), "tune cache: write failed: " + tmp);
+    }
+    if (std::rename(tmp.c_str(), path) != 0) {
+        std::remove(tmp.c_str());
+        throw Error(PS3D_ERR_ARG, std::string("tune cache: cannot rename ") + tmp + " to " + path);
+    }
+}
+
 void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
 {
     PS3D_REQUIRE(next_tau >= 1 && next_tau <= 3, "tune: next_tau must be 1..3");
     for (int f = 0; f < 4; ++f) PS3D_REQUIRE(m.cur[f] == 0, "tune: friction results not yet folded in by ab_update");
     reps = reps < 1 ? 1 : reps;
-    std::map<std::string, int> cache;
-    if (cache_path) {
-        std::ifstream in(cache_path);
-        std::string st, sh;
-        int v;
-        while (in >> st >> sh >> v) cache[st + " " + sh] = v;
-    }
+    std::map<std::string, int> cache = tune_cache_read(cache_path);
     const std::string sh = shape_of(m);
     const int nz = m.g.nz;
     std::vector<int> levels;
@@ -129,13 +170,20 @@ void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
         cache[key] = best;
     };
 
-    pick("tendency", levels, m.tend_kc, reps, nullptr, [&] {
-        if (m.mixed) launch_tendency_mixed(m, next_tau, m.pint);
-        else launch_tendency(m, next_tau);
-    });
-    if (m.mixed || (fused_friction_ok(m) && m.fuse_friction))
-        pick("friction_ab", levels, m.fric_kc, reps, nullptr,
-             [&] { launch_friction_ab_raw(m, 3, next_tau, t1, t2); });
+    if (fused_tendency_active(m)) {
+        // the fused kernel writes slot next_tau (overwritten by the next
+        // step) and the second buffers
+        pick("tend_friction_ab", levels, m.tfric_kc, reps, nullptr,
+             [&] { launch_friction_ab_raw(m, 3, next_tau, t1, t2, true); });
+    } else {
+        pick("tendency", levels, m.tend_kc, reps, nullptr, [&] {
+            if (m.mixed) launch_tendency_mixed(m, next_tau, m.pint);
+            else launch_tendency(m, next_tau);
+        });
+        if (m.mixed || (fused_friction_ok(m) && m.fuse_friction))
+            pick("friction_ab", levels, m.fric_kc, reps, nullptr,
+                 [&] { launch_friction_ab_raw(m, 3, next_tau, t1, t2); });
+    }
     if (!m.mixed && m.prm.enable_vertical_boundaries && !cache.count("thomas " + sh)) {
         double2 *scratch = nullptr;
         void *flush = nullptr;
@@ -146,7 +194,13 @@ void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
                                           m.stream));
             // the table form reads per-column factors that stay cached when
             // launched back to back: timed with cold caches, as in the step
-            const std::vector<int> cpw = {32, 16, 64};
+            // columns per work-group whose LDS keeps several work-groups on
+            // a CU (64 KiB), measured faster in the step than wider groups
+            // that a back-to-back timing can prefer
+            std::vector<int> cpw;
+            for (int c : {32, 16, 64})
+                if ((sizeof(double2) + sizeof(double)) * (size_t)c * m.g.nz <= 64 * 1024) cpw.push_back(c);
+            if (cpw.empty()) cpw.push_back(16);
             pick("thomas", cpw, m.thomas_cpw, 2 * reps, flush, [&] { launch_thomas(m, scratch); });
         } catch (...) {
             (void)hipFree(scratch);
@@ -159,18 +213,19 @@ void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
     } else if (!m.mixed && m.prm.enable_vertical_boundaries) {
         m.thomas_cpw = cache["thomas " + sh];
     }
-    if (cache_path) {
-        std::ofstream out(cache_path, std::ios::trunc);
-        for (auto &kv : cache) out << kv.first << " " << kv.second << "\n";
-    }
+    tune_cache_write(cache_path, cache);
 }
 
 std::string describe_tuning(const Model &m)
 {
     std::ostringstream o;
     const std::string sh = shape_of(m);
-    o << "tendency " << sh << " " << m.tend_kc << "\n";
-    o << "friction_ab " << sh << " " << m.fric_kc << "\n";
+    if (fused_tendency_active(m)) {
+        o << "tend_friction_ab " << sh << " " << m.tfric_kc << "\n";
+    } else {
+        o << "tendency " << sh << " " << m.tend_kc << "\n";
+        o << "friction_ab " << sh << " " << m.fric_kc << "\n";
+    }
     o << "thomas " << sh << " " << m.thomas_cpw << "\n";
     return o.str();
 }

@@ -19,7 +19,12 @@
 !    is one of these: the host arrays (u, v, w, b, p, du..db) and the
 !    parameters are pushed, the step runs, and the results come back to the
 !    host, where such a caller reads them.  The main loop's next call then
-!    starts again from the host arrays, which the caller has restored.
+!    starts again from the host arrays, which the caller has restored --
+!    also when its counters happen to continue the outside caller's last
+!    call (model_run keeps itt = 0 and rotates the tau slots, so the main
+!    loop's first call after read_restart at itt = 1 can look like its
+!    continuation): after a call from outside the main loop the next call
+!    always starts from the host arrays.
 ! Around set_forcing, when the config's forcing works on the fields
 ! (ps3d_forcing_on_host = .true.), tendencies and fields go to the host,
 ! set_forcing runs there (integrate.f90:82), and the results come back.
@@ -41,6 +46,9 @@ module ps3d_gpu_state
  integer :: ps3d_last(4) = -1
  real(8) :: ps3d_dyn(13) = 0d0
  logical :: ps3d_switches(2) = .false.
+ ! the last call came from outside the main loop: its caller may have
+ ! changed the host arrays since, so the next call starts from them
+ logical :: ps3d_host_ahead = .false.
 end module ps3d_gpu_state
 
 
@@ -170,7 +178,10 @@ subroutine integrate
    ps3d_switches = sw
   endif
   if (.not. first) call ps3d_gpu_sync(.true.)
+ else if (ps3d_host_ahead) then
+  call ps3d_gpu_sync(.true.)
  endif
+ ps3d_host_ahead = .not. resumes .and. .not. first
  ps3d_last = (/itt, tau, taum1, taum2/)
 
  call tic('integrate')

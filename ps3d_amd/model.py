@@ -282,6 +282,11 @@ class Model:
         self._forcing_cb = _lib.FORCING_FN(tramp)
         check(self.lib.ps3d_set_forcing(self.h, ctypes.cast(self._forcing_cb, ctypes.c_void_p), None))
 
+    def set_fused_tendency(self, on: bool):
+        """Form the tendencies inside the friction + AB kernel (True) or in a
+        kernel of their own (False, the default): ps3d_set_fused_tendency."""
+        check(self.lib.ps3d_set_fused_tendency(self.h, 1 if on else 0))
+
     def device_ptr(self, name: str, slot: int | None = None) -> int:
         """Device address of a field array (ps3d_device_ptr)."""
         out = ctypes.c_void_p()

@@ -57,8 +57,8 @@ def test_version_and_kernel_names():
     (dict(dt=0.0), "dt"),
     # the mixed form: fused friction only, compiled-in FFT lengths only
     (dict(precision=1, Avbi=1e-2), "vertical biharmonic"),
-    (dict(precision=1, nx=40), "compiled-in FFT lengths"),
-    (dict(precision=1, nz=30, enable_vertical_boundaries=False), "compiled-in FFT lengths"),
+    (dict(precision=1, nx=36), "compiled-in FFT lengths"),
+    (dict(precision=1, nz=18, enable_vertical_boundaries=False), "compiled-in FFT lengths"),
 ])
 def test_create_rejects_bad_parameters(bad, msg):
     from ps3d_amd import _lib
@@ -132,7 +132,7 @@ def test_fortran_dropin_links_against_reference_module(tmp_path):
 
 @pytest.mark.parametrize("bad,msg", [
     (dict(Avbi=1e-2), "vertical biharmonic"),
-    (dict(nx=40), "compiled-in FFT lengths"),
+    (dict(nx=36), "compiled-in FFT lengths"),
     (dict(precision=1), "share one precision form"),
     (dict(precision=7), "unknown precision form"),
 ])
@@ -153,17 +153,17 @@ def test_ensemble_create_rejects_bad_members(bad, msg):
 
 def test_mixed_ensemble_rejects_tall_lid_members():
     """A mixed ensemble's Thomas stage keeps 32 columns of FP64 factors in
-    LDS: rigid-lid members of more than 85 levels are refused at creation,
-    before any device work."""
+    LDS (at most 160 KiB per work-group on gfx950): rigid-lid members of more
+    than 213 levels are refused at creation, before any device work."""
     from ps3d_amd import _lib
     from ps3d_amd.params import CParams, ModelParams, PS3D_MIXED
     lib = _lib.load()
     base = dict(nx=32, ny=32, Lx=1.0, Ly=1.0, Lz=1.0, dt=0.1, enable_vertical_boundaries=True,
                 precision=PS3D_MIXED)
     arr = (CParams * 2)(CParams.from_params(ModelParams(nz=16, **base)),
-                        CParams.from_params(ModelParams(nz=96, **base)))
+                        CParams.from_params(ModelParams(nz=256, **base)))
     h = ctypes.c_void_p()
     code = lib.ps3d_ensemble_create(2, arr, None, ctypes.byref(h))
     assert code == 1
     assert not h.value
-    assert "member 1" in _lib.last_error() and "at most 85 levels" in _lib.last_error()
+    assert "member 1" in _lib.last_error() and "at most 213 levels" in _lib.last_error()

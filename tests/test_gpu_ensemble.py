@@ -73,7 +73,7 @@ def test_ensemble_tuning_keeps_results(tmp_path):
     ens.tune(cache, reps=3)
     desc = ens.describe()
     print(desc, flush=True)
-    assert "tendency" in desc and "friction_ab" in desc
+    assert "ens_tendency" in desc and "ens_friction_ab" in desc
     ens.run(6)
     for i, (prm, fields) in enumerate(members):
         ref = _single(prm, fields, 10)

@@ -131,3 +131,39 @@ def test_mixed_1000_steps_c2_size():
             assert v <= TOL, (upto, k, v)
     g64.close()
     gmx.close()
+
+
+@pytest.mark.parametrize("case", ["eddy1_120x120x10", "jets_100x100x20"])
+def test_mixed_at_reference_config_sizes(case):
+    """The mixed form at the grid sizes of the reference's own configs --
+    eddy1.f90:15 (120 x 120 x 10: transforms of 60 and 120) and
+    jets_scaled.f90:9 / jets_unscaled.f90:17 (100 x 100 x 20: 50 and 100),
+    lengths with factors 3 and 5 -- against the CPU oracle (FP64) after 1
+    and 100 steps, within the mixed form's bound."""
+    import oracle
+    from ps3d_amd import configs, diagnose
+    from ps3d_amd.model import Model
+    from ps3d_amd.params import PS3D_MIXED
+    if case.startswith("eddy1"):
+        prm, fields = configs.eddy(120, 120, 10, seed=1)
+    else:
+        prm, fields = configs.jets(100, 100, 20, seed=2)
+    gmx = Model(prm.replace(precision=PS3D_MIXED))
+    orc = oracle.OracleModel(prm)
+    for n, a in fields.items():
+        gmx.f[n][...] = a
+        orc.f[n][...] = a
+    gmx.push()
+    done = 0
+    for upto in (1, 100):
+        gmx.run(upto - done)
+        orc.run(upto - done)
+        done = upto
+        gmx.pull()
+        ref = {n: orc.f[n] for n in ("u", "v", "w", "b", "p")}
+        err = diagnose.compare(gmx.f, ref, prm)
+        print(f"mixed {case} step {upto}: " + " ".join(f"{k}={v:.2e}" for k, v in err.items()), flush=True)
+        for k, v in err.items():
+            assert v <= TOL, (upto, k, v)
+    gmx.close()
+    orc.close()

@@ -251,5 +251,43 @@ def test_tuned_shapes_give_the_same_bits(case, tmp_path):
     for n in ("u", "v", "w", "b", "p"):
         assert np.array_equal(gpu.f[n], orc.f[n]), n
     text = open(cache).read()
-    assert "tendency" in text
+    assert "tend_friction_ab" in text or "tendency" in text
+    gpu.close()
+
+
+FUSED = ["lid_fused", "periodic_fused", "periodic_pow2", "thin_lid_fused"]
+
+
+@pytest.mark.parametrize("case", FUSED)
+def test_fused_tendency_kernel_100_steps(case):
+    """The step with the tendencies formed inside the friction + AB kernel
+    (k_tend_fric_ab) instead of written to slot tau by a kernel of their own
+    and read back: bit for bit with the oracle as well."""
+    prm, gpu, orc = _pair(case)
+    gpu.set_fused_tendency(True)
+    gpu.run(100)
+    orc.run(100)
+    gpu.pull()
+    for n in ("u", "v", "w", "b", "p"):
+        assert np.array_equal(gpu.f[n], orc.f[n]), n
+    gpu.close()
+
+
+@pytest.mark.parametrize("case", FUSED)
+@pytest.mark.parametrize("steps", [1, 2, 5])
+def test_fused_tendency_writes_slot_tau(case, steps):
+    """Euler, AB2 and AB3 steps of the fused kernel: the tendency slots it
+    leaves (interior: the contract of the tendency arrays) and every field
+    equal the oracle's bit for bit."""
+    prm, gpu, orc = _pair(case, seed=9)
+    gpu.set_fused_tendency(True)
+    gpu.run(steps)
+    orc.run(steps)
+    gpu.pull()
+    for n in ("u", "v", "w", "b", "p"):
+        assert np.array_equal(gpu.f[n], orc.f[n]), n
+    for n in ("du", "dv", "dw", "db"):
+        g, o = gpu.f[n], orc.f[n]
+        for slot in range(3):
+            assert np.array_equal(interior(g[..., slot], prm), interior(o[..., slot], prm)), (n, slot)
     gpu.close()
