@@ -43,6 +43,11 @@ METRIC = "grid-pt steps/s FP64 vs mixed @≤1e-5 rel L2; ensemble member-steps/s
 HBM_PEAK_GBS = 8000.0
 
 
+def log(msg: str) -> None:
+    """A progress line on stderr (the JSON result is the one stdout line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def algorithmic_bytes(name: str, prm, order: int = 3) -> float:
     """Compulsory HBM bytes of one launch of a kernel (element size 8 B)."""
     N = prm.nx * prm.ny * prm.nz
@@ -126,10 +131,13 @@ def mixed_leg(prm, fields, steps: int, warmup: int, fp64_value: float, check_ste
     m.push()
     m.run(max(warmup, 3))
     m.tune(reps=5)
+    plans = m.fft_plan
+    tuning = m.tuning().strip().splitlines()
     el = time_steps(m, steps)
     value = prm.npoints * steps / el
     m.close()
-    # accuracy: both forms from the same initial state
+    # accuracy: both forms from the same initial state, the mixed one with
+    # the radix plans its tuner picked
     g64 = Model(prm.replace(precision=PS3D_FP64))
     gmx = Model(pm)
     for n, a in fields.items():
@@ -137,6 +145,7 @@ def mixed_leg(prm, fields, steps: int, warmup: int, fp64_value: float, check_ste
         gmx.f[n][...] = a
     g64.push()
     gmx.push()
+    gmx.fft_plan = plans
     errs = {}
     done = 0
     for upto in check_steps:
@@ -151,6 +160,7 @@ def mixed_leg(prm, fields, steps: int, warmup: int, fp64_value: float, check_ste
     worst = max(max(e.values()) for e in errs.values())
     return {"value": value, "unit": "grid-pt steps/s", "ms_per_step": 1e3 * el / steps,
             "speedup_vs_fp64": value / fp64_value, "rel_l2_vs_fp64": errs, "max_rel_l2": worst,
+            "tuning": tuning,
             "dtype": "f32 stencils/FFT, f64 solve + accumulation"}
 
 
@@ -161,21 +171,8 @@ def ensemble_leg(steps: int, warm: int = 3, precision: int = 0) -> dict:
     from ps3d_amd.model import Model
     members = configs.ensemble_c4(precision=precision)
     n = len(members)
-    # one after another: each member alone through the single-model path
-    seq = 0.0
-    keep = {}
-    for i, (prm, fields) in enumerate(members):
-        m = Model(prm)
-        for k, a in fields.items():
-            m.f[k][...] = a
-        m.push()
-        m.run(warm)
-        seq += time_steps(m, steps)
-        if i in (0, n - 1):
-            m.pull(["u", "w", "p"])
-            keep[i] = {k: m.f[k].copy() for k in ("u", "w", "p")}
-        m.close()
-    # batched
+    # batched: warm-up steps, then the autotuner (tile shapes; radix plans of
+    # the mixed form's FP32 transforms)
     ens = Ensemble([p for p, _ in members])
     for i, (_, fields) in enumerate(members):
         for k, a in fields.items():
@@ -184,25 +181,41 @@ def ensemble_leg(steps: int, warm: int = 3, precision: int = 0) -> dict:
     t0 = time.perf_counter()
     ens.tune(os.path.join("/tmp", f"ps3d_ensemble_tune_{precision}.txt"), reps=5)
     tune_s = time.perf_counter() - t0
+    plans = [ens.fft_plan(i) for i in range(n)]
+    # one after another: each member alone through the single-model path,
+    # with the plans the ensemble's tuner picked
+    seq = 0.0
+    for i, (prm, fields) in enumerate(members):
+        m = Model(prm)
+        for k, a in fields.items():
+            m.f[k][...] = a
+        m.push()
+        if precision:
+            m.fft_plan = plans[i]
+        m.run(warm)
+        seq += time_steps(m, steps)
+        if i % 16 == 15:
+            log(f"ensemble ({'mixed' if precision else 'FP64'}): {i + 1} of {n} members run one after another")
+        m.close()
     ens.run(1, use_graph=True)   # capture
     ens.synchronize()
-    # the capture step above is one extra step: the comparison below uses a
-    # separate count
     t0 = time.perf_counter()
     ens.run(steps, use_graph=True)
     ens.synchronize()
     bat = time.perf_counter() - t0
+    # the first and last members against single models that take the same
+    # path: warm steps before the tuning, the tuned plans for the rest
     same = True
-    # compare with the single runs after warm + steps steps: rerun the two
-    # members one step short of the ensemble's count is not needed -- the
-    # ensemble ran warm + 1 + steps; advance the kept singles by one step
-    for i in keep:
+    for i in (0, n - 1):
         prm, fields = members[i]
         m = Model(prm)
         for k, a in fields.items():
             m.f[k][...] = a
         m.push()
-        m.run(warm + 1 + steps)
+        m.run(warm)
+        if precision:
+            m.fft_plan = plans[i]
+        m.run(1 + steps)
         m.pull(["u", "w", "p"])
         for k in ("u", "w", "p"):
             same = same and bool(np.array_equal(m.f[k], ens.get(i, k)))
@@ -366,9 +379,29 @@ def reference_cpu_leg(prm, procs: int, budget_s: float = 4.0, small: bool = Fals
     import oracle
     if not oracle.reference_available():
         return None
-    ctx = mp.get_context("spawn")
-    with ctx.Pool(procs) as pool:
-        res = pool.map(_reference_worker, [(budget_s, small)] * procs)
+    import resource
+    from concurrent.futures import ProcessPoolExecutor, as_completed
+    # integrate.f90 keeps grid-sized automatic arrays, which amdflang places on
+    # the stack (~250 MB at 256x256x64): the workers are started with the
+    # stack limit raised to the hard limit (a process's main stack is sized
+    # from the limit it starts with)
+    soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    need = 1 << 30
+    if hard != resource.RLIM_INFINITY and hard < need and not small:
+        log(f"CPU baseline: stack hard limit {hard} bytes is below what the reference needs; skipped")
+        return None
+    resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+    try:
+        # a pool that reports a worker's death (multiprocessing.Pool would wait
+        # for it for ever)
+        with ProcessPoolExecutor(procs, mp_context=mp.get_context("spawn")) as pool:
+            futs = [pool.submit(_reference_worker, (budget_s, small)) for _ in range(procs)]
+            res = []
+            for f in as_completed(futs):
+                res.append(f.result())
+                log(f"CPU baseline (reference): {len(res)} of {procs} processes done")
+    finally:
+        resource.setrlimit(resource.RLIMIT_STACK, (soft, hard))
     N = prm.npoints
 
     def rate(tag):
@@ -426,7 +459,9 @@ def main() -> int:
 
     from ps3d_amd.model import Model, kernel_names
 
+    log("HBM probe")
     hbm = hbm_probe()
+    log(f"probe copy {hbm['ceiling']:.0f} GB/s; building the 256x256x64 jets workload")
     prm, fields = build_workload()
     model = Model(prm)
     if args.tendency_form != "default":
@@ -451,6 +486,7 @@ def main() -> int:
     model.profile_kernel(None)
     dominant = max(shares, key=lambda k: shares[k][1])
 
+    log(f"kernel shares measured; timing {args.steps} steps")
     # timed region: exactly --steps steps, the dominant kernel under events
     model.profile_kernel(dominant)
     el = time_steps(model, args.steps)
@@ -488,17 +524,23 @@ def main() -> int:
         out["kernel_shares_ms_per_step"] = {k: v[1] for k, v in sorted(shares.items(), key=lambda x: -x[1][1])}
     out["tuning"] = model.tuning().strip().splitlines()
     model.close()
+    log(f"FP64: {1e3 * el / args.steps:.4f} ms per step")
     if not args.no_mixed:
         checks = tuple(int(x) for x in args.mixed_check.split(",") if x.strip())
         out["mixed"] = mixed_leg(prm, fields, args.steps, args.warmup, value, check_steps=checks)
+        log(f"mixed: {out['mixed']['ms_per_step']:.4f} ms per step, worst rel L2 {out['mixed']['max_rel_l2']:.2e}")
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ens_steps)
+        log(f"ensemble FP64: {out['ensemble']['speedup_vs_sequential']:.2f}x sequential")
         if not args.no_mixed:
             out["ensemble_mixed"] = ensemble_leg(args.ens_steps, precision=1)
+            log(f"ensemble mixed: {out['ensemble_mixed']['speedup_vs_sequential']:.2f}x sequential")
     if args.configs:
         out["configs"] = configs_leg(cases=tuple(c.strip() for c in args.configs.split(",") if c.strip()))
     if not args.no_cpu:
+        log("CPU baseline: the C port on the host cores")
         port = cpu_baseline(prm, fields, threads=args.cpu_threads)
+        log("CPU baseline: the reference's integrate.f90 on the host cores")
         ref = reference_cpu_leg(prm, port["cores"])
         if ref is not None:
             # the reference's own code where it builds; the port's whole step beside it

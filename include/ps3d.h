@@ -137,6 +137,17 @@ int ps3d_synchronize(ps3d_model *m);
    and measurements. */
 int ps3d_set_fused_tendency(ps3d_model *m, int on);
 
+/* Radix plans of a mixed-form model's FP32 transforms along x, y and z
+   (plans[0..2]): 0 = radix-4 passes first, then 2, 3, 5 (the plan of every
+   FP64 transform, which the oracle follows bit for bit); 1 = radix-8 passes
+   first (fewer passes through LDS; lengths without a factor 8 run plan 0
+   either way).  ps3d_tune picks them per transform length by timing.  A
+   FP64 model accepts only 0.  The reference leaves the factorisation to
+   FFTW's run-time planner behind P3DFFT (my_fft.f90:5-82, p3dfft_setup);
+   this is its counterpart for the device transforms. */
+int ps3d_set_fft_plan(ps3d_model *m, const int32_t plans[3]);
+int ps3d_get_fft_plan(ps3d_model *m, int32_t plans[3]);
+
 /* ---- forcing: the set_forcing hook of integrate.f90:82 ----------------
    A forcing hook runs where the reference calls set_forcing: after the
    friction block (:59-78) and before the Adams-Bashforth update (:87-102),
@@ -367,6 +378,21 @@ int ps3d_ensemble_destroy(ps3d_ensemble *e);
 /* field ids and layouts as for ps3d_set_field / ps3d_get_field */
 int ps3d_ensemble_set_field(ps3d_ensemble *e, int member, int field, const double *host);
 int ps3d_ensemble_get_field(ps3d_ensemble *e, int member, int field, double *host);
+/* Forcing of one member, as ps3d_set_background_flow / ps3d_set_body_force
+   set it on a single model (lee_wave1.f90:239-314, a steady body force):
+   background-flow advection and body force formed on the device from the
+   post-friction state (integrate.f90:82).  Forced members run the friction
+   chain into a third set of buffers, then a halo copy and one forcing +
+   Adams-Bashforth stage (each one launch over the forced members), and stay
+   bit-identical to a single model with the same forcing.  PS3D_FP64
+   ensembles. */
+int ps3d_ensemble_set_background_flow(ps3d_ensemble *e, int member, const double *u_back, const double *b_back);
+int ps3d_ensemble_set_body_force(ps3d_ensemble *e, int member, const double *const f[4]);
+/* Radix plans of a member's FP32 transforms (as ps3d_set_fft_plan; PS3D_MIXED
+   ensembles); ps3d_ensemble_tune picks them per transform length.  A member
+   with the same plans as a single mixed model gives that model's bits. */
+int ps3d_ensemble_set_fft_plan(ps3d_ensemble *e, int member, const int32_t plans[3]);
+int ps3d_ensemble_get_fft_plan(ps3d_ensemble *e, int member, int32_t plans[3]);
 /* nsteps steps of every member; state = {itt, tau, taum1, taum2}, shared by
    the members and advanced as by ps3d_run; use_graph replays a captured step */
 int ps3d_ensemble_run(ps3d_ensemble *e, int nsteps, int32_t *state, int use_graph);

@@ -49,6 +49,8 @@ PROTOTYPES = {
     "ps3d_run": (_c_int, [_vp, _c_int, _i32p, _c_int]),
     "ps3d_synchronize": (_c_int, [_vp]),
     "ps3d_set_fused_tendency": (_c_int, [_vp, _c_int]),
+    "ps3d_set_fft_plan": (_c_int, [_vp, _i32p]),
+    "ps3d_get_fft_plan": (_c_int, [_vp, _i32p]),
     "ps3d_tendency": (_c_int, [_vp, _c_int]),
     "ps3d_buoyancy_advection": (_c_int, [_vp, _c_int]),
     "ps3d_momentum_advection": (_c_int, [_vp, _c_int]),
@@ -90,6 +92,10 @@ PROTOTYPES = {
     "ps3d_ensemble_destroy": (_c_int, [_vp]),
     "ps3d_ensemble_set_field": (_c_int, [_vp, _c_int, _c_int, _dp]),
     "ps3d_ensemble_get_field": (_c_int, [_vp, _c_int, _c_int, _dp]),
+    "ps3d_ensemble_set_background_flow": (_c_int, [_vp, _c_int, _dp, _dp]),
+    "ps3d_ensemble_set_body_force": (_c_int, [_vp, _c_int, ctypes.POINTER(_dp)]),
+    "ps3d_ensemble_set_fft_plan": (_c_int, [_vp, _c_int, _i32p]),
+    "ps3d_ensemble_get_fft_plan": (_c_int, [_vp, _c_int, _i32p]),
     "ps3d_ensemble_run": (_c_int, [_vp, _c_int, _i32p, _c_int]),
     "ps3d_ensemble_synchronize": (_c_int, [_vp]),
     "ps3d_ensemble_tune": (_c_int, [_vp, ctypes.c_char_p, _c_int]),

@@ -945,6 +945,29 @@ int ps3d_set_fused_tendency(ps3d_model *m, int on)
     });
 }
 
+int ps3d_set_fft_plan(ps3d_model *m, const int32_t plans[3])
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(plans, "null plans");
+        for (int q = 0; q < 3; ++q) {
+            PS3D_REQUIRE(plans[q] == 0 || plans[q] == 1, "radix plans are 0 or 1");
+            PS3D_REQUIRE(m->mixed || plans[q] == 0,
+                         "FP64 transforms keep plan 0 (the factorisation the oracle reproduces bit for bit)");
+        }
+        for (int q = 0; q < 3; ++q) m->fft_plan[q] = plans[q];
+    });
+}
+
+int ps3d_get_fft_plan(ps3d_model *m, int32_t plans[3])
+{
+    return guard([&] {
+        check_model(m);
+        PS3D_REQUIRE(plans, "null plans");
+        for (int q = 0; q < 3; ++q) plans[q] = m->fft_plan[q];
+    });
+}
+
 int ps3d_diagnose(ps3d_model *m, double *out, int32_t *nan_seen)
 {
     return guard([&] {

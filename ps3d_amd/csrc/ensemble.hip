@@ -21,7 +21,7 @@ using namespace ps3d;
 namespace {
 
 // S_PTS: 256 points per work-group over the H points of a member
-enum Stage { S_TEND, S_FRIC, S_R2C, S_COLS, S_THOMAS, S_ZSOLVE, S_PROJ, S_PTS, S_NSTAGE };
+enum Stage { S_TEND, S_FRIC, S_R2C, S_COLS, S_THOMAS, S_ZSOLVE, S_PROJ, S_PTS, S_HCOPY, S_BGAB, S_NSTAGE };
 
 // tile shapes of the k-marching kernels (stencil.hip)
 constexpr int kTTX = TEND_TX, kTTY = TEND_TY, kFTX = FRIC_TX, kFTY = FRIC_TY;
@@ -49,6 +49,9 @@ struct ps3d_ensemble {
     // and it and the next one add a correction solve (as a single model)
     bool mixed = false, cold = true;
     int refine = 0;
+    // every rigid-lid member's Thomas solve has the register form
+    bool thomas_reg = false;
+    std::vector<void *> tmp_alloc;      // third buffers of forced members
 };
 
 namespace {
@@ -143,6 +146,7 @@ void fill_member(EnsMember &e, ps3d_model &m)
         e.twzf = m.pz.twf;
         e.wkf = m.twf_r2c;
     }
+    for (int q = 0; q < 3; ++q) e.fft_plan[q] = m.mixed ? m.fft_plan[q] : 0;
 }
 
 void build_maps(ps3d_ensemble &E)
@@ -161,10 +165,12 @@ void build_maps(ps3d_ensemble &E)
             case S_FRIC: nb = (int64_t)e.fric_tx * e.fric_ty * e.fric_nchunk * 4; break;
             case S_R2C: nb = ens_r2c_blocks(e); break;
             case S_COLS: nb = ens_cols_blocks(e, g.ny, g.nz); break;
-            case S_THOMAS: nb = lid ? cdiv((int64_t)g.nkx * g.ny, ens_thomas_cols()) : 0; break;
+            case S_THOMAS: nb = lid ? cdiv((int64_t)g.nkx * g.ny, ens_thomas_cols(E.thomas_reg)) : 0; break;
             case S_ZSOLVE: nb = lid ? 0 : ens_cols_blocks(e, g.nz, g.ny); break;
             case S_PROJ: nb = (int64_t)cdiv(g.nx, 64) * cdiv(g.ny, 4) * g.nz; break;
             case S_PTS: nb = E.mixed ? cdiv(g.H, 256) : 0; break;
+            case S_HCOPY: nb = e.forced ? 4 * cdiv(g.H - g.N, 256) : 0; break;
+            case S_BGAB: nb = e.forced ? (int64_t)cdiv(g.nx, 64) * cdiv(g.ny, 4) * g.nz : 0; break;
             }
             E.maps[s].start[i] = (int)acc;
             acc += nb;
@@ -234,7 +240,7 @@ void enqueue_solve_f(ps3d_ensemble &E, int which)
     hipStream_t s = E.stream;
     ens_r2c_f(s, E.dev_tab, E.maps[S_R2C], E.dev_state, E.lds_x);
     ens_cols_y_f(s, E.dev_tab, E.maps[S_COLS], -1, E.ny_max, E.lds_y);
-    ens_thomas_f(s, E.dev_tab, E.maps[S_THOMAS], E.nzl_max);
+    ens_thomas_f(s, E.dev_tab, E.maps[S_THOMAS], E.nzl_max, E.thomas_reg);
     ens_zsolve_f(s, E.dev_tab, E.maps[S_ZSOLVE], E.nz_max, E.lds_z);
     ens_cols_y_f(s, E.dev_tab, E.maps[S_COLS], +1, E.ny_max, E.lds_y);
     ens_c2r_f(s, E.dev_tab, E.maps[S_R2C], E.dev_state, which, E.lds_x);
@@ -280,9 +286,13 @@ void enqueue_step(ps3d_ensemble &E)
     hipStream_t s = E.stream;
     ens_tendency(s, E.dev_tab, E.maps[S_TEND], E.dev_state);
     ens_friction_ab(s, E.dev_tab, E.maps[S_FRIC], E.dev_state);   // writes the halo images too
+    // forced members (their friction went to the third buffers): the
+    // pre-friction halos, then set_forcing (integrate.f90:82) and AB
+    ens_halo_copy(s, E.dev_tab, E.maps[S_HCOPY], E.dev_state);
+    ens_background_ab(s, E.dev_tab, E.maps[S_BGAB], E.dev_state);
     ens_r2c(s, E.dev_tab, E.maps[S_R2C], E.dev_state, E.lds_x);
     ens_cols_y(s, E.dev_tab, E.maps[S_COLS], -1, E.ny_max, E.lds_y);
-    ens_thomas(s, E.dev_tab, E.maps[S_THOMAS], E.nzl_max);
+    ens_thomas(s, E.dev_tab, E.maps[S_THOMAS], E.nzl_max, E.thomas_reg);
     ens_zsolve(s, E.dev_tab, E.maps[S_ZSOLVE], E.nz_max, E.lds_z);
     ens_cols_y(s, E.dev_tab, E.maps[S_COLS], +1, E.ny_max, E.lds_y);
     ens_c2r(s, E.dev_tab, E.maps[S_R2C], E.lds_x);
@@ -324,15 +334,155 @@ void destroy(ps3d_ensemble *e)
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
     for (ps3d_model *m : e->mem) ps3d_destroy(m);
+    for (void *p : e->tmp_alloc) (void)hipFree(p);
     if (e->dev_tab) (void)hipFree(e->dev_tab);
     if (e->dev_state) (void)hipFree(e->dev_state);
     if (e->own_stream && e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
 
+// a member's forcing as its model holds it (set by the single-model calls)
+void refresh_forcing(ps3d_ensemble &E, int i)
+{
+    ps3d_model &m = *E.mem[i];
+    EnsMember &e = E.tab[i];
+    e.ub = m.bg_on ? m.bg_u : nullptr;
+    e.bb = (m.bg_on && m.bg_b_on) ? m.bg_b : nullptr;
+    bool any = e.ub != nullptr;
+    for (int q = 0; q < 4; ++q) {
+        e.bf[q] = m.bf_on[q] ? m.bf[q] : nullptr;
+        any = any || e.bf[q];
+    }
+    if (any && !e.tmp[0]) {
+        for (int q = 0; q < 4; ++q) {
+            void *p = nullptr;
+            PS3D_HIP_CHECK(hipMalloc(&p, sizeof(double) * (size_t)m.g.H));
+            E.tmp_alloc.push_back(p);
+            e.tmp[q] = static_cast<double *>(p);
+        }
+    }
+    e.forced = any ? 1 : 0;
+    build_maps(E);
+    upload_table(E);
+}
+
+// Radix plans of a mixed ensemble's FP32 transforms, per axis and transform
+// length: each candidate is set on every member with that length and the
+// stages that run it are timed over the whole ensemble (x: the forward x
+// pass; y: the forward x pass and both y passes; z: those and the periodic
+// z solve).  They write only the spectral scratch FSf, which every solve
+// starts by overwriting.
+void tune_fft_plans(ps3d_ensemble &E, std::map<std::string, int> &cache, int reps, hipEvent_t t0, hipEvent_t t1)
+{
+    const char *axis_name[3] = {"ens_fft_plan_x", "ens_fft_plan_y", "ens_fft_plan_z"};
+    auto len_of = [&](size_t i, int ax) {
+        const Grid &g = E.mem[i]->g;
+        return ax == 0 ? g.nx / 2 : (ax == 1 ? g.ny : g.nz);
+    };
+    auto uses_axis = [&](size_t i, int ax) { return ax < 2 || !E.tab[i].c.lid; };
+    auto set_plan = [&](int ax, int len, int plan) {
+        for (size_t i = 0; i < E.mem.size(); ++i)
+            if (uses_axis(i, ax) && len_of(i, ax) == len) {
+                E.mem[i]->fft_plan[ax] = plan;
+                E.tab[i].fft_plan[ax] = plan;
+            }
+    };
+    auto run = [&](int ax) {
+        hipStream_t s = E.stream;
+        ens_r2c_f(s, E.dev_tab, E.maps[S_R2C], E.dev_state, E.lds_x);
+        if (ax >= 1) ens_cols_y_f(s, E.dev_tab, E.maps[S_COLS], -1, E.ny_max, E.lds_y);
+        if (ax == 2) ens_zsolve_f(s, E.dev_tab, E.maps[S_ZSOLVE], E.nz_max, E.lds_z);
+        if (ax >= 1) ens_cols_y_f(s, E.dev_tab, E.maps[S_COLS], +1, E.ny_max, E.lds_y);
+    };
+    for (int ax = 0; ax < 3; ++ax) {
+        std::vector<int> lens;
+        for (size_t i = 0; i < E.mem.size(); ++i) {
+            const int L = len_of(i, ax);
+            if (uses_axis(i, ax) && fft_plan_choice(L) && std::find(lens.begin(), lens.end(), L) == lens.end())
+                lens.push_back(L);
+        }
+        for (int L : lens) {
+            const std::string key = std::string(axis_name[ax]) + " " + std::to_string(L) + ":mixed";
+            auto hit = cache.find(key);
+            if (hit != cache.end()) {
+                set_plan(ax, L, hit->second);
+                E.tuned[key] = hit->second;
+                continue;
+            }
+            int best = 0;
+            float best_ms = 1e30f;
+            for (int plan : {0, 1}) {
+                set_plan(ax, L, plan);
+                upload_table(E);
+                run(ax);   // warm-up
+                PS3D_HIP_CHECK(hipEventRecord(t0, E.stream));
+                for (int r = 0; r < reps; ++r) run(ax);
+                PS3D_HIP_CHECK(hipEventRecord(t1, E.stream));
+                PS3D_HIP_CHECK(hipEventSynchronize(t1));
+                float ms = 0.f;
+                PS3D_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = plan;
+                }
+            }
+            set_plan(ax, L, best);
+            upload_table(E);
+            E.tuned[key] = best;
+            cache[key] = best;
+        }
+    }
+}
+
+
 }  // namespace
 
 extern "C" {
+
+int ps3d_ensemble_set_background_flow(ps3d_ensemble *e, int member, const double *u_back, const double *b_back)
+{
+    return guard([&] {
+        check_member(e, member);
+        PS3D_REQUIRE(!e->mixed, "ensemble forcing: PS3D_FP64 members only");
+        PS3D_HIP_CHECK(hipStreamSynchronize(e->stream));
+        const int rc = ps3d_set_background_flow(e->mem[member], u_back, b_back);
+        if (rc != PS3D_OK) throw Error(rc, error_slot());
+        refresh_forcing(*e, member);
+    });
+}
+
+int ps3d_ensemble_set_body_force(ps3d_ensemble *e, int member, const double *const f[4])
+{
+    return guard([&] {
+        check_member(e, member);
+        PS3D_REQUIRE(!e->mixed, "ensemble forcing: PS3D_FP64 members only");
+        PS3D_HIP_CHECK(hipStreamSynchronize(e->stream));
+        const int rc = ps3d_set_body_force(e->mem[member], f);
+        if (rc != PS3D_OK) throw Error(rc, error_slot());
+        refresh_forcing(*e, member);
+    });
+}
+
+int ps3d_ensemble_set_fft_plan(ps3d_ensemble *e, int member, const int32_t plans[3])
+{
+    return guard([&] {
+        check_member(e, member);
+        PS3D_HIP_CHECK(hipStreamSynchronize(e->stream));
+        const int rc = ps3d_set_fft_plan(e->mem[member], plans);
+        if (rc != PS3D_OK) throw Error(rc, error_slot());
+        for (int q = 0; q < 3; ++q) e->tab[member].fft_plan[q] = e->mem[member]->fft_plan[q];
+        upload_table(*e);
+    });
+}
+
+int ps3d_ensemble_get_fft_plan(ps3d_ensemble *e, int member, int32_t plans[3])
+{
+    return guard([&] {
+        check_member(e, member);
+        PS3D_REQUIRE(plans, "null plans");
+        for (int q = 0; q < 3; ++q) plans[q] = e->tab[member].fft_plan[q];
+    });
+}
 
 int ps3d_ensemble_create(int n, const ps3d_params *prms, void *stream, ps3d_ensemble **out)
 {
@@ -346,7 +496,7 @@ int ps3d_ensemble_create(int n, const ps3d_params *prms, void *stream, ps3d_ense
             PS3D_REQUIRE(p.precision == PS3D_FP64 || p.precision == PS3D_MIXED, who + "unknown precision form");
             PS3D_REQUIRE(p.precision == prms[0].precision, who + "the members of an ensemble share one precision form");
             PS3D_REQUIRE(p.precision != PS3D_MIXED || !p.enable_vertical_boundaries ||
-                             (sizeof(double2) + sizeof(double)) * (size_t)ens_thomas_cols() * (size_t)p.nz <=
+                             (sizeof(double2) + sizeof(double)) * (size_t)ens_thomas_cols(false) * (size_t)p.nz <=
                                  160 * 1024,
                          who + "mixed ensembles: rigid-lid members of at most 213 levels");
             PS3D_REQUIRE(!(p.Avbi > 0.) && !(p.Kvbi > 0.),
@@ -378,6 +528,11 @@ int ps3d_ensemble_create(int n, const ps3d_params *prms, void *stream, ps3d_ense
                 set_chunks(m->g.nz, default_tend_kc(m->g), E->tab[i].tend_kc, E->tab[i].tend_nchunk);
                 set_chunks(m->g.nz, default_fric_kc(m->g), E->tab[i].fric_kc, E->tab[i].fric_nchunk);
             }
+            E->thomas_reg = true;
+            for (ps3d_model *m : E->mem)
+                if (m->prm.enable_vertical_boundaries)
+                    E->thomas_reg = E->thomas_reg && m->thomas_reg &&
+                                    thomas_reg_ok(m->g.nz, (uint64_t)m->g.S * (E->mixed ? sizeof(float2) : sizeof(double2)));
             build_maps(*E);
             PS3D_HIP_CHECK(hipMalloc(&E->dev_tab, sizeof(EnsMember) * n));
             PS3D_HIP_CHECK(hipMalloc(&E->dev_state, sizeof(EnsState)));
@@ -581,6 +736,9 @@ int ps3d_ensemble_tune(ps3d_ensemble *e, const char *cache_path, int reps)
                     cache[key] = best;
                 }
             }
+            build_maps(*e);
+            upload_table(*e);
+            if (e->mixed) tune_fft_plans(*e, cache, reps, t0, t1);
         } catch (...) {
             (void)hipEventDestroy(t0);
             (void)hipEventDestroy(t1);
@@ -606,6 +764,20 @@ int ps3d_ensemble_describe(ps3d_ensemble *e, char *buf, int len)
         for (auto &kv : seen) {
             o << "ens_tendency " << kv.first << " " << kv.second.first << "\n";
             o << "ens_friction_ab " << kv.first << " " << kv.second.second << "\n";
+        }
+        if (e->mixed) {
+            // radix plans by axis and transform length
+            std::map<std::string, int> plans;
+            const char *names[3] = {"ens_fft_plan_x", "ens_fft_plan_y", "ens_fft_plan_z"};
+            for (size_t i = 0; i < e->mem.size(); ++i) {
+                const Grid &g = e->mem[i]->g;
+                const int lens[3] = {g.nx / 2, g.ny, g.nz};
+                for (int ax = 0; ax < 3; ++ax)
+                    if (ax < 2 || !e->tab[i].c.lid)
+                        plans[std::string(names[ax]) + " " + std::to_string(lens[ax]) + ":mixed"] =
+                            e->tab[i].fft_plan[ax];
+            }
+            for (auto &kv : plans) o << kv.first << " " << kv.second << "\n";
         }
         const std::string s = o.str();
         n = (int)s.size();

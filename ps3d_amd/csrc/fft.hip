@@ -21,6 +21,7 @@
 #include "ps3d_internal.h"
 
 #include <cmath>
+#include <type_traits>
 
 // the factorisation (radix 4 passes, then 2, 3, 5), the twiddle table and the
 // butterfly formulas fix the operation order; with contraction off the
@@ -402,18 +403,47 @@ __device__ __forceinline__ void dftc(C *v)
         v[4] = sub(m1, q1);
         v[2] = add(m2, q2);
         v[3] = sub(m2, q2);
+    } else if constexpr (R == 8) {
+        // two radix-4 butterflies on the even and odd inputs, joined by the
+        // eighth roots w^q = e^(SIGN i pi q / 4): X(q) = A(q) + w^q B(q),
+        // X(q + 4) = A(q) - w^q B(q)
+        C a[4] = {v[0], v[2], v[4], v[6]}, b[4] = {v[1], v[3], v[5], v[7]};
+        dftc<4, SIGN>(a);
+        dftc<4, SIGN>(b);
+        const T h = T(0.70710678118654752440);   // 1/sqrt(2)
+        const C b1 = scl(add(b[1], isg<C, SIGN>(b[1])), h);
+        const C b2 = isg<C, SIGN>(b[2]);
+        const C b3 = scl(sub(isg<C, SIGN>(b[3]), b[3]), h);
+        v[0] = add(a[0], b[0]);
+        v[4] = sub(a[0], b[0]);
+        v[1] = add(a[1], b1);
+        v[5] = sub(a[1], b1);
+        v[2] = add(a[2], b2);
+        v[6] = sub(a[2], b2);
+        v[3] = add(a[3], b3);
+        v[7] = sub(a[3], b3);
     }
 }
+
+// Radix plans of the compiled-in bodies.  Plan 0: radix-4 passes, then 2,
+// 3, 5 -- the FP64 plan, the order the oracle's transforms follow, so FP64
+// results are reproducible bit for bit.  Plan 1: radix-8 passes first, then
+// 4, 2, 3, 5 -- fewer passes through LDS; a different rounding, offered to
+// the FP32 transforms of the mixed form, where the autotuner picks the plan
+// per transform length.
+constexpr bool plan_differs(int N) { return N % 8 == 0; }
 
 struct Fac {
     int n;
     int f[MAX_FAC];
 };
 
-constexpr Fac factorize(int N)
+constexpr Fac factorize(int N, int PL)
 {
     Fac F{0, {}};
     int r = N;
+    if (PL == 1)
+        while (r % 8 == 0) { F.f[F.n++] = 8; r /= 8; }
     while (r % 4 == 0) { F.f[F.n++] = 4; r /= 4; }
     while (r % 2 == 0) { F.f[F.n++] = 2; r /= 2; }
     while (r % 3 == 0) { F.f[F.n++] = 3; r /= 3; }
@@ -466,26 +496,26 @@ __device__ __forceinline__ void pass(C *buf, const C *__restrict__ tw)
     __syncthreads();
 }
 
-template <int N, int P, int NS, int NCOL, int T, int ES, int CS, bool COLFAST, int SIGN, class C>
+template <int N, int P, int NS, int NCOL, int T, int ES, int CS, bool COLFAST, int SIGN, int PL, class C>
 __device__ __forceinline__ void passes(C *buf, const C *__restrict__ tw)
 {
-    constexpr Fac F = factorize(N);
+    constexpr Fac F = factorize(N, PL);
     if constexpr (P < F.n) {
         constexpr int R = F.f[P];
         pass<R, N, NS, NCOL, T, ES, CS, COLFAST, SIGN>(buf, tw);
-        passes<N, P + 1, NS * R, NCOL, T, ES, CS, COLFAST, SIGN>(buf, tw);
+        passes<N, P + 1, NS * R, NCOL, T, ES, CS, COLFAST, SIGN, PL>(buf, tw);
     }
 }
 
-template <int N, int NCOL, int T, int ES, int CS, bool COLFAST, int SIGN, class C>
+template <int N, int NCOL, int T, int ES, int CS, bool COLFAST, int SIGN, int PL = 0, class C>
 __device__ __forceinline__ void fft(C *buf, const C *__restrict__ tw)
 {
-    passes<N, 0, 1, NCOL, T, ES, CS, COLFAST, SIGN>(buf, tw);
+    passes<N, 0, 1, NCOL, T, ES, CS, COLFAST, SIGN, PL>(buf, tw);
 }
 
 // x-pass, forward r2c of ROWS rows (see k_r2c_x); the post-processing works
 // on the pair (k, n2-k) in place.  RT = double (drop-in) or float (mixed).
-template <int N2, int ROWS, int T, bool FUSE_DIV, class RT>
+template <int N2, int ROWS, int T, bool FUSE_DIV, class RT, int PL = 0>
 __device__ __forceinline__ void r2c_body(typename cplx_of<RT>::type *lds, int blk, const StepConsts &c,
                                          const typename cplx_of<RT>::type *__restrict__ tw,
                                          const typename cplx_of<RT>::type *__restrict__ wk,
@@ -525,7 +555,7 @@ __device__ __forceinline__ void r2c_body(typename cplx_of<RT>::type *lds, int bl
         }
     }
     __syncthreads();
-    fft<N2, ROWS, T, 1, CS, false, -1>(lds, tw);
+    fft<N2, ROWS, T, 1, CS, false, -1, PL>(lds, tw);
     constexpr int HALF = N2 / 2 + 1;   // k = 0..N2/2 handles (k, N2-k)
 #pragma unroll
     for (int s = 0; s < (ROWS * HALF + T - 1) / T; ++s) {
@@ -557,7 +587,7 @@ __device__ __forceinline__ void r2c_body(typename cplx_of<RT>::type *lds, int bl
     }
 }
 
-template <int N2, int ROWS, int T, bool FUSE_DIV, class RT>
+template <int N2, int ROWS, int T, bool FUSE_DIV, class RT, int PL = 0>
 __global__ __launch_bounds__(T) void r2c_x(StepConsts c, const typename cplx_of<RT>::type *__restrict__ tw,
                                            const typename cplx_of<RT>::type *__restrict__ wk,
                                            const RT *__restrict__ src, const RT *__restrict__ u,
@@ -566,11 +596,11 @@ __global__ __launch_bounds__(T) void r2c_x(StepConsts c, const typename cplx_of<
 {
     __shared__ typename cplx_of<RT>::type lds[ROWS * (N2 + 1)];
     // (measured: the XCD renumbering of the other passes makes this one slower)
-    r2c_body<N2, ROWS, T, FUSE_DIV, RT>(lds, (int)blockIdx.x, c, tw, wk, src, u, v, w, dst, nrows, dstride);
+    r2c_body<N2, ROWS, T, FUSE_DIV, RT, PL>(lds, (int)blockIdx.x, c, tw, wk, src, u, v, w, dst, nrows, dstride);
 }
 
 // x-pass, inverse c2r of ROWS rows (see k_c2r_x), pre-processing in place
-template <int N2, int ROWS, int T, class RT>
+template <int N2, int ROWS, int T, class RT, int PL = 0>
 __device__ __forceinline__ void c2r_body(typename cplx_of<RT>::type *lds, int blk, const StepConsts &c,
                                          const typename cplx_of<RT>::type *__restrict__ tw,
                                          const typename cplx_of<RT>::type *__restrict__ wk,
@@ -611,7 +641,7 @@ __device__ __forceinline__ void c2r_body(typename cplx_of<RT>::type *lds, int bl
         }
     }
     __syncthreads();
-    fft<N2, ROWS, T, 1, CS, false, +1>(lds, tw);
+    fft<N2, ROWS, T, 1, CS, false, +1, PL>(lds, tw);
 #pragma unroll
     for (int s = 0; s < (ROWS * N2 + T - 1) / T; ++s) {
         const int t = (int)threadIdx.x + s * T;
@@ -635,7 +665,7 @@ __device__ __forceinline__ void c2r_body(typename cplx_of<RT>::type *lds, int bl
     }
 }
 
-template <int N2, int ROWS, int T, class RT>
+template <int N2, int ROWS, int T, class RT, int PL = 0>
 __global__ __launch_bounds__(T) void c2r_x(StepConsts c, const typename cplx_of<RT>::type *__restrict__ tw,
                                            const typename cplx_of<RT>::type *__restrict__ wk,
                                            const typename cplx_of<RT>::type *__restrict__ src,
@@ -643,12 +673,12 @@ __global__ __launch_bounds__(T) void c2r_x(StepConsts c, const typename cplx_of<
                                            double rinv_scale, int64_t nrows, int sstride)
 {
     __shared__ typename cplx_of<RT>::type lds[ROWS * (N2 + 1)];
-    c2r_body<N2, ROWS, T, RT>(lds, xcd_remap((int)blockIdx.x, (int)gridDim.x), c, tw, wk, src, dst, dst_is_field,
+    c2r_body<N2, ROWS, T, RT, PL>(lds, xcd_remap((int)blockIdx.x, (int)gridDim.x), c, tw, wk, src, dst, dst_is_field,
                               inv_scale, rinv_scale, nrows, sstride);
 }
 
 // strided c2c pass over TX consecutive kx columns (see k_c2c_cols)
-template <int N, int TX, int T, int SIGN, class C>
+template <int N, int TX, int T, int SIGN, class C, int PL = 0>
 __device__ __forceinline__ void cols_body(C *lds, int bx, int by, const C *__restrict__ tw, C *__restrict__ data,
                                           int nkx, int64_t estride, int64_t lstride, double s1, double s2, double s3,
                                           int nscale)
@@ -666,7 +696,7 @@ __device__ __forceinline__ void cols_body(C *lds, int bx, int by, const C *__res
         }
     }
     __syncthreads();
-    fft<N, TX, T, TX, 1, true, SIGN>(lds, tw);
+    fft<N, TX, T, TX, 1, true, SIGN, PL>(lds, tw);
 #pragma unroll
     for (int s = 0; s < (N * TX + T - 1) / T; ++s) {
         const int t = (int)threadIdx.x + s * T;
@@ -683,7 +713,7 @@ __device__ __forceinline__ void cols_body(C *lds, int bx, int by, const C *__res
     }
 }
 
-template <int N, int TX, int T, int SIGN, class C>
+template <int N, int TX, int T, int SIGN, class C, int PL = 0>
 __global__ __launch_bounds__(T) void c2c_cols(const C *__restrict__ tw, C *__restrict__ data, int nkx,
                                               int64_t estride, int64_t lstride, double s1, double s2,
                                               double s3, int nscale)
@@ -693,12 +723,12 @@ __global__ __launch_bounds__(T) void c2c_cols(const C *__restrict__ tw, C *__res
     // rows are not line-aligned): keep them on one XCD's L2
     const int gx = (int)gridDim.x;
     const int r = xcd_remap((int)blockIdx.x + gx * (int)blockIdx.y, gx * (int)gridDim.y);
-    cols_body<N, TX, T, SIGN, C>(lds, r % gx, r / gx, tw, data, nkx, estride, lstride, s1, s2, s3, nscale);
+    cols_body<N, TX, T, SIGN, C, PL>(lds, r % gx, r / gx, tw, data, nkx, estride, lstride, s1, s2, s3, nscale);
 }
 
 // periodic z solve (see k_z_periodic_solve); the scaling and the division by
 // ksqr are done in FP64 whatever the storage type
-template <int N, int TX, int T, class C>
+template <int N, int TX, int T, class C, int PL = 0>
 __device__ __forceinline__ void zsolve_body(C *lds, int bx, int by, const C *__restrict__ tw, C *__restrict__ data,
                                             const double *__restrict__ ksqr, int nkx, int nkxs, int ny, double dx,
                                             double dy, double dz)
@@ -719,7 +749,7 @@ __device__ __forceinline__ void zsolve_body(C *lds, int bx, int by, const C *__r
         }
     }
     __syncthreads();
-    fft<N, TX, T, TX, 1, true, -1>(lds, tw);
+    fft<N, TX, T, TX, 1, true, -1, PL>(lds, tw);
 #pragma unroll
     for (int s = 0; s < (N * TX + T - 1) / T; ++s) {
         const int t = (int)threadIdx.x + s * T;
@@ -736,7 +766,7 @@ __device__ __forceinline__ void zsolve_body(C *lds, int bx, int by, const C *__r
         }
     }
     __syncthreads();
-    fft<N, TX, T, TX, 1, true, +1>(lds, tw);
+    fft<N, TX, T, TX, 1, true, +1, PL>(lds, tw);
 #pragma unroll
     for (int s = 0; s < (N * TX + T - 1) / T; ++s) {
         const int t = (int)threadIdx.x + s * T;
@@ -747,7 +777,7 @@ __device__ __forceinline__ void zsolve_body(C *lds, int bx, int by, const C *__r
     }
 }
 
-template <int N, int TX, int T, class C>
+template <int N, int TX, int T, class C, int PL = 0>
 __global__ __launch_bounds__(T) void z_periodic_solve(const C *__restrict__ tw, C *__restrict__ data,
                                                       const double *__restrict__ ksqr, int nkx, int nkxs, int ny,
                                                       double dx, double dy, double dz)
@@ -755,7 +785,7 @@ __global__ __launch_bounds__(T) void z_periodic_solve(const C *__restrict__ tw, 
     __shared__ C lds[N * TX];
     const int gx = (int)gridDim.x;
     const int r = xcd_remap((int)blockIdx.x + gx * (int)blockIdx.y, gx * (int)gridDim.y);
-    zsolve_body<N, TX, T, C>(lds, r % gx, r / gx, tw, data, ksqr, nkx, nkxs, ny, dx, dy, dz);
+    zsolve_body<N, TX, T, C, PL>(lds, r % gx, r / gx, tw, data, ksqr, nkx, nkxs, ny, dx, dy, dz);
 }
 
 // launch configurations: x-pass rows so that ROWS*N2 = 1024 values; 256
@@ -771,56 +801,96 @@ template <class C> constexpr int TCOLS(int n) { return n * TCOL<C> / 16 > 256 ? 
 template <class C, int N> constexpr int TCOLN = N >= 1024 ? TCOL<C> / 2 : TCOL<C>;
 template <class C, int N> constexpr int TCOLSN = N * TCOLN<C, N> / 16 > 256 ? N * TCOLN<C, N> / 16 : 256;
 
-template <int N2, class RT>
-bool launch_r2c(Model &m, const RT *src, const RT *u, const RT *v, const RT *w, int fuse_div,
-                typename cplx_of<RT>::type *dst, const typename cplx_of<RT>::type *tw,
-                const typename cplx_of<RT>::type *wk, int64_t nrows, int dstride)
+// plan 1 bodies are compiled for the FP32 transforms of lengths it changes
+template <int N, class RT> constexpr bool has_plan1 = std::is_same<RT, float>::value && plan_differs(N);
+
+template <int N2, class RT, int PL>
+void launch_r2c_pl(Model &m, const RT *src, const RT *u, const RT *v, const RT *w, int fuse_div,
+                   typename cplx_of<RT>::type *dst, const typename cplx_of<RT>::type *tw,
+                   const typename cplx_of<RT>::type *wk, int64_t nrows, int dstride)
 {
     constexpr int ROWS = XROWS(N2);
     const dim3 grid((unsigned)((nrows + ROWS - 1) / ROWS));
     if (fuse_div)
-        hipLaunchKernelGGL((r2c_x<N2, ROWS, TTHREADS, true, RT>), grid, dim3(TTHREADS), 0, m.stream, m.c, tw, wk,
+        hipLaunchKernelGGL((r2c_x<N2, ROWS, TTHREADS, true, RT, PL>), grid, dim3(TTHREADS), 0, m.stream, m.c, tw, wk,
                            nullptr, u, v, w, dst, nrows, dstride);
     else
-        hipLaunchKernelGGL((r2c_x<N2, ROWS, TTHREADS, false, RT>), grid, dim3(TTHREADS), 0, m.stream, m.c, tw, wk,
+        hipLaunchKernelGGL((r2c_x<N2, ROWS, TTHREADS, false, RT, PL>), grid, dim3(TTHREADS), 0, m.stream, m.c, tw, wk,
                            src, nullptr, nullptr, nullptr, dst, nrows, dstride);
+}
+
+template <int N2, class RT>
+bool launch_r2c(Model &m, const RT *src, const RT *u, const RT *v, const RT *w, int fuse_div,
+                typename cplx_of<RT>::type *dst, const typename cplx_of<RT>::type *tw,
+                const typename cplx_of<RT>::type *wk, int64_t nrows, int dstride, int plan)
+{
+    if constexpr (has_plan1<N2, RT>)
+        if (plan == 1) {
+            launch_r2c_pl<N2, RT, 1>(m, src, u, v, w, fuse_div, dst, tw, wk, nrows, dstride);
+            return true;
+        }
+    launch_r2c_pl<N2, RT, 0>(m, src, u, v, w, fuse_div, dst, tw, wk, nrows, dstride);
     return true;
 }
 
 template <int N2, class RT>
 bool launch_c2r(Model &m, const typename cplx_of<RT>::type *src, RT *dst, int dst_is_field, double inv_scale,
                 const typename cplx_of<RT>::type *tw, const typename cplx_of<RT>::type *wk, int64_t nrows,
-                int sstride)
+                int sstride, int plan)
 {
     constexpr int ROWS = XROWS(N2);
     const dim3 grid((unsigned)((nrows + ROWS - 1) / ROWS));
-    hipLaunchKernelGGL((c2r_x<N2, ROWS, TTHREADS, RT>), grid, dim3(TTHREADS), 0, m.stream, m.c, tw, wk, src, dst,
+    if constexpr (has_plan1<N2, RT>)
+        if (plan == 1) {
+            hipLaunchKernelGGL((c2r_x<N2, ROWS, TTHREADS, RT, 1>), grid, dim3(TTHREADS), 0, m.stream, m.c, tw, wk, src,
+                               dst, dst_is_field, inv_scale, 1.0 / inv_scale, nrows, sstride);
+            return true;
+        }
+    hipLaunchKernelGGL((c2r_x<N2, ROWS, TTHREADS, RT, 0>), grid, dim3(TTHREADS), 0, m.stream, m.c, tw, wk, src, dst,
                        dst_is_field, inv_scale, 1.0 / inv_scale, nrows, sstride);
     return true;
 }
 
-template <int N, class C>
-bool launch_cols(const C *tw, hipStream_t st, C *data, int nkx, int64_t estride, int64_t lstride,
-                 unsigned ny_or_nz, int sign, double s1, double s2, double s3, int nscale)
+template <int N, class C, int PL>
+void launch_cols_pl(const C *tw, hipStream_t st, C *data, int nkx, int64_t estride, int64_t lstride,
+                    unsigned ny_or_nz, int sign, double s1, double s2, double s3, int nscale)
 {
     const dim3 grid((unsigned)((nkx + TCOLN<C, N> - 1) / TCOLN<C, N>), ny_or_nz);
     constexpr int T = TCOLSN<C, N>;
     if (sign < 0)
-        hipLaunchKernelGGL((c2c_cols<N, TCOLN<C, N>, T, -1, C>), grid, dim3(T), 0, st, tw, data, nkx, estride, lstride,
-                           s1, s2, s3, nscale);
+        hipLaunchKernelGGL((c2c_cols<N, TCOLN<C, N>, T, -1, C, PL>), grid, dim3(T), 0, st, tw, data, nkx, estride,
+                           lstride, s1, s2, s3, nscale);
     else
-        hipLaunchKernelGGL((c2c_cols<N, TCOLN<C, N>, T, +1, C>), grid, dim3(T), 0, st, tw, data, nkx, estride, lstride,
-                           s1, s2, s3, nscale);
+        hipLaunchKernelGGL((c2c_cols<N, TCOLN<C, N>, T, +1, C, PL>), grid, dim3(T), 0, st, tw, data, nkx, estride,
+                           lstride, s1, s2, s3, nscale);
+}
+
+template <int N, class C>
+bool launch_cols(const C *tw, hipStream_t st, C *data, int nkx, int64_t estride, int64_t lstride,
+                 unsigned ny_or_nz, int sign, double s1, double s2, double s3, int nscale, int plan)
+{
+    if constexpr (has_plan1<N, typename real_of<C>::type>)
+        if (plan == 1) {
+            launch_cols_pl<N, C, 1>(tw, st, data, nkx, estride, lstride, ny_or_nz, sign, s1, s2, s3, nscale);
+            return true;
+        }
+    launch_cols_pl<N, C, 0>(tw, st, data, nkx, estride, lstride, ny_or_nz, sign, s1, s2, s3, nscale);
     return true;
 }
 
 template <int N, class C>
-bool launch_zsolve(Model &m, C *data, const C *tw, int nkxs)
+bool launch_zsolve(Model &m, C *data, const C *tw, int nkxs, int plan)
 {
     const dim3 grid((unsigned)((m.g.nkx + TCOLN<C, N> - 1) / TCOLN<C, N>), (unsigned)m.g.ny);
     constexpr int T = TCOLSN<C, N>;
-    hipLaunchKernelGGL((z_periodic_solve<N, TCOLN<C, N>, T, C>), grid, dim3(T), 0, m.stream, tw, data, m.ksqr, m.g.nkx,
-                       nkxs, m.g.ny, m.prm.dx, m.prm.dy, m.prm.dz);
+    if constexpr (has_plan1<N, typename real_of<C>::type>)
+        if (plan == 1) {
+            hipLaunchKernelGGL((z_periodic_solve<N, TCOLN<C, N>, T, C, 1>), grid, dim3(T), 0, m.stream, tw, data, m.ksqr,
+                               m.g.nkx, nkxs, m.g.ny, m.prm.dx, m.prm.dy, m.prm.dz);
+            return true;
+        }
+    hipLaunchKernelGGL((z_periodic_solve<N, TCOLN<C, N>, T, C, 0>), grid, dim3(T), 0, m.stream, tw, data, m.ksqr,
+                       m.g.nkx, nkxs, m.g.ny, m.prm.dx, m.prm.dy, m.prm.dz);
     return true;
 }
 
@@ -831,10 +901,10 @@ bool launch_zsolve(Model &m, C *data, const C *tw, int nkxs)
 template <class RT>
 bool dispatch_r2c(Model &m, const RT *src, const RT *u, const RT *v, const RT *w, int fuse_div,
                   typename cplx_of<RT>::type *dst, const typename cplx_of<RT>::type *tw,
-                  const typename cplx_of<RT>::type *wk, int64_t nrows, int dstride)
+                  const typename cplx_of<RT>::type *wk, int64_t nrows, int dstride, int plan = 0)
 {
     switch (m.px.n) {
-#define CASE(n) case n: return launch_r2c<n, RT>(m, src, u, v, w, fuse_div, dst, tw, wk, nrows, dstride);
+#define CASE(n) case n: return launch_r2c<n, RT>(m, src, u, v, w, fuse_div, dst, tw, wk, nrows, dstride, plan);
         PS3D_FFT_SIZES(CASE)
 #undef CASE
     default: return false;
@@ -844,10 +914,10 @@ bool dispatch_r2c(Model &m, const RT *src, const RT *u, const RT *v, const RT *w
 template <class RT>
 bool dispatch_c2r(Model &m, const typename cplx_of<RT>::type *src, RT *dst, int dst_is_field, double inv_scale,
                   const typename cplx_of<RT>::type *tw, const typename cplx_of<RT>::type *wk, int64_t nrows,
-                  int sstride)
+                  int sstride, int plan = 0)
 {
     switch (m.px.n) {
-#define CASE(n) case n: return launch_c2r<n, RT>(m, src, dst, dst_is_field, inv_scale, tw, wk, nrows, sstride);
+#define CASE(n) case n: return launch_c2r<n, RT>(m, src, dst, dst_is_field, inv_scale, tw, wk, nrows, sstride, plan);
         PS3D_FFT_SIZES(CASE)
 #undef CASE
     default: return false;
@@ -856,10 +926,10 @@ bool dispatch_c2r(Model &m, const typename cplx_of<RT>::type *src, RT *dst, int 
 
 template <class C>
 bool dispatch_cols(int n, const C *tw, hipStream_t st, C *data, int nkx, int64_t estride, int64_t lstride,
-                   unsigned ny_or_nz, int sign, double s1, double s2, double s3, int nscale)
+                   unsigned ny_or_nz, int sign, double s1, double s2, double s3, int nscale, int plan = 0)
 {
     switch (n) {
-#define CASE(n) case n: return launch_cols<n, C>(tw, st, data, nkx, estride, lstride, ny_or_nz, sign, s1, s2, s3, nscale);
+#define CASE(n) case n: return launch_cols<n, C>(tw, st, data, nkx, estride, lstride, ny_or_nz, sign, s1, s2, s3, nscale, plan);
         PS3D_FFT_SIZES(CASE)
 #undef CASE
     default: return false;
@@ -867,10 +937,10 @@ bool dispatch_cols(int n, const C *tw, hipStream_t st, C *data, int nkx, int64_t
 }
 
 template <class C>
-bool dispatch_zsolve(Model &m, C *data, const C *tw, int nkxs)
+bool dispatch_zsolve(Model &m, C *data, const C *tw, int nkxs, int plan = 0)
 {
     switch (m.pz.n) {
-#define CASE(n) case n: return launch_zsolve<n, C>(m, data, tw, nkxs);
+#define CASE(n) case n: return launch_zsolve<n, C>(m, data, tw, nkxs, plan);
         PS3D_FFT_SIZES(CASE)
 #undef CASE
     default: return false;
@@ -994,6 +1064,12 @@ __global__ __launch_bounds__(TTHREADS) void k_ens_r2c_f(const EnsMember *__restr
     switch (e.n2) {
 #define CASE(n)                                                                                              \
     case n:                                                                                                  \
+        if constexpr (plan_differs(n))                                                                       \
+            if (e.fft_plan[0] == 1) {                                                                        \
+                r2c_body<n, XROWS(n), TTHREADS, true, float, 1>(lds, lb, c, e.twxf, e.wkf, nullptr, e.fh[0][nb], \
+                                                                e.fh[1][nb], e.fh[2][nb], e.FSf, nrows, e.nkx); \
+                break;                                                                                       \
+            }                                                                                                \
         r2c_body<n, XROWS(n), TTHREADS, true, float>(lds, lb, c, e.twxf, e.wkf, nullptr, e.fh[0][nb],         \
                                                      e.fh[1][nb], e.fh[2][nb], e.FSf, nrows, e.nkx);          \
         break;
@@ -1018,6 +1094,12 @@ __global__ __launch_bounds__(TTHREADS) void k_ens_c2r_f(const EnsMember *__restr
     switch (e.n2) {
 #define CASE(n)                                                                                              \
     case n:                                                                                                  \
+        if constexpr (plan_differs(n))                                                                       \
+            if (e.fft_plan[0] == 1) {                                                                        \
+                c2r_body<n, XROWS(n), TTHREADS, float, 1>(lds, lb, c, e.twxf, e.wkf, e.FSf, dst, 1, inv, rinv, \
+                                                          nrows, e.nkx);                                     \
+                break;                                                                                       \
+            }                                                                                                \
         c2r_body<n, XROWS(n), TTHREADS, float>(lds, lb, c, e.twxf, e.wkf, e.FSf, dst, 1, inv, rinv, nrows,    \
                                                e.nkx);                                                       \
         break;
@@ -1042,9 +1124,17 @@ __global__ __launch_bounds__(TTHREADS) void k_ens_cols_y_f(const EnsMember *__re
     switch (e.c.ny) {
 #define CASE(n)                                                                                              \
     case n:                                                                                                  \
-        if constexpr (n <= NMAX && TCOLS<float2>(n) == TTHREADS)                                             \
+        if constexpr (n <= NMAX && TCOLS<float2>(n) == TTHREADS) {                                           \
+            if constexpr (plan_differs(n))                                                                   \
+                if (e.fft_plan[1] == 1) {                                                                    \
+                    cols_body<n, TCOL<float2>, TTHREADS, SIGN, float2, 1>(lds, bx, by, e.twyf, e.FSf, e.nkx,  \
+                                                                          (int64_t)e.nkx, (int64_t)e.nkx * e.c.ny, \
+                                                                          s1, s2, 0.0, nscale);              \
+                    break;                                                                                   \
+                }                                                                                            \
             cols_body<n, TCOL<float2>, TTHREADS, SIGN, float2>(lds, bx, by, e.twyf, e.FSf, e.nkx, (int64_t)e.nkx,     \
                                                        (int64_t)e.nkx * e.c.ny, s1, s2, 0.0, nscale);        \
+        }                                                                                                    \
         break;
         PS3D_FFT_SIZES(CASE)
 #undef CASE
@@ -1065,9 +1155,16 @@ __global__ __launch_bounds__(TTHREADS) void k_ens_zsolve_f(const EnsMember *__re
     switch (e.c.nz) {
 #define CASE(n)                                                                                              \
     case n:                                                                                                  \
-        if constexpr (n <= NMAX && TCOLS<float2>(n) == TTHREADS)                                             \
+        if constexpr (n <= NMAX && TCOLS<float2>(n) == TTHREADS) {                                           \
+            if constexpr (plan_differs(n))                                                                   \
+                if (e.fft_plan[2] == 1) {                                                                    \
+                    zsolve_body<n, TCOL<float2>, TTHREADS, float2, 1>(lds, bx, by, e.twzf, e.FSf, e.ksqr, e.nkx, \
+                                                                      e.nkx, e.c.ny, e.c.dx, e.c.dy, e.c.dz); \
+                    break;                                                                                   \
+                }                                                                                            \
             zsolve_body<n, TCOL<float2>, TTHREADS, float2>(lds, bx, by, e.twzf, e.FSf, e.ksqr, e.nkx, e.nkx, e.c.ny,  \
                                                    e.c.dx, e.c.dy, e.c.dz);                                  \
+        }                                                                                                    \
         break;
         PS3D_FFT_SIZES(CASE)
 #undef CASE
@@ -1249,7 +1346,7 @@ void fft_r2c_x_f(Model &m, const float *u, const float *v, const float *w, float
     const Grid &g = m.g;
     ProfScope ps(m, "fft_x_r2c");
     PS3D_REQUIRE(ft::dispatch_r2c<float>(m, nullptr, u, v, w, 1, dst, m.px.twf, m.twf_r2c, (int64_t)g.ny * g.nz,
-                                         g.nkx),
+                                         g.nkx, m.fft_plan[0]),
                  "mixed precision: nx/2 must be one of the compiled-in FFT lengths");
     PS3D_HIP_CHECK(hipGetLastError());
 }
@@ -1258,7 +1355,8 @@ void fft_c2r_x_f(Model &m, const float2 *src, float *dst, double inv_scale)
 {
     const Grid &g = m.g;
     ProfScope ps(m, "fft_x_c2r");
-    PS3D_REQUIRE(ft::dispatch_c2r<float>(m, src, dst, 1, inv_scale, m.px.twf, m.twf_r2c, (int64_t)g.ny * g.nz, g.nkx),
+    PS3D_REQUIRE(ft::dispatch_c2r<float>(m, src, dst, 1, inv_scale, m.px.twf, m.twf_r2c, (int64_t)g.ny * g.nz, g.nkx,
+                                         m.fft_plan[0]),
                  "mixed precision: nx/2 must be one of the compiled-in FFT lengths");
     PS3D_HIP_CHECK(hipGetLastError());
 }
@@ -1269,7 +1367,8 @@ void fft_c2c_y_f(Model &m, float2 *data, int sign, double scale1, double scale2)
     const int nscale = (scale1 != 0.0) ? 2 : 0;
     ProfScope ps(m, sign < 0 ? "fft_y_fwd" : "fft_y_inv");
     PS3D_REQUIRE(ft::dispatch_cols<float2>(m.py.n, m.py.twf, m.stream, data, g.nkx, (int64_t)g.nkx,
-                                           (int64_t)g.nkx * g.ny, (unsigned)g.nz, sign, scale1, scale2, 0.0, nscale),
+                                           (int64_t)g.nkx * g.ny, (unsigned)g.nz, sign, scale1, scale2, 0.0, nscale,
+                                           m.fft_plan[1]),
                  "mixed precision: ny must be one of the compiled-in FFT lengths");
     PS3D_HIP_CHECK(hipGetLastError());
 }
@@ -1277,7 +1376,7 @@ void fft_c2c_y_f(Model &m, float2 *data, int sign, double scale1, double scale2)
 void fft_z_periodic_solve_f(Model &m, float2 *data)
 {
     ProfScope ps(m, "z_solve");
-    PS3D_REQUIRE(ft::dispatch_zsolve<float2>(m, data, m.pz.twf, m.g.nkx),
+    PS3D_REQUIRE(ft::dispatch_zsolve<float2>(m, data, m.pz.twf, m.g.nkx, m.fft_plan[2]),
                  "mixed precision: nz must be one of the compiled-in FFT lengths");
     PS3D_HIP_CHECK(hipGetLastError());
 }
@@ -1291,6 +1390,8 @@ bool fft_sizes_compiled(int n)
     default: return false;
     }
 }
+
+bool fft_plan_choice(int n) { return fft_sizes_compiled(n) && ft::plan_differs(n); }
 
 std::string fft_sizes_text()
 {

@@ -62,7 +62,13 @@ constexpr int ONX = 2;   // halo width, main_module.f90:93
 #ifndef PS3D_FRIC_TY
 #define PS3D_FRIC_TY 8
 #endif
-constexpr int TEND_TX = 64, TEND_TY = 8;
+#ifndef PS3D_TEND_TX
+#define PS3D_TEND_TX 64
+#endif
+#ifndef PS3D_TEND_TY
+#define PS3D_TEND_TY 8
+#endif
+constexpr int TEND_TX = PS3D_TEND_TX, TEND_TY = PS3D_TEND_TY;
 constexpr int FRIC_TX = 64, FRIC_TY = PS3D_FRIC_TY;
 
 // Extents of one model.  Physical fields are (X, Y, Z) = (nx+4, ny+4, nz+4)
@@ -226,6 +232,10 @@ struct Model {
     // FP64 Thomas solve with whole columns in registers where nz is one of
     // the compiled-in column lengths (k_thomas_reg), else k_thomas_fly
     bool thomas_reg = true;
+    // radix plans of the x, y and z transforms (0: radix 4 first, the FP64
+    // plan; 1: radix 8 first); FP64 transforms always use plan 0, FP32 ones
+    // (mixed form) the plan set here (ps3d_set_fft_plan, ps3d_tune)
+    int fft_plan[3] = {0, 0, 0};
 
     // mixed form (PS3D_MIXED): prognostic fields as FP32 (hi, lo) pairs whose
     // sum carries the FP64-accumulated state.  The pressure is carried in
@@ -419,6 +429,13 @@ struct EnsMember {
     float *dpr[3];
     float2 *FSf;
     const float2 *twxf, *twyf, *twzf, *wkf;
+    int fft_plan[3];   // radix plans of the FP32 x, y, z transforms (0, 1)
+    // forcing (FP64 members): background flow u_back, b_back and the body
+    // force (null: none); forced members run friction into the third
+    // buffers tmp, then the halo copy and the forcing + AB stages
+    int forced;
+    double *tmp[4];
+    const double *ub, *bb, *bf[4];
 };
 
 // member of block g: the last member whose first block is <= g (members
@@ -444,6 +461,10 @@ struct EnsState {
 void ens_tendency(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st);
 void ens_friction_ab(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st);
 void ens_project(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st);
+// forced FP64 members: pre-friction halos into the third buffers (4 fields x
+// halo points per member), then forcing + AB (interior points per member)
+void ens_halo_copy(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st);
+void ens_background_ab(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st);
 void ens_advance(hipStream_t s, EnsState *st);
 void ens_r2c(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, size_t lds);
 // nmax: the longest column among the members (selects the compiled variant)
@@ -452,8 +473,15 @@ void ens_zsolve(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nmax
 void ens_c2r(hipStream_t s, const EnsMember *mem, const EnsMap &map, size_t lds);
 // Thomas stage: ens_thomas_cols() columns per work-group; nz_max: most levels
 // of a rigid-lid member (sizes the LDS of the on-the-fly form)
-int ens_thomas_cols();
-void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max);
+// reg: the register form (k_ens_thomas_reg, 64 columns per work-group), for
+// ensembles whose rigid-lid members all have thomas_reg_ok levels
+int ens_thomas_cols(bool reg);
+// nz levels have a compiled-in register Thomas solve, and the spectral array
+// (fs_bytes) is addressed with 32-bit offsets
+bool thomas_reg_ok(int nz, uint64_t fs_bytes);
+// a compiled-in FP32 transform length whose plan 1 differs from plan 0
+bool fft_plan_choice(int n);
+void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max, bool reg);
 // the same stages for members of the mixed form.  cold: the provisional
 // Euler step of the cold start (tendency without the reference pressure,
 // friction + Euler); which: the increment a solve writes and a projection
@@ -471,7 +499,7 @@ void ens_r2c_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const Ens
 void ens_cols_y_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int sign, int nmax, size_t lds);
 void ens_zsolve_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nmax, size_t lds);
 void ens_c2r_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, int which, size_t lds);
-void ens_thomas_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max);
+void ens_thomas_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max, bool reg);
 // block counts and LDS bytes of the batched FFT stages for one member
 int ens_r2c_blocks(const EnsMember &e);
 int ens_cols_blocks(const EnsMember &e, int n_len, int lines);

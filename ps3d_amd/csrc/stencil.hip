@@ -663,12 +663,9 @@ __global__ __launch_bounds__(BX * BY) void k_ab(StepConsts c, AbArgs a)
 // the new values written to the canonical ones, so no point is read after it
 // is written).  The slot tau keeps the forced tendency for later steps; the
 // same expressions as k_background then k_ab.
-__global__ __launch_bounds__(BX * BY) void k_background_ab(StepConsts c, BgArgs<double> bg, AbArgs a)
+__device__ __forceinline__ void background_ab_point(const StepConsts &c, const BgArgs<double> &bg, const AbArgs &a,
+                                                    int i, int j, int k)
 {
-    const int i = blockIdx.x * (int)blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * (int)blockDim.y + threadIdx.y + 1;
-    if (i > c.nx || j > c.ny) return;
-    const int k = (int)blockIdx.z + 1;
     const int64_t o = idx(c, i, j, k);
     double d[4];
     bg_point<double>(c, bg, k, o, d[0], d[1], d[2], d[3]);
@@ -690,6 +687,14 @@ __global__ __launch_bounds__(BX * BY) void k_background_ab(StepConsts c, BgArgs<
         a.dst[f][o] = val;
         if (a.halo) write_images<double>(c, a.dst[f], i, j, k, val, 0);
     }
+}
+
+__global__ __launch_bounds__(BX * BY) void k_background_ab(StepConsts c, BgArgs<double> bg, AbArgs a)
+{
+    const int i = blockIdx.x * (int)blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * (int)blockDim.y + threadIdx.y + 1;
+    if (i > c.nx || j > c.ny) return;
+    background_ab_point(c, bg, a, i, j, (int)blockIdx.z + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -880,6 +885,12 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
         return r;
     };
     Pt nxt = load_pt(kb);
+    // harmonic: the vertical flux through each point's upper face, for the
+    // next level's lower face
+    constexpr int NSQ1 = (R1X * R1Y + FTHREADS - 1) / FTHREADS;
+    T ftc[NSQ1];
+#pragma unroll
+    for (int sq = 0; sq < NSQ1; ++sq) ftc[sq] = zero;
     // fused tendency: the other fields' values at level k (e0), k - 1 (em:
     // W, carried from the level before) and k + 1 (ep, field w only)
     double e0[6] = {}, em[2] = {}, ep[6] = {};
@@ -912,7 +923,7 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
         __syncthreads();
         const int sm = slot(k - 1), sc = slot(k), sp = slot(k + 1);
 #pragma unroll
-        for (int sq = 0; sq < (R1X * R1Y + FTHREADS - 1) / FTHREADS; ++sq) {
+        for (int sq = 0; sq < NSQ1; ++sq) {
             const int q = t + sq * FTHREADS;
             if (q >= R1X * R1Y) break;
             const int rx = q % R1X, ry = q / R1X;
@@ -927,7 +938,13 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
                 const T fn1 = sdiv<T>(Hm * (f0p[sc][oy + 1][ox] - v0), dy, rdy);
                 const T fn0 = sdiv<T>(Hm * (v0 - f0p[sc][oy - 1][ox]), dy, rdy);
                 const T ft1 = (lid && k >= nz) ? zero : sdiv<T>(Vm * (f0p[sp][oy][ox] - v0), dz, rdz);
-                const T ft0 = (lid && k - 1 <= 0) ? zero : sdiv<T>(Vm * (v0 - f0p[sm][oy][ox]), dz, rdz);
+                // the flux through the face below is the face above of the
+                // level before (the same operands, so the same bits), carried
+                // by the thread, which keeps its point from level to level;
+                // formed here at the chunk's first level
+                const T ft0 = k > kb ? ftc[sq]
+                                     : ((lid && k - 1 <= 0) ? zero : sdiv<T>(Vm * (v0 - f0p[sm][oy][ox]), dz, rdz));
+                ftc[sq] = ft1;
                 inc = dt * (sdiv<T>(fe1 - fe0, dx, rdx) + sdiv<T>(fn1 - fn0, dy, rdy) + sdiv<T>(ft1 - ft0, dz, rdz));
                 f1 = v0 + inc;
             }
@@ -1332,7 +1349,89 @@ __global__ __launch_bounds__(FTHREADS) void k_ens_fric(const EnsMember *__restri
     F.Hbi = e.Hbi[q];
     F.zero_top = (f == 2 && F.harm && c.lid) ? 1 : 0;
     a.halo = 1;
+    if (e.forced) {
+        // set_forcing sees the post-friction state (integrate.f90:82): the
+        // friction chain alone into the third buffers; the halo copy and the
+        // forcing + AB stages follow (launch_friction_fused_only, then
+        // launch_background_ab, of a single model)
+        a.order = 0;
+        F.out = e.tmp[f];
+        a.halo = 0;
+    }
     fric_body<double>(c, a, F, lb);
+}
+
+// forced members: the third buffers' halos take the pre-friction values (the
+// halo the reference's in-place friction leaves), k_halo_copy per member
+__global__ __launch_bounds__(256) void k_ens_halo_copy(const EnsMember *__restrict__ mem, EnsMap map,
+                                                        const EnsState *__restrict__ st)
+{
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const StepConsts c = e.c;
+    const int64_t nhalo = e.H - (int64_t)c.nx * c.ny * c.nz;
+    const int64_t per = (nhalo + 255) / 256;
+    const int64_t lb = g - map.start[mi];
+    const int f = (int)(lb / per);
+    const int64_t t = (lb % per) * 256 + threadIdx.x;
+    if (f > 3 || t >= nhalo) return;
+    int i, j, k;
+    halo_ijk(c, t, i, j, k);
+    const int64_t o = idx(c, i, j, k);
+    e.tmp[f][o] = e.f[f][st->par][o];
+}
+
+// forced members: background flow / body force and the Adams-Bashforth
+// update in one pass (k_background_ab of a single model), from the
+// post-friction third buffers into the state's next buffers, halo images
+// included
+__global__ __launch_bounds__(BX * BY) void k_ens_background_ab(const EnsMember *__restrict__ mem, EnsMap map,
+                                                                const EnsState *__restrict__ st)
+{
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const StepConsts c = e.c;
+    const int lb = g - map.start[mi];
+    const int bx = (c.nx + BX - 1) / BX, by = (c.ny + BY - 1) / BY;
+    const int k = lb / (bx * by) + 1, r = lb % (bx * by);
+    const int i = (r % bx) * BX + (int)threadIdx.x + 1;
+    const int j = (r / bx) * BY + (int)threadIdx.y + 1;
+    if (i > c.nx || j > c.ny) return;
+    const int par = st->par, itt = st->itt;
+    const int64_t H = e.H;
+    BgArgs<double> bg{};
+    bg.u = e.tmp[0]; bg.v = e.tmp[1]; bg.w = e.tmp[2]; bg.b = e.tmp[3];
+    const int64_t o0 = (int64_t)(st->tau - 1) * H;
+    bg.du = e.tend[0] + o0; bg.dv = e.tend[1] + o0; bg.dw = e.tend[2] + o0; bg.db = e.tend[3] + o0;
+    bg.ub = e.ub;
+    bg.bb = e.bb;
+    AbArgs a{};
+    for (int q = 0; q < 4; ++q) {
+        bg.f[q] = e.bf[q];
+        a.src[q] = e.tmp[q];
+        a.dst[q] = e.f[q][par ^ 1];
+        a.t0[q] = e.tend[q] + o0;
+        a.t1[q] = e.tend[q] + (int64_t)(st->taum1 - 1) * H;
+        a.t2[q] = e.tend[q] + (int64_t)(st->taum2 - 1) * H;
+        a.scale[q] = c.dt;
+    }
+    a.scale[2] = c.dt / c.dsqr;
+    if (itt == 0) {
+        a.order = 1;
+    } else if (itt == 1 || !e.ab3) {
+        a.order = 2;
+        a.ca = 1.5 + e.eps_ab;
+        a.cb = 0.5 + e.eps_ab;
+    } else {
+        a.order = 3;
+        a.ca = 23.0 / 12.0;
+        a.cb = -16.0 / 12.0;
+        a.cc = 5.0 / 12.0;
+    }
+    a.halo = 1;
+    background_ab_point(c, bg, a, i, j, k);
 }
 
 __global__ __launch_bounds__(BX * BY) void k_ens_project(const EnsMember *__restrict__ mem, EnsMap map,
@@ -2123,6 +2222,20 @@ void ens_friction_ab(hipStream_t s, const EnsMember *mem, const EnsMap &map, con
 {
     if (!map.start[map.n]) return;
     hipLaunchKernelGGL(k_ens_fric, dim3((unsigned)map.start[map.n]), dim3(FTHREADS), 0, s, mem, map, st);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_halo_copy(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st)
+{
+    if (!map.start[map.n]) return;
+    hipLaunchKernelGGL(k_ens_halo_copy, dim3((unsigned)map.start[map.n]), dim3(256), 0, s, mem, map, st);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_background_ab(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st)
+{
+    if (!map.start[map.n]) return;
+    hipLaunchKernelGGL(k_ens_background_ab, dim3((unsigned)map.start[map.n]), dim3(BX, BY), 0, s, mem, map, st);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 

@@ -15,6 +15,8 @@
 // (else left in FS), and the backward sweep writes the solution.
 #include "ps3d_internal.h"
 
+#include <type_traits>
+
 #pragma clang fp contract(off)
 
 namespace ps3d {
@@ -245,24 +247,23 @@ __global__ __launch_bounds__(64) void k_thomas_fly(C *__restrict__ fs, const dou
 // the solution.  With one wave per work-group and little LDS, every wave of
 // the solve is resident at once, so the sequential sweeps of all columns
 // run concurrently.  The arithmetic is that of thomas_col_fly, operation for
-// operation: the same bits.
-template <int NZ>
-__global__ __launch_bounds__(64, 1) void k_thomas_reg(double2 *__restrict__ fs, const double *__restrict__ ksqr,
-                                                      int64_t ncols, double fxa, double dz2, double dsqr)
+// operation: the same bits.  C = double2: the FP64 path; C = float2: the
+// mixed form (FP32 right-hand side and solution, FP64 elimination).
+// cpl: NZ x 64 doubles of LDS; columns col (this lane's) of ncols.
+template <int NZ, class C>
+__device__ __forceinline__ void thomas_reg_col(C *__restrict__ fs, const double *__restrict__ ksqr, int64_t ncols,
+                                               int64_t col, double fxa, double dz2, double dsqr, double *cpl)
 {
-    const int64_t col = (int64_t)xcd_remap((int)blockIdx.x, (int)gridDim.x) * 64 + threadIdx.x;
-    __shared__ double cpl[NZ][64];
-    if (col >= ncols) return;
-    double *cpk = &cpl[0][threadIdx.x];   // c'(level q) at cpk[64 q]
-    // 32-bit byte offsets from the array base (the launcher checks that FS
+    double *cpk = cpl + threadIdx.x;   // c'(level q) at cpk[64 q]
+    // 32-bit byte offsets from the array base (the launchers check that FS
     // is below 4 GiB), so each access is one base register pair plus one
     // offset register
     char *base = reinterpret_cast<char *>(fs);
-    const uint32_t o0 = (uint32_t)col * 16u, st = (uint32_t)ncols * 16u;
-    auto at = [&](int k) -> double2 & { return *reinterpret_cast<double2 *>(base + (o0 + (uint32_t)k * st)); };
+    constexpr uint32_t E = sizeof(C);
+    const uint32_t o0 = (uint32_t)col * E, st = (uint32_t)ncols * E;
     double2 d[NZ];
 #pragma unroll
-    for (int k = 0; k < NZ; ++k) d[k] = at(k);
+    for (int k = 0; k < NZ; ++k) d[k] = to_d2(*reinterpret_cast<const C *>(base + (o0 + (uint32_t)k * st)));
     const double ks = ksqr[col];
     if (ks > 0) {
         const double b1 = -ks - fxa;
@@ -284,7 +285,8 @@ __global__ __launch_bounds__(64, 1) void k_thomas_reg(double2 *__restrict__ fs, 
             d[q] = dp;
         }
         double2 x = dp;
-        // c'(k) is read back from LDS (not kept in registers across the sweep)
+        // c'(k) is read back from LDS (not kept in registers across the
+        // sweep); one wave per work-group, so the barrier holds nobody up
         __syncthreads();
 #pragma unroll
         for (int q = NZ - 2; q >= 0; --q) {
@@ -318,8 +320,44 @@ __global__ __launch_bounds__(64, 1) void k_thomas_reg(double2 *__restrict__ fs, 
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
         asm volatile("" : "+v"(off));
-        *reinterpret_cast<double2 *>(base + off) = d[k];
+        *reinterpret_cast<C *>(base + off) = from_d2<C>(d[k]);
         off += st;
+    }
+}
+
+template <int NZ, class C>
+__global__ __launch_bounds__(64, 1) void k_thomas_reg(C *__restrict__ fs, const double *__restrict__ ksqr,
+                                                      int64_t ncols, double fxa, double dz2, double dsqr)
+{
+    __shared__ double cpl[NZ * 64];
+    const int64_t col = (int64_t)xcd_remap((int)blockIdx.x, (int)gridDim.x) * 64 + threadIdx.x;
+    if (col >= ncols) return;
+    thomas_reg_col<NZ, C>(fs, ksqr, ncols, col, fxa, dz2, dsqr, cpl);
+}
+
+// the same for the rigid-lid members of a batched ensemble, 64 columns per
+// work-group; every member's nz is one of the compiled-in lengths
+// (thomas_reg_ok), selected at run time
+template <class C>
+__global__ __launch_bounds__(64, 1) void k_ens_thomas_reg(const EnsMember *__restrict__ mem, EnsMap map)
+{
+    __shared__ double cpl[64 * 64];
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const int64_t ncols = (int64_t)e.nkx * e.c.ny;
+    const int64_t col = (int64_t)(g - map.start[mi]) * 64 + threadIdx.x;
+    if (col >= ncols) return;
+    C *fs;
+    if constexpr (std::is_same<C, float2>::value) fs = e.FSf;
+    else fs = e.FS;
+    const double fxa = e.fxa, dz2 = e.c.dz * e.c.dz, dsqr = e.c.dsqr;
+    switch (e.c.nz) {
+    case 16: thomas_reg_col<16, C>(fs, e.ksqr, ncols, col, fxa, dz2, dsqr, cpl); break;
+    case 32: thomas_reg_col<32, C>(fs, e.ksqr, ncols, col, fxa, dz2, dsqr, cpl); break;
+    case 48: thomas_reg_col<48, C>(fs, e.ksqr, ncols, col, fxa, dz2, dsqr, cpl); break;
+    case 64: thomas_reg_col<64, C>(fs, e.ksqr, ncols, col, fxa, dz2, dsqr, cpl); break;
+    default: break;
     }
 }
 
@@ -474,6 +512,11 @@ static void allow_lds(K kernel, size_t bytes)
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
+bool thomas_reg_ok(int nz, uint64_t fs_bytes)
+{
+    return (nz == 16 || nz == 32 || nz == 48 || nz == 64) && fs_bytes < (uint64_t(1) << 32);
+}
+
 void tridiag_factor(Model &m)
 {
     if (!m.prm.enable_vertical_boundaries) return;
@@ -488,7 +531,7 @@ void launch_thomas(Model &m, double2 *data)
 {
     const int64_t ncols = (int64_t)m.g.nkx * m.g.ny;
     const double fxa = 1.0 / (m.prm.dz * m.prm.dz) / m.prm.dsqr;
-    if (m.thomas_reg && (uint64_t)m.g.S * sizeof(double2) < (uint64_t(1) << 32)) {
+    if (m.thomas_reg && thomas_reg_ok(m.g.nz, (uint64_t)m.g.S * sizeof(double2))) {
         const dim3 grid((unsigned)((ncols + 63) / 64));
         const double dz2 = m.prm.dz * m.prm.dz, dsqr = m.prm.dsqr;
         bool done = true;
@@ -496,7 +539,8 @@ void launch_thomas(Model &m, double2 *data)
         switch (m.g.nz) {
 #define CASE(n)                                                                                             \
     case n:                                                                                                 \
-        hipLaunchKernelGGL(k_thomas_reg<n>, grid, dim3(64), 0, m.stream, data, m.ksqr, ncols, fxa, dz2, dsqr); \
+        hipLaunchKernelGGL((k_thomas_reg<n, double2>), grid, dim3(64), 0, m.stream, data, m.ksqr, ncols, fxa, dz2, \
+                           dsqr);                                                                           \
         break;
             CASE(16) CASE(32) CASE(48) CASE(64)
 #undef CASE
@@ -544,6 +588,21 @@ void launch_thomas_f(Model &m, float2 *data)
     const double fxa = 1.0 / (m.prm.dz * m.prm.dz) / m.prm.dsqr;
     const unsigned nb = (unsigned)((ncols + 63) / 64);
     ProfScope ps(m, "thomas");
+    if (m.thomas_reg && thomas_reg_ok(m.g.nz, (uint64_t)m.g.S * sizeof(float2))) {
+        const dim3 grid((unsigned)((ncols + 63) / 64));
+        const double dz2 = m.prm.dz * m.prm.dz, dsqr = m.prm.dsqr;
+        switch (m.g.nz) {
+#define CASE(n)                                                                                             \
+    case n:                                                                                                 \
+        hipLaunchKernelGGL((k_thomas_reg<n, float2>), grid, dim3(64), 0, m.stream, data, m.ksqr, ncols, fxa, dz2,  \
+                           dsqr);                                                                           \
+        break;
+            CASE(16) CASE(32) CASE(48) CASE(64)
+#undef CASE
+        }
+        PS3D_HIP_CHECK(hipGetLastError());
+        return;
+    }
     constexpr int cpw = 32;
     const size_t lds_fly = thomas_fly_lds(cpw, m.g.nz);
     if (lds_fly <= kLdsMax) {
@@ -558,11 +617,16 @@ void launch_thomas_f(Model &m, float2 *data)
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
-int ens_thomas_cols() { return ENS_TC; }
+int ens_thomas_cols(bool reg) { return reg ? 64 : ENS_TC; }
 
-void ens_thomas_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max)
+void ens_thomas_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max, bool reg)
 {
     if (!map.start[map.n]) return;
+    if (reg) {
+        hipLaunchKernelGGL(k_ens_thomas_reg<float2>, dim3((unsigned)map.start[map.n]), dim3(64), 0, s, mem, map);
+        PS3D_HIP_CHECK(hipGetLastError());
+        return;
+    }
     const size_t lds = thomas_fly_lds(ENS_TC, nz_max);
     PS3D_REQUIRE(lds <= kLdsMax, "mixed ensemble: rigid-lid members of at most 213 levels");
     allow_lds(k_ens_thomas_f, lds);
@@ -570,9 +634,14 @@ void ens_thomas_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
-void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max)
+void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max, bool reg)
 {
     if (!map.start[map.n]) return;
+    if (reg) {
+        hipLaunchKernelGGL(k_ens_thomas_reg<double2>, dim3((unsigned)map.start[map.n]), dim3(64), 0, s, mem, map);
+        PS3D_HIP_CHECK(hipGetLastError());
+        return;
+    }
     const size_t lds = thomas_fly_lds(ENS_TC, nz_max);
     if (lds <= kLdsMax) {
         allow_lds(k_ens_thomas<true>, lds);

@@ -1,6 +1,8 @@
 // tune.hip -- autotuner of one model's launch shapes: levels per work-group
-// of the two k-marching kernels (tendency; friction + Adams-Bashforth) and
-// columns per wavefront of the Thomas solve.  Each candidate is timed with
+// of the two k-marching kernels (tendency; friction + Adams-Bashforth), the
+// form of the Thomas solve (whole columns in registers, or columns per
+// wavefront of the LDS form) and, in the mixed form, the radix plan of each
+// FP32 transform axis.  Each candidate is timed with
 // HIP events on the live model; the launches timed do not change the model's
 // state (the tendency writes the slot the next step overwrites anyway, the
 // friction kernel writes the second buffers, Thomas runs on a scratch copy of
@@ -184,6 +186,41 @@ void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
             pick("friction_ab", levels, m.fric_kc, reps, nullptr,
                  [&] { launch_friction_ab_raw(m, 3, next_tau, t1, t2); });
     }
+    if (m.mixed) {
+        // radix plans of the FP32 transforms, on scratch arrays: the forward
+        // x pass reads the state and writes the scratch spectrum, the other
+        // passes work on that
+        float2 *fs = nullptr;
+        float *pd = nullptr;
+        try {
+            const int lens[3] = {m.g.nx / 2, m.g.ny, m.g.nz};
+            const char *names[3] = {"fft_plan_x", "fft_plan_y", "fft_plan_z"};
+            for (int ax = 0; ax < 3; ++ax) {
+                if (ax == 2 && m.prm.enable_vertical_boundaries) continue;
+                if (!fft_plan_choice(lens[ax])) continue;
+                if (!fs) {
+                    PS3D_HIP_CHECK(hipMalloc(&fs, sizeof(float2) * (size_t)m.g.S));
+                    PS3D_HIP_CHECK(hipMalloc(&pd, sizeof(float) * (size_t)m.g.H));
+                }
+                pick(names[ax], {0, 1}, m.fft_plan[ax], reps, nullptr, [&] {
+                    fft_r2c_x_f(m, m.fh[0], m.fh[1], m.fh[2], fs);
+                    if (ax == 0) fft_c2r_x_f(m, fs, pd, 1.0);
+                    if (ax >= 1) fft_c2c_y_f(m, fs, -1, 0.0, 0.0);
+                    if (ax == 2) fft_z_periodic_solve_f(m, fs);
+                    if (ax >= 1) fft_c2c_y_f(m, fs, +1, 0.0, 0.0);
+                });
+            }
+        } catch (...) {
+            if (fs) (void)hipFree(fs);
+            if (pd) (void)hipFree(pd);
+            throw;
+        }
+        if (fs) {
+            PS3D_HIP_CHECK(hipStreamSynchronize(m.stream));
+            PS3D_HIP_CHECK(hipFree(fs));
+            PS3D_HIP_CHECK(hipFree(pd));
+        }
+    }
     if (!m.mixed && m.prm.enable_vertical_boundaries && !cache.count("thomas " + sh)) {
         double2 *scratch = nullptr;
         void *flush = nullptr;
@@ -197,11 +234,21 @@ void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
             // columns per work-group whose LDS keeps several work-groups on
             // a CU (64 KiB), measured faster in the step than wider groups
             // that a back-to-back timing can prefer
+            // value 0: whole columns in registers (k_thomas_reg), where nz
+            // is one of its compiled-in lengths
             std::vector<int> cpw;
+            if (thomas_reg_ok(m.g.nz, sizeof(double2) * (uint64_t)m.g.S)) cpw.push_back(0);
             for (int c : {32, 16, 64})
                 if ((sizeof(double2) + sizeof(double)) * (size_t)c * m.g.nz <= 64 * 1024) cpw.push_back(c);
             if (cpw.empty()) cpw.push_back(16);
-            pick("thomas", cpw, m.thomas_cpw, 2 * reps, flush, [&] { launch_thomas(m, scratch); });
+            int form = 0;
+            pick("thomas", cpw, form, 2 * reps, flush, [&] {
+                m.thomas_reg = form == 0;
+                m.thomas_cpw = form;
+                launch_thomas(m, scratch);
+            });
+            m.thomas_reg = form == 0;
+            m.thomas_cpw = form;
         } catch (...) {
             (void)hipFree(scratch);
             if (flush) (void)hipFree(flush);
@@ -212,6 +259,7 @@ void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
         PS3D_HIP_CHECK(hipFree(flush));
     } else if (!m.mixed && m.prm.enable_vertical_boundaries) {
         m.thomas_cpw = cache["thomas " + sh];
+        m.thomas_reg = m.thomas_cpw == 0;
     }
     tune_cache_write(cache_path, cache);
 }
@@ -227,6 +275,10 @@ std::string describe_tuning(const Model &m)
         o << "friction_ab " << sh << " " << m.fric_kc << "\n";
     }
     o << "thomas " << sh << " " << m.thomas_cpw << "\n";
+    if (m.mixed) {
+        const char *names[3] = {"fft_plan_x", "fft_plan_y", "fft_plan_z"};
+        for (int ax = 0; ax < 3; ++ax) o << names[ax] << " " << sh << " " << m.fft_plan[ax] << "\n";
+    }
     return o.str();
 }
 

@@ -62,6 +62,34 @@ class Ensemble:
         check(self.lib.ps3d_ensemble_get_field(self.h, member, FIELDS[name], _dp(a)))
         return a
 
+    def set_background_flow(self, member: int, u_back=None, b_back=None):
+        """Advection by the background flow (u_back, 0, 0) and of b_back for
+        one member (ps3d_ensemble_set_background_flow); u_back None: off."""
+        ub = None if u_back is None else np.asfortranarray(u_back, dtype=np.float64)
+        bb = None if b_back is None else np.asfortranarray(b_back, dtype=np.float64)
+        self._keep = (ub, bb)
+        check(self.lib.ps3d_ensemble_set_background_flow(self.h, member, None if ub is None else _dp(ub),
+                                                         None if bb is None else _dp(bb)))
+
+    def set_body_force(self, member: int, force: dict | None):
+        """Steady body force of one member ({'du': .., 'dw': ..};
+        ps3d_ensemble_set_body_force); None: off."""
+        force = force or {}
+        arrs = [None if force.get(n) is None else np.asfortranarray(force[n], dtype=np.float64)
+                for n in ("du", "dv", "dw", "db")]
+        tab = (ctypes.POINTER(ctypes.c_double) * 4)(*[None if a is None else _dp(a) for a in arrs])
+        check(self.lib.ps3d_ensemble_set_body_force(self.h, member, tab))
+
+    def fft_plan(self, member: int) -> tuple:
+        """Radix plans (x, y, z) of one member's FP32 transforms
+        (ps3d_ensemble_get_fft_plan)."""
+        a = (ctypes.c_int32 * 3)()
+        check(self.lib.ps3d_ensemble_get_fft_plan(self.h, member, a))
+        return tuple(a)
+
+    def set_fft_plan(self, member: int, plans):
+        check(self.lib.ps3d_ensemble_set_fft_plan(self.h, member, (ctypes.c_int32 * 3)(*plans)))
+
     def run(self, nsteps: int, use_graph: bool = True):
         st = (ctypes.c_int32 * 4)(self.itt, self.tau, self.taum1, self.taum2)
         check(self.lib.ps3d_ensemble_run(self.h, nsteps, st, int(use_graph)))

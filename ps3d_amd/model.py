@@ -287,6 +287,18 @@ class Model:
         kernel of their own (False, the default): ps3d_set_fused_tendency."""
         check(self.lib.ps3d_set_fused_tendency(self.h, 1 if on else 0))
 
+    @property
+    def fft_plan(self) -> tuple:
+        """Radix plans (x, y, z) of the FP32 transforms (ps3d_get_fft_plan):
+        0 radix 4 first, 1 radix 8 first."""
+        a = (ctypes.c_int32 * 3)()
+        check(self.lib.ps3d_get_fft_plan(self.h, a))
+        return tuple(a)
+
+    @fft_plan.setter
+    def fft_plan(self, plans):
+        check(self.lib.ps3d_set_fft_plan(self.h, (ctypes.c_int32 * 3)(*plans)))
+
     def device_ptr(self, name: str, slot: int | None = None) -> int:
         """Device address of a field array (ps3d_device_ptr)."""
         out = ctypes.c_void_p()

@@ -28,12 +28,18 @@ def _members():
     return out
 
 
-def _single(prm, fields, steps):
+def _single(prm, fields, steps, plans=None, plans_from=0):
+    """A single model run for steps steps; plans: the FP32 radix plans it
+    switches to after plans_from steps (as an ensemble tuned then)."""
     from ps3d_amd.model import Model
     m = Model(prm)
     for n, a in fields.items():
         m.f[n][...] = a
     m.push()
+    if plans is not None:
+        m.run(plans_from)
+        m.fft_plan = plans
+        steps -= plans_from
     m.run(steps)
     m.pull()
     out = {n: m.f[n].copy() for n in NAMES}
@@ -115,6 +121,29 @@ def test_mixed_ensemble_matches_single_mixed_models(use_graph):
     ens.close()
 
 
+def test_mixed_ensemble_radix8_plans_match_single_models():
+    """Members set to the radix-8 plan on every axis (plan 1; the lengths
+    without a factor 8 run plan 0 either way) equal single mixed models with
+    the same plans bit for bit; a FP64 ensemble refuses plan 1."""
+    members = _mixed_members()
+    ens = _ensemble(members)
+    for i in range(len(members)):
+        ens.set_fft_plan(i, (1, 1, 1))
+        assert ens.fft_plan(i) == (1, 1, 1)
+    ens.run(8, use_graph=True)
+    for i, (prm, fields) in enumerate(members):
+        ref = _single(prm, fields, 8, plans=(1, 1, 1), plans_from=0)
+        for n in NAMES:
+            got = ens.get(i, n)
+            assert np.array_equal(got, ref[n]), (i, n, float(np.abs(got - ref[n]).max()))
+    ens.close()
+    from ps3d_amd import _lib
+    e64 = _ensemble(_members()[:1])
+    with pytest.raises(_lib.Ps3dError, match="plan 0"):
+        e64.set_fft_plan(0, (1, 0, 0))
+    e64.close()
+
+
 def test_mixed_ensemble_tuning_keeps_results(tmp_path):
     members = _mixed_members()
     ens = _ensemble(members)
@@ -123,9 +152,11 @@ def test_mixed_ensemble_tuning_keeps_results(tmp_path):
     desc = ens.describe()
     print(desc, flush=True)
     assert ":mixed" in desc
+    # the radix plans picked per transform length (lengths with a factor 8)
+    assert "ens_fft_plan_x 64:mixed" in desc and "ens_fft_plan_y 64:mixed" in desc
     ens.run(3)
     for i, (prm, fields) in enumerate(members):
-        ref = _single(prm, fields, 6)
+        ref = _single(prm, fields, 6, plans=ens.fft_plan(i), plans_from=3)
         for n in NAMES:
             assert np.array_equal(ens.get(i, n), ref[n]), (i, n)
     ens.close()
@@ -156,8 +187,50 @@ def test_c4_ensemble_500_steps(mixed):
     ens.run(497, use_graph=True)
     for i in (0, 16, 47):
         prm, fields = members[i]
-        ref = _single(prm, fields, 500)
+        ref = _single(prm, fields, 500, plans=ens.fft_plan(i) if mixed else None, plans_from=3)
         for n in NAMES:
             got = ens.get(i, n)
             assert np.array_equal(got, ref[n]), (i, n, float(np.abs(got - ref[n]).max()))
+    ens.close()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_forced_members_match_single_forced_models(use_graph):
+    """Forcing in the batched ensemble: members with background-flow
+    advection (lee_wave1.f90:239-314) and a body force, next to unforced
+    ones, each bit-identical to a single model with the same forcing -- the
+    friction chain into the third buffers, the pre-friction halos copied,
+    then forcing + Adams-Bashforth, as a single forced model steps."""
+    from ps3d_amd import configs
+    from ps3d_amd.ensemble import Ensemble
+    from ps3d_amd.model import Model
+    from test_gpu_lee_wave import _forcing
+    members = [configs.random_state(32, 24, 12, seed=6, lid=True, Avbi=0.0, Kvbi=0.0),
+               configs.random_state(24, 16, 8, seed=7, lid=False, Avbi=0.0, Kvbi=0.0),
+               configs.random_state(16, 16, 16, seed=8, lid=True, Avbi=0.0, Kvbi=0.0)]
+    forcing = {0: _forcing(members[0][0], seed=4), 1: _forcing(members[1][0], seed=5)}
+    forcing[1] = (forcing[1][0], None, forcing[1][2])   # no background buoyancy
+    ens = Ensemble([p for p, _ in members])
+    for i, (_, fields) in enumerate(members):
+        for k, a in fields.items():
+            ens.set(i, k, a)
+    for i, (ub, bb, force) in forcing.items():
+        ens.set_background_flow(i, ub, bb)
+        ens.set_body_force(i, force)
+    ens.run(12, use_graph=use_graph)
+    for i, (prm, fields) in enumerate(members):
+        m = Model(prm)
+        for k, a in fields.items():
+            m.f[k][...] = a
+        m.push()
+        if i in forcing:
+            ub, bb, force = forcing[i]
+            m.set_background_flow(ub, bb)
+            m.set_body_force(force)
+        m.run(12)
+        m.pull()
+        for n in NAMES:
+            got = ens.get(i, n)
+            assert np.array_equal(got, m.f[n]), (i, n, float(np.abs(got - m.f[n]).max()))
+        m.close()
     ens.close()

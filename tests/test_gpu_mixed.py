@@ -167,3 +167,70 @@ def test_mixed_at_reference_config_sizes(case):
             assert v <= TOL, (upto, k, v)
     gmx.close()
     orc.close()
+
+
+@pytest.mark.parametrize("case", ["lid_128x128x32", "periodic_64x64x32"])
+def test_radix8_plans_vs_fp64(case):
+    """The FP32 transforms with radix-8 passes first (plan 1) on every axis
+    that has a factor 8 -- x (n/2 = 64 or 32), y and, periodic, z: within
+    the stated accuracy of the FP64 path after 100 steps, and not the same
+    bits as plan 0 (so the other factorisation ran)."""
+    from ps3d_amd import configs, diagnose
+    from ps3d_amd.model import Model
+    from ps3d_amd.params import PS3D_MIXED
+    if case.startswith("lid"):
+        make = lambda **kw: configs.jets(128, 128, 32, seed=5, **kw)
+        plans = (1, 1, 0)
+    else:
+        make = lambda **kw: configs.random_state(64, 64, 32, seed=8, lid=False, Avbi=0.0, Kvbi=0.0, **kw)
+        plans = (1, 1, 1)
+    prm, g64, gmx = _models(make)
+    gmx.fft_plan = plans
+    assert gmx.fft_plan == plans
+    _, fields = make()
+    g0 = Model(prm.replace(precision=PS3D_MIXED))
+    for n, a in fields.items():
+        g0.f[n][...] = a
+    g0.push()
+    assert g0.fft_plan == (0, 0, 0)
+    for m in (g64, gmx, g0):
+        m.run(100)
+        m.pull()
+    err = diagnose.compare(gmx.f, g64.f, prm)
+    print(f"plans {plans} {case} step 100: " + " ".join(f"{k}={v:.2e}" for k, v in err.items()), flush=True)
+    for k, v in err.items():
+        assert v <= TOL, (k, v)
+    assert any(not np.array_equal(gmx.f[n], g0.f[n]) for n in ("u", "v", "w", "b", "p"))
+    for m in (g64, gmx, g0):
+        m.close()
+
+
+def test_fft_plan_rules_and_tuning(tmp_path):
+    """FP64 models keep plan 0 (the oracle's factorisation); plans are 0 or
+    1; the tuner of a mixed model picks a plan for each transform axis with
+    a choice (lengths with a factor 8) and reports it."""
+    from ps3d_amd import _lib, configs
+    from ps3d_amd.model import Model
+    from ps3d_amd.params import PS3D_MIXED
+    prm, fields = configs.jets(64, 64, 32, seed=6)
+    m64 = Model(prm)
+    with pytest.raises(_lib.Ps3dError, match="plan 0"):
+        m64.fft_plan = (1, 0, 0)
+    assert m64.fft_plan == (0, 0, 0)
+    m64.close()
+    m = Model(prm.replace(precision=PS3D_MIXED))
+    with pytest.raises(_lib.Ps3dError, match="0 or 1"):
+        m.fft_plan = (2, 0, 0)
+    for n, a in fields.items():
+        m.f[n][...] = a
+    m.push()
+    m.run(3)
+    m.tune(str(tmp_path / "tune.txt"), reps=3)
+    desc = m.tuning()
+    print(desc, flush=True)
+    lines = {ln.split()[0]: int(ln.split()[2]) for ln in desc.strip().splitlines()}
+    assert lines["fft_plan_x"] in (0, 1) and lines["fft_plan_y"] in (0, 1)
+    assert (lines["fft_plan_x"], lines["fft_plan_y"]) == m.fft_plan[:2]
+    cached = open(tmp_path / "tune.txt").read()
+    assert "fft_plan_x 64x64x32:lid:mixed" in cached and "fft_plan_y 64x64x32:lid:mixed" in cached
+    m.close()
