@@ -77,8 +77,8 @@ def algorithmic_bytes(name: str, prm, order: int = 3) -> float:
 
 def measured_traffic(kernel: str):
     """HBM bytes per launch of the kernel on this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/), calibrated on a plain
-    device copy; None where no profile exists."""
+    rocprofv3 runs of the TCC request counters by request size (profiles/;
+    exact on the library's streaming copy); None where no profile exists."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_fp64_256x256x64_jets.json")
     try:
         with open(path) as f:
@@ -381,12 +381,12 @@ def reference_cpu_leg(prm, procs: int, budget_s: float = 4.0, small: bool = Fals
         return None
     import resource
     from concurrent.futures import ProcessPoolExecutor, as_completed
-    # integrate.f90 keeps grid-sized automatic arrays, which amdflang places on
-    # the stack (~250 MB at 256x256x64): the workers are started with the
-    # stack limit raised to the hard limit (a process's main stack is sized
-    # from the limit it starts with)
+    # the friction routines keep grid-sized automatic arrays (mixing.f90:11-12,
+    # ~75 MB at 256x256x64), which amdflang places on the stack: the workers
+    # are started with the stack limit raised to the hard limit (a process's
+    # main stack is sized from the limit it starts with)
     soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
-    need = 1 << 30
+    need = 256 << 20
     if hard != resource.RLIM_INFINITY and hard < need and not small:
         log(f"CPU baseline: stack hard limit {hard} bytes is below what the reference needs; skipped")
         return None
