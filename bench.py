@@ -239,7 +239,8 @@ def configs_leg(steps: int = 50, warm: int = 5, cases=("eddy", "lee_wave", "jets
     """Throughput of the other configurations, each form from the same
     initial state after warm-up steps and tuning: the 64x64x32 baroclinic
     eddy, the 512x128x128 lee wave (background flow and body force on the
-    device) and the 1024x1024x128 jets and eddies."""
+    device), the 1024x1024x128 jets and eddies and the 256x256x64 jets at
+    0.5 m/s with noise on every field."""
     from ps3d_amd import configs
     from ps3d_amd.model import Model
     from ps3d_amd.params import PS3D_FP64, PS3D_MIXED
@@ -248,7 +249,9 @@ def configs_leg(steps: int = 50, warm: int = 5, cases=("eddy", "lee_wave", "jets
              "lee_wave": ("lee wave 512x128x128, rigid lid, background flow + body force",
                           lambda: configs.lee_wave(512, 128, 128, seed=3)),
              "jets_eddies": ("jets and eddies 1024x1024x128, rigid lid",
-                             lambda: configs.jets_eddies(1024, 1024, 128, seed=5) + (None,))}
+                             lambda: configs.jets_eddies(1024, 1024, 128, seed=5) + (None,)),
+             "jets_c2": ("jets 256x256x64 at 0.5 m/s over Ly/10, noise on u, v, w, b, N0 raised for stability",
+                         lambda: configs.jets_c2(256, 256, 64) + (None,))}
     out = {}
     for key in cases:
         name, make = table[key]
@@ -434,9 +437,9 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--ens-steps", type=int, default=100, help="timed steps of the ensemble leg")
-    ap.add_argument("--configs", nargs="?", const="eddy,lee_wave,jets_eddies", default=None,
-                    help="also time the eddy, lee-wave and 1024x1024x128 configurations (both forms); "
-                         "optionally a comma list of them")
+    ap.add_argument("--configs", nargs="?", const="eddy,lee_wave,jets_eddies,jets_c2", default=None,
+                    help="also time the eddy, lee-wave, 1024x1024x128 and 0.5 m/s jets configurations (both "
+                         "forms); optionally a comma list of them")
     ap.add_argument("--mixed-check", default="1,100,1000",
                     help="steps at which the mixed leg reports its error against FP64 (comma list)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",

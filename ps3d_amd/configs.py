@@ -132,27 +132,29 @@ def eddy(nx=64, ny=64, nz=32, seed=1, radius_frac=1.0 / 6.0, noise=1e-3, cfl=0.3
     return prm, _finish(prm, {"u": u, "v": v, "w": np.zeros_like(u), "b": b})
 
 
-def jets(nx=256, ny=256, nz=64, seed=2, umax=0.1, noise=1e-3, friction=True, precision=PS3D_FP64):
+def jets(nx=256, ny=256, nz=64, seed=2, umax=0.1, noise=1e-3, friction=True, precision=PS3D_FP64,
+         width_frac=0.02, noise_fields=("u", "v"), stable_N0=False):
     """Two zonal jets of opposite sign, geostrophically balanced, rigid lid.
 
     The set-up of for_config/jets_unscaled.f90:5-95 at any resolution:
     Lx = Ly = 500 km, Lz = 500 m with the lid, f0 = 1e-4, N0 = 50 f0,
     dt = 0.5/N0; u = umax (jets at y = Ly/4, 3Ly/4 of Gaussian width
-    0.02 Lx + 0.02 sin(10 pi x/Lx) sin(2 pi y/Ly)) cos(pi z/Lz); b from the
-    thermal-wind cumsum in y (:80-95).  Added: N(0, noise) on u and v, and
-    harmonic (Ah, Av, Kh, Kv) and horizontal biharmonic (Ahbi, Khbi)
-    friction sized to the grid.  (Stronger, wider jets -- 0.5 m/s over
-    Ly/10 -- or noise on b of 1e-3 make the thermal-wind stratification
-    statically unstable at N0 = 50 f0, d(b)/dz ~ 1.7e-4 > N0^2 = 2.5e-5,
-    and the run convects and blows up within ~50 steps.)
+    width_frac Lx + 0.02 sin(10 pi x/Lx) sin(2 pi y/Ly)) cos(pi z/Lz); b from
+    the thermal-wind cumsum in y (:80-95).  Added: N(0, noise) on the
+    noise_fields, and harmonic (Ah, Av, Kh, Kv) and horizontal biharmonic
+    (Ahbi, Khbi) friction sized to the grid.  (Stronger, wider jets -- 0.5
+    m/s over Ly/10 -- or noise on b of 1e-3 make the thermal-wind
+    stratification statically unstable at N0 = 50 f0, d(b)/dz ~ 1.7e-4 >
+    N0^2 = 2.5e-5, and the run convects and blows up within ~50 steps;
+    stable_N0 raises N0 until N0^2 is four times the largest -d(b)/dz of the
+    jets' buoyancy, with dt = 0.5/N0 following it.)
     """
     f0 = 1e-4
     N0 = 50 * f0
     Lx = Ly = 500e3
     Lz = 500.0
     dx, dy, dz = Lx / nx, Ly / ny, Lz / nz
-    dt = 0.5 / N0
-    width = Lx * 0.02
+    width = Lx * width_frac
     xi = (np.arange(1, nx + 1) - 1) * dx
     yj = (np.arange(1, ny + 1) - 1) * dy
     zk = (np.arange(1, nz + 1) - 1) * dz
@@ -170,9 +172,13 @@ def jets(nx=256, ny=256, nz=64, seed=2, umax=0.1, noise=1e-3, friction=True, pre
         else:
             b[:, :, k] = -dy * (up[:, :, k + 2] - up[:, :, k]) / (2 * dz) * f0
     b = np.cumsum(b, axis=1)
-    u = u + noise * normal(seed, "u", nx, ny, nz)
-    v = noise * normal(seed, "v", nx, ny, nz)
-    w = np.zeros_like(u)
+    if stable_N0:
+        bz_min = float(np.min(np.diff(b, axis=2))) / dz
+        N0 = max(N0, math.sqrt(max(0.0, -4.0 * bz_min)))
+    dt = 0.5 / N0
+    fields = {"u": u, "v": np.zeros_like(u), "w": np.zeros_like(u), "b": b}
+    for n in noise_fields:
+        fields[n] = fields[n] + noise * normal(seed, n, nx, ny, nz)
     kw = {}
     if friction:
         kw = dict(Ah=1e-3 * dx ** 2 / dt, Av=1e-2 * dz ** 2 / dt, Kh=1e-3 * dx ** 2 / dt,
@@ -180,7 +186,16 @@ def jets(nx=256, ny=256, nz=64, seed=2, umax=0.1, noise=1e-3, friction=True, pre
     prm = ModelParams(nx=nx, ny=ny, nz=nz, Lx=Lx, Ly=Ly, Lz=Lz, dx=dx, dy=dy, dz=dz, dt=dt,
                       f0=f0, N0=N0, enable_vertical_boundaries=True, enable_AB_3_order=True,
                       eps_ab=0.0, precision=precision, **kw)
-    return prm, _finish(prm, {"u": u, "v": v, "w": w, "b": b})
+    return prm, _finish(prm, fields)
+
+
+def jets_c2(nx=256, ny=256, nz=64, seed=2, precision=PS3D_FP64):
+    """The 256x256x64 jets state as the benchmark configuration describes
+    it: jets of 0.5 m/s and width Ly/10, N(0, 1e-3) on u, v, w and b; N0
+    raised for static stability (see jets: 269 f0, dt = 18.6 s at
+    256x256x64)."""
+    return jets(nx, ny, nz, seed=seed, umax=0.5, noise=1e-3, precision=precision, width_frac=0.1,
+                noise_fields=("u", "v", "w", "b"), stable_N0=True)
 
 
 def jets_eddies(nx=1024, ny=1024, nz=128, seed=5, n_eddies=8, eddy_amp=0.02, radius_frac=1.0 / 20.0,

@@ -234,3 +234,20 @@ def test_fft_plan_rules_and_tuning(tmp_path):
     cached = open(tmp_path / "tune.txt").read()
     assert "fft_plan_x 64x64x32:lid:mixed" in cached and "fft_plan_y 64x64x32:lid:mixed" in cached
     m.close()
+
+
+def test_mixed_c2_state_100_steps():
+    """The mixed form on the benchmark configuration's own jets state
+    (configs.jets_c2, 256x256x64) against the FP64 path after 100 steps."""
+    from ps3d_amd import configs, diagnose
+    prm, g64, gmx = _models(lambda **kw: configs.jets_c2(256, 256, 64, **kw))
+    g64.run(100)
+    gmx.run(100)
+    g64.pull()
+    gmx.pull()
+    err = diagnose.compare(gmx.f, g64.f, prm)
+    print("mixed C2 state step 100: " + " ".join(f"{k}={v:.2e}" for k, v in err.items()), flush=True)
+    for k, v in err.items():
+        assert v <= TOL, (k, v)
+    g64.close()
+    gmx.close()

@@ -131,6 +131,35 @@ def test_bench_workload_ten_steps_bitwise(jets256):
     orc.close()
 
 
+def test_c2_state_steps_bitwise():
+    """The benchmark configuration's own jets state (0.5 m/s over Ly/10,
+    N(0, 1e-3) on u, v, w, b; N0 raised for static stability,
+    configs.jets_c2) at 256x256x64: 5 steps bit for bit against the oracle;
+    then 300 more on the GPU stay bounded (the state is stable)."""
+    from ps3d_amd import configs
+    from ps3d_amd.model import Model
+    prm, fields = configs.jets_c2(256, 256, 64)
+    gpu = Model(prm)
+    orc = oracle.OracleModel(prm)
+    for n, a in fields.items():
+        gpu.f[n][...] = a
+        orc.f[n][...] = a
+    gpu.push()
+    print("256x256x64 C2 jets: 5 steps against the oracle", flush=True)
+    gpu.run(5)
+    orc.run(5)
+    gpu.pull()
+    for n in ("u", "v", "w", "b", "p"):
+        assert np.array_equal(gpu.f[n], orc.f[n]), n
+    orc.close()
+    gpu.run(300)
+    gpu.pull(["u", "v", "w"])
+    umax = max(float(np.abs(gpu.f[n]).max()) for n in ("u", "v", "w"))
+    print(f"  after 305 steps: max |u, v, w| = {umax:.3f} m/s", flush=True)
+    assert np.isfinite(umax) and umax < 1.0
+    gpu.close()
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_ensemble_members_match_oracle(mixed):
     from ps3d_amd import configs
