@@ -251,8 +251,13 @@ template <class T>
 __device__ __forceinline__ void tendency_body(const StepConsts &c, const TendArgs<T> &a, int bid)
 {
     __shared__ T sp[4][3][TPY][TPX];
-    const int ch = bid % a.nchunk;
-    const int tile = bid / a.nchunk;
+    // tiles fastest: the work-groups resident on an XCD at once hold many
+    // rows of tiles of the same chunk, which march through the same levels
+    // together, so the apron rows a tile shares with its y-neighbours are
+    // read from HBM once and then found in L2
+    const int ntile = a.tiles_x * a.tiles_y;
+    const int tile = bid % ntile;
+    const int ch = bid / ntile;
     const int i0 = (tile % a.tiles_x) * TTX + 1;
     const int j0 = (tile / a.tiles_x) * TTY + 1;
     const int kb = 1 + ch * a.kc;
