@@ -440,6 +440,7 @@ def main() -> int:
     ap.add_argument("--configs", nargs="?", const="eddy,lee_wave,jets_eddies,jets_c2", default=None,
                     help="also time the eddy, lee-wave, 1024x1024x128 and 0.5 m/s jets configurations (both "
                          "forms); optionally a comma list of them")
+    ap.add_argument("--config-steps", type=int, default=50, help="timed steps of each --configs case")
     ap.add_argument("--mixed-check", default="1,100,1000",
                     help="steps at which the mixed leg reports its error against FP64 (comma list)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -539,7 +540,8 @@ def main() -> int:
             out["ensemble_mixed"] = ensemble_leg(args.ens_steps, precision=1)
             log(f"ensemble mixed: {out['ensemble_mixed']['speedup_vs_sequential']:.2f}x sequential")
     if args.configs:
-        out["configs"] = configs_leg(cases=tuple(c.strip() for c in args.configs.split(",") if c.strip()))
+        out["configs"] = configs_leg(steps=args.config_steps,
+                                     cases=tuple(c.strip() for c in args.configs.split(",") if c.strip()))
     if not args.no_cpu:
         log("CPU baseline: the C port on the host cores")
         port = cpu_baseline(prm, fields, threads=args.cpu_threads)

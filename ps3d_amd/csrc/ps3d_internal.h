@@ -225,12 +225,13 @@ struct Model {
     bool fuse_halo = false;
     // launch shapes (results do not depend on them; set by ps3d_tune):
     // levels per work-group of the k-marching kernels (0: heuristic) and
-    // columns per 64-lane wave of the Thomas solve (64, 32 or 16; 0: the
-    // widest whose on-the-fly form fits its LDS)
+    // the form of the FP64 Thomas solve (launch_thomas: 1 whole columns in
+    // registers; 16, 32, 64 columns per wave on the fly; -16, -32, -64 the
+    // factor tables; 0: the heuristic)
     int tend_kc = 0, fric_kc = 0, thomas_cpw = 0;
     int tfric_kc = 0;   // the same for the kernel with the tendency fused in
-    // FP64 Thomas solve with whole columns in registers where nz is one of
-    // the compiled-in column lengths (k_thomas_reg), else k_thomas_fly
+    // the heuristic's first choice: whole columns in registers where nz is
+    // one of the compiled-in column lengths (k_thomas_reg)
     bool thomas_reg = true;
     // radix plans of the x, y and z transforms (0: radix 4 first, the FP64
     // plan; 1: radix 8 first); FP64 transforms always use plan 0, FP32 ones

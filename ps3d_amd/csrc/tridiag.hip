@@ -531,10 +531,12 @@ void launch_thomas(Model &m, double2 *data)
 {
     const int64_t ncols = (int64_t)m.g.nkx * m.g.ny;
     const double fxa = 1.0 / (m.prm.dz * m.prm.dz) / m.prm.dsqr;
-    if (m.thomas_reg && thomas_reg_ok(m.g.nz, (uint64_t)m.g.S * sizeof(double2))) {
+    const int form = m.thomas_cpw;
+    const bool reg = thomas_reg_ok(m.g.nz, (uint64_t)m.g.S * sizeof(double2)) &&
+                     (form == 1 || (form == 0 && m.thomas_reg));
+    if (reg) {
         const dim3 grid((unsigned)((ncols + 63) / 64));
         const double dz2 = m.prm.dz * m.prm.dz, dsqr = m.prm.dsqr;
-        bool done = true;
         ProfScope ps(m, "thomas");
         switch (m.g.nz) {
 #define CASE(n)                                                                                             \
@@ -544,25 +546,35 @@ void launch_thomas(Model &m, double2 *data)
         break;
             CASE(16) CASE(32) CASE(48) CASE(64)
 #undef CASE
-        default: done = false;
         }
-        if (done) {
-            PS3D_HIP_CHECK(hipGetLastError());
-            return;
-        }
+        PS3D_HIP_CHECK(hipGetLastError());
+        return;
     }
-    int cpw = m.thomas_cpw;
-    if (cpw != 16 && cpw != 32 && cpw != 64) {
-        // the widest group whose on-the-fly form fits 64 KiB (several
-        // work-groups per CU), else 16 columns in up to 160 KiB
+    // form: 16, 32, 64 columns per wave on the fly (c' and d' in LDS);
+    // -16, -32, -64 the factor tables (d' in LDS where it fits 64 KiB, else
+    // in FS); 0 (untuned): on the fly with 64 or 32 columns where that
+    // keeps several work-groups on a CU (LDS <= 64 KiB), else the tables
+    // with 64 columns (measured faster at nz = 128 than 16 columns on the
+    // fly: a quarter of the lanes would idle)
+    int cpw;
+    bool table;
+    if (form == 16 || form == 32 || form == 64) {
+        cpw = form;
+        table = thomas_fly_lds(cpw, m.g.nz) > kLdsMax;
+    } else if (form == -16 || form == -32 || form == -64) {
+        cpw = -form;
+        table = true;
+    } else {
         cpw = 64;
-        while (cpw > 16 && thomas_fly_lds(cpw, m.g.nz) > 64 * 1024) cpw /= 2;
+        while (cpw > 32 && thomas_fly_lds(cpw, m.g.nz) > 64 * 1024) cpw /= 2;
+        table = thomas_fly_lds(cpw, m.g.nz) > 64 * 1024;
+        if (table) cpw = 64;
     }
     const size_t lds = sizeof(double2) * (size_t)cpw * m.g.nz;
     const size_t lds_fly = thomas_fly_lds(cpw, m.g.nz);
     const dim3 grid((unsigned)((ncols + cpw - 1) / cpw));
     ProfScope ps(m, "thomas");
-    if (lds_fly <= kLdsMax) {
+    if (!table) {
         allow_lds(k_thomas_fly<double2>, lds_fly);
         hipLaunchKernelGGL(k_thomas_fly<double2>, grid, dim3(64), lds_fly, m.stream, data, m.ksqr, ncols, m.g.nz,
                            fxa, m.prm.dz * m.prm.dz, m.prm.dsqr, cpw);
@@ -603,9 +615,12 @@ void launch_thomas_f(Model &m, float2 *data)
         PS3D_HIP_CHECK(hipGetLastError());
         return;
     }
+    // on the fly with 32 columns per wave where its LDS keeps several
+    // work-groups on a CU, else the factor tables with FP64 d' in a scratch
+    // array (faster than 16 columns on the fly for tall columns)
     constexpr int cpw = 32;
     const size_t lds_fly = thomas_fly_lds(cpw, m.g.nz);
-    if (lds_fly <= kLdsMax) {
+    if (lds_fly <= 64 * 1024) {
         allow_lds(k_thomas_fly<float2>, lds_fly);
         hipLaunchKernelGGL(k_thomas_fly<float2>, dim3((unsigned)((ncols + cpw - 1) / cpw)), dim3(64), lds_fly,
                            m.stream, data, m.ksqr, ncols, m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr, cpw);

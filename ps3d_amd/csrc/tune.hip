@@ -234,20 +234,20 @@ void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
             // columns per work-group whose LDS keeps several work-groups on
             // a CU (64 KiB), measured faster in the step than wider groups
             // that a back-to-back timing can prefer
-            // value 0: whole columns in registers (k_thomas_reg), where nz
-            // is one of its compiled-in lengths
+            // forms (launch_thomas): 1 whole columns in registers
+            // (k_thomas_reg), where nz is one of its compiled-in lengths;
+            // 16, 32, 64 columns per wave on the fly; -16, -32, -64 the
+            // factor tables
             std::vector<int> cpw;
-            if (thomas_reg_ok(m.g.nz, sizeof(double2) * (uint64_t)m.g.S)) cpw.push_back(0);
+            if (thomas_reg_ok(m.g.nz, sizeof(double2) * (uint64_t)m.g.S)) cpw.push_back(1);
             for (int c : {32, 16, 64})
                 if ((sizeof(double2) + sizeof(double)) * (size_t)c * m.g.nz <= 64 * 1024) cpw.push_back(c);
-            if (cpw.empty()) cpw.push_back(16);
+            for (int c : {-64, -32, -16}) cpw.push_back(c);
             int form = 0;
             pick("thomas", cpw, form, 2 * reps, flush, [&] {
-                m.thomas_reg = form == 0;
                 m.thomas_cpw = form;
                 launch_thomas(m, scratch);
             });
-            m.thomas_reg = form == 0;
             m.thomas_cpw = form;
         } catch (...) {
             (void)hipFree(scratch);
@@ -259,7 +259,6 @@ void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
         PS3D_HIP_CHECK(hipFree(flush));
     } else if (!m.mixed && m.prm.enable_vertical_boundaries) {
         m.thomas_cpw = cache["thomas " + sh];
-        m.thomas_reg = m.thomas_cpw == 0;
     }
     tune_cache_write(cache_path, cache);
 }

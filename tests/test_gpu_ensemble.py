@@ -218,6 +218,7 @@ def test_forced_members_match_single_forced_models(use_graph):
         ens.set_background_flow(i, ub, bb)
         ens.set_body_force(i, force)
     ens.run(12, use_graph=use_graph)
+    singles = []
     for i, (prm, fields) in enumerate(members):
         m = Model(prm)
         for k, a in fields.items():
@@ -232,5 +233,21 @@ def test_forced_members_match_single_forced_models(use_graph):
         for n in NAMES:
             got = ens.get(i, n)
             assert np.array_equal(got, m.f[n]), (i, n, float(np.abs(got - m.f[n]).max()))
+        singles.append(m)
+    # forcing switched off for member 0 (its background flow and force) and
+    # member 1's body force: 4 more steps
+    ens.set_background_flow(0, None)
+    ens.set_body_force(0, None)
+    ens.set_body_force(1, None)
+    singles[0].set_background_flow(None)
+    singles[0].set_body_force(None)
+    singles[1].set_body_force(None)
+    ens.run(4, use_graph=use_graph)
+    for i, m in enumerate(singles):
+        m.run(4)
+        m.pull()
+        for n in NAMES:
+            got = ens.get(i, n)
+            assert np.array_equal(got, m.f[n]), ("after switching off", i, n, float(np.abs(got - m.f[n]).max()))
         m.close()
     ens.close()

@@ -255,6 +255,40 @@ def test_tuned_shapes_give_the_same_bits(case, tmp_path):
     gpu.close()
 
 
+@pytest.mark.parametrize("nz", [48, 128])
+def test_every_thomas_form_gives_the_same_bits(nz, tmp_path):
+    """Each form of the FP64 Thomas solve, forced through a tuning cache --
+    whole columns in registers (nz of 16, 32, 48, 64), 16/32/64 columns per
+    wave on the fly, the factor tables with 16/32/64 columns (d' in LDS or,
+    where that exceeds 64 KiB, in FS) -- steps bit for bit like the oracle."""
+    import oracle
+    from ps3d_amd import configs
+    from ps3d_amd.model import Model
+    prm, fields = configs.random_state(24, 16, nz, seed=41, lid=True, Avbi=0.0, Kvbi=0.0)
+    orc = oracle.OracleModel(prm)
+    for n, a in fields.items():
+        orc.f[n][...] = a
+    orc.run(3)
+    shape = f"24x16x{nz}:lid:fp64"
+    forms = ([1] if nz in (16, 32, 48, 64) else []) + [16, 32, 64, -16, -32, -64]
+    for form in forms:
+        print(f"nz {nz}: Thomas form {form}", flush=True)
+        cache = tmp_path / f"thomas_{nz}_{form}.txt"
+        cache.write_text(f"friction_ab {shape} 8\ntendency {shape} 8\nthomas {shape} {form}\n")
+        m = Model(prm)
+        for n, a in fields.items():
+            m.f[n][...] = a
+        m.push()
+        m.tune(str(cache), reps=1)
+        assert f"thomas {shape} {form}" in m.tuning()
+        m.run(3)
+        m.pull()
+        for n in ("u", "v", "w", "b", "p"):
+            assert np.array_equal(m.f[n], orc.f[n]), (form, n)
+        m.close()
+    orc.close()
+
+
 FUSED = ["lid_fused", "periodic_fused", "periodic_pow2", "thin_lid_fused"]
 
 
