@@ -14,6 +14,7 @@
 
 #include <unistd.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <fstream>
 #include <map>
@@ -158,14 +159,23 @@ void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
             field = hit->second;
             return;
         }
+        // three rounds over the candidates, interleaved, and the median of
+        // each candidate's rounds: the device's clocks drift over a tuning
+        // run by more than the few per cent that separate close candidates
+        constexpr int kRounds = 3;
+        std::vector<std::vector<float>> t(cand.size());
+        for (int round = 0; round < kRounds; ++round)
+            for (size_t q = 0; q < cand.size(); ++q) {
+                field = cand[q];
+                t[q].push_back(flush ? time_it_cold(m, nrep, flush, kFlushBytes, launch) : time_it(m, nrep, launch));
+            }
         int best = cand[0];
         float best_ms = 1e30f;
-        for (int v : cand) {
-            field = v;
-            const float ms = flush ? time_it_cold(m, nrep, flush, kFlushBytes, launch) : time_it(m, nrep, launch);
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = v;
+        for (size_t q = 0; q < cand.size(); ++q) {
+            std::sort(t[q].begin(), t[q].end());
+            if (t[q][kRounds / 2] < best_ms) {
+                best_ms = t[q][kRounds / 2];
+                best = cand[q];
             }
         }
         field = best;
