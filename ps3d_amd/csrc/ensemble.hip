@@ -409,23 +409,23 @@ void tune_fft_plans(ps3d_ensemble &E, std::map<std::string, int> &cache, int rep
                 E.tuned[key] = hit->second;
                 continue;
             }
-            int best = 0;
-            float best_ms = 1e30f;
-            for (int plan : {0, 1}) {
-                set_plan(ax, L, plan);
-                upload_table(E);
-                run(ax);   // warm-up
-                PS3D_HIP_CHECK(hipEventRecord(t0, E.stream));
-                for (int r = 0; r < reps; ++r) run(ax);
-                PS3D_HIP_CHECK(hipEventRecord(t1, E.stream));
-                PS3D_HIP_CHECK(hipEventSynchronize(t1));
-                float ms = 0.f;
-                PS3D_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best = plan;
+            constexpr int kRounds = 3;   // interleaved rounds, the median
+            std::vector<float> tm[2];
+            for (int round = 0; round < kRounds; ++round)
+                for (int plan : {0, 1}) {
+                    set_plan(ax, L, plan);
+                    upload_table(E);
+                    run(ax);   // warm-up
+                    PS3D_HIP_CHECK(hipEventRecord(t0, E.stream));
+                    for (int r = 0; r < reps; ++r) run(ax);
+                    PS3D_HIP_CHECK(hipEventRecord(t1, E.stream));
+                    PS3D_HIP_CHECK(hipEventSynchronize(t1));
+                    float ms = 0.f;
+                    PS3D_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+                    tm[plan].push_back(ms);
                 }
-            }
+            for (auto &v : tm) std::sort(v.begin(), v.end());
+            const int best = tm[1][kRounds / 2] < tm[0][kRounds / 2] ? 1 : 0;
             set_plan(ax, L, best);
             upload_table(E);
             E.tuned[key] = best;
@@ -713,22 +713,31 @@ int ps3d_ensemble_tune(ps3d_ensemble *e, const char *cache_path, int reps)
                     std::vector<int> cand;
                     for (int kc = 1; kc < nz; kc *= 2) cand.push_back(kc);
                     cand.push_back(nz);
+                    // three interleaved rounds, the median of each
+                    // candidate's (as the single-model tuner)
+                    constexpr int kRounds = 3;
+                    std::vector<std::vector<float>> tm(cand.size());
+                    for (int round = 0; round < kRounds; ++round)
+                        for (size_t q = 0; q < cand.size(); ++q) {
+                            apply_tiles(*e, stage, sh, cand[q]);
+                            build_maps(*e);
+                            upload_table(*e);
+                            launch_stage(*e, stage);   // warm-up
+                            PS3D_HIP_CHECK(hipEventRecord(t0, e->stream));
+                            for (int r = 0; r < reps; ++r) launch_stage(*e, stage);
+                            PS3D_HIP_CHECK(hipEventRecord(t1, e->stream));
+                            PS3D_HIP_CHECK(hipEventSynchronize(t1));
+                            float ms = 0.f;
+                            PS3D_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+                            tm[q].push_back(ms);
+                        }
                     int best = cand[0];
                     float best_ms = 1e30f;
-                    for (int kc : cand) {
-                        apply_tiles(*e, stage, sh, kc);
-                        build_maps(*e);
-                        upload_table(*e);
-                        launch_stage(*e, stage);   // warm-up
-                        PS3D_HIP_CHECK(hipEventRecord(t0, e->stream));
-                        for (int r = 0; r < reps; ++r) launch_stage(*e, stage);
-                        PS3D_HIP_CHECK(hipEventRecord(t1, e->stream));
-                        PS3D_HIP_CHECK(hipEventSynchronize(t1));
-                        float ms = 0.f;
-                        PS3D_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
-                        if (ms < best_ms) {
-                            best_ms = ms;
-                            best = kc;
+                    for (size_t q = 0; q < cand.size(); ++q) {
+                        std::sort(tm[q].begin(), tm[q].end());
+                        if (tm[q][kRounds / 2] < best_ms) {
+                            best_ms = tm[q][kRounds / 2];
+                            best = cand[q];
                         }
                     }
                     apply_tiles(*e, stage, sh, best);
