@@ -384,8 +384,10 @@ int ps3d_ensemble_get_field(ps3d_ensemble *e, int member, int field, double *hos
    post-friction state (integrate.f90:82).  Forced members run the friction
    chain into a third set of buffers, then a halo copy and one forcing +
    Adams-Bashforth stage (each one launch over the forced members), and stay
-   bit-identical to a single model with the same forcing.  PS3D_FP64
-   ensembles. */
+   bit-identical to a single model of their form with the same forcing (in
+   the mixed form the third buffers hold (hi, lo) pairs, and the cold start
+   forms the forcing on the pre-friction state, as a single mixed model
+   does). */
 int ps3d_ensemble_set_background_flow(ps3d_ensemble *e, int member, const double *u_back, const double *b_back);
 int ps3d_ensemble_set_body_force(ps3d_ensemble *e, int member, const double *const f[4]);
 /* Radix plans of a member's FP32 transforms (as ps3d_set_fft_plan; PS3D_MIXED

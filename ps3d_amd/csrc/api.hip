@@ -130,8 +130,9 @@ void get_field_mixed(Model &m, int field, double *host, int64_t l)
     }
     if (field > PS3D_B) throw Error(PS3D_ERR_STATE, kMixedFields);
     launch_convert(m, 1, l, nullptr, m.stage, m.fh[field], nullptr, m.fl[field], nullptr);
-    // after a step the lo parts' halos are stale (the stencils read the hi
-    // parts only): rebuild the halo of the joined field
+    // after a step, the joined field's halo is rebuilt from its interior as
+    // border_exchg_3D would (the same values as the halo images of the hi and
+    // lo parts that the step's kernels write, summed)
     if (m.lo_stale[field]) launch_border_exchg(m, m.stage, 0);
     PS3D_HIP_CHECK(hipMemcpyAsync(host, m.stage, sizeof(double) * l, hipMemcpyDeviceToHost, m.stream));
 }

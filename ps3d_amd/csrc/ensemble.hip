@@ -253,6 +253,9 @@ void enqueue_cold_mixed(ps3d_ensemble &E)
 {
     hipStream_t s = E.stream;
     ens_tendency_f(s, E.dev_tab, E.maps[S_TEND], E.dev_state, 1);
+    // forced members: the forcing on the pre-friction state (a single
+    // model's provisional step)
+    ens_background_f(s, E.dev_tab, E.maps[S_BGAB], E.dev_state);
     ens_friction_ab_f(s, E.dev_tab, E.maps[S_FRIC], E.dev_state, 1);
     enqueue_solve_f(E, 0);
     ens_cold_pint(s, E.dev_tab, E.maps[S_PROJ]);
@@ -266,6 +269,10 @@ void enqueue_step_mixed(ps3d_ensemble &E, bool refine)
     hipStream_t s = E.stream;
     ens_tendency_f(s, E.dev_tab, E.maps[S_TEND], E.dev_state, 0);
     ens_friction_ab_f(s, E.dev_tab, E.maps[S_FRIC], E.dev_state, 0);   // writes the halo images too
+    // forced members (their friction went to the third (hi, lo) buffers):
+    // the pre-friction hi halos, then forcing + Adams-Bashforth
+    ens_halo_copy(s, E.dev_tab, E.maps[S_HCOPY], E.dev_state, true);
+    ens_background_ab_f(s, E.dev_tab, E.maps[S_BGAB], E.dev_state);
     enqueue_solve_f(E, 1);
     ens_project_f(s, E.dev_tab, E.maps[S_PROJ], E.dev_state, 1);
     if (refine) {
@@ -288,7 +295,7 @@ void enqueue_step(ps3d_ensemble &E)
     ens_friction_ab(s, E.dev_tab, E.maps[S_FRIC], E.dev_state);   // writes the halo images too
     // forced members (their friction went to the third buffers): the
     // pre-friction halos, then set_forcing (integrate.f90:82) and AB
-    ens_halo_copy(s, E.dev_tab, E.maps[S_HCOPY], E.dev_state);
+    ens_halo_copy(s, E.dev_tab, E.maps[S_HCOPY], E.dev_state, false);
     ens_background_ab(s, E.dev_tab, E.maps[S_BGAB], E.dev_state);
     ens_r2c(s, E.dev_tab, E.maps[S_R2C], E.dev_state, E.lds_x);
     ens_cols_y(s, E.dev_tab, E.maps[S_COLS], -1, E.ny_max, E.lds_y);
@@ -353,12 +360,20 @@ void refresh_forcing(ps3d_ensemble &E, int i)
         e.bf[q] = m.bf_on[q] ? m.bf[q] : nullptr;
         any = any || e.bf[q];
     }
-    if (any && !e.tmp[0]) {
+    if (any && !E.mixed && !e.tmp[0]) {
         for (int q = 0; q < 4; ++q) {
             void *p = nullptr;
             PS3D_HIP_CHECK(hipMalloc(&p, sizeof(double) * (size_t)m.g.H));
             E.tmp_alloc.push_back(p);
             e.tmp[q] = static_cast<double *>(p);
+        }
+    }
+    if (any && E.mixed && !e.tmph[0]) {   // the (hi, lo) pair of the mixed form
+        for (int q = 0; q < 8; ++q) {
+            void *p = nullptr;
+            PS3D_HIP_CHECK(hipMalloc(&p, sizeof(float) * (size_t)m.g.H));
+            E.tmp_alloc.push_back(p);
+            (q < 4 ? e.tmph[q] : e.tmpl[q - 4]) = static_cast<float *>(p);
         }
     }
     e.forced = any ? 1 : 0;
@@ -443,7 +458,6 @@ int ps3d_ensemble_set_background_flow(ps3d_ensemble *e, int member, const double
 {
     return guard([&] {
         check_member(e, member);
-        PS3D_REQUIRE(!e->mixed, "ensemble forcing: PS3D_FP64 members only");
         PS3D_HIP_CHECK(hipStreamSynchronize(e->stream));
         const int rc = ps3d_set_background_flow(e->mem[member], u_back, b_back);
         if (rc != PS3D_OK) throw Error(rc, error_slot());
@@ -455,7 +469,6 @@ int ps3d_ensemble_set_body_force(ps3d_ensemble *e, int member, const double *con
 {
     return guard([&] {
         check_member(e, member);
-        PS3D_REQUIRE(!e->mixed, "ensemble forcing: PS3D_FP64 members only");
         PS3D_HIP_CHECK(hipStreamSynchronize(e->stream));
         const int rc = ps3d_set_body_force(e->mem[member], f);
         if (rc != PS3D_OK) throw Error(rc, error_slot());
@@ -654,8 +667,9 @@ int ps3d_ensemble_run(ps3d_ensemble *e, int nsteps, int32_t *st, int use_graph)
             e->state.dp_cur = (e->state.dp_cur + 1) % 3;
         }
         if (e->mixed && nsteps > 0) {
-            // as after a single model's step: the lo parts' halos are stale
-            // and a pressure set by the host is superseded
+            // as after a single model's step: the joined fields' halos are
+            // rebuilt on the way out and a pressure set by the host is
+            // superseded
             for (ps3d_model *m : e->mem) {
                 for (int f = 0; f < 4; ++f) m->lo_stale[f] = true;
                 m->p_host.clear();

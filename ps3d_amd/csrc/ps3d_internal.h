@@ -257,7 +257,7 @@ struct Model {
     double2 *dp64 = nullptr;          // forward-sweep values of the Thomas solve
     double *stage = nullptr;          // 3H doubles for host conversions
     float2 *twf_r2c = nullptr;
-    bool lo_stale[4] = {};            // lo halo not exchanged since the last step
+    bool lo_stale[4] = {};            // a step since the joined field's halo was last rebuilt
     std::vector<double> p_host;       // p as set by the host, returned until the next step
 
     // profiling of one kernel by name
@@ -431,11 +431,13 @@ struct EnsMember {
     float2 *FSf;
     const float2 *twxf, *twyf, *twzf, *wkf;
     int fft_plan[3];   // radix plans of the FP32 x, y, z transforms (0, 1)
-    // forcing (FP64 members): background flow u_back, b_back and the body
-    // force (null: none); forced members run friction into the third
-    // buffers tmp, then the halo copy and the forcing + AB stages
+    // forcing: background flow u_back, b_back and the body force (null:
+    // none); forced members run friction into the third buffers (FP64: tmp;
+    // mixed: the (hi, lo) pair tmph, tmpl), then the halo copy and the
+    // forcing + AB stages
     int forced;
     double *tmp[4];
+    float *tmph[4], *tmpl[4];
     const double *ub, *bb, *bf[4];
 };
 
@@ -464,8 +466,13 @@ void ens_friction_ab(hipStream_t s, const EnsMember *mem, const EnsMap &map, con
 void ens_project(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st);
 // forced FP64 members: pre-friction halos into the third buffers (4 fields x
 // halo points per member), then forcing + AB (interior points per member)
-void ens_halo_copy(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st);
+void ens_halo_copy(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, bool mixed);
 void ens_background_ab(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st);
+// mixed form: the forcing on the pre-friction state into slot tau (the cold
+// start's provisional step), and forcing + Adams-Bashforth from the third
+// (hi, lo) buffers into the state's next buffers (the steady step)
+void ens_background_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st);
+void ens_background_ab_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st);
 void ens_advance(hipStream_t s, EnsState *st);
 void ens_r2c(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, size_t lds);
 // nmax: the longest column among the members (selects the compiled variant)

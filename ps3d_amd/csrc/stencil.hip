@@ -635,10 +635,13 @@ __global__ __launch_bounds__(BX * BY) void k_ab_mixed(StepConsts c, AbMixedArgs 
         gq = s * ((qn - qo) * a.grd[f]);
     }
     const double x = ((double)a.ph[f][o] + (double)a.pl[f][o]) + (ab - gq);
-    const float hi = (float)x;
+    const float hi = (float)x, lo = (float)(x - (double)hi);
     a.oh[f][o] = hi;
-    a.ol[f][o] = (float)(x - (double)hi);
-    if (a.halo) write_images<float>(c, a.oh[f], i, j, k, hi, 0);
+    a.ol[f][o] = lo;
+    if (a.halo) {
+        write_images<float>(c, a.oh[f], i, j, k, hi, 0);
+        write_images<float>(c, a.ol[f], i, j, k, lo, 0);
+    }
 }
 
 __global__ __launch_bounds__(BX * BY) void k_ab(StepConsts c, AbArgs a)
@@ -1031,10 +1034,15 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
                     gq = F.scale * ((qn - qo) * F.grd);
                 }
                 const double x = base + ((zt ? 0.0 : (double)hinc[py][px]) + (double)binc + ab - gq);
-                const float hi = (float)x;
+                const float hi = (float)x, lo = (float)(x - (double)hi);
                 F.out[o] = hi;
-                F.out_lo[o] = (float)(x - (double)hi);
-                if (a.halo) write_images<float>(c, F.out, i, j, k, hi, 0);
+                F.out_lo[o] = lo;
+                // the images of both parts: the mixed tendency reads lo at
+                // the neighbours of a point, halos included
+                if (a.halo) {
+                    write_images<float>(c, F.out, i, j, k, hi, 0);
+                    write_images<float>(c, F.out_lo, i, j, k, lo, 0);
+                }
             }
         }
         // no barrier here: the next level's first barrier orders this level's
@@ -1207,13 +1215,17 @@ __device__ __forceinline__ void project_f_point(const StepConsts &c, float *__re
     const double pe = dp[idx(c, i == c.nx ? 1 : i + 1, j, k)];
     const double pn = dp[idx(c, i, j == c.ny ? 1 : j + 1, k)];
     const double pt = dp[idx(c, i, j, k == c.nz ? (c.lid ? c.nz : 1) : k + 1)];
-    // with halo set, the halo images of the hi parts, of the increment and of
-    // pint are written too (Neumann in z under a lid for the last two)
+    // with halo set, the halo images of the hi and lo parts, of the
+    // increment and of pint are written too (Neumann in z under a lid for
+    // the last two)
     auto put = [&](float *h, float *l, double x) {
-        const float hi = (float)x;
+        const float hi = (float)x, lo = (float)(x - (double)hi);
         h[o] = hi;
-        l[o] = (float)(x - (double)hi);
-        if (halo) write_images<float>(c, h, i, j, k, hi, 0);
+        l[o] = lo;
+        if (halo) {
+            write_images<float>(c, h, i, j, k, hi, 0);
+            write_images<float>(c, l, i, j, k, lo, 0);
+        }
     };
     put(uh, ul, ((double)uh[o] + (double)ul[o]) - c.dt * ((pe - pc) * c.rdx));
     put(vh, vl, ((double)vh[o] + (double)vl[o]) - c.dt * ((pn - pc) * c.rdy));
@@ -1369,7 +1381,7 @@ __global__ __launch_bounds__(FTHREADS) void k_ens_fric(const EnsMember *__restri
 // forced members: the third buffers' halos take the pre-friction values (the
 // halo the reference's in-place friction leaves), k_halo_copy per member
 __global__ __launch_bounds__(256) void k_ens_halo_copy(const EnsMember *__restrict__ mem, EnsMap map,
-                                                        const EnsState *__restrict__ st)
+                                                        const EnsState *__restrict__ st, int mixed)
 {
     const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
     const int mi = ens_find(map, g);
@@ -1384,7 +1396,8 @@ __global__ __launch_bounds__(256) void k_ens_halo_copy(const EnsMember *__restri
     int i, j, k;
     halo_ijk(c, t, i, j, k);
     const int64_t o = idx(c, i, j, k);
-    e.tmp[f][o] = e.f[f][st->par][o];
+    if (mixed) e.tmph[f][o] = e.fh[f][st->par][o];   // the hi parts (halo_copy<float>)
+    else e.tmp[f][o] = e.f[f][st->par][o];
 }
 
 // forced members: background flow / body force and the Adams-Bashforth
@@ -1437,6 +1450,125 @@ __global__ __launch_bounds__(BX * BY) void k_ens_background_ab(const EnsMember *
     }
     a.halo = 1;
     background_ab_point(c, bg, a, i, j, k);
+}
+
+// mixed forced members: background flow / body force on the pre-friction hi
+// parts into the FP32 slot tau (k_background<float> of a single model: the
+// cold start's provisional step)
+__global__ __launch_bounds__(BX * BY) void k_ens_background_f(const EnsMember *__restrict__ mem, EnsMap map,
+                                                               const EnsState *__restrict__ st)
+{
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const StepConsts c = e.c;
+    const int lb = g - map.start[mi];
+    const int bx = (c.nx + BX - 1) / BX, by = (c.ny + BY - 1) / BY;
+    const int k = lb / (bx * by) + 1, r = lb % (bx * by);
+    const int i = (r % bx) * BX + (int)threadIdx.x + 1;
+    const int j = (r / bx) * BY + (int)threadIdx.y + 1;
+    if (i > c.nx || j > c.ny) return;
+    const int par = st->par;
+    const int64_t o0 = (int64_t)(st->tau - 1) * e.H;
+    BgArgs<float> bg{};
+    bg.u = e.fh[0][par]; bg.v = e.fh[1][par]; bg.w = e.fh[2][par]; bg.b = e.fh[3][par];
+    bg.du = e.tf[0] + o0; bg.dv = e.tf[1] + o0; bg.dw = e.tf[2] + o0; bg.db = e.tf[3] + o0;
+    bg.ub = e.ub;
+    bg.bb = e.bb;
+    for (int q = 0; q < 4; ++q) bg.f[q] = e.bf[q];
+    const int64_t o = idx(c, i, j, k);
+    double du, dv, dw, db;
+    bg_point<float>(c, bg, k, o, du, dv, dw, db);
+    bg.du[o] = (float)du;
+    bg.dv[o] = (float)dv;
+    bg.dw[o] = (float)dw;
+    bg.db[o] = (float)db;
+}
+
+// mixed forced members, the steady step: the forcing on the post-friction
+// hi parts of the third buffers into the FP32 slot tau (k_background<float>),
+// then the Adams-Bashforth update with the pressure-history term from the
+// third (hi, lo) buffers into the state's next buffers, halo images of both
+// parts included (k_ab_mixed) -- per point, with the slot's FP32 values,
+// so the same bits as the two kernels of a single model
+__global__ __launch_bounds__(BX * BY) void k_ens_background_ab_f(const EnsMember *__restrict__ mem, EnsMap map,
+                                                                  const EnsState *__restrict__ st)
+{
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const StepConsts c = e.c;
+    const int lb = g - map.start[mi];
+    const int bx = (c.nx + BX - 1) / BX, by = (c.ny + BY - 1) / BY;
+    const int k = lb / (bx * by) + 1, r = lb % (bx * by);
+    const int i = (r % bx) * BX + (int)threadIdx.x + 1;
+    const int j = (r / bx) * BY + (int)threadIdx.y + 1;
+    if (i > c.nx || j > c.ny) return;
+    const int par = st->par, itt = st->itt, cur = st->dp_cur;
+    const int64_t H = e.H;
+    const int64_t o0 = (int64_t)(st->tau - 1) * H;
+    BgArgs<float> bg{};
+    bg.u = e.tmph[0]; bg.v = e.tmph[1]; bg.w = e.tmph[2]; bg.b = e.tmph[3];
+    bg.du = e.tf[0] + o0; bg.dv = e.tf[1] + o0; bg.dw = e.tf[2] + o0; bg.db = e.tf[3] + o0;
+    bg.ub = e.ub;
+    bg.bb = e.bb;
+    for (int q = 0; q < 4; ++q) bg.f[q] = e.bf[q];
+    const int64_t o = idx(c, i, j, k);
+    double fd[4];
+    bg_point<float>(c, bg, k, o, fd[0], fd[1], fd[2], fd[3]);
+    float sl[4];
+    for (int q = 0; q < 4; ++q) sl[q] = (float)fd[q];
+    bg.du[o] = sl[0];
+    bg.dv[o] = sl[1];
+    bg.dw[o] = sl[2];
+    bg.db[o] = sl[3];
+    // the AB order and pressure-history coefficients of the step
+    // (integrate_mixed of a single model)
+    int order;
+    double ca = 0.0, cb = 0.0, cc = 0.0, qc1 = 0.0, qc2 = 0.0;
+    if (itt == 0) {
+        order = 1;
+    } else if (itt == 1 || !e.ab3) {
+        order = 2;
+        ca = 1.5 + e.eps_ab;
+        cb = 0.5 + e.eps_ab;
+        qc1 = -(0.5 + e.eps_ab);
+    } else {
+        order = 3;
+        ca = 23.0 / 12.0;
+        cb = -16.0 / 12.0;
+        cc = 5.0 / 12.0;
+        qc1 = -16.0 / 12.0 + 5.0 / 12.0;
+        qc2 = 5.0 / 12.0;
+    }
+    const float *d1 = qc1 != 0.0 ? e.dpr[cur] : nullptr;
+    const float *d2 = qc2 != 0.0 ? e.dpr[(cur + 2) % 3] : nullptr;
+    for (int f = 0; f < 4; ++f) {
+        const double s = (f == 2) ? c.dt / c.dsqr : c.dt;
+        const float *t1 = e.tf[f] + (int64_t)(st->taum1 - 1) * H;
+        const float *t2 = e.tf[f] + (int64_t)(st->taum2 - 1) * H;
+        double ab;
+        if (order == 1) ab = s * (double)sl[f];
+        else if (order == 2) ab = s * (ca * (double)sl[f] - cb * (double)t1[o]);
+        else ab = s * (ca * (double)sl[f] + cb * (double)t1[o] + cc * (double)t2[o]);
+        double gq = 0.0;
+        const int64_t gs = f == 0 ? 1 : (f == 1 ? c.X : (f == 2 ? c.XY : 0));
+        if (gs && d1) {
+            const double grd = f == 0 ? c.rdx : (f == 1 ? c.rdy : c.rdz);
+            double qo = qc1 * (double)d1[o], qn = qc1 * (double)d1[o + gs];
+            if (d2) {
+                qo += qc2 * (double)d2[o];
+                qn += qc2 * (double)d2[o + gs];
+            }
+            gq = s * ((qn - qo) * grd);
+        }
+        const double x = ((double)e.tmph[f][o] + (double)e.tmpl[f][o]) + (ab - gq);
+        const float hi = (float)x, lo = (float)(x - (double)hi);
+        e.fh[f][par ^ 1][o] = hi;
+        e.fl[f][par ^ 1][o] = lo;
+        write_images<float>(c, e.fh[f][par ^ 1], i, j, k, hi, 0);
+        write_images<float>(c, e.fl[f][par ^ 1], i, j, k, lo, 0);
+    }
 }
 
 __global__ __launch_bounds__(BX * BY) void k_ens_project(const EnsMember *__restrict__ mem, EnsMap map,
@@ -1544,6 +1676,17 @@ __global__ __launch_bounds__(FTHREADS) void k_ens_fric_f(const EnsMember *__rest
     F.gstride = f == 0 ? 1 : (f == 1 ? c.X : (f == 2 ? c.XY : 0));
     F.grd = f == 0 ? c.rdx : (f == 1 ? c.rdy : c.rdz);
     a.halo = 1;
+    if (e.forced && !cold) {
+        // the forcing sees the post-friction state: friction alone into the
+        // third (hi, lo) buffers (launch_friction_forcing_ab_mixed of a
+        // single model); the halo copy and forcing + AB stages follow
+        a.order = 0;
+        a.qc1 = a.qc2 = 0.0;
+        a.d1 = a.d2 = nullptr;
+        F.out = e.tmph[f];
+        F.out_lo = e.tmpl[f];
+        a.halo = 0;
+    }
     fric_body<float>(c, a, F, lb);
 }
 
@@ -2036,13 +2179,16 @@ void launch_friction_ab_mixed(Model &m, int itt, int tau, int taum1, int taum2, 
         std::swap(m.fh[f], m.ah[f]);
         std::swap(m.fl[f], m.al[f]);
     }
-    // the stencils and the divergence read the hi parts' halos (written by the
-    // kernel itself when fuse_halo); the lo parts' halos are rebuilt when a
-    // field is copied out
+    // the stencils and the divergence read the hi parts' halos, the mixed
+    // tendency the lo parts' too (written by the kernel itself when
+    // fuse_halo)
     if (m.fuse_halo) return;
     HaloArgsT<float> h{};
-    for (int f = 0; f < 4; ++f) h.f[f] = m.fh[f];
-    halo_launch(m, h, 4);
+    for (int f = 0; f < 4; ++f) {
+        h.f[f] = m.fh[f];
+        h.f[4 + f] = m.fl[f];
+    }
+    halo_launch(m, h, 8);
 }
 
 // the mixed step with background forcing: friction alone into the second
@@ -2094,8 +2240,11 @@ void launch_friction_forcing_ab_mixed(Model &m, int itt, int tau, int taum1, int
     PS3D_HIP_CHECK(hipGetLastError());
     if (!m.fuse_halo) {
         HaloArgsT<float> h{};
-        for (int f = 0; f < 4; ++f) h.f[f] = m.fh[f];
-        halo_launch(m, h, 4);
+        for (int f = 0; f < 4; ++f) {
+            h.f[f] = m.fh[f];
+            h.f[4 + f] = m.fl[f];
+        }
+        halo_launch(m, h, 8);
     }
 }
 
@@ -2194,8 +2343,9 @@ void launch_project_exchange_f(Model &m, float *dp)
     if (m.fuse_halo) return;
     HaloArgsT<float> h{};
     h.f[0] = m.fh[0]; h.f[1] = m.fh[1]; h.f[2] = m.fh[2]; h.f[3] = dp;
+    h.f[4] = m.fl[0]; h.f[5] = m.fl[1]; h.f[6] = m.fl[2];
     h.neumann[3] = m.prm.enable_vertical_boundaries;
-    halo_launch(m, h, 4);
+    halo_launch(m, h, 7);
     launch_border_exchg(m, m.pint, m.prm.enable_vertical_boundaries);
 }
 
@@ -2230,10 +2380,25 @@ void ens_friction_ab(hipStream_t s, const EnsMember *mem, const EnsMap &map, con
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
-void ens_halo_copy(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st)
+void ens_halo_copy(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, bool mixed)
 {
     if (!map.start[map.n]) return;
-    hipLaunchKernelGGL(k_ens_halo_copy, dim3((unsigned)map.start[map.n]), dim3(256), 0, s, mem, map, st);
+    hipLaunchKernelGGL(k_ens_halo_copy, dim3((unsigned)map.start[map.n]), dim3(256), 0, s, mem, map, st,
+                       mixed ? 1 : 0);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_background_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st)
+{
+    if (!map.start[map.n]) return;
+    hipLaunchKernelGGL(k_ens_background_f, dim3((unsigned)map.start[map.n]), dim3(BX, BY), 0, s, mem, map, st);
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
+void ens_background_ab_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st)
+{
+    if (!map.start[map.n]) return;
+    hipLaunchKernelGGL(k_ens_background_ab_f, dim3((unsigned)map.start[map.n]), dim3(BX, BY), 0, s, mem, map, st);
     PS3D_HIP_CHECK(hipGetLastError());
 }
 

@@ -194,20 +194,27 @@ def test_c4_ensemble_500_steps(mixed):
     ens.close()
 
 
+@pytest.mark.parametrize("mixed", [False, True])
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_forced_members_match_single_forced_models(use_graph):
+def test_forced_members_match_single_forced_models(use_graph, mixed):
     """Forcing in the batched ensemble: members with background-flow
     advection (lee_wave1.f90:239-314) and a body force, next to unforced
-    ones, each bit-identical to a single model with the same forcing -- the
-    friction chain into the third buffers, the pre-friction halos copied,
-    then forcing + Adams-Bashforth, as a single forced model steps."""
+    ones, each bit-identical to a single model of its form with the same
+    forcing -- the friction chain into the third buffers ((hi, lo) pairs in
+    the mixed form), the pre-friction halos copied, then forcing +
+    Adams-Bashforth, as a single forced model steps; the mixed form through
+    its cold start (forcing on the pre-friction state) and correction
+    steps."""
+    import dataclasses
     from ps3d_amd import configs
     from ps3d_amd.ensemble import Ensemble
     from ps3d_amd.model import Model
+    from ps3d_amd.params import PS3D_FP64, PS3D_MIXED
     from test_gpu_lee_wave import _forcing
     members = [configs.random_state(32, 24, 12, seed=6, lid=True, Avbi=0.0, Kvbi=0.0),
                configs.random_state(24, 16, 8, seed=7, lid=False, Avbi=0.0, Kvbi=0.0),
                configs.random_state(16, 16, 16, seed=8, lid=True, Avbi=0.0, Kvbi=0.0)]
+    members = [(dataclasses.replace(p, precision=PS3D_MIXED if mixed else PS3D_FP64), f) for p, f in members]
     forcing = {0: _forcing(members[0][0], seed=4), 1: _forcing(members[1][0], seed=5)}
     forcing[1] = (forcing[1][0], None, forcing[1][2])   # no background buoyancy
     ens = Ensemble([p for p, _ in members])
