@@ -314,9 +314,17 @@ __device__ __forceinline__ void tendency_body(const StepConsts &c, const TendArg
             if (i <= nx && j <= ny) {
                 const int64_t o = idx(c, i, j, k);
                 const int64_t X = c.X, XY = c.XY;
-                lu[0] = a.ul[o - 1]; lu[1] = a.ul[o]; lu[2] = a.ul[o - 1 + X]; lu[3] = a.ul[o + X];
-                lv[0] = a.vl[o]; lv[1] = a.vl[o - X]; lv[2] = a.vl[o + 1]; lv[3] = a.vl[o + 1 - X];
-                lb[0] = a.bl[o]; lb[1] = a.bl[o + XY];
+                // the lo parts at the neighbours' periodic partners inside
+                // the domain (the kernels keep only the hi parts' halos
+                // current; under a lid b's level nz + 1 is not used)
+                const int im = i == 1 ? nx : i - 1, ip = i == nx ? 1 : i + 1;
+                const int jm = j == 1 ? ny : j - 1, jp = j == ny ? 1 : j + 1;
+                const int kp = k == nz ? 1 : k + 1;
+                lu[0] = a.ul[idx(c, im, j, k)]; lu[1] = a.ul[o]; lu[2] = a.ul[idx(c, im, jp, k)];
+                lu[3] = a.ul[idx(c, i, jp, k)];
+                lv[0] = a.vl[o]; lv[1] = a.vl[idx(c, i, jm, k)]; lv[2] = a.vl[idx(c, ip, j, k)];
+                lv[3] = a.vl[idx(c, ip, jm, k)];
+                lb[0] = a.bl[o]; lb[1] = a.bl[idx(c, i, j, kp)];
                 if (a.pint) {
                     pp[0] = a.pint[o]; pp[1] = a.pint[o + 1]; pp[2] = a.pint[o + X]; pp[3] = a.pint[o + XY];
                 }
@@ -635,13 +643,10 @@ __global__ __launch_bounds__(BX * BY) void k_ab_mixed(StepConsts c, AbMixedArgs 
         gq = s * ((qn - qo) * a.grd[f]);
     }
     const double x = ((double)a.ph[f][o] + (double)a.pl[f][o]) + (ab - gq);
-    const float hi = (float)x, lo = (float)(x - (double)hi);
+    const float hi = (float)x;
     a.oh[f][o] = hi;
-    a.ol[f][o] = lo;
-    if (a.halo) {
-        write_images<float>(c, a.oh[f], i, j, k, hi, 0);
-        write_images<float>(c, a.ol[f], i, j, k, lo, 0);
-    }
+    a.ol[f][o] = (float)(x - (double)hi);
+    if (a.halo) write_images<float>(c, a.oh[f], i, j, k, hi, 0);
 }
 
 __global__ __launch_bounds__(BX * BY) void k_ab(StepConsts c, AbArgs a)
@@ -1034,15 +1039,10 @@ __device__ __forceinline__ void fric_body(const StepConsts &c, const FricArgs<T>
                     gq = F.scale * ((qn - qo) * F.grd);
                 }
                 const double x = base + ((zt ? 0.0 : (double)hinc[py][px]) + (double)binc + ab - gq);
-                const float hi = (float)x, lo = (float)(x - (double)hi);
+                const float hi = (float)x;
                 F.out[o] = hi;
-                F.out_lo[o] = lo;
-                // the images of both parts: the mixed tendency reads lo at
-                // the neighbours of a point, halos included
-                if (a.halo) {
-                    write_images<float>(c, F.out, i, j, k, hi, 0);
-                    write_images<float>(c, F.out_lo, i, j, k, lo, 0);
-                }
+                F.out_lo[o] = (float)(x - (double)hi);
+                if (a.halo) write_images<float>(c, F.out, i, j, k, hi, 0);
             }
         }
         // no barrier here: the next level's first barrier orders this level's
@@ -1215,17 +1215,13 @@ __device__ __forceinline__ void project_f_point(const StepConsts &c, float *__re
     const double pe = dp[idx(c, i == c.nx ? 1 : i + 1, j, k)];
     const double pn = dp[idx(c, i, j == c.ny ? 1 : j + 1, k)];
     const double pt = dp[idx(c, i, j, k == c.nz ? (c.lid ? c.nz : 1) : k + 1)];
-    // with halo set, the halo images of the hi and lo parts, of the
-    // increment and of pint are written too (Neumann in z under a lid for
-    // the last two)
+    // with halo set, the halo images of the hi parts, of the increment and of
+    // pint are written too (Neumann in z under a lid for the last two)
     auto put = [&](float *h, float *l, double x) {
-        const float hi = (float)x, lo = (float)(x - (double)hi);
+        const float hi = (float)x;
         h[o] = hi;
-        l[o] = lo;
-        if (halo) {
-            write_images<float>(c, h, i, j, k, hi, 0);
-            write_images<float>(c, l, i, j, k, lo, 0);
-        }
+        l[o] = (float)(x - (double)hi);
+        if (halo) write_images<float>(c, h, i, j, k, hi, 0);
     };
     put(uh, ul, ((double)uh[o] + (double)ul[o]) - c.dt * ((pe - pc) * c.rdx));
     put(vh, vl, ((double)vh[o] + (double)vl[o]) - c.dt * ((pn - pc) * c.rdy));
@@ -1488,8 +1484,8 @@ __global__ __launch_bounds__(BX * BY) void k_ens_background_f(const EnsMember *_
 // mixed forced members, the steady step: the forcing on the post-friction
 // hi parts of the third buffers into the FP32 slot tau (k_background<float>),
 // then the Adams-Bashforth update with the pressure-history term from the
-// third (hi, lo) buffers into the state's next buffers, halo images of both
-// parts included (k_ab_mixed) -- per point, with the slot's FP32 values,
+// third (hi, lo) buffers into the state's next buffers, halo images of the
+// hi parts included (k_ab_mixed) -- per point, with the slot's FP32 values,
 // so the same bits as the two kernels of a single model
 __global__ __launch_bounds__(BX * BY) void k_ens_background_ab_f(const EnsMember *__restrict__ mem, EnsMap map,
                                                                   const EnsState *__restrict__ st)
@@ -1563,11 +1559,10 @@ __global__ __launch_bounds__(BX * BY) void k_ens_background_ab_f(const EnsMember
             gq = s * ((qn - qo) * grd);
         }
         const double x = ((double)e.tmph[f][o] + (double)e.tmpl[f][o]) + (ab - gq);
-        const float hi = (float)x, lo = (float)(x - (double)hi);
+        const float hi = (float)x;
         e.fh[f][par ^ 1][o] = hi;
-        e.fl[f][par ^ 1][o] = lo;
+        e.fl[f][par ^ 1][o] = (float)(x - (double)hi);
         write_images<float>(c, e.fh[f][par ^ 1], i, j, k, hi, 0);
-        write_images<float>(c, e.fl[f][par ^ 1], i, j, k, lo, 0);
     }
 }
 
@@ -2179,16 +2174,13 @@ void launch_friction_ab_mixed(Model &m, int itt, int tau, int taum1, int taum2, 
         std::swap(m.fh[f], m.ah[f]);
         std::swap(m.fl[f], m.al[f]);
     }
-    // the stencils and the divergence read the hi parts' halos, the mixed
-    // tendency the lo parts' too (written by the kernel itself when
-    // fuse_halo)
+    // the stencils and the divergence read the hi parts' halos (written by the
+    // kernel itself when fuse_halo); the lo parts are read at interior points
+    // only (the mixed tendency wraps its neighbours' indices)
     if (m.fuse_halo) return;
     HaloArgsT<float> h{};
-    for (int f = 0; f < 4; ++f) {
-        h.f[f] = m.fh[f];
-        h.f[4 + f] = m.fl[f];
-    }
-    halo_launch(m, h, 8);
+    for (int f = 0; f < 4; ++f) h.f[f] = m.fh[f];
+    halo_launch(m, h, 4);
 }
 
 // the mixed step with background forcing: friction alone into the second
@@ -2240,11 +2232,8 @@ void launch_friction_forcing_ab_mixed(Model &m, int itt, int tau, int taum1, int
     PS3D_HIP_CHECK(hipGetLastError());
     if (!m.fuse_halo) {
         HaloArgsT<float> h{};
-        for (int f = 0; f < 4; ++f) {
-            h.f[f] = m.fh[f];
-            h.f[4 + f] = m.fl[f];
-        }
-        halo_launch(m, h, 8);
+        for (int f = 0; f < 4; ++f) h.f[f] = m.fh[f];
+        halo_launch(m, h, 4);
     }
 }
 
@@ -2343,9 +2332,8 @@ void launch_project_exchange_f(Model &m, float *dp)
     if (m.fuse_halo) return;
     HaloArgsT<float> h{};
     h.f[0] = m.fh[0]; h.f[1] = m.fh[1]; h.f[2] = m.fh[2]; h.f[3] = dp;
-    h.f[4] = m.fl[0]; h.f[5] = m.fl[1]; h.f[6] = m.fl[2];
     h.neumann[3] = m.prm.enable_vertical_boundaries;
-    halo_launch(m, h, 7);
+    halo_launch(m, h, 4);
     launch_border_exchg(m, m.pint, m.prm.enable_vertical_boundaries);
 }
 
