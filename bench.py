@@ -120,6 +120,14 @@ def time_steps(model, steps: int) -> float:
     return time.perf_counter() - t0
 
 
+def time_steps_graph(model, steps: int) -> float:
+    model.synchronize()
+    t0 = time.perf_counter()
+    model.run(steps, use_graph=True)
+    model.synchronize()
+    return time.perf_counter() - t0
+
+
 def mixed_leg(prm, fields, steps: int, warmup: int, fp64_value: float, check_steps=(1, 100, 1000)) -> dict:
     from ps3d_amd import diagnose
     from ps3d_amd.model import Model
@@ -527,8 +535,13 @@ def main() -> int:
     if args.profile_all:
         out["kernel_shares_ms_per_step"] = {k: v[1] for k, v in sorted(shares.items(), key=lambda x: -x[1][1])}
     out["tuning"] = model.tuning().strip().splitlines()
+    # the same steps replayed from ps3d_run's captured six-step hipGraph
+    model.run(12, use_graph=True)   # capture
+    el_graph = time_steps_graph(model, args.steps)
+    out["graph_replay"] = {"ms_per_step": 1e3 * el_graph / args.steps,
+                           "value": prm.npoints * args.steps / el_graph, "unit": "grid-pt steps/s"}
     model.close()
-    log(f"FP64: {1e3 * el / args.steps:.4f} ms per step")
+    log(f"FP64: {1e3 * el / args.steps:.4f} ms per step ({1e3 * el_graph / args.steps:.4f} as graph replays)")
     if not args.no_mixed:
         checks = tuple(int(x) for x in args.mixed_check.split(",") if x.strip())
         out["mixed"] = mixed_leg(prm, fields, args.steps, args.warmup, value, check_steps=checks)

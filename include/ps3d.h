@@ -119,9 +119,14 @@ int ps3d_integrate(ps3d_model *m, int itt, int tau, int taum1, int taum2);
 /* main.f90:139-161 without diagnostics: nsteps calls of integrate with the
    tau rotation.  state = {itt, tau, taum1, taum2}, updated on return.
    The steps are queued back to back on the model's stream and the call
-   returns when they are queued.  use_graph is accepted for interface
-   stability and has no effect on a single model (its kernels are long
-   enough to hide launch cost; ps3d_ensemble_run does replay a graph). */
+   returns when they are queued.  use_graph = 1: past the first two steps
+   (and the mixed form's cold start and correction steps) the steps run as
+   replays of a hipGraph of six steps (the period of the tau rotation, the
+   buffer parity and the mixed form's increment ring), captured from the
+   model's state on first use and again whenever that state's signature
+   (buffers, constants, launch shapes, options) changes; the same bits as
+   queued steps.  Steps with a forcing hook or the wavemaker, and steps
+   whose kernels are being timed, are always queued directly. */
 int ps3d_run(ps3d_model *m, int nsteps, int32_t *state, int use_graph);
 
 /* wait for the model's stream */

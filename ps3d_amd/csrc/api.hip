@@ -309,6 +309,108 @@ void integrate(Model &m, int itt, int tau, int taum1, int taum2)
     pressure(m);
 }
 
+// ---- graph replay of ps3d_run ---------------------------------------------
+// Six steps repeat the whole host-side state of a step: the tau rotation
+// (period 3), the buffer parity of the friction pass (2) and the mixed
+// form's ring of pressure increments (3).  Captured once from a state, the
+// six steps replay from every later state with the same signature.
+constexpr int kGraphSteps = 6;
+
+void rotate(int32_t *st)
+{
+    st[1] = st[1] % 3 + 1;
+    st[2] = st[2] % 3 + 1;
+    st[3] = st[3] % 3 + 1;
+    st[0] += 1;
+}
+
+// host-side work per step (a forcing hook, the time-dependent wavemaker) or
+// per-kernel timing events cannot be replayed
+bool graph_capable(const Model &m)
+{
+    return !m.forcing && !m.wm_on && m.prof_name.empty() && m.stream != nullptr;
+}
+
+// the AB order no longer changes; the mixed form's history is started
+bool graph_steady(const Model &m, const int32_t *st)
+{
+    return st[0] >= 2 && (!m.mixed || (!m.cold && m.refine == 0));
+}
+
+uint64_t fnv(uint64_t h, const void *p, size_t n)
+{
+    const unsigned char *b = static_cast<const unsigned char *>(p);
+    for (size_t q = 0; q < n; ++q) h = (h ^ b[q]) * 1099511628211ull;
+    return h;
+}
+
+// everything a captured step depends on: the tau slots, the buffers the
+// fields are in, the ring index, the constants, launch shapes and options
+std::vector<uint64_t> graph_signature(const Model &m, const int32_t *st)
+{
+    std::vector<uint64_t> k = {(uint64_t)st[1], (uint64_t)st[2], (uint64_t)st[3], (uint64_t)m.dp_cur};
+    for (int f = 0; f < 4; ++f) k.push_back((uint64_t)m.cur[f]);
+    for (const void *p : {(const void *)m.u, (const void *)m.v, (const void *)m.w, (const void *)m.b,
+                          (const void *)m.alt[0], (const void *)m.alt[1], (const void *)m.alt[2],
+                          (const void *)m.alt[3]})
+        k.push_back((uint64_t)(uintptr_t)p);
+    for (int f = 0; f < 4; ++f) {
+        k.push_back((uint64_t)(uintptr_t)m.fh[f]);
+        k.push_back((uint64_t)(uintptr_t)m.ah[f]);
+        k.push_back((uint64_t)(uintptr_t)m.fl[f]);
+        k.push_back((uint64_t)(uintptr_t)m.al[f]);
+    }
+    uint64_t h = 1469598103934665603ull;
+    h = fnv(h, &m.c, sizeof(m.c));
+    h = fnv(h, &m.prm, sizeof(m.prm));
+    const int opts[] = {m.tend_kc, m.fric_kc, m.tfric_kc, m.thomas_cpw, (int)m.thomas_reg, (int)m.fuse_tendency,
+                        (int)m.fuse_friction, (int)m.fuse_halo, m.fft_plan[0], m.fft_plan[1], m.fft_plan[2],
+                        (int)m.bg_on, (int)m.bg_b_on, (int)m.bf_on[0], (int)m.bf_on[1], (int)m.bf_on[2],
+                        (int)m.bf_on[3]};
+    h = fnv(h, opts, sizeof(opts));
+    k.push_back(h);
+    return k;
+}
+
+void drop_graph(Model &m)
+{
+    if (m.step_exec) (void)hipGraphExecDestroy(m.step_exec);
+    if (m.step_graph) (void)hipGraphDestroy(m.step_graph);
+    m.step_exec = nullptr;
+    m.step_graph = nullptr;
+    m.graph_key.clear();
+}
+
+// capture six steps from state st (which is left as it is: the captured
+// steps have not run); the host-side state goes round its period and comes
+// back to where it was, which the signature checks
+void capture_steps(Model &m, const int32_t *st)
+{
+    drop_graph(m);
+    const std::vector<uint64_t> key = graph_signature(m, st);
+    int32_t cs[4] = {st[0], st[1], st[2], st[3]};
+    PS3D_HIP_CHECK(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    try {
+        for (int q = 0; q < kGraphSteps; ++q) {
+            integrate(m, cs[0], cs[1], cs[2], cs[3]);
+            rotate(cs);
+        }
+    } catch (...) {
+        (void)hipStreamEndCapture(m.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    PS3D_HIP_CHECK(hipStreamEndCapture(m.stream, &g));
+    if (graph_signature(m, st) != key) {
+        (void)hipGraphDestroy(g);
+        throw Error(PS3D_ERR_STATE, "graph capture: the step's state has no period of six steps");
+    }
+    m.step_graph = g;
+    PS3D_HIP_CHECK(hipGraphInstantiate(&m.step_exec, g, nullptr, nullptr, 0));
+    m.graph_key = key;
+}
+
 const char *kKernelNames =
     "tendency,friction_ab,tend_friction_ab,harmonic,biharmonic_h,biharmonic_v,ab_update,halo,fft_x_r2c,fft_y_fwd,thomas,"
     "fft_y_inv,fft_x_c2r,z_solve,fft_z_fwd,fft_z_inv,project";
@@ -494,6 +596,7 @@ int ps3d_destroy(ps3d_model *m)
     return guard([&] {
         if (!m) return;
         if (m->stream) (void)hipStreamSynchronize(m->stream);
+        drop_graph(*m);
         balance_release(*m);
         for (auto ev : m->prof_ev) (void)hipEventDestroy(ev);
         for (void *p : m->allocations) (void)hipFree(p);
@@ -596,15 +699,32 @@ int ps3d_run(ps3d_model *m, int nsteps, int32_t *st, int use_graph)
     return guard([&] {
         check_model(m);
         PS3D_REQUIRE(st && nsteps >= 0, "bad arguments");
-        (void)use_graph;
-        for (int s = 0; s < nsteps; ++s) {
+        int s = 0;
+        auto eager = [&] {
             check_tau(st[1], st[2], st[3]);
             integrate(*m, st[0], st[1], st[2], st[3]);
-            st[1] = st[1] % 3 + 1;
-            st[2] = st[2] % 3 + 1;
-            st[3] = st[3] % 3 + 1;
-            st[0] += 1;
+            rotate(st);
+            ++s;
+        };
+        if (use_graph) {
+            // steps that cannot be replayed run first: the first two (Euler,
+            // AB2) and the mixed form's cold start and correction steps
+            while (s < nsteps && graph_capable(*m) && !graph_steady(*m, st)) eager();
+            if (graph_capable(*m) && nsteps - s >= kGraphSteps) {
+                if (!m->step_exec || m->graph_key != graph_signature(*m, st)) {
+                    // one eager step first, so that whatever the step sets up
+                    // on first use is in place before the capture
+                    eager();
+                    if (nsteps - s >= kGraphSteps) capture_steps(*m, st);
+                }
+                while (m->step_exec && nsteps - s >= kGraphSteps) {
+                    PS3D_HIP_CHECK(hipGraphLaunch(m->step_exec, m->stream));
+                    for (int q = 0; q < kGraphSteps; ++q) rotate(st);
+                    s += kGraphSteps;
+                }
+            }
         }
+        while (s < nsteps) eager();
     });
 }
 

@@ -233,6 +233,12 @@ struct Model {
     // the heuristic's first choice: whole columns in registers where nz is
     // one of the compiled-in column lengths (k_thomas_reg)
     bool thomas_reg = true;
+    // ps3d_run's captured steps: a hipGraph of six steps (the period of the
+    // tau rotation, the buffer parity and the pressure-increment ring) and
+    // the signature of the state it was captured from
+    hipGraph_t step_graph = nullptr;
+    hipGraphExec_t step_exec = nullptr;
+    std::vector<uint64_t> graph_key;
     // radix plans of the x, y and z transforms (0: radix 4 first, the FP64
     // plan; 1: radix 8 first); FP64 transforms always use plan 0, FP32 ones
     // (mixed form) the plan set here (ps3d_set_fft_plan, ps3d_tune)
