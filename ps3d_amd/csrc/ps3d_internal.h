@@ -225,9 +225,10 @@ struct Model {
     bool fuse_halo = false;
     // launch shapes (results do not depend on them; set by ps3d_tune):
     // levels per work-group of the k-marching kernels (0: heuristic) and
-    // the form of the FP64 Thomas solve (launch_thomas: 1 whole columns in
-    // registers; 16, 32, 64 columns per wave on the fly; -16, -32, -64 the
-    // factor tables; 0: the heuristic)
+    // the form of the Thomas solve (launch_thomas, launch_thomas_f: 1 whole
+    // columns in registers; 2 the real and imaginary parts of a column on two
+    // lanes; 16, 32, 64 columns per wave on the fly; -16, -32, -64 the factor
+    // tables; 0: the heuristic)
     int tend_kc = 0, fric_kc = 0, thomas_cpw = 0;
     int tfric_kc = 0;   // the same for the kernel with the tendency fused in
     // the heuristic's first choice: whole columns in registers where nz is
@@ -493,6 +494,9 @@ int ens_thomas_cols(bool reg);
 // nz levels have a compiled-in register Thomas solve, and the spectral array
 // (fs_bytes) is addressed with 32-bit offsets
 bool thomas_reg_ok(int nz, uint64_t fs_bytes);
+// the same for the form with the real and imaginary parts on two lanes
+// (k_thomas_split), which has taller compiled-in lengths
+bool thomas_split_ok(int nz, uint64_t fs_bytes);
 // a compiled-in FP32 transform length whose plan 1 differs from plan 0
 bool fft_plan_choice(int n);
 void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max, bool reg);

@@ -335,6 +335,96 @@ __global__ __launch_bounds__(64, 1) void k_thomas_reg(C *__restrict__ fs, const 
     thomas_reg_col<NZ, C>(fs, ksqr, ncols, col, fxa, dz2, dsqr, cpl);
 }
 
+// Taller columns in registers: the real and imaginary parts of a column on
+// two neighbouring lanes (32 columns per wave), so a lane keeps NZ values of
+// d' instead of 2 NZ and columns of 128 levels still fit its registers.  Both
+// lanes of a column run its c' recurrence (the same operations on the same
+// operands, so the same bits; the even lane writes c' to LDS) and each runs
+// the d' sweeps of its part with the operations thomas_reg_col applies to
+// that component: the same bits as every other form.  R = double: the FP64
+// path; R = float: the mixed form (FP32 right-hand side and solution, FP64
+// elimination).  cpl: NZ x 32 doubles of LDS.
+template <int NZ, class R>
+__device__ __forceinline__ void thomas_split_col(R *__restrict__ fs, const double *__restrict__ ksqr, int64_t ncols,
+                                                 int64_t col, double fxa, double dz2, double dsqr, double *cpl)
+{
+    const int part = threadIdx.x & 1;
+    double *cpk = cpl + (threadIdx.x >> 1);   // c'(level q) at cpk[32 q]
+    // component `part` of column col at level k is element 2 (k ncols + col)
+    // + part; 32-bit byte offsets as in thomas_reg_col
+    char *base = reinterpret_cast<char *>(fs);
+    constexpr uint32_t E = sizeof(R);
+    const uint32_t o0 = ((uint32_t)col * 2 + (uint32_t)part) * E, st = (uint32_t)ncols * 2 * E;
+    double d[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) d[k] = (double)*reinterpret_cast<const R *>(base + (o0 + (uint32_t)k * st));
+    const double ks = ksqr[col];
+    const bool solve = ks > 0;
+    double x = 0.0;
+    if (solve) {
+        const double b1 = -ks - fxa;
+        double cprev = fxa / b1;
+        if (part == 0) cpk[0] = cprev;
+        double dp = d[0] / b1;
+        d[0] = dp;
+#pragma unroll
+        for (int q = 1; q < NZ; ++q) {
+            const int k = q + 1;   // level, 1-based
+            const double bk = (k == NZ) ? -ks - fxa : -ks - 2 * fxa;
+            const double mm = bk - cprev * fxa;
+            const double r = 1.0 / mm;
+            const double ck = (k == NZ) ? 0.0 : fxa * r;
+            if (part == 0) cpk[32 * q] = ck;
+            cprev = ck;
+            dp = (d[q] - dp * fxa) * r;
+            d[q] = dp;
+        }
+        x = dp;
+    }
+    // the odd lanes read back the c' their even neighbours wrote
+    __syncthreads();
+    if (solve) {
+#pragma unroll
+        for (int q = NZ - 2; q >= 0; --q) {
+            x = d[q] - cpk[32 * q] * x;
+            d[q] = x;
+        }
+    } else {
+        // zero mode, pressure_solver.f90:44-50, as in thomas_reg_col
+        double pm1 = 0.0;
+        double pk = d[0] * dz2 * dsqr;
+        double bk = d[1];
+        d[0] = pm1;
+        d[1] = pk;
+#pragma unroll
+        for (int k = 2; k < NZ; ++k) {
+            const double nxt = d[k];
+            const double pn = 2 * pk - pm1 + bk * dz2 * dsqr;
+            d[k] = pn;
+            pm1 = pk;
+            pk = pn;
+            bk = nxt;
+        }
+    }
+    uint32_t off = o0;
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        asm volatile("" : "+v"(off));
+        *reinterpret_cast<R *>(base + off) = (R)d[k];
+        off += st;
+    }
+}
+
+template <int NZ, class R>
+__global__ __launch_bounds__(64, 1) void k_thomas_split(R *__restrict__ fs, const double *__restrict__ ksqr,
+                                                        int64_t ncols, double fxa, double dz2, double dsqr)
+{
+    __shared__ double cpl[NZ * 32];
+    const int64_t col = (int64_t)xcd_remap((int)blockIdx.x, (int)gridDim.x) * 32 + (threadIdx.x >> 1);
+    if (col >= ncols) return;
+    thomas_split_col<NZ, R>(fs, ksqr, ncols, col, fxa, dz2, dsqr, cpl);
+}
+
 // the same for the rigid-lid members of a batched ensemble, 64 columns per
 // work-group; every member's nz is one of the compiled-in lengths
 // (thomas_reg_ok), selected at run time
@@ -517,6 +607,32 @@ bool thomas_reg_ok(int nz, uint64_t fs_bytes)
     return (nz == 16 || nz == 32 || nz == 48 || nz == 64) && fs_bytes < (uint64_t(1) << 32);
 }
 
+bool thomas_split_ok(int nz, uint64_t fs_bytes)
+{
+    return (nz == 16 || nz == 32 || nz == 48 || nz == 64 || nz == 96 || nz == 128) &&
+           fs_bytes < (uint64_t(1) << 32);
+}
+
+// the split register form (k_thomas_split) over data of element type R
+template <class R>
+static void launch_thomas_split(Model &m, R *data)
+{
+    const int64_t ncols = (int64_t)m.g.nkx * m.g.ny;
+    const double fxa = 1.0 / (m.prm.dz * m.prm.dz) / m.prm.dsqr;
+    const double dz2 = m.prm.dz * m.prm.dz, dsqr = m.prm.dsqr;
+    const dim3 grid((unsigned)((ncols + 31) / 32));
+    switch (m.g.nz) {
+#define CASE(n)                                                                                                 \
+    case n:                                                                                                     \
+        hipLaunchKernelGGL((k_thomas_split<n, R>), grid, dim3(64), 0, m.stream, data, m.ksqr, ncols, fxa, dz2, dsqr); \
+        break;
+        CASE(16) CASE(32) CASE(48) CASE(64) CASE(96) CASE(128)
+#undef CASE
+    default: throw Error(PS3D_ERR_ARG, "split Thomas form: no compiled-in column length " + std::to_string(m.g.nz));
+    }
+    PS3D_HIP_CHECK(hipGetLastError());
+}
+
 void tridiag_factor(Model &m)
 {
     if (!m.prm.enable_vertical_boundaries) return;
@@ -532,8 +648,15 @@ void launch_thomas(Model &m, double2 *data)
     const int64_t ncols = (int64_t)m.g.nkx * m.g.ny;
     const double fxa = 1.0 / (m.prm.dz * m.prm.dz) / m.prm.dsqr;
     const int form = m.thomas_cpw;
-    const bool reg = thomas_reg_ok(m.g.nz, (uint64_t)m.g.S * sizeof(double2)) &&
-                     (form == 1 || (form == 0 && m.thomas_reg));
+    const uint64_t fsb = (uint64_t)m.g.S * sizeof(double2);
+    const bool reg = thomas_reg_ok(m.g.nz, fsb) && (form == 1 || (form == 0 && m.thomas_reg));
+    // form 2, or untuned where the whole-column form has no length: the
+    // real and imaginary parts on two lanes
+    if (!reg && thomas_split_ok(m.g.nz, fsb) && (form == 2 || (form == 0 && m.thomas_reg))) {
+        ProfScope ps(m, "thomas");
+        launch_thomas_split<double>(m, reinterpret_cast<double *>(data));
+        return;
+    }
     if (reg) {
         const dim3 grid((unsigned)((ncols + 63) / 64));
         const double dz2 = m.prm.dz * m.prm.dz, dsqr = m.prm.dsqr;
@@ -600,7 +723,17 @@ void launch_thomas_f(Model &m, float2 *data)
     const double fxa = 1.0 / (m.prm.dz * m.prm.dz) / m.prm.dsqr;
     const unsigned nb = (unsigned)((ncols + 63) / 64);
     ProfScope ps(m, "thomas");
-    if (m.thomas_reg && thomas_reg_ok(m.g.nz, (uint64_t)m.g.S * sizeof(float2))) {
+    // forms as in launch_thomas: 1 whole columns in registers, 2 the parts
+    // on two lanes, 32 on the fly, -64 the factor tables; 0 (untuned): the
+    // first of these that applies
+    const int form = m.thomas_cpw;
+    const uint64_t fsb = (uint64_t)m.g.S * sizeof(float2);
+    const bool reg = thomas_reg_ok(m.g.nz, fsb) && (form == 1 || (form == 0 && m.thomas_reg));
+    if (!reg && thomas_split_ok(m.g.nz, fsb) && (form == 2 || (form == 0 && m.thomas_reg))) {
+        launch_thomas_split<float>(m, reinterpret_cast<float *>(data));
+        return;
+    }
+    if (reg) {
         const dim3 grid((unsigned)((ncols + 63) / 64));
         const double dz2 = m.prm.dz * m.prm.dz, dsqr = m.prm.dsqr;
         switch (m.g.nz) {
@@ -620,7 +753,10 @@ void launch_thomas_f(Model &m, float2 *data)
     // array (faster than 16 columns on the fly for tall columns)
     constexpr int cpw = 32;
     const size_t lds_fly = thomas_fly_lds(cpw, m.g.nz);
-    if (lds_fly <= 64 * 1024) {
+    bool fly = lds_fly <= 64 * 1024;
+    if (form == 32) fly = lds_fly <= kLdsMax;
+    else if (form == -64 && m.dp64) fly = false;
+    if (fly) {
         allow_lds(k_thomas_fly<float2>, lds_fly);
         hipLaunchKernelGGL(k_thomas_fly<float2>, dim3((unsigned)((ncols + cpw - 1) / cpw)), dim3(64), lds_fly,
                            m.stream, data, m.ksqr, ncols, m.g.nz, fxa, m.prm.dz * m.prm.dz, m.prm.dsqr, cpw);

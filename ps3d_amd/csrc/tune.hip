@@ -231,43 +231,54 @@ void tune_model(Model &m, int next_tau, const char *cache_path, int reps)
             PS3D_HIP_CHECK(hipFree(pd));
         }
     }
-    if (!m.mixed && m.prm.enable_vertical_boundaries && !cache.count("thomas " + sh)) {
-        double2 *scratch = nullptr;
-        void *flush = nullptr;
-        PS3D_HIP_CHECK(hipMalloc(&scratch, sizeof(double2) * (size_t)m.g.S));
+    if (m.prm.enable_vertical_boundaries && !cache.count("thomas " + sh)) {
+        // forms (launch_thomas, launch_thomas_f): 1 whole columns in
+        // registers (k_thomas_reg) and 2 the real and imaginary parts on two
+        // lanes (k_thomas_split), where nz is one of their compiled-in
+        // lengths; FP64: 16, 32, 64 columns per wave on the fly, whose LDS
+        // keeps several work-groups on a CU (64 KiB; wider groups, which a
+        // back-to-back timing can prefer, measured slower in the step), and
+        // -16, -32, -64 the factor tables; mixed: 32 columns on the fly, and
+        // -64 the factor tables with FP64 d' in the scratch array where the
+        // model has one
+        const bool mx = m.mixed;
+        const size_t esz = mx ? sizeof(float2) : sizeof(double2);
+        const uint64_t fsb = esz * (uint64_t)m.g.S;
+        std::vector<int> forms;
+        if (thomas_reg_ok(m.g.nz, fsb)) forms.push_back(1);
+        if (thomas_split_ok(m.g.nz, fsb)) forms.push_back(2);
+        if (mx) {
+            if ((sizeof(double2) + sizeof(double)) * 32 * (size_t)m.g.nz <= 160 * 1024) forms.push_back(32);
+            if (m.dp64) forms.push_back(-64);
+        } else {
+            for (int c : {32, 16, 64})
+                if ((sizeof(double2) + sizeof(double)) * (size_t)c * m.g.nz <= 64 * 1024) forms.push_back(c);
+            for (int c : {-64, -32, -16}) forms.push_back(c);
+        }
+        void *scratch = nullptr, *flush = nullptr;
         try {
+            PS3D_HIP_CHECK(hipMalloc(&scratch, esz * (size_t)m.g.S));
             PS3D_HIP_CHECK(hipMalloc(&flush, kFlushBytes));
-            PS3D_HIP_CHECK(hipMemcpyAsync(scratch, m.FS, sizeof(double2) * (size_t)m.g.S, hipMemcpyDeviceToDevice,
-                                          m.stream));
+            PS3D_HIP_CHECK(hipMemcpyAsync(scratch, mx ? (const void *)m.FSf : (const void *)m.FS,
+                                          esz * (size_t)m.g.S, hipMemcpyDeviceToDevice, m.stream));
             // the table form reads per-column factors that stay cached when
             // launched back to back: timed with cold caches, as in the step
-            // columns per work-group whose LDS keeps several work-groups on
-            // a CU (64 KiB), measured faster in the step than wider groups
-            // that a back-to-back timing can prefer
-            // forms (launch_thomas): 1 whole columns in registers
-            // (k_thomas_reg), where nz is one of its compiled-in lengths;
-            // 16, 32, 64 columns per wave on the fly; -16, -32, -64 the
-            // factor tables
-            std::vector<int> cpw;
-            if (thomas_reg_ok(m.g.nz, sizeof(double2) * (uint64_t)m.g.S)) cpw.push_back(1);
-            for (int c : {32, 16, 64})
-                if ((sizeof(double2) + sizeof(double)) * (size_t)c * m.g.nz <= 64 * 1024) cpw.push_back(c);
-            for (int c : {-64, -32, -16}) cpw.push_back(c);
             int form = 0;
-            pick("thomas", cpw, form, 2 * reps, flush, [&] {
+            pick("thomas", forms, form, 2 * reps, flush, [&] {
                 m.thomas_cpw = form;
-                launch_thomas(m, scratch);
+                if (mx) launch_thomas_f(m, static_cast<float2 *>(scratch));
+                else launch_thomas(m, static_cast<double2 *>(scratch));
             });
             m.thomas_cpw = form;
         } catch (...) {
-            (void)hipFree(scratch);
+            if (scratch) (void)hipFree(scratch);
             if (flush) (void)hipFree(flush);
             throw;
         }
         PS3D_HIP_CHECK(hipStreamSynchronize(m.stream));
         PS3D_HIP_CHECK(hipFree(scratch));
         PS3D_HIP_CHECK(hipFree(flush));
-    } else if (!m.mixed && m.prm.enable_vertical_boundaries) {
+    } else if (m.prm.enable_vertical_boundaries) {
         m.thomas_cpw = cache["thomas " + sh];
     }
     tune_cache_write(cache_path, cache);

@@ -251,3 +251,53 @@ def test_mixed_c2_state_100_steps():
         assert v <= TOL, (k, v)
     g64.close()
     gmx.close()
+
+
+@pytest.mark.parametrize("nz", [48, 128])
+def test_every_mixed_thomas_form_gives_the_same_bits(nz, tmp_path):
+    """Each form of the mixed form's Thomas solve (FP32 right-hand side, FP64
+    elimination), forced through a tuning cache -- whole columns in
+    registers (nz of 16-64), the real and imaginary parts on two lanes, 32
+    columns per wave on the fly, the factor tables with FP64 d' in a scratch
+    array (the model has one where the on-the-fly form's LDS exceeds 64 KiB)
+    -- steps bit for bit like the others, and within the stated bound of the
+    FP64 path."""
+    from ps3d_amd import configs, diagnose
+    from ps3d_amd.model import Model
+    from ps3d_amd.params import PS3D_MIXED
+    prm, fields = configs.random_state(32, 16, nz, seed=43, lid=True, Avbi=0.0, Kvbi=0.0)
+    g64 = Model(prm)
+    for n, a in fields.items():
+        g64.f[n][...] = a
+    g64.push()
+    g64.run(3)
+    g64.pull()
+    prm_m = prm.replace(precision=PS3D_MIXED)
+    shape = f"32x16x{nz}:lid:mixed"
+    forms = ([1] if nz <= 64 else []) + [2, 32] + ([-64] if 24 * 32 * nz > 64 * 1024 else [])
+    ref = None
+    for form in forms:
+        print(f"mixed nz {nz}: Thomas form {form}", flush=True)
+        cache = tmp_path / f"thomas_{nz}_{form}.txt"
+        cache.write_text(f"friction_ab {shape} 8\ntendency {shape} 8\nthomas {shape} {form}\n"
+                         f"fft_plan_x {shape} 0\nfft_plan_y {shape} 0\n")
+        m = Model(prm_m)
+        for n, a in fields.items():
+            m.f[n][...] = a
+        m.push()
+        m.tune(str(cache), reps=1)
+        assert f"thomas {shape} {form}" in m.tuning()
+        m.run(3)
+        m.pull()
+        state = {n: m.f[n].copy() for n in ("u", "v", "w", "b", "p")}
+        m.close()
+        if ref is None:
+            ref = state
+            err = diagnose.compare(state, g64.f, prm)
+            print(f"mixed nz {nz} step 3: " + " ".join(f"{k}={v:.2e}" for k, v in err.items()), flush=True)
+            for k, v in err.items():
+                assert v <= TOL, (k, v)
+        else:
+            for n in ("u", "v", "w", "b", "p"):
+                assert np.array_equal(state[n], ref[n]), (form, n)
+    g64.close()
