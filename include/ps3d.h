@@ -253,10 +253,12 @@ int ps3d_fft_backward(ps3d_fft *f, const char *op, const double *qs, double *qp)
 /* ---- launch-shape autotuner ------------------------------------------ */
 
 /* Times the candidate launch shapes of this model's step on the live model
-   (levels per work-group of the tendency and friction + AB kernels, columns
-   per wavefront of the Thomas solve) and keeps the fastest; the choice is
-   cached per (stage, grid, boundary, precision) in cache_path (NULL or "":
-   none).
+   (levels per work-group of the tendency and friction + AB kernels; the
+   form of the Thomas solve: whole columns in registers, the real and
+   imaginary parts on two lanes, columns per wavefront on the fly or with
+   factor tables; in the mixed form also the radix plan of each FP32
+   transform axis) and keeps the fastest; the choice is cached per (stage,
+   grid, boundary, precision) in cache_path (NULL or "": none).
    next_tau is the slot the next step writes (it is used as scratch).  The
    model's state is unchanged and results do not depend on the shapes.  No
    counterpart in the reference (its loop shapes are fixed by the compiler). */
