@@ -51,6 +51,7 @@ struct ps3d_ensemble {
     int refine = 0;
     // every rigid-lid member's Thomas solve has the register form
     bool thomas_reg = false;
+    bool thomas_split = false;   // (when not thomas_reg) k_ens_thomas_split
     std::vector<void *> tmp_alloc;      // third buffers of forced members
 };
 
@@ -165,7 +166,7 @@ void build_maps(ps3d_ensemble &E)
             case S_FRIC: nb = (int64_t)e.fric_tx * e.fric_ty * e.fric_nchunk * 4; break;
             case S_R2C: nb = ens_r2c_blocks(e); break;
             case S_COLS: nb = ens_cols_blocks(e, g.ny, g.nz); break;
-            case S_THOMAS: nb = lid ? cdiv((int64_t)g.nkx * g.ny, ens_thomas_cols(E.thomas_reg)) : 0; break;
+            case S_THOMAS: nb = lid ? cdiv((int64_t)g.nkx * g.ny, ens_thomas_cols(E.thomas_reg, E.thomas_split)) : 0; break;
             case S_ZSOLVE: nb = lid ? 0 : ens_cols_blocks(e, g.nz, g.ny); break;
             case S_PROJ: nb = (int64_t)cdiv(g.nx, 64) * cdiv(g.ny, 4) * g.nz; break;
             case S_PTS: nb = E.mixed ? cdiv(g.H, 256) : 0; break;
@@ -240,7 +241,7 @@ void enqueue_solve_f(ps3d_ensemble &E, int which)
     hipStream_t s = E.stream;
     ens_r2c_f(s, E.dev_tab, E.maps[S_R2C], E.dev_state, E.lds_x);
     ens_cols_y_f(s, E.dev_tab, E.maps[S_COLS], -1, E.ny_max, E.lds_y);
-    ens_thomas_f(s, E.dev_tab, E.maps[S_THOMAS], E.nzl_max, E.thomas_reg);
+    ens_thomas_f(s, E.dev_tab, E.maps[S_THOMAS], E.nzl_max, E.thomas_reg, E.thomas_split);
     ens_zsolve_f(s, E.dev_tab, E.maps[S_ZSOLVE], E.nz_max, E.lds_z);
     ens_cols_y_f(s, E.dev_tab, E.maps[S_COLS], +1, E.ny_max, E.lds_y);
     ens_c2r_f(s, E.dev_tab, E.maps[S_R2C], E.dev_state, which, E.lds_x);
@@ -299,7 +300,7 @@ void enqueue_step(ps3d_ensemble &E)
     ens_background_ab(s, E.dev_tab, E.maps[S_BGAB], E.dev_state);
     ens_r2c(s, E.dev_tab, E.maps[S_R2C], E.dev_state, E.lds_x);
     ens_cols_y(s, E.dev_tab, E.maps[S_COLS], -1, E.ny_max, E.lds_y);
-    ens_thomas(s, E.dev_tab, E.maps[S_THOMAS], E.nzl_max, E.thomas_reg);
+    ens_thomas(s, E.dev_tab, E.maps[S_THOMAS], E.nzl_max, E.thomas_reg, E.thomas_split);
     ens_zsolve(s, E.dev_tab, E.maps[S_ZSOLVE], E.nz_max, E.lds_z);
     ens_cols_y(s, E.dev_tab, E.maps[S_COLS], +1, E.ny_max, E.lds_y);
     ens_c2r(s, E.dev_tab, E.maps[S_R2C], E.lds_x);
@@ -509,7 +510,7 @@ int ps3d_ensemble_create(int n, const ps3d_params *prms, void *stream, ps3d_ense
             PS3D_REQUIRE(p.precision == PS3D_FP64 || p.precision == PS3D_MIXED, who + "unknown precision form");
             PS3D_REQUIRE(p.precision == prms[0].precision, who + "the members of an ensemble share one precision form");
             PS3D_REQUIRE(p.precision != PS3D_MIXED || !p.enable_vertical_boundaries ||
-                             (sizeof(double2) + sizeof(double)) * (size_t)ens_thomas_cols(false) * (size_t)p.nz <=
+                             (sizeof(double2) + sizeof(double)) * (size_t)ens_thomas_cols(false, false) * (size_t)p.nz <=
                                  160 * 1024,
                          who + "mixed ensembles: rigid-lid members of at most 213 levels");
             PS3D_REQUIRE(!(p.Avbi > 0.) && !(p.Kvbi > 0.),
@@ -541,11 +542,15 @@ int ps3d_ensemble_create(int n, const ps3d_params *prms, void *stream, ps3d_ense
                 set_chunks(m->g.nz, default_tend_kc(m->g), E->tab[i].tend_kc, E->tab[i].tend_nchunk);
                 set_chunks(m->g.nz, default_fric_kc(m->g), E->tab[i].fric_kc, E->tab[i].fric_nchunk);
             }
-            E->thomas_reg = true;
+            bool reg = true, split = true;
             for (ps3d_model *m : E->mem)
-                if (m->prm.enable_vertical_boundaries)
-                    E->thomas_reg = E->thomas_reg && m->thomas_reg &&
-                                    thomas_reg_ok(m->g.nz, (uint64_t)m->g.S * (E->mixed ? sizeof(float2) : sizeof(double2)));
+                if (m->prm.enable_vertical_boundaries) {
+                    const uint64_t fsb = (uint64_t)m->g.S * (E->mixed ? sizeof(float2) : sizeof(double2));
+                    reg = reg && m->thomas_reg && thomas_reg_ok(m->g.nz, fsb);
+                    split = split && m->thomas_reg && thomas_split_ok(m->g.nz, fsb);
+                }
+            E->thomas_reg = reg;
+            E->thomas_split = !reg && split;
             build_maps(*E);
             PS3D_HIP_CHECK(hipMalloc(&E->dev_tab, sizeof(EnsMember) * n));
             PS3D_HIP_CHECK(hipMalloc(&E->dev_state, sizeof(EnsState)));

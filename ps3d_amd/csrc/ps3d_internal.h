@@ -489,8 +489,11 @@ void ens_c2r(hipStream_t s, const EnsMember *mem, const EnsMap &map, size_t lds)
 // Thomas stage: ens_thomas_cols() columns per work-group; nz_max: most levels
 // of a rigid-lid member (sizes the LDS of the on-the-fly form)
 // reg: the register form (k_ens_thomas_reg, 64 columns per work-group), for
-// ensembles whose rigid-lid members all have thomas_reg_ok levels
-int ens_thomas_cols(bool reg);
+// ensembles whose rigid-lid members all have thomas_reg_ok levels; split
+// (when not reg): the real and imaginary parts on two lanes
+// (k_ens_thomas_split, 32 columns per work-group), where they all have
+// thomas_split_ok levels
+int ens_thomas_cols(bool reg, bool split);
 // nz levels have a compiled-in register Thomas solve, and the spectral array
 // (fs_bytes) is addressed with 32-bit offsets
 bool thomas_reg_ok(int nz, uint64_t fs_bytes);
@@ -499,7 +502,7 @@ bool thomas_reg_ok(int nz, uint64_t fs_bytes);
 bool thomas_split_ok(int nz, uint64_t fs_bytes);
 // a compiled-in FP32 transform length whose plan 1 differs from plan 0
 bool fft_plan_choice(int n);
-void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max, bool reg);
+void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max, bool reg, bool split);
 // the same stages for members of the mixed form.  cold: the provisional
 // Euler step of the cold start (tendency without the reference pressure,
 // friction + Euler); which: the increment a solve writes and a projection
@@ -517,7 +520,7 @@ void ens_r2c_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const Ens
 void ens_cols_y_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int sign, int nmax, size_t lds);
 void ens_zsolve_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nmax, size_t lds);
 void ens_c2r_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, const EnsState *st, int which, size_t lds);
-void ens_thomas_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max, bool reg);
+void ens_thomas_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max, bool reg, bool split);
 // block counts and LDS bytes of the batched FFT stages for one member
 int ens_r2c_blocks(const EnsMember &e);
 int ens_cols_blocks(const EnsMember &e, int n_len, int lines);

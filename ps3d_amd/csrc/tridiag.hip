@@ -451,6 +451,35 @@ __global__ __launch_bounds__(64, 1) void k_ens_thomas_reg(const EnsMember *__res
     }
 }
 
+// the split form for the rigid-lid members of a batched ensemble, 32
+// columns per work-group, where every member's nz is one of its compiled-in
+// lengths (thomas_split_ok) and some is too tall for k_ens_thomas_reg
+template <class C>
+__global__ __launch_bounds__(64, 1) void k_ens_thomas_split(const EnsMember *__restrict__ mem, EnsMap map)
+{
+    __shared__ double cpl[128 * 32];
+    using R = typename std::conditional<std::is_same<C, float2>::value, float, double>::type;
+    const int g = xcd_remap((int)blockIdx.x, map.start[map.n]);
+    const int mi = ens_find(map, g);
+    const EnsMember &e = mem[mi];
+    const int64_t ncols = (int64_t)e.nkx * e.c.ny;
+    const int64_t col = (int64_t)(g - map.start[mi]) * 32 + (threadIdx.x >> 1);
+    if (col >= ncols) return;
+    R *fs;
+    if constexpr (std::is_same<C, float2>::value) fs = reinterpret_cast<R *>(e.FSf);
+    else fs = reinterpret_cast<R *>(e.FS);
+    const double fxa = e.fxa, dz2 = e.c.dz * e.c.dz, dsqr = e.c.dsqr;
+    switch (e.c.nz) {
+    case 16: thomas_split_col<16, R>(fs, e.ksqr, ncols, col, fxa, dz2, dsqr, cpl); break;
+    case 32: thomas_split_col<32, R>(fs, e.ksqr, ncols, col, fxa, dz2, dsqr, cpl); break;
+    case 48: thomas_split_col<48, R>(fs, e.ksqr, ncols, col, fxa, dz2, dsqr, cpl); break;
+    case 64: thomas_split_col<64, R>(fs, e.ksqr, ncols, col, fxa, dz2, dsqr, cpl); break;
+    case 96: thomas_split_col<96, R>(fs, e.ksqr, ncols, col, fxa, dz2, dsqr, cpl); break;
+    case 128: thomas_split_col<128, R>(fs, e.ksqr, ncols, col, fxa, dz2, dsqr, cpl); break;
+    default: break;
+    }
+}
+
 // the Thomas stage of a batched ensemble (rigid-lid members), ENS_TC columns
 // per 64-lane work-group; FLY: the on-the-fly form with LDS for ENS_TC
 // columns of nz_max levels, else the factor tables
@@ -768,13 +797,14 @@ void launch_thomas_f(Model &m, float2 *data)
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
-int ens_thomas_cols(bool reg) { return reg ? 64 : ENS_TC; }
+int ens_thomas_cols(bool reg, bool split) { return reg ? 64 : (split ? 32 : ENS_TC); }
 
-void ens_thomas_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max, bool reg)
+void ens_thomas_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max, bool reg, bool split)
 {
     if (!map.start[map.n]) return;
-    if (reg) {
-        hipLaunchKernelGGL(k_ens_thomas_reg<float2>, dim3((unsigned)map.start[map.n]), dim3(64), 0, s, mem, map);
+    if (reg || split) {
+        if (reg) hipLaunchKernelGGL(k_ens_thomas_reg<float2>, dim3((unsigned)map.start[map.n]), dim3(64), 0, s, mem, map);
+        else hipLaunchKernelGGL(k_ens_thomas_split<float2>, dim3((unsigned)map.start[map.n]), dim3(64), 0, s, mem, map);
         PS3D_HIP_CHECK(hipGetLastError());
         return;
     }
@@ -785,11 +815,12 @@ void ens_thomas_f(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz
     PS3D_HIP_CHECK(hipGetLastError());
 }
 
-void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max, bool reg)
+void ens_thomas(hipStream_t s, const EnsMember *mem, const EnsMap &map, int nz_max, bool reg, bool split)
 {
     if (!map.start[map.n]) return;
-    if (reg) {
-        hipLaunchKernelGGL(k_ens_thomas_reg<double2>, dim3((unsigned)map.start[map.n]), dim3(64), 0, s, mem, map);
+    if (reg || split) {
+        if (reg) hipLaunchKernelGGL(k_ens_thomas_reg<double2>, dim3((unsigned)map.start[map.n]), dim3(64), 0, s, mem, map);
+        else hipLaunchKernelGGL(k_ens_thomas_split<double2>, dim3((unsigned)map.start[map.n]), dim3(64), 0, s, mem, map);
         PS3D_HIP_CHECK(hipGetLastError());
         return;
     }

@@ -162,6 +162,34 @@ def test_mixed_ensemble_tuning_keeps_results(tmp_path):
     ens.close()
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_tall_members_take_the_register_thomas_forms(mixed):
+    """Rigid-lid members of 96 and 128 levels beside one of 32: every member
+    has a register Thomas length, so the stage runs k_ens_thomas_reg with
+    whole columns for 32 levels and the real and imaginary parts on two
+    lanes for 96 and 128 (32 columns per work-group; 16x16 and 24x32
+    grids leave the last work-group of a member partly empty); members
+    bit-identical to single models of their form."""
+    import dataclasses
+    from ps3d_amd import configs
+    from ps3d_amd.params import PS3D_MIXED
+    members = [configs.random_state(16, 16, 128, seed=31, lid=True, Avbi=0.0, Kvbi=0.0),
+               configs.random_state(32, 24, 96, seed=32, lid=True, Avbi=0.0, Kvbi=0.0),
+               configs.eddy(64, 64, 32, seed=33)]
+    if mixed:
+        members = [(dataclasses.replace(p, precision=PS3D_MIXED), f) for p, f in members]
+    ens = _ensemble(members)
+    print(f"tall members, mixed={mixed}", flush=True)
+    ens.run(4)
+    ens.run(4, use_graph=True)
+    for i, (prm, fields) in enumerate(members):
+        ref = _single(prm, fields, 8)
+        for n in NAMES:
+            got = ens.get(i, n)
+            assert np.array_equal(got, ref[n]), (i, n, float(np.abs(got - ref[n]).max()))
+    ens.close()
+
+
 def test_ensemble_rejects_mixed_forms():
     import dataclasses
     from ps3d_amd.params import PS3D_MIXED
