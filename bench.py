@@ -107,6 +107,19 @@ def hbm_probe(mib: int = 1024, reps: int = 10) -> dict:
     return r
 
 
+def fma_probe(reps: int = 5) -> dict:
+    """Vector FMA issue rates of the library's probe (ps3d_fma_probe,
+    probe.hip), TFLOP/s: FP32 as the library compiles it (without the packed
+    FP32 instructions) and FP64."""
+    import ctypes
+    import numpy as np
+    from ps3d_amd import _lib
+    lib = _lib.load()
+    out = np.zeros(2)
+    _lib.check(lib.ps3d_fma_probe(reps, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return {"fp32_fma": float(out[0]), "fp64_fma": float(out[1]), "unit": "TFLOP/s"}
+
+
 def build_workload(precision=0):
     from ps3d_amd import configs
     return configs.jets(256, 256, 64, seed=2, precision=precision)
@@ -545,6 +558,8 @@ def main() -> int:
     if not args.no_mixed:
         checks = tuple(int(x) for x in args.mixed_check.split(",") if x.strip())
         out["mixed"] = mixed_leg(prm, fields, args.steps, args.warmup, value, check_steps=checks)
+        # the rate the FP32 stencils issue at, beside FP64's
+        out["mixed"]["vector_issue"] = fma_probe()
         log(f"mixed: {out['mixed']['ms_per_step']:.4f} ms per step, worst rel L2 {out['mixed']['max_rel_l2']:.2e}")
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ens_steps)

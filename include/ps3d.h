@@ -364,6 +364,13 @@ const char *ps3d_kernel_names(void);
    achievable bandwidth that the step's kernels are compared with. */
 int ps3d_hbm_probe(int64_t bytes, int reps, int grid, double *out);
 
+/* Vector FMA issue rate on the current GPU, TFLOP/s: out[0] FP32, out[1]
+   FP64 (eight independent FMA chains per lane, the best of reps launches
+   after one warm-up).  FP32 here is compiled, like every kernel of the
+   library, without the packed FP32 instructions, so it shows the rate the
+   mixed form's FP32 stencils issue at.  No counterpart in the reference. */
+int ps3d_fma_probe(int reps, double *out);
+
 /* ---- batched ensemble ------------------------------------------------
    No counterpart in the reference, which advances one model per process
    (main.f90:138-162); an ensemble there is a set of separate runs.  Here

@@ -72,6 +72,7 @@ PROTOTYPES = {
     "ps3d_tuning": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
     "ps3d_diagnose": (_c_int, [_vp, _dp, _i32p]),
     "ps3d_hbm_probe": (_c_int, [_c_i64, _c_int, _c_int, _dp]),
+    "ps3d_fma_probe": (_c_int, [_c_int, _dp]),
     "ps3d_diagnose_noalloc": (_c_int, [ctypes.POINTER(CParams), _vp, _vp, _vp, _vp, _vp, _dp, _i32p]),
     "ps3d_set_forcing": (_c_int, [_vp, _vp, _vp]),
     "ps3d_device_ptr": (_c_int, [_vp, _c_int, _c_int, ctypes.POINTER(_vp)]),

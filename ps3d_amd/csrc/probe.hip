@@ -2,7 +2,9 @@
 // GPU: copy (read + write), read only and write only, over buffers far
 // larger than the caches.  They set the achievable ceiling that the step's
 // kernels are compared with (beside the 8 TB/s vendor peak), and the copy is
-// the known-size kernel that calibrates the HBM byte counters.
+// the known-size kernel that calibrates the HBM byte counters.  And the
+// vector FMA issue rate in FP32 and FP64, which bounds the stencil kernels'
+// arithmetic in the two precision forms.
 #include "ps3d_internal.h"
 
 #include <algorithm>
@@ -74,6 +76,29 @@ __global__ __launch_bounds__(PT) void k_write(vec2 *__restrict__ dst, int64_t n,
             }
         }
     }
+}
+
+// vector issue rate: FMI independent FMA chains per lane, so the SIMDs issue
+// back to back; T = float is compiled, like every kernel here, without the
+// packed FP32 instructions
+constexpr int FMI = 8;
+template <class T>
+__global__ __launch_bounds__(PT) void k_fma(T *__restrict__ sink, T a, T b, int iters)
+{
+    T x[FMI];
+#pragma unroll
+    for (int q = 0; q < FMI; ++q) x[q] = (T)(threadIdx.x + q);
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int q = 0; q < FMI; ++q) {
+            if constexpr (sizeof(T) == 4) x[q] = __builtin_fmaf(x[q], a, b);
+            else x[q] = __builtin_fma(x[q], a, b);
+        }
+    }
+    T s = T(0);
+#pragma unroll
+    for (int q = 0; q < FMI; ++q) s += x[q];
+    sink[(int64_t)blockIdx.x * PT + threadIdx.x] = s;
 }
 
 // best of reps launches, in seconds
@@ -151,6 +176,42 @@ extern "C" int ps3d_hbm_probe(int64_t bytes, int reps, int grid, double *out)
         PS3D_HIP_CHECK(hipStreamSynchronize(s));
         (void)hipFree(a);
         (void)hipFree(b);
+        (void)hipFree(sink);
+        (void)hipStreamDestroy(s);
+    });
+}
+
+extern "C" int ps3d_fma_probe(int reps, double *out)
+{
+    return guard([&] {
+        PS3D_REQUIRE(out, "null output");
+        PS3D_REQUIRE(reps >= 1 && reps <= 1000, "reps: 1 .. 1000");
+        int dev = 0, ncu = 0;
+        PS3D_HIP_CHECK(hipGetDevice(&dev));
+        PS3D_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        const int nb = 32 * std::max(ncu, 1), iters = 4096;
+        double *sink = nullptr;
+        hipStream_t s = nullptr;
+        PS3D_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        try {
+            PS3D_HIP_CHECK(hipMalloc(&sink, sizeof(double) * (size_t)nb * PT));
+            const double flops = 2.0 * FMI * iters * (double)nb * PT;
+            const double t32 = best_time(s, reps, [&] {
+                hipLaunchKernelGGL(k_fma<float>, dim3(nb), dim3(PT), 0, s, reinterpret_cast<float *>(sink), 0.999f,
+                                   0.001f, iters);
+            });
+            const double t64 = best_time(s, reps, [&] {
+                hipLaunchKernelGGL(k_fma<double>, dim3(nb), dim3(PT), 0, s, sink, 0.999, 0.001, iters);
+            });
+            out[0] = flops / t32 * 1e-12;
+            out[1] = flops / t64 * 1e-12;
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(sink);
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        PS3D_HIP_CHECK(hipStreamSynchronize(s));
         (void)hipFree(sink);
         (void)hipStreamDestroy(s);
     });
