@@ -343,13 +343,10 @@ __global__ __launch_bounds__(64, 1) void k_thomas_reg(C *__restrict__ fs, const 
 // the d' sweeps of its part with the operations thomas_reg_col applies to
 // that component: the same bits as every other form.  R = double: the FP64
 // path; R = float: the mixed form (FP32 right-hand side and solution, FP64
-// elimination).  cpl: NZ x 32 doubles of LDS.  Every lane of the
-// work-group calls it and reaches its barrier; lanes past the last column
-// (valid false) load, write and store nothing.
+// elimination).  cpl: NZ x 32 doubles of LDS.
 template <int NZ, class R>
 __device__ __forceinline__ void thomas_split_col(R *__restrict__ fs, const double *__restrict__ ksqr, int64_t ncols,
-                                                 int64_t col, bool valid, double fxa, double dz2, double dsqr,
-                                                 double *cpl)
+                                                 int64_t col, double fxa, double dz2, double dsqr, double *cpl)
 {
     const int part = threadIdx.x & 1;
     double *cpk = cpl + (threadIdx.x >> 1);   // c'(level q) at cpk[32 q]
@@ -360,10 +357,9 @@ __device__ __forceinline__ void thomas_split_col(R *__restrict__ fs, const doubl
     const uint32_t o0 = ((uint32_t)col * 2 + (uint32_t)part) * E, st = (uint32_t)ncols * 2 * E;
     double d[NZ];
 #pragma unroll
-    for (int k = 0; k < NZ; ++k)
-        d[k] = valid ? (double)*reinterpret_cast<const R *>(base + (o0 + (uint32_t)k * st)) : 0.0;
-    const double ks = valid ? ksqr[col] : 0.0;
-    const bool solve = valid && ks > 0;
+    for (int k = 0; k < NZ; ++k) d[k] = (double)*reinterpret_cast<const R *>(base + (o0 + (uint32_t)k * st));
+    const double ks = ksqr[col];
+    const bool solve = ks > 0;
     double x = 0.0;
     if (solve) {
         const double b1 = -ks - fxa;
@@ -393,7 +389,7 @@ __device__ __forceinline__ void thomas_split_col(R *__restrict__ fs, const doubl
             x = d[q] - cpk[32 * q] * x;
             d[q] = x;
         }
-    } else if (valid) {
+    } else {
         // zero mode, pressure_solver.f90:44-50, as in thomas_reg_col
         double pm1 = 0.0;
         double pk = d[0] * dz2 * dsqr;
@@ -410,7 +406,6 @@ __device__ __forceinline__ void thomas_split_col(R *__restrict__ fs, const doubl
             bk = nxt;
         }
     }
-    if (!valid) return;
     uint32_t off = o0;
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
@@ -426,7 +421,8 @@ __global__ __launch_bounds__(64, 1) void k_thomas_split(R *__restrict__ fs, cons
 {
     __shared__ double cpl[NZ * 32];
     const int64_t col = (int64_t)xcd_remap((int)blockIdx.x, (int)gridDim.x) * 32 + (threadIdx.x >> 1);
-    thomas_split_col<NZ, R>(fs, ksqr, ncols, col, col < ncols, fxa, dz2, dsqr, cpl);
+    if (col >= ncols) return;
+    thomas_split_col<NZ, R>(fs, ksqr, ncols, col, fxa, dz2, dsqr, cpl);
 }
 
 // the same for the rigid-lid members of a batched ensemble, 64 columns per
@@ -468,18 +464,18 @@ __global__ __launch_bounds__(64, 1) void k_ens_thomas_split(const EnsMember *__r
     const EnsMember &e = mem[mi];
     const int64_t ncols = (int64_t)e.nkx * e.c.ny;
     const int64_t col = (int64_t)(g - map.start[mi]) * 32 + (threadIdx.x >> 1);
-    const bool valid = col < ncols;
+    if (col >= ncols) return;
     R *fs;
     if constexpr (std::is_same<C, float2>::value) fs = reinterpret_cast<R *>(e.FSf);
     else fs = reinterpret_cast<R *>(e.FS);
     const double fxa = e.fxa, dz2 = e.c.dz * e.c.dz, dsqr = e.c.dsqr;
     switch (e.c.nz) {
-    case 16: thomas_split_col<16, R>(fs, e.ksqr, ncols, col, valid, fxa, dz2, dsqr, cpl); break;
-    case 32: thomas_split_col<32, R>(fs, e.ksqr, ncols, col, valid, fxa, dz2, dsqr, cpl); break;
-    case 48: thomas_split_col<48, R>(fs, e.ksqr, ncols, col, valid, fxa, dz2, dsqr, cpl); break;
-    case 64: thomas_split_col<64, R>(fs, e.ksqr, ncols, col, valid, fxa, dz2, dsqr, cpl); break;
-    case 96: thomas_split_col<96, R>(fs, e.ksqr, ncols, col, valid, fxa, dz2, dsqr, cpl); break;
-    case 128: thomas_split_col<128, R>(fs, e.ksqr, ncols, col, valid, fxa, dz2, dsqr, cpl); break;
+    case 16: thomas_split_col<16, R>(fs, e.ksqr, ncols, col, fxa, dz2, dsqr, cpl); break;
+    case 32: thomas_split_col<32, R>(fs, e.ksqr, ncols, col, fxa, dz2, dsqr, cpl); break;
+    case 48: thomas_split_col<48, R>(fs, e.ksqr, ncols, col, fxa, dz2, dsqr, cpl); break;
+    case 64: thomas_split_col<64, R>(fs, e.ksqr, ncols, col, fxa, dz2, dsqr, cpl); break;
+    case 96: thomas_split_col<96, R>(fs, e.ksqr, ncols, col, fxa, dz2, dsqr, cpl); break;
+    case 128: thomas_split_col<128, R>(fs, e.ksqr, ncols, col, fxa, dz2, dsqr, cpl); break;
     default: break;
     }
 }
